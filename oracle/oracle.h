@@ -1,0 +1,138 @@
+/*
+ * oracle.h -- CPU restatement of smartcore v0.4.0's distance-bound path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the parity checker and the
+ * single-core CPU baseline.  Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may load it.  The product path
+ * (libsmartcore_amd.so) never links, loads or calls it.
+ *
+ * Every function states the reference file:line it restates.  Arithmetic
+ * order is the reference's: sequential left folds, separate roundings, no
+ * FMA contraction (compile with -ffp-contract=off, no -ffast-math).
+ *
+ * Pinning status (what checks each function beyond its own restatement):
+ *   or_squared_distance_* / or_distance   KATs euclidian.rs:83-91,
+ *       manhattan.rs:73-81, minkowski.rs:78-99; numpy sequential sums.
+ *   or_heap_*                              exact arrays heap_select.rs:103-177.
+ *   or_linear_find                         linear_search.rs:151-195 KAT; brute
+ *       force (sorted by distance, index) on tie-free inputs; equivalence
+ *       with the add()-driven heap (pinned by heap_select.rs KATs) on
+ *       tie-heavy integer inputs.  Heap-array order is pinned only through
+ *       that equivalence.
+ *   or_linear_find_radius                  linear_search.rs:166-174 KAT;
+ *       brute-force filter.
+ *   or_knn_*_predict_row                   knn_classifier.rs:328-344,
+ *       knn_regressor.rs:296-336 KATs.
+ *   or_bbd_* / or_kmeans_fit_bbd           bbd_tree.rs:324-364 KAT; labels equal
+ *       brute-force argmin on tie-free blobs.  Centroid bits and distortion
+ *       bits follow tree order and are "parity unpinned" beyond tolerance.
+ *   or_kmeans_fit_brute / or_kmeans_predict kmeans.rs:327-352 rule; labels equal
+ *       the BBD restatement on tie-free blobs.
+ */
+#ifndef SMARTCORE_AMD_ORACLE_H
+#define SMARTCORE_AMD_ORACLE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { OR_F64 = 0, OR_F32 = 1 };
+enum { OR_EUCLIDIAN = 0, OR_MANHATTAN = 1, OR_MINKOWSKI = 2 };
+
+/* ---- distances (src/metrics/distance/{euclidian,manhattan,minkowski}.rs) ---- */
+double or_squared_distance_f64(const double* a, const double* b, size_t d);
+double or_squared_distance_f32(const float* a, const float* b, size_t d);
+/* Distance<T>::distance for the three metrics; returns NaN and sets *err=1
+ * for Minkowski p < 1 (the reference panics "p must be at least 1"). */
+double or_distance(int metric, unsigned p, int dtype, const void* a,
+                   const void* b, size_t d, int* err);
+
+/* ---- HeapSelection<KNNPoint> (src/algorithm/sort/heap_select.rs) ---- */
+typedef struct {
+  double distance;
+  int64_t index; /* -1 stands for Option::None */
+} or_knn_point;
+
+typedef struct {
+  size_t k;
+  size_t n;
+  int sorted;
+  size_t len;
+  or_knn_point* heap; /* capacity k */
+} or_heap;
+
+void or_heap_init(or_heap* h, or_knn_point* storage, size_t k);
+void or_heap_add(or_heap* h, double distance, int64_t index);
+void or_heap_heapify(or_heap* h);
+void or_sift_down(or_knn_point* heap, size_t k, size_t n);
+
+/* ---- LinearKNNSearch (src/algorithm/neighbour/linear_search.rs) ---- */
+/* Returns 0, or 4 (FindFailed) when k < 1 or k > n, or 7 for p < 1.
+ * out_idx/out_dist receive up to k entries in heap-array order; *out_count
+ * is the number kept (entries whose index stayed None are dropped). */
+int or_linear_find(const void* data, int dtype, size_t n, size_t d,
+                   const void* query, size_t k, int metric, unsigned p,
+                   int64_t* out_idx, double* out_dist, size_t* out_count);
+/* q queries, outputs q*k (rows padded with -1 / NaN past count). */
+int or_linear_find_batch(const void* data, int dtype, size_t n, size_t d,
+                         const void* queries, size_t q, size_t k, int metric,
+                         unsigned p, int64_t* out_idx, double* out_dist,
+                         uint32_t* out_count);
+/* Same search driven through or_heap_add() instead of peek_mut+heapify. */
+int or_linear_find_via_add(const void* data, int dtype, size_t n, size_t d,
+                           const void* query, size_t k, int metric, unsigned p,
+                           int64_t* out_idx, double* out_dist,
+                           size_t* out_count);
+/* find_radius: writes at most cap entries, returns the full match count via
+ * *out_count; returns 4 when radius <= 0. */
+int or_linear_find_radius(const void* data, int dtype, size_t n, size_t d,
+                          const void* query, double radius, int metric,
+                          unsigned p, int64_t* out_idx, double* out_dist,
+                          size_t cap, size_t* out_count);
+
+/* ---- KNN estimators (src/neighbors/) ---- */
+/* KNNWeightFunction: 0 = Uniform, 1 = Distance. */
+void or_calc_weights(int weight, const double* dist, size_t k, double* w);
+/* predict_for_row of the classifier: y_class[] are class positions. */
+size_t or_knn_classifier_predict_row(const int64_t* idx, const double* dist,
+                                     size_t count, const int64_t* y_class,
+                                     size_t n_classes, int weight);
+double or_knn_regressor_predict_row(const int64_t* idx, const double* dist,
+                                    size_t count, const double* y, int weight);
+
+/* ---- BBDTree (src/algorithm/neighbour/bbd_tree.rs) ---- */
+typedef struct or_bbd or_bbd;
+or_bbd* or_bbd_new(const void* data, int dtype, size_t n, size_t d);
+void or_bbd_free(or_bbd* t);
+size_t or_bbd_num_nodes(const or_bbd* t);
+double or_bbd_clustering(const or_bbd* t, const double* centroids, size_t k,
+                         double* sums, uint64_t* counts, uint32_t* membership);
+
+/* ---- KMeans (src/cluster/kmeans.rs) ---- */
+/* Initial centroids from initial labels, kmeans.rs:275-292. */
+void or_kmeans_means_from_labels(const void* data, int dtype, size_t n,
+                                 size_t d, const uint32_t* y, size_t k,
+                                 double* centroids, uint64_t* size);
+/* The fit loop kmeans.rs:294-310 starting from given centroids, with the
+ * BBD tree (use_bbd=1, the reference) or brute-force assignment (0).
+ * Returns 0 or 1 (FitFailed for k < 2 / max_iter == 0). */
+int or_kmeans_fit(const void* data, int dtype, size_t n, size_t d, size_t k,
+                  size_t max_iter, int use_bbd, double* centroids_inout,
+                  uint32_t* labels, uint64_t* size, double* distortion,
+                  uint32_t* iters);
+/* One brute-force Lloyd step: assignment by KMeans::predict's rule
+ * (kmeans.rs:333-347), sums in point order, distortion = sum of the
+ * minimum squared distances in point order. */
+double or_lloyd_step_brute(const void* data, int dtype, size_t n, size_t d,
+                           const double* centroids, size_t k, double* sums,
+                           uint64_t* counts, uint32_t* membership);
+void or_kmeans_predict(const void* data, int dtype, size_t n, size_t d,
+                       const double* centroids, size_t k, uint32_t* labels);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

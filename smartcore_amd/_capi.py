@@ -1,0 +1,276 @@
+"""ctypes binding of libsmartcore_amd.so, the C ABI in include/smartcore_amd.h.
+
+The library is loaded from this package's directory (built in-tree by
+``smartcore_amd.build``).  There is no fallback: if the library is missing
+or cannot be loaded, every entry point raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import re
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libsmartcore_amd.so")
+HEADER = os.path.join(os.path.dirname(HERE), "include", "smartcore_amd.h")
+
+SC_OK = 0
+SC_FIT_FAILED = 1
+SC_PREDICT_FAILED = 2
+SC_TRANSFORM_FAILED = 3
+SC_FIND_FAILED = 4
+SC_DECOMPOSITION_FAILED = 5
+SC_SOLUTION_FAILED = 6
+SC_PARAMETERS_ERROR = 7
+SC_INVALID_STATE = 8
+SC_ERR_PANIC = 64
+SC_ERR_DEVICE = 65
+SC_ERR_OUT_OF_MEMORY = 66
+SC_ERR_UNSUPPORTED = 67
+
+SC_F64, SC_F32 = 0, 1
+SC_EUCLIDIAN, SC_MANHATTAN, SC_MINKOWSKI = 0, 1, 2
+SC_PLAIN_FP64, SC_MIXED = 0, 1
+
+# smartcore's FailedError Display strings (src/error/mod.rs:108-121)
+_KIND_TEXT = {
+    SC_FIT_FAILED: "Fit failed",
+    SC_PREDICT_FAILED: "Predict failed",
+    SC_TRANSFORM_FAILED: "Transform failed",
+    SC_FIND_FAILED: "Find failed",
+    SC_DECOMPOSITION_FAILED: "Decomposition failed",
+    SC_SOLUTION_FAILED: "Can't find solution",
+    SC_PARAMETERS_ERROR: "Error in input, check parameters",
+    SC_INVALID_STATE: "Invalid state, this should never happen",
+}
+
+
+class Failed(Exception):
+    """smartcore's Failed{err, msg}; str() is "{err}: {msg}" (error/mod.rs:124-127)."""
+
+    def __init__(self, code: int, msg: str):
+        self.code = code
+        self.msg = msg
+        kind = _KIND_TEXT.get(code)
+        super().__init__(f"{kind}: {msg}" if kind else msg)
+
+
+class Panic(Failed):
+    """A condition the reference turns into a panic (dimension mismatch, p < 1)."""
+
+
+class DeviceError(Failed):
+    """HIP runtime failure or device memory exhaustion."""
+
+
+class sc_stats(C.Structure):
+    _fields_ = [
+        ("queries", C.c_uint64),
+        ("candidates", C.c_uint64),
+        ("overflow_queries", C.c_uint64),
+        ("shortlist_rescored", C.c_uint64),
+        ("uncertified", C.c_uint64),
+    ]
+
+
+_lib = None
+
+
+def declared_symbols() -> list[str]:
+    """Entry points declared in include/smartcore_amd.h."""
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(sc_[a-z0-9_]+)\s*\(", text)))
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(
+            f"{LIB_PATH} is missing: build it with `python -m smartcore_amd.build` "
+            "(there is no CPU fallback)")
+    L = C.CDLL(LIB_PATH)
+    vp, sz, i32, u16, dbl = C.c_void_p, C.c_size_t, C.c_int, C.c_uint16, C.c_double
+    L.sc_index_create.argtypes = [vp, i32, sz, sz, C.POINTER(vp)]
+    L.sc_index_create_device.argtypes = [vp, i32, sz, sz, C.POINTER(vp)]
+    L.sc_index_destroy.argtypes = [vp]
+    L.sc_index_rows.argtypes = [vp]
+    L.sc_index_rows.restype = sz
+    L.sc_index_dim.argtypes = [vp]
+    L.sc_index_dim.restype = sz
+    L.sc_knn_search.argtypes = [vp, vp, sz, sz, i32, u16, i32, vp, vp, vp]
+    L.sc_knn_search_device.argtypes = [vp, vp, sz, sz, i32, u16, i32, vp, vp, vp, vp]
+    L.sc_radius_search.argtypes = [vp, vp, sz, dbl, i32, u16, C.POINTER(vp)]
+    L.sc_radius_result_total.argtypes = [vp]
+    L.sc_radius_result_total.restype = sz
+    L.sc_radius_result_copy.argtypes = [vp, vp, vp, vp]
+    L.sc_radius_result_free.argtypes = [vp]
+    L.sc_radius_result_free.restype = None
+    L.sc_kmeans_step.argtypes = [vp, vp, sz, vp, vp, vp, C.POINTER(dbl)]
+    L.sc_kmeans_run.argtypes = [vp, sz, sz, i32, vp, vp, vp, C.POINTER(dbl), C.POINTER(C.c_uint32)]
+    L.sc_kmeans_predict.argtypes = [vp, i32, sz, sz, vp, sz, vp]
+    L.sc_last_error.argtypes = [C.c_char_p, sz]
+    L.sc_last_error.restype = sz
+    L.sc_last_stats.argtypes = [C.POINTER(sc_stats)]
+    for name in declared_symbols():
+        fn = getattr(L, name)
+        if fn.restype is C.c_int and name not in ("sc_radius_result_free",):
+            fn.restype = C.c_int
+    _lib = L
+    return L
+
+
+def last_error() -> str:
+    n = lib().sc_last_error(None, 0)
+    buf = C.create_string_buffer(n + 1)
+    lib().sc_last_error(buf, n + 1)
+    return buf.value.decode()
+
+
+def last_stats() -> dict:
+    st = sc_stats()
+    lib().sc_last_stats(C.byref(st))
+    return {f: getattr(st, f) for f, _ in sc_stats._fields_}
+
+
+def check(rc: int) -> None:
+    if rc == SC_OK:
+        return
+    msg = last_error()
+    if rc == SC_ERR_PANIC:
+        raise Panic(rc, msg)
+    if rc in (SC_ERR_DEVICE, SC_ERR_OUT_OF_MEMORY):
+        raise DeviceError(rc, msg)
+    raise Failed(rc, msg)
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def dtype_code(dt) -> int:
+    dt = np.dtype(dt)
+    if dt == np.float64:
+        return SC_F64
+    if dt == np.float32:
+        return SC_F32
+    raise TypeError(f"rows must be float64 or float32, got {dt}")
+
+
+class Index:
+    """Device-resident reference rows (an sc_index handle)."""
+
+    def __init__(self, rows: np.ndarray | None = None, *, device_ptr: int | None = None,
+                 n: int = 0, d: int = 0, dtype=np.float64):
+        h = C.c_void_p()
+        if device_ptr is not None:
+            self.dtype = np.dtype(dtype)
+            check(lib().sc_index_create_device(C.c_void_p(device_ptr), dtype_code(self.dtype), n, d,
+                                               C.byref(h)))
+        else:
+            rows = np.ascontiguousarray(rows)
+            if rows.ndim != 2:
+                raise ValueError("rows must be 2-D")
+            self.dtype = rows.dtype
+            n, d = rows.shape
+            check(lib().sc_index_create(_ptr(rows), dtype_code(rows.dtype), n, d, C.byref(h)))
+        self._h = h
+        self.n, self.d = n, d
+
+    @property
+    def handle(self):
+        return self._h
+
+    def close(self) -> None:
+        if getattr(self, "_h", None) and self._h.value:
+            lib().sc_index_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def knn(self, queries: np.ndarray, k: int, metric: int = SC_EUCLIDIAN, p: int = 2,
+            mode: int = SC_PLAIN_FP64):
+        """sc_knn_search: (idx[q,k], dist[q,k], count[q]) in heap-array order."""
+        queries = np.ascontiguousarray(queries, dtype=self.dtype)
+        if queries.ndim == 1:
+            queries = queries[None, :]
+        q = queries.shape[0]
+        if q and queries.shape[1] != self.d:
+            raise Panic(SC_ERR_PANIC, "Input vector sizes are different.")
+        kk = max(int(k), 0)
+        idx = np.empty((q, max(kk, 1)), np.int64)
+        dist = np.empty((q, max(kk, 1)), np.float64)
+        cnt = np.empty(max(q, 1), np.uint32)
+        check(lib().sc_knn_search(self._h, _ptr(queries), q, kk, metric, p, mode, _ptr(idx),
+                                  _ptr(dist), _ptr(cnt)))
+        return idx[:, :kk], dist[:, :kk], cnt[:q]
+
+    def knn_device(self, q_ptr: int, q: int, k: int, idx_ptr: int, dist_ptr: int, cnt_ptr: int,
+                   metric: int = SC_EUCLIDIAN, p: int = 2, mode: int = SC_PLAIN_FP64,
+                   stream: int = 0) -> None:
+        check(lib().sc_knn_search_device(self._h, C.c_void_p(q_ptr), q, k, metric, p, mode,
+                                         C.c_void_p(idx_ptr), C.c_void_p(dist_ptr),
+                                         C.c_void_p(cnt_ptr), C.c_void_p(stream)))
+
+    def radius(self, queries: np.ndarray, radius: float, metric: int = SC_EUCLIDIAN, p: int = 2):
+        """sc_radius_search: list of (idx, dist) arrays, one per query, index order."""
+        queries = np.ascontiguousarray(queries, dtype=self.dtype)
+        if queries.ndim == 1:
+            queries = queries[None, :]
+        q = queries.shape[0]
+        r = C.c_void_p()
+        check(lib().sc_radius_search(self._h, _ptr(queries), q, float(radius), metric, p,
+                                     C.byref(r)))
+        try:
+            total = lib().sc_radius_result_total(r)
+            off = np.empty(q + 1, np.uint64)
+            idx = np.empty(max(total, 1), np.int64)
+            dist = np.empty(max(total, 1), np.float64)
+            check(lib().sc_radius_result_copy(r, _ptr(off), _ptr(idx), _ptr(dist)))
+        finally:
+            lib().sc_radius_result_free(r)
+        return [(idx[off[i]:off[i + 1]].copy(), dist[off[i]:off[i + 1]].copy()) for i in range(q)]
+
+    def kmeans_step(self, centroids: np.ndarray):
+        """sc_kmeans_step: (distortion, sums[k,d], counts[k], membership[n])."""
+        centroids = np.ascontiguousarray(centroids, dtype=np.float64)
+        k = centroids.shape[0]
+        sums = np.zeros((k, self.d))
+        counts = np.zeros(k, np.uint64)
+        memb = np.zeros(max(self.n, 1), np.uint32)
+        dist = C.c_double()
+        check(lib().sc_kmeans_step(self._h, _ptr(centroids), k, _ptr(sums), _ptr(counts),
+                                   _ptr(memb), C.byref(dist)))
+        return dist.value, sums, counts, memb[: self.n]
+
+    def kmeans_run(self, init_centroids: np.ndarray, max_iter: int, mode: int = SC_PLAIN_FP64):
+        """sc_kmeans_run: (centroids, labels, size, distortion, iters)."""
+        cent = np.array(init_centroids, dtype=np.float64, order="C", copy=True)
+        k = cent.shape[0] if cent.ndim == 2 else 0
+        if cent.ndim != 2:
+            cent = np.zeros((max(k, 1), self.d))
+        labels = np.zeros(max(self.n, 1), np.uint32)
+        size = np.zeros(max(k, 1), np.uint64)
+        dist = C.c_double()
+        iters = C.c_uint32()
+        check(lib().sc_kmeans_run(self._h, k, max_iter, mode, _ptr(cent), _ptr(labels), _ptr(size),
+                                  C.byref(dist), C.byref(iters)))
+        return cent, labels[: self.n], size[:k], dist.value, iters.value
+
+
+def kmeans_predict(x: np.ndarray, centroids: np.ndarray) -> np.ndarray:
+    x = np.ascontiguousarray(x)
+    centroids = np.ascontiguousarray(centroids, dtype=np.float64)
+    n, d = x.shape
+    labels = np.zeros(max(n, 1), np.uint32)
+    check(lib().sc_kmeans_predict(_ptr(x), dtype_code(x.dtype), n, d, _ptr(centroids),
+                                  centroids.shape[0], _ptr(labels)))
+    return labels[:n]

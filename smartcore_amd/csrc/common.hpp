@@ -1,0 +1,111 @@
+// Shared host/device helpers of libsmartcore_amd: status plumbing, HIP
+// checks, the device-resident index, and per-call workspace.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+
+#include "../../include/smartcore_amd.h"
+
+namespace sc {
+
+// Row stride (elements) used for every row block in HBM: rows are padded
+// with zeros up to a multiple of kRowPad so that kernels stage whole
+// dimension chunks.  A zero pad adds exact zeros to every distance sum
+// (r = 0 - 0 = 0, |0| = 0, 0^p = 0) and therefore leaves it bit-identical.
+constexpr int kRowPad = 16;
+
+inline size_t padded_dim(size_t d) {
+  const size_t r = (d + kRowPad - 1) / kRowPad * kRowPad;
+  return r ? r : kRowPad;
+}
+
+struct Status {
+  int code = SC_OK;
+  std::string msg;
+};
+
+// Sets the calling thread's last error and returns code.
+int fail(int code, const std::string& msg);
+// Clears the calling thread's last error.
+void clear_error();
+// Thread-local counters of the last search.
+sc_stats& last_stats();
+
+// Converts a HIP error to SC_ERR_DEVICE (or SC_ERR_OUT_OF_MEMORY).
+int hip_fail(hipError_t e, const char* what);
+
+#define SC_HIP(expr)                                   \
+  do {                                                 \
+    hipError_t _e = (expr);                            \
+    if (_e != hipSuccess) return ::sc::hip_fail(_e, #expr); \
+  } while (0)
+
+inline size_t elem_size(sc_dtype t) { return t == SC_F64 ? 8 : 4; }
+
+}  // namespace sc
+
+struct sc_index {
+  sc_dtype dtype;
+  size_t n;
+  size_t d;
+  size_t ld;        // padded row stride in elements
+  int device;       // HIP device ordinal the rows live on
+  void* rows;       // n x ld, dtype, zero padded
+  // Lazily built derived data for the mixed-precision form (guarded by mu).
+  void* rows_bf16;  // n x ldb bf16 (hi part)
+  float* norms2;    // n squared norms (FP32 of the exact FP64 value)
+  size_t ldb;
+  void* mu;         // std::mutex*
+};
+
+namespace sc {
+
+// Stream-ordered scratch allocation that is released when the object dies.
+struct DevBuf {
+  void* p = nullptr;
+  hipStream_t s = nullptr;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() {
+    if (p) (void)hipFreeAsync(p, s);
+  }
+  hipError_t alloc(size_t bytes, hipStream_t stream) {
+    s = stream;
+    return hipMallocAsync(&p, bytes ? bytes : 16, stream);
+  }
+  template <typename T>
+  T* as() const {
+    return static_cast<T*>(p);
+  }
+};
+
+// Copies `n` host rows of `d` elements into a zero-padded device block with
+// stride ld (async on `stream`).
+int upload_padded(const void* host, sc_dtype dtype, size_t n, size_t d, size_t ld, void* dev,
+                  hipStream_t stream);
+// Same for device-resident rows (row stride d).
+int copy_padded_device(const void* src, sc_dtype dtype, size_t n, size_t d, size_t ld, void* dev,
+                       hipStream_t stream);
+
+// The calling host thread's stream for the index's device.
+hipStream_t stream_for(const sc_index* index);
+hipStream_t stream_for_device(int device);
+
+// ---- kernels entry points shared between translation units ----
+
+// Exact (reference arithmetic) kNN over an index; queries are padded device
+// rows (stride index->ld).  Outputs device arrays of q*k / q.
+int knn_exact_device(const sc_index* index, const void* d_queries, size_t q, size_t k,
+                     sc_metric metric, int p, int64_t* d_idx, double* d_dist, uint32_t* d_count,
+                     hipStream_t stream, sc_stats* stats);
+
+int knn_mixed_device(const sc_index* index, const void* d_queries, size_t q, size_t k,
+                     int64_t* d_idx, double* d_dist, uint32_t* d_count, hipStream_t stream,
+                     sc_stats* stats);
+
+}  // namespace sc

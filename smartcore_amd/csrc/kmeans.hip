@@ -1,0 +1,460 @@
+// k-means Lloyd step (BBDTree::clustering, bbd_tree.rs:62-87) and the loop of
+// KMeans::fit (kmeans.rs:294-310), FP64 form.
+//
+//   assign   Tiled FP64 squared distances of every row (widened to f64 as
+//            KMeans::predict does, kmeans.rs:338-342) to every centroid, in the
+//            reference's element order; argmin with strict < so the lowest
+//            index wins ties (kmeans.rs:343-346).  64 rows x 64 centroids per
+//            tile, 4x4 register micro-tiles, LDS-staged dimension chunks.
+//   partial  One workgroup per block of kBlockRows rows: per-cluster sums of
+//            the block's rows in row order (one wave per cluster, lane =
+//            dimension, ballot over labels), per-cluster counts, and the
+//            block's distortion over a fixed-shape tree.
+//   reduce   Sums the block partials in block order and applies
+//            centroid_i = sums_i / size_i when size_i > 0 (kmeans.rs:297-303).
+//   control  The stop rule `if distortion <= dist break` (kmeans.rs:305-309)
+//            evaluated on the device; every kernel of later iterations
+//            returns at once when it has fired, so a whole fit is enqueued
+//            without host round trips.
+// The reduction order is fixed by kBlockRows alone, so results do not
+// depend on launch configuration.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cfloat>
+#include <string>
+#include <vector>
+
+#include "common.hpp"
+#include "kmeans.hpp"
+#include "metric.hpp"
+
+namespace sc {
+namespace {
+
+constexpr int PT = 64;  // rows per assign tile
+constexpr int CT = 64;  // centroids per assign tile
+constexpr int DCK = 16;
+constexpr int NT = 256;
+
+__device__ __forceinline__ void lexmin(double& v, uint32_t& i, double v2, uint32_t i2) {
+  if (v2 < v || (v2 == v && i2 < i)) {
+    v = v2;
+    i = i2;
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(NT) void kmeans_assign_kernel(const T* __restrict__ rows, int64_t n,
+                                                           int ld, const double* __restrict__ cent,
+                                                           int k, uint32_t* __restrict__ labels,
+                                                           double* __restrict__ mind,
+                                                           const KmState* __restrict__ st) {
+  if (st && st->done) return;
+  __shared__ double Ps[DCK][PT + 1];
+  __shared__ double Cs[DCK][CT + 1];
+  const int tid = threadIdx.x;
+  const int tp = tid >> 4;  // rows tp*4 .. tp*4+3
+  const int tc = tid & 15;  // centroids tc + 16*j
+  const int64_t p0 = static_cast<int64_t>(blockIdx.x) * PT;
+
+  double bv[4];
+  uint32_t bi[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    bv[i] = INFINITY;
+    bi[i] = 0xffffffffu;
+  }
+  for (int ct0 = 0; ct0 < k; ct0 += CT) {
+    double acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) acc[i][j] = 0.0;
+    for (int c0 = 0; c0 < ld; c0 += DCK) {
+      __syncthreads();
+      for (int e = tid; e < PT * DCK; e += NT) {
+        const int r = e / DCK, c = e % DCK;
+        const int64_t row = p0 + r;
+        Ps[c][r] = row < n ? static_cast<double>(rows[row * ld + c0 + c]) : 0.0;
+      }
+      for (int e = tid; e < CT * DCK; e += NT) {
+        const int r = e / DCK, c = e % DCK;
+        const int cr = ct0 + r;
+        Cs[c][r] = cr < k ? cent[static_cast<int64_t>(cr) * ld + c0 + c] : 0.0;
+      }
+      __syncthreads();
+#pragma unroll 4
+      for (int kk = 0; kk < DCK; ++kk) {
+        double a[4], b[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) a[i] = Ps[kk][tp * 4 + i];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) b[j] = Cs[kk][tc + 16 * j];
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+          for (int j = 0; j < 4; ++j) acc[i][j] = acc[i][j] + Term<double, kEuclid>::f(a[i], b[j], 2.0);
+      }
+    }
+    // best of this tile per row: own 4 centroids in index order, then the
+    // 16 lanes sharing the row (lexicographic (value, index) minimum equals
+    // the first strict minimum of a sequential scan).
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      double v = INFINITY;
+      uint32_t ix = 0xffffffffu;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int c = ct0 + tc + 16 * j;
+        if (c < k) lexmin(v, ix, acc[i][j], static_cast<uint32_t>(c));
+      }
+#pragma unroll
+      for (int off = 8; off >= 1; off >>= 1) {
+        const double v2 = __shfl_xor(v, off, 16);
+        const uint32_t i2 = __shfl_xor(ix, off, 16);
+        lexmin(v, ix, v2, i2);
+      }
+      if (v < bv[i]) {
+        bv[i] = v;
+        bi[i] = ix;
+      }
+    }
+  }
+  if (tc == 0) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int64_t row = p0 + tp * 4 + i;
+      if (row < n) {
+        // kmeans.rs:334-347: min starts at f64::MAX; nothing below it keeps 0
+        const bool hit = bv[i] < DBL_MAX;
+        labels[row] = hit ? bi[i] : 0u;
+        if (mind) mind[row] = hit ? bv[i] : DBL_MAX;
+      }
+    }
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(NT) void kmeans_partial_kernel(
+    const T* __restrict__ rows, int64_t n, int ld, int d, const uint32_t* __restrict__ labels,
+    const double* __restrict__ mind, int k, double* __restrict__ psum, uint32_t* __restrict__ pcount,
+    double* __restrict__ pdist, const KmState* __restrict__ st) {
+  if (st && st->done) return;
+  __shared__ uint32_t lab[kBlockRows];
+  __shared__ double red[NT];
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int64_t b = blockIdx.x;
+  const int64_t r0 = b * kBlockRows;
+  const int rows_here = static_cast<int>(min<int64_t>(kBlockRows, n - r0));
+  for (int i = tid; i < kBlockRows; i += NT) lab[i] = i < rows_here ? labels[r0 + i] : 0xffffffffu;
+
+  // block distortion: 8 consecutive rows per thread, then a fixed tree
+  {
+    double s = 0.0;
+    const int per = kBlockRows / NT;
+    for (int j = 0; j < per; ++j) {
+      const int i = tid * per + j;
+      if (i < rows_here) s = s + mind[r0 + i];
+    }
+    red[tid] = s;
+  }
+  __syncthreads();
+  for (int w = NT / 2; w >= 1; w >>= 1) {
+    if (tid < w) red[tid] = red[tid] + red[tid + w];
+    __syncthreads();
+  }
+  if (tid == 0) pdist[b] = red[0];
+
+  double* my_psum = psum + b * static_cast<int64_t>(k) * d;
+  for (int c = wave; c < k; c += NT / 64) {
+    uint32_t cnt = 0;
+    for (int dc0 = 0; dc0 < d; dc0 += 64) {
+      const int dim = dc0 + lane;
+      const bool live = dim < d;
+      double acc = 0.0;
+      for (int i0 = 0; i0 < rows_here; i0 += 64) {
+        const int i = i0 + lane;
+        const bool mine = i < rows_here && lab[i] == static_cast<uint32_t>(c);
+        uint64_t m = __ballot(mine);
+        if (dc0 == 0) cnt += static_cast<uint32_t>(__popcll(m));
+        while (m) {
+          // up to 8 rows in flight, added in row order
+          int idx[8];
+          int nb = 0;
+          while (m && nb < 8) {
+            idx[nb++] = __builtin_ctzll(m);
+            m &= m - 1;
+          }
+          double v[8];
+#pragma unroll
+          for (int t = 0; t < 8; ++t)
+            v[t] = (t < nb && live) ? static_cast<double>(rows[(r0 + i0 + idx[t]) * ld + dim]) : 0.0;
+#pragma unroll
+          for (int t = 0; t < 8; ++t)
+            if (t < nb) acc = acc + v[t];
+        }
+      }
+      if (live) my_psum[static_cast<int64_t>(c) * d + dim] = acc;
+    }
+    if (lane == 0) pcount[b * k + c] = cnt;
+  }
+}
+
+__global__ void kmeans_reduce_kernel(int64_t nb, int k, int d, int ld,
+                                     const double* __restrict__ psum,
+                                     const uint32_t* __restrict__ pcount,
+                                     const double* __restrict__ pdist, double* __restrict__ sums,
+                                     uint64_t* __restrict__ counts, double* __restrict__ cent,
+                                     KmState* __restrict__ st, int update) {
+  if (st && st->done) return;
+  const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t == 0) {
+    double s = 0.0;
+    for (int64_t b = 0; b < nb; ++b) s = s + pdist[b];
+    st->dist = s;
+  }
+  if (t >= static_cast<int64_t>(k) * d) return;
+  const int c = static_cast<int>(t / d), j = static_cast<int>(t % d);
+  double s = 0.0;
+  uint64_t cnt = 0;
+  for (int64_t b = 0; b < nb; ++b) {
+    s = s + psum[(b * k + c) * static_cast<int64_t>(d) + j];
+    cnt += pcount[b * k + c];
+  }
+  if (sums) sums[t] = s;
+  if (j == 0 && counts) counts[c] = cnt;
+  if (update && cnt > 0) cent[static_cast<int64_t>(c) * ld + j] = s / static_cast<double>(cnt);
+}
+
+__global__ void kmeans_control_kernel(KmState* st) {
+  if (st->done) return;
+  st->iters += 1;
+  if (st->distortion <= st->dist) {
+    st->done = 1;
+  } else {
+    st->distortion = st->dist;
+  }
+}
+
+__global__ void kmeans_state_init_kernel(KmState* st) {
+  st->distortion = DBL_MAX;
+  st->dist = 0.0;
+  st->iters = 0;
+  st->done = 0;
+  st->fallback = 0;
+}
+
+}  // namespace
+
+int kmeans_assign_launch(const sc_index* ix, const void* rows, int64_t n, const double* d_cent,
+                         int k, uint32_t* d_labels, double* d_mind, const KmState* st,
+                         hipStream_t s) {
+  const unsigned grid = static_cast<unsigned>((n + PT - 1) / PT);
+  if (grid == 0) return SC_OK;
+  if (ix->dtype == SC_F64)
+    hipLaunchKernelGGL(kmeans_assign_kernel<double>, dim3(grid), dim3(NT), 0, s,
+                       static_cast<const double*>(rows), n, static_cast<int>(ix->ld), d_cent, k,
+                       d_labels, d_mind, st);
+  else
+    hipLaunchKernelGGL(kmeans_assign_kernel<float>, dim3(grid), dim3(NT), 0, s,
+                       static_cast<const float*>(rows), n, static_cast<int>(ix->ld), d_cent, k,
+                       d_labels, d_mind, st);
+  SC_HIP(hipGetLastError());
+  return SC_OK;
+}
+
+int kmeans_update_launch(const sc_index* ix, KmWork& w, int k, KmState* st, double* d_sums,
+                         uint64_t* d_counts, double* d_cent, bool update, hipStream_t s) {
+  const int64_t n = static_cast<int64_t>(ix->n);
+  const int d = static_cast<int>(ix->d);
+  const int64_t nb = (n + kBlockRows - 1) / kBlockRows;
+  if (nb > 0) {
+    if (ix->dtype == SC_F64)
+      hipLaunchKernelGGL(kmeans_partial_kernel<double>, dim3(static_cast<unsigned>(nb)), dim3(NT),
+                         0, s, static_cast<const double*>(ix->rows), n, static_cast<int>(ix->ld),
+                         d, w.labels, w.mind, k, w.psum, w.pcount, w.pdist, st);
+    else
+      hipLaunchKernelGGL(kmeans_partial_kernel<float>, dim3(static_cast<unsigned>(nb)), dim3(NT),
+                         0, s, static_cast<const float*>(ix->rows), n, static_cast<int>(ix->ld), d,
+                         w.labels, w.mind, k, w.psum, w.pcount, w.pdist, st);
+    SC_HIP(hipGetLastError());
+  }
+  const int64_t threads = std::max<int64_t>(1, static_cast<int64_t>(k) * d);
+  hipLaunchKernelGGL(kmeans_reduce_kernel, dim3(static_cast<unsigned>((threads + 255) / 256)),
+                     dim3(256), 0, s, nb, k, d, static_cast<int>(ix->ld), w.psum, w.pcount,
+                     w.pdist, d_sums, d_counts, d_cent, st, update ? 1 : 0);
+  SC_HIP(hipGetLastError());
+  return SC_OK;
+}
+
+int KmWork::alloc(const sc_index* ix, int k, hipStream_t s) {
+  const int64_t n = static_cast<int64_t>(ix->n);
+  const int64_t nb = (n + kBlockRows - 1) / kBlockRows;
+  SC_HIP(b_labels.alloc(sizeof(uint32_t) * std::max<int64_t>(n, 1), s));
+  SC_HIP(b_mind.alloc(sizeof(double) * std::max<int64_t>(n, 1), s));
+  SC_HIP(b_psum.alloc(sizeof(double) * std::max<int64_t>(nb, 1) * k * std::max<size_t>(ix->d, 1), s));
+  SC_HIP(b_pcount.alloc(sizeof(uint32_t) * std::max<int64_t>(nb, 1) * k, s));
+  SC_HIP(b_pdist.alloc(sizeof(double) * std::max<int64_t>(nb, 1), s));
+  SC_HIP(b_state.alloc(sizeof(KmState), s));
+  labels = b_labels.as<uint32_t>();
+  mind = b_mind.as<double>();
+  psum = b_psum.as<double>();
+  pcount = b_pcount.as<uint32_t>();
+  pdist = b_pdist.as<double>();
+  state = b_state.as<KmState>();
+  return SC_OK;
+}
+
+int kmeans_state_init(KmState* st, hipStream_t s) {
+  hipLaunchKernelGGL(kmeans_state_init_kernel, dim3(1), dim3(1), 0, s, st);
+  SC_HIP(hipGetLastError());
+  return SC_OK;
+}
+
+int kmeans_control_launch(KmState* st, hipStream_t s) {
+  hipLaunchKernelGGL(kmeans_control_kernel, dim3(1), dim3(1), 0, s, st);
+  SC_HIP(hipGetLastError());
+  return SC_OK;
+}
+
+}  // namespace sc
+
+using namespace sc;
+
+namespace {
+
+int upload_centroids(const double* host, int k, size_t d, size_t ld, double* dev, hipStream_t s) {
+  SC_HIP(hipMemsetAsync(dev, 0, sizeof(double) * k * ld, s));
+  if (d > 0)
+    SC_HIP(hipMemcpy2DAsync(dev, ld * sizeof(double), host, d * sizeof(double), d * sizeof(double),
+                            k, hipMemcpyHostToDevice, s));
+  return SC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int sc_kmeans_step(const sc_index* index, const double* centroids, size_t k, double* sums,
+                   uint64_t* counts, uint32_t* membership, double* distortion) {
+  clear_error();
+  if (!index || !centroids || !sums || !counts || !membership || !distortion)
+    return fail(SC_PARAMETERS_ERROR, "null argument");
+  if (k < 1) return fail(SC_PARAMETERS_ERROR, "k must be >= 1");
+  if (index->n == 0) {
+    for (size_t i = 0; i < k; ++i) counts[i] = 0;
+    for (size_t i = 0; i < k * index->d; ++i) sums[i] = 0.0;
+    *distortion = 0.0;
+    return SC_OK;
+  }
+  SC_HIP(hipSetDevice(index->device));
+  hipStream_t s = stream_for(index);
+  KmWork w;
+  if (int rc = w.alloc(index, static_cast<int>(k), s)) return rc;
+  DevBuf cent, dsums, dcounts;
+  SC_HIP(cent.alloc(sizeof(double) * k * index->ld, s));
+  SC_HIP(dsums.alloc(sizeof(double) * k * std::max<size_t>(index->d, 1), s));
+  SC_HIP(dcounts.alloc(sizeof(uint64_t) * k, s));
+  if (int rc = upload_centroids(centroids, static_cast<int>(k), index->d, index->ld,
+                                cent.as<double>(), s))
+    return rc;
+  if (int rc = kmeans_state_init(w.state, s)) return rc;
+  if (int rc = kmeans_assign_launch(index, index->rows, static_cast<int64_t>(index->n),
+                                    cent.as<double>(), static_cast<int>(k), w.labels, w.mind,
+                                    w.state, s))
+    return rc;
+  if (int rc = kmeans_update_launch(index, w, static_cast<int>(k), w.state, dsums.as<double>(),
+                                    dcounts.as<uint64_t>(), cent.as<double>(), false, s))
+    return rc;
+  KmState hs{};
+  SC_HIP(hipMemcpyAsync(sums, dsums.p, sizeof(double) * k * index->d, hipMemcpyDeviceToHost, s));
+  SC_HIP(hipMemcpyAsync(counts, dcounts.p, sizeof(uint64_t) * k, hipMemcpyDeviceToHost, s));
+  SC_HIP(hipMemcpyAsync(membership, w.labels, sizeof(uint32_t) * index->n, hipMemcpyDeviceToHost, s));
+  SC_HIP(hipMemcpyAsync(&hs, w.state, sizeof(KmState), hipMemcpyDeviceToHost, s));
+  SC_HIP(hipStreamSynchronize(s));
+  *distortion = hs.dist;
+  return SC_OK;
+}
+
+int sc_kmeans_run(const sc_index* index, size_t k, size_t max_iter, sc_mode mode,
+                  double* centroids, uint32_t* labels, uint64_t* size, double* distortion,
+                  uint32_t* iters) {
+  clear_error();
+  // kmeans.rs:257-269
+  if (k < 2) return fail(SC_FIT_FAILED, "invalid number of clusters: " + std::to_string(k));
+  if (max_iter == 0)
+    return fail(SC_FIT_FAILED, "invalid maximum number of iterations: " + std::to_string(max_iter));
+  if (!index || !centroids || !labels || !size || !distortion || !iters)
+    return fail(SC_PARAMETERS_ERROR, "null argument");
+  if (mode != SC_PLAIN_FP64 && mode != SC_MIXED) return fail(SC_ERR_UNSUPPORTED, "unknown mode");
+  SC_HIP(hipSetDevice(index->device));
+  hipStream_t s = stream_for(index);
+  if (mode == SC_MIXED)
+    return kmeans_run_mixed(index, static_cast<int>(k), max_iter, centroids, labels, size,
+                            distortion, iters, s);
+  KmWork w;
+  if (int rc = w.alloc(index, static_cast<int>(k), s)) return rc;
+  DevBuf cent, dcounts;
+  SC_HIP(cent.alloc(sizeof(double) * k * index->ld, s));
+  SC_HIP(dcounts.alloc(sizeof(uint64_t) * k, s));
+  if (int rc = upload_centroids(centroids, static_cast<int>(k), index->d, index->ld,
+                                cent.as<double>(), s))
+    return rc;
+  if (int rc = kmeans_state_init(w.state, s)) return rc;
+  for (size_t it = 0; it < max_iter; ++it) {
+    if (int rc = kmeans_assign_launch(index, index->rows, static_cast<int64_t>(index->n),
+                                      cent.as<double>(), static_cast<int>(k), w.labels, w.mind,
+                                      w.state, s))
+      return rc;
+    if (int rc = kmeans_update_launch(index, w, static_cast<int>(k), w.state, nullptr,
+                                      dcounts.as<uint64_t>(), cent.as<double>(), true, s))
+      return rc;
+    if (int rc = kmeans_control_launch(w.state, s)) return rc;
+  }
+  KmState hs{};
+  SC_HIP(hipMemcpy2DAsync(centroids, index->d * sizeof(double), cent.p, index->ld * sizeof(double),
+                          index->d * sizeof(double), k, hipMemcpyDeviceToHost, s));
+  SC_HIP(hipMemcpyAsync(labels, w.labels, sizeof(uint32_t) * index->n, hipMemcpyDeviceToHost, s));
+  SC_HIP(hipMemcpyAsync(size, dcounts.p, sizeof(uint64_t) * k, hipMemcpyDeviceToHost, s));
+  SC_HIP(hipMemcpyAsync(&hs, w.state, sizeof(KmState), hipMemcpyDeviceToHost, s));
+  SC_HIP(hipStreamSynchronize(s));
+  *distortion = hs.distortion;
+  *iters = hs.iters;
+  return SC_OK;
+}
+
+int sc_kmeans_predict(const void* x, sc_dtype dtype, size_t n, size_t d, const double* centroids,
+                      size_t k, uint32_t* labels) {
+  clear_error();
+  if (!centroids || !labels || (!x && n > 0)) return fail(SC_PARAMETERS_ERROR, "null argument");
+  if (dtype != SC_F64 && dtype != SC_F32) return fail(SC_ERR_UNSUPPORTED, "dtype must be f64 or f32");
+  if (k < 1) return fail(SC_PARAMETERS_ERROR, "k must be >= 1");
+  if (n == 0) return SC_OK;
+  int dev = 0;
+  SC_HIP(hipGetDevice(&dev));
+  hipStream_t s = stream_for_device(dev);
+  sc_index tmp{};
+  tmp.dtype = dtype;
+  tmp.n = n;
+  tmp.d = d;
+  tmp.ld = padded_dim(d);
+  tmp.device = dev;
+  DevBuf rows, cent, lab;
+  SC_HIP(rows.alloc(elem_size(dtype) * n * tmp.ld, s));
+  SC_HIP(cent.alloc(sizeof(double) * k * tmp.ld, s));
+  SC_HIP(lab.alloc(sizeof(uint32_t) * n, s));
+  if (int rc = upload_padded(x, dtype, n, d, tmp.ld, rows.p, s)) return rc;
+  if (int rc = upload_centroids(centroids, static_cast<int>(k), d, tmp.ld, cent.as<double>(), s))
+    return rc;
+  if (int rc = kmeans_assign_launch(&tmp, rows.p, static_cast<int64_t>(n), cent.as<double>(),
+                                    static_cast<int>(k), lab.as<uint32_t>(), nullptr, nullptr, s))
+    return rc;
+  SC_HIP(hipMemcpyAsync(labels, lab.p, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, s));
+  SC_HIP(hipStreamSynchronize(s));
+  return SC_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,406 @@
+// Exact brute-force kNN (LinearKNNSearch::find, linear_search.rs:52-84) for
+// the Euclidian, Manhattan and Minkowski metrics on f64 / f32 rows.
+//
+// Three kernels:
+//   scan    Tiled distance kernel.  A workgroup owns 32 queries x one
+//           contiguous segment of the reference rows and sweeps the segment
+//           in index order, 128 rows per tile, staging dimension chunks of
+//           queries and rows through LDS.  Each thread folds a 4x4 register
+//           micro-tile of sums in the reference's element order.  Per query
+//           the workgroup keeps the k smallest sums seen so far in the
+//           segment; a row whose sum is below that running k-th sum is
+//           appended (index, sum) to the query's candidate list for the
+//           segment.  Those lists are a superset of the rows that enter the
+//           reference heap (an entrant's distance is below the global running
+//           k-th distance, which never exceeds the segment's).
+//   replay  One thread per query replays HeapSelection over the candidate
+//           lists in segment order, i.e. in index order, exactly as
+//           linear_search.rs:69-77 does over all rows: rows that would not
+//           enter the heap are no-ops there too, so the heap array (order
+//           included) is the reference's.
+//   serial  For queries whose candidate list overflowed its capacity (or k
+//           above the tiled limit): one workgroup per query computes 256
+//           distances at a time and lane 0 replays the heap over those that
+//           beat the current root.  Exact for any input, slower.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "common.hpp"
+#include "metric.hpp"
+#include "tile.hpp"
+
+namespace sc {
+namespace {
+
+constexpr int QT = kTileQ;    // queries per workgroup
+constexpr int RT = kTileR;    // reference rows per tile
+constexpr int KMAX_TILED = 64;
+constexpr int NT = kTileThreads;
+
+static_assert(kTileD == kRowPad, "chunk must divide the row padding");
+
+// HeapSelection::sift_down (heap_select.rs:71-86) on (dist, idx) arrays:
+// children of kk are 2kk and 2kk+1; break on Equal or Greater.
+__device__ inline void sift_down(double* hd, int64_t* hi, int kk, int n) {
+  while (2 * kk <= n) {
+    int j = 2 * kk;
+    if (j < n && hd[j] < hd[j + 1]) j += 1;
+    if (hd[kk] == hd[j] || hd[kk] > hd[j]) break;
+    double td = hd[kk];
+    hd[kk] = hd[j];
+    hd[j] = td;
+    int64_t ti = hi[kk];
+    hi[kk] = hi[j];
+    hi[j] = ti;
+    kk = j;
+  }
+}
+
+// Replacing the root of a valid heap and running heapify()
+// (heap_select.rs:42-50) only moves elements in its final sift_down(0, n-1):
+// every sift_down(i >= 1) meets a node already >= its children and breaks.
+__device__ inline void heap_push_root(double* hd, int64_t* hi, int k, double d, int64_t idx) {
+  hd[0] = d;
+  hi[0] = idx;
+  if (k > 1) sift_down(hd, hi, 0, k - 1);
+}
+
+template <typename T, int M>
+__global__ __launch_bounds__(NT) void knn_scan_kernel(
+    const T* __restrict__ refs, int64_t n, int ld, const T* __restrict__ queries, int64_t q, int k,
+    double p, int64_t seg_len, int nseg, uint32_t* __restrict__ cand_idx,
+    double* __restrict__ cand_s, uint32_t* __restrict__ cand_count, int cap) {
+  __shared__ TileSmem<T> sm;
+  __shared__ double sbuf[QT][RT];
+  __shared__ uint32_t mask[QT][RT / 32];
+  __shared__ double topk[QT][KMAX_TILED];
+  __shared__ double thr[QT];
+
+  const int tid = threadIdx.x;
+  const int tq = tid >> 5;  // 0..7 -> queries tq*4 .. tq*4+3
+  const int tr = tid & 31;  // 0..31 -> rows tr + 32*j
+  const int64_t q0 = static_cast<int64_t>(blockIdx.x) * QT;
+  const int seg = blockIdx.y;
+  const int64_t r_begin = static_cast<int64_t>(seg) * seg_len;
+  const int64_t r_end = min(n, r_begin + seg_len);
+
+  for (int e = tid; e < QT * KMAX_TILED; e += NT) (&topk[0][0])[e] = INFINITY;
+  for (int e = tid; e < QT * (RT / 32); e += NT) (&mask[0][0])[e] = 0u;
+  if (tid < QT) thr[tid] = INFINITY;
+
+  uint32_t emitted = 0;  // candidates of query q0+tid (tid < QT)
+  const int64_t my_q = q0 + tid;
+  const size_t cand_base = (tid < QT && my_q < q)
+                               ? (static_cast<size_t>(my_q) * nseg + seg) * static_cast<size_t>(cap)
+                               : 0;
+
+  for (int64_t t0 = r_begin; t0 < r_end; t0 += RT) {
+    double acc[4][4];
+    tile_sums<T, M>(sm, acc, refs, t0, r_end, queries, q0, q, ld, p);
+
+    // Epilogue: mark rows whose sum beats the query's running k-th sum.
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int qi = tq * 4 + i;
+      const double th = thr[qi];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int rl = tr + 32 * j;
+        if (q0 + qi < q && t0 + rl < r_end && acc[i][j] < th) {
+          sbuf[qi][rl] = acc[i][j];
+          atomicOr(&mask[qi][rl >> 5], 1u << (rl & 31));
+        }
+      }
+    }
+    __syncthreads();
+    if (tid < QT && my_q < q) {
+      double* tk = topk[tid];
+      for (int w = 0; w < RT / 32; ++w) {
+        uint32_t bits = mask[tid][w];
+        mask[tid][w] = 0u;
+        while (bits) {
+          const int b = __builtin_ctz(bits);
+          bits &= bits - 1;
+          const int rl = w * 32 + b;
+          const double s = sbuf[tid][rl];
+          if (s < tk[k - 1]) {
+            int pos = k - 1;
+            while (pos > 0 && tk[pos - 1] > s) {
+              tk[pos] = tk[pos - 1];
+              --pos;
+            }
+            tk[pos] = s;
+            if (emitted < static_cast<uint32_t>(cap)) {
+              cand_idx[cand_base + emitted] = static_cast<uint32_t>(t0 + rl);
+              cand_s[cand_base + emitted] = s;
+            }
+            ++emitted;
+          }
+        }
+      }
+      thr[tid] = tk[k - 1];
+    }
+  }
+  if (tid < QT && my_q < q) cand_count[static_cast<size_t>(my_q) * nseg + seg] = emitted;
+}
+
+template <int M>
+__global__ void knn_replay_kernel(int64_t q, int k, double p, int nseg, int cap,
+                                  const uint32_t* __restrict__ cand_idx,
+                                  const double* __restrict__ cand_s,
+                                  const uint32_t* __restrict__ cand_count, int64_t* __restrict__ out_idx,
+                                  double* __restrict__ out_dist, uint32_t* __restrict__ out_count,
+                                  uint32_t* __restrict__ overflow_list,
+                                  uint32_t* __restrict__ overflow_n,
+                                  unsigned long long* __restrict__ total_cand) {
+  const int64_t qi = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (qi >= q) return;
+  double* hd = out_dist + qi * k;
+  int64_t* hi = out_idx + qi * k;
+  for (int j = 0; j < k; ++j) {
+    hd[j] = INFINITY;
+    hi[j] = -1;
+  }
+  unsigned long long seen = 0;
+  for (int s = 0; s < nseg; ++s) {
+    const uint32_t cnt = cand_count[static_cast<size_t>(qi) * nseg + s];
+    if (cnt > static_cast<uint32_t>(cap)) {
+      const uint32_t slot = atomicAdd(overflow_n, 1u);
+      overflow_list[slot] = static_cast<uint32_t>(qi);
+      out_count[qi] = 0;
+      return;
+    }
+    const size_t base = (static_cast<size_t>(qi) * nseg + s) * static_cast<size_t>(cap);
+    for (uint32_t c = 0; c < cnt; ++c) {
+      const double d = finalize<M>(cand_s[base + c], p);
+      if (d < hd[0]) heap_push_root(hd, hi, k, d, static_cast<int64_t>(cand_idx[base + c]));
+    }
+    seen += cnt;
+  }
+  atomicAdd(total_cand, seen);
+  // linear_search.rs:79-83: keep heap-array order, drop None entries.
+  int c = 0;
+  for (int j = 0; j < k; ++j) {
+    if (hi[j] >= 0) {
+      hi[c] = hi[j];
+      hd[c] = hd[j];
+      ++c;
+    }
+  }
+  for (int j = c; j < k; ++j) {
+    hi[j] = -1;
+    hd[j] = NAN;
+  }
+  out_count[qi] = static_cast<uint32_t>(c);
+}
+
+// One workgroup per listed query; persistent over the list so the grid can
+// be launched before the list length is known.
+template <typename T, int M>
+__global__ __launch_bounds__(NT) void knn_serial_kernel(
+    const T* __restrict__ refs, int64_t n, int ld, const T* __restrict__ queries, int k, double p,
+    const uint32_t* __restrict__ list, const uint32_t* __restrict__ list_n, int64_t* __restrict__ out_idx,
+    double* __restrict__ out_dist, uint32_t* __restrict__ out_count) {
+  __shared__ double dvals[NT];
+  __shared__ uint32_t flags[NT / 32];
+  __shared__ double root;
+  const int tid = threadIdx.x;
+  const uint32_t total = *list_n;
+  for (uint32_t li = blockIdx.x; li < total; li += gridDim.x) {
+    const int64_t qi = list[li];
+    const T* qrow = queries + qi * ld;
+    double* hd = out_dist + qi * k;
+    int64_t* hi = out_idx + qi * k;
+    if (tid == 0) {
+      for (int j = 0; j < k; ++j) {
+        hd[j] = INFINITY;
+        hi[j] = -1;
+      }
+      root = INFINITY;
+    }
+    if (tid < NT / 32) flags[tid] = 0u;
+    __syncthreads();
+    for (int64_t c0 = 0; c0 < n; c0 += NT) {
+      const int64_t r = c0 + tid;
+      double d = NAN;
+      if (r < n) {
+        const T* rrow = refs + r * ld;
+        double s = 0.0;
+        for (int j = 0; j < ld; ++j) s = s + Term<T, M>::f(qrow[j], rrow[j], p);
+        d = finalize<M>(s, p);
+      }
+      const double cur = root;
+      dvals[tid] = d;
+      if (d < cur) atomicOr(&flags[tid >> 5], 1u << (tid & 31));
+      __syncthreads();
+      if (tid == 0) {
+        for (int w = 0; w < NT / 32; ++w) {
+          uint32_t bits = flags[w];
+          flags[w] = 0u;
+          while (bits) {
+            const int b = __builtin_ctz(bits);
+            bits &= bits - 1;
+            const int t = w * 32 + b;
+            const double dv = dvals[t];
+            if (dv < hd[0]) heap_push_root(hd, hi, k, dv, c0 + t);
+          }
+        }
+        root = hd[0];
+      }
+      __syncthreads();
+    }
+    if (tid == 0) {
+      int c = 0;
+      for (int j = 0; j < k; ++j) {
+        if (hi[j] >= 0) {
+          hi[c] = hi[j];
+          hd[c] = hd[j];
+          ++c;
+        }
+      }
+      for (int j = c; j < k; ++j) {
+        hi[j] = -1;
+        hd[j] = NAN;
+      }
+      out_count[qi] = static_cast<uint32_t>(c);
+    }
+    __syncthreads();
+  }
+}
+
+__global__ void iota_list_kernel(uint32_t* list, uint32_t* list_n, int64_t q) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < q) list[i] = static_cast<uint32_t>(i);
+  if (i == 0) *list_n = static_cast<uint32_t>(q);
+}
+
+struct Plan {
+  int nseg;
+  int64_t seg_len;
+  int cap;
+};
+
+Plan make_plan(int64_t n, int64_t q, int k) {
+  const int64_t nqt = (q + QT - 1) / QT;
+  const int64_t tiles = (n + RT - 1) / RT;
+  int64_t nseg = std::max<int64_t>(1, (4096 + nqt - 1) / nqt);
+  nseg = std::min<int64_t>(nseg, tiles);
+  nseg = std::min<int64_t>(nseg, 65535);
+  int64_t seg_tiles = (tiles + nseg - 1) / nseg;
+  nseg = (tiles + seg_tiles - 1) / seg_tiles;
+  const int64_t seg_len = seg_tiles * RT;
+  // Expected number of running-k-th improvements over L rows is about
+  // k (1 + ln(L / k)); keep three times that plus slack.
+  const double L = static_cast<double>(std::min<int64_t>(seg_len, n));
+  double expect = k * (1.0 + std::log(std::max(1.0, L / k)));
+  int64_t cap = static_cast<int64_t>(3.0 * expect) + 64;
+  cap = std::min<int64_t>(cap, seg_len);
+  return Plan{static_cast<int>(nseg), seg_len, static_cast<int>(cap)};
+}
+
+template <typename T, int M>
+int run_exact(const sc_index* index, const T* d_queries, int64_t q, int k, double p, int64_t* d_idx,
+              double* d_dist, uint32_t* d_count, hipStream_t stream, sc_stats* stats) {
+  const int64_t n = static_cast<int64_t>(index->n);
+  const int ld = static_cast<int>(index->ld);
+  const T* refs = static_cast<const T*>(index->rows);
+
+  DevBuf flags;  // [0] overflow count, [1..2] candidate total (u64)
+  SC_HIP(flags.alloc(64, stream));
+  SC_HIP(hipMemsetAsync(flags.p, 0, 64, stream));
+  uint32_t* overflow_n = flags.as<uint32_t>();
+  unsigned long long* total_cand = reinterpret_cast<unsigned long long*>(flags.as<char>() + 8);
+  DevBuf olist;
+  SC_HIP(olist.alloc(sizeof(uint32_t) * std::max<int64_t>(q, 1), stream));
+
+  if (k <= KMAX_TILED) {
+    // Bound the candidate store to ~1 GiB by processing query batches.
+    Plan plan = make_plan(n, q, k);
+    const size_t per_query = static_cast<size_t>(plan.nseg) * plan.cap * 12 + plan.nseg * 4;
+    int64_t qb = std::max<int64_t>(QT, static_cast<int64_t>((size_t(1) << 30) / per_query) / QT * QT);
+    qb = std::min(qb, q);
+    DevBuf cidx, cs, ccnt;
+    SC_HIP(cidx.alloc(sizeof(uint32_t) * qb * plan.nseg * plan.cap, stream));
+    SC_HIP(cs.alloc(sizeof(double) * qb * plan.nseg * plan.cap, stream));
+    SC_HIP(ccnt.alloc(sizeof(uint32_t) * qb * plan.nseg, stream));
+    for (int64_t b0 = 0; b0 < q; b0 += qb) {
+      const int64_t qn = std::min(qb, q - b0);
+      dim3 grid(static_cast<unsigned>((qn + QT - 1) / QT), static_cast<unsigned>(plan.nseg));
+      hipLaunchKernelGGL((knn_scan_kernel<T, M>), grid, dim3(NT), 0, stream, refs, n, ld,
+                         d_queries + b0 * ld, qn, k, p, plan.seg_len, plan.nseg,
+                         cidx.as<uint32_t>(), cs.as<double>(), ccnt.as<uint32_t>(), plan.cap);
+      SC_HIP(hipGetLastError());
+      // overflow list indices are absolute query ids: replay writes qi in
+      // batch space, so shift the outputs and fix ids in the serial pass.
+      hipLaunchKernelGGL((knn_replay_kernel<M>), dim3(static_cast<unsigned>((qn + 127) / 128)),
+                         dim3(128), 0, stream, qn, k, p, plan.nseg, plan.cap, cidx.as<uint32_t>(),
+                         cs.as<double>(), ccnt.as<uint32_t>(), d_idx + b0 * k, d_dist + b0 * k,
+                         d_count + b0, olist.as<uint32_t>(), overflow_n, total_cand);
+      SC_HIP(hipGetLastError());
+      hipLaunchKernelGGL((knn_serial_kernel<T, M>), dim3(512), dim3(NT), 0, stream, refs, n, ld,
+                         d_queries + b0 * ld, k, p, olist.as<uint32_t>(), overflow_n,
+                         d_idx + b0 * k, d_dist + b0 * k, d_count + b0);
+      SC_HIP(hipGetLastError());
+      if (stats) {
+        uint32_t ov = 0;
+        SC_HIP(hipMemcpyAsync(&ov, overflow_n, 4, hipMemcpyDeviceToHost, stream));
+        SC_HIP(hipStreamSynchronize(stream));
+        stats->overflow_queries += ov;
+      }
+      SC_HIP(hipMemsetAsync(overflow_n, 0, 4, stream));
+    }
+  } else {
+    hipLaunchKernelGGL(iota_list_kernel, dim3(static_cast<unsigned>((q + 255) / 256)), dim3(256), 0,
+                       stream, olist.as<uint32_t>(), overflow_n, q);
+    SC_HIP(hipGetLastError());
+    hipLaunchKernelGGL((knn_serial_kernel<T, M>), dim3(1024), dim3(NT), 0, stream, refs, n, ld,
+                       d_queries, k, p, olist.as<uint32_t>(), overflow_n, d_idx, d_dist, d_count);
+    SC_HIP(hipGetLastError());
+    if (stats) stats->overflow_queries += static_cast<uint64_t>(q);
+  }
+  if (stats) {
+    unsigned long long tc = 0;
+    SC_HIP(hipMemcpyAsync(&tc, total_cand, 8, hipMemcpyDeviceToHost, stream));
+    SC_HIP(hipStreamSynchronize(stream));
+    stats->candidates += tc;
+    stats->queries += static_cast<uint64_t>(q);
+  }
+  return SC_OK;
+}
+
+template <typename T>
+int dispatch_metric(const sc_index* index, const void* d_queries, size_t q, size_t k,
+                    sc_metric metric, int p, int64_t* d_idx, double* d_dist, uint32_t* d_count,
+                    hipStream_t stream, sc_stats* stats) {
+  const T* qs = static_cast<const T*>(d_queries);
+  switch (metric) {
+    case SC_EUCLIDIAN:
+      return run_exact<T, kEuclid>(index, qs, q, static_cast<int>(k), 2.0, d_idx, d_dist, d_count,
+                                   stream, stats);
+    case SC_MANHATTAN:
+      return run_exact<T, kManhattan>(index, qs, q, static_cast<int>(k), 1.0, d_idx, d_dist,
+                                      d_count, stream, stats);
+    case SC_MINKOWSKI:
+      return run_exact<T, kMinkowski>(index, qs, q, static_cast<int>(k), static_cast<double>(p),
+                                      d_idx, d_dist, d_count, stream, stats);
+  }
+  return fail(SC_ERR_UNSUPPORTED, "unknown metric");
+}
+
+}  // namespace
+
+int knn_exact_device(const sc_index* index, const void* d_queries, size_t q, size_t k,
+                     sc_metric metric, int p, int64_t* d_idx, double* d_dist, uint32_t* d_count,
+                     hipStream_t stream, sc_stats* stats) {
+  if (q == 0) return SC_OK;
+  if (index->dtype == SC_F64)
+    return dispatch_metric<double>(index, d_queries, q, k, metric, p, d_idx, d_dist, d_count,
+                                   stream, stats);
+  return dispatch_metric<float>(index, d_queries, q, k, metric, p, d_idx, d_dist, d_count, stream,
+                                stats);
+}
+
+}  // namespace sc

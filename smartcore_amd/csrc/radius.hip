@@ -1,0 +1,247 @@
+// Brute-force radius search (LinearKNNSearch::find_radius,
+// linear_search.rs:89-108): every row with distance <= radius, in index order.
+//
+// Two passes over the same tiled distance block as the kNN scan: `count`
+// tallies matches per (query, segment); an exclusive scan turns the tallies
+// into CSR offsets; `fill` recomputes the block and writes each query's
+// matches of a tile in index order at its running offset.  A row is tested
+// with a conservative bound on the sum first and then exactly on the
+// finalised distance (sqrt / identity / powf(1/p)), so the decision is the
+// reference's `d <= radius`.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "common.hpp"
+#include "metric.hpp"
+#include "tile.hpp"
+
+struct sc_radius_result {
+  size_t q = 0;
+  std::vector<uint64_t> offsets;
+  std::vector<int64_t> idx;
+  std::vector<double> dist;
+};
+
+namespace sc {
+namespace {
+
+constexpr int QT = kTileQ;
+constexpr int RT = kTileR;
+constexpr int NT = kTileThreads;
+
+template <typename T, int M, bool FILL>
+__global__ __launch_bounds__(NT) void radius_kernel(const T* __restrict__ refs, int64_t n, int ld,
+                                                    const T* __restrict__ queries, int64_t q,
+                                                    double p, double radius, double sum_bound,
+                                                    int64_t seg_len, int nseg,
+                                                    uint64_t* __restrict__ counts,
+                                                    int64_t* __restrict__ out_idx,
+                                                    double* __restrict__ out_dist) {
+  __shared__ TileSmem<T> sm;
+  __shared__ double dbuf[QT][RT];
+  __shared__ uint32_t mask[QT][RT / 32];
+  const int tid = threadIdx.x;
+  const int tq = tid >> 5;
+  const int tr = tid & 31;
+  const int64_t q0 = static_cast<int64_t>(blockIdx.x) * QT;
+  const int seg = blockIdx.y;
+  const int64_t r_begin = static_cast<int64_t>(seg) * seg_len;
+  const int64_t r_end = min(n, r_begin + seg_len);
+  for (int e = tid; e < QT * (RT / 32); e += NT) (&mask[0][0])[e] = 0u;
+  const int64_t my_q = q0 + tid;
+  uint64_t run = 0;
+  if (FILL && tid < QT && my_q < q) run = counts[static_cast<size_t>(my_q) * nseg + seg];
+
+  for (int64_t t0 = r_begin; t0 < r_end; t0 += RT) {
+    double acc[4][4];
+    tile_sums<T, M>(sm, acc, refs, t0, r_end, queries, q0, q, ld, p);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int qi = tq * 4 + i;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int rl = tr + 32 * j;
+        if (q0 + qi < q && t0 + rl < r_end && acc[i][j] <= sum_bound) {
+          const double dv = finalize<M>(acc[i][j], p);
+          if (dv <= radius) {
+            dbuf[qi][rl] = dv;
+            atomicOr(&mask[qi][rl >> 5], 1u << (rl & 31));
+          }
+        }
+      }
+    }
+    __syncthreads();
+    if (tid < QT && my_q < q) {
+      for (int w = 0; w < RT / 32; ++w) {
+        uint32_t bits = mask[tid][w];
+        mask[tid][w] = 0u;
+        if (!FILL) {
+          run += static_cast<uint64_t>(__popc(bits));
+          continue;
+        }
+        while (bits) {
+          const int b = __builtin_ctz(bits);
+          bits &= bits - 1;
+          const int rl = w * 32 + b;
+          out_idx[run] = t0 + rl;
+          out_dist[run] = dbuf[tid][rl];
+          ++run;
+        }
+      }
+    }
+  }
+  if (!FILL && tid < QT && my_q < q) counts[static_cast<size_t>(my_q) * nseg + seg] = run;
+}
+
+template <typename T, int M>
+int run_radius(const sc_index* ix, const T* dq, int64_t q, double p, double radius,
+               sc_radius_result* res, hipStream_t s) {
+  const int64_t n = static_cast<int64_t>(ix->n);
+  const int ld = static_cast<int>(ix->ld);
+  const int64_t nqt = (q + QT - 1) / QT;
+  const int64_t tiles = std::max<int64_t>(1, (n + RT - 1) / RT);
+  int64_t nseg = std::min<int64_t>(std::max<int64_t>(1, (4096 + nqt - 1) / nqt), tiles);
+  nseg = std::min<int64_t>(nseg, 65535);
+  const int64_t seg_tiles = (tiles + nseg - 1) / nseg;
+  nseg = (tiles + seg_tiles - 1) / seg_tiles;
+  const int64_t seg_len = seg_tiles * RT;
+  // Conservative bound on the sum for the exact test d <= radius.
+  double bound;
+  if (M == kEuclid)
+    bound = radius * radius * (1.0 + 1e-12) + 1e-300;
+  else if (M == kManhattan)
+    bound = radius;
+  else
+    bound = std::pow(radius, p) * (1.0 + 1e-9) + 1e-300;
+  if (!std::isfinite(bound)) bound = INFINITY;
+
+  DevBuf cnt;
+  SC_HIP(cnt.alloc(sizeof(uint64_t) * q * nseg, s));
+  dim3 grid(static_cast<unsigned>(nqt), static_cast<unsigned>(nseg));
+  hipLaunchKernelGGL((radius_kernel<T, M, false>), grid, dim3(NT), 0, s, static_cast<const T*>(ix->rows),
+                     n, ld, dq, q, p, radius, bound, seg_len, static_cast<int>(nseg),
+                     cnt.as<uint64_t>(), nullptr, nullptr);
+  SC_HIP(hipGetLastError());
+  std::vector<uint64_t> hc(static_cast<size_t>(q * nseg));
+  SC_HIP(hipMemcpyAsync(hc.data(), cnt.p, sizeof(uint64_t) * hc.size(), hipMemcpyDeviceToHost, s));
+  SC_HIP(hipStreamSynchronize(s));
+  // exclusive scan in (query, segment) order = CSR offsets in index order
+  res->q = static_cast<size_t>(q);
+  res->offsets.assign(static_cast<size_t>(q) + 1, 0);
+  uint64_t total = 0;
+  for (int64_t i = 0; i < q; ++i) {
+    res->offsets[i] = total;
+    for (int64_t sg = 0; sg < nseg; ++sg) {
+      const uint64_t c = hc[i * nseg + sg];
+      hc[i * nseg + sg] = total;
+      total += c;
+    }
+  }
+  res->offsets[q] = total;
+  res->idx.resize(total);
+  res->dist.resize(total);
+  if (total == 0) return SC_OK;
+  DevBuf oi, od;
+  SC_HIP(oi.alloc(sizeof(int64_t) * total, s));
+  SC_HIP(od.alloc(sizeof(double) * total, s));
+  SC_HIP(hipMemcpyAsync(cnt.p, hc.data(), sizeof(uint64_t) * hc.size(), hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL((radius_kernel<T, M, true>), grid, dim3(NT), 0, s, static_cast<const T*>(ix->rows),
+                     n, ld, dq, q, p, radius, bound, seg_len, static_cast<int>(nseg),
+                     cnt.as<uint64_t>(), oi.as<int64_t>(), od.as<double>());
+  SC_HIP(hipGetLastError());
+  SC_HIP(hipMemcpyAsync(res->idx.data(), oi.p, sizeof(int64_t) * total, hipMemcpyDeviceToHost, s));
+  SC_HIP(hipMemcpyAsync(res->dist.data(), od.p, sizeof(double) * total, hipMemcpyDeviceToHost, s));
+  SC_HIP(hipStreamSynchronize(s));
+  return SC_OK;
+}
+
+template <typename T>
+int radius_dispatch(const sc_index* ix, const void* dq, int64_t q, sc_metric metric, int p,
+                    double radius, sc_radius_result* res, hipStream_t s) {
+  const T* qs = static_cast<const T*>(dq);
+  switch (metric) {
+    case SC_EUCLIDIAN:
+      return run_radius<T, kEuclid>(ix, qs, q, 2.0, radius, res, s);
+    case SC_MANHATTAN:
+      return run_radius<T, kManhattan>(ix, qs, q, 1.0, radius, res, s);
+    case SC_MINKOWSKI:
+      return run_radius<T, kMinkowski>(ix, qs, q, static_cast<double>(p), radius, res, s);
+  }
+  return fail(SC_ERR_UNSUPPORTED, "unknown metric");
+}
+
+}  // namespace
+}  // namespace sc
+
+using namespace sc;
+
+extern "C" {
+
+int sc_radius_search(const sc_index* index, const void* queries, size_t q, double radius,
+                     sc_metric metric, uint16_t p, sc_radius_result** out) {
+  clear_error();
+  if (!index || !out) return fail(SC_PARAMETERS_ERROR, "null argument");
+  *out = nullptr;
+  // linear_search.rs:90-95
+  if (!(radius > 0.0)) return fail(SC_FIND_FAILED, "radius should be > 0");
+  if (metric != SC_EUCLIDIAN && metric != SC_MANHATTAN && metric != SC_MINKOWSKI)
+    return fail(SC_ERR_UNSUPPORTED, "unknown metric");
+  if (metric == SC_MINKOWSKI && p < 1) return fail(SC_ERR_PANIC, "p must be at least 1");
+  auto* res = new sc_radius_result();
+  res->q = q;
+  res->offsets.assign(q + 1, 0);
+  if (q == 0 || index->n == 0) {
+    *out = res;
+    return SC_OK;
+  }
+  if (!queries) {
+    delete res;
+    return fail(SC_PARAMETERS_ERROR, "null queries");
+  }
+  hipError_t e = hipSetDevice(index->device);
+  if (e != hipSuccess) {
+    delete res;
+    return hip_fail(e, "hipSetDevice");
+  }
+  hipStream_t s = stream_for(index);
+  int rc;
+  {
+    DevBuf dq;
+    e = dq.alloc(q * index->ld * elem_size(index->dtype), s);
+    if (e != hipSuccess) {
+      delete res;
+      return hip_fail(e, "alloc queries");
+    }
+    rc = upload_padded(queries, index->dtype, q, index->d, index->ld, dq.p, s);
+    if (rc == SC_OK) {
+      rc = index->dtype == SC_F64
+               ? radius_dispatch<double>(index, dq.p, static_cast<int64_t>(q), metric, p, radius, res, s)
+               : radius_dispatch<float>(index, dq.p, static_cast<int64_t>(q), metric, p, radius, res, s);
+    }
+  }
+  if (rc != SC_OK) {
+    delete res;
+    return rc;
+  }
+  *out = res;
+  return SC_OK;
+}
+
+size_t sc_radius_result_total(const sc_radius_result* r) { return r ? r->idx.size() : 0; }
+
+int sc_radius_result_copy(const sc_radius_result* r, uint64_t* offsets, int64_t* idx,
+                          double* dist) {
+  clear_error();
+  if (!r) return fail(SC_PARAMETERS_ERROR, "null result");
+  if (offsets) std::copy(r->offsets.begin(), r->offsets.end(), offsets);
+  if (idx) std::copy(r->idx.begin(), r->idx.end(), idx);
+  if (dist) std::copy(r->dist.begin(), r->dist.end(), dist);
+  return SC_OK;
+}
+
+void sc_radius_result_free(sc_radius_result* r) { delete r; }
+
+}  // extern "C"

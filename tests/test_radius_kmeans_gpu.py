@@ -1,0 +1,109 @@
+"""Parity of the HIP radius search and k-means Lloyd step with the oracle.
+
+Radius search restates linear_search.rs:89-108 (index order, d <= radius);
+the k-means step is BBDTree::clustering (bbd_tree.rs:62-87), whose labels
+equal KMeans::predict's brute-force rule off exact ties; the loop is
+KMeans::fit's (kmeans.rs:294-310).
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from smartcore_amd import _capi as S
+from smartcore_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+IRIS20 = np.array([
+    [5.1, 3.5, 1.4, 0.2], [4.9, 3.0, 1.4, 0.2], [4.7, 3.2, 1.3, 0.2], [4.6, 3.1, 1.5, 0.2],
+    [5.0, 3.6, 1.4, 0.2], [5.4, 3.9, 1.7, 0.4], [4.6, 3.4, 1.4, 0.3], [5.0, 3.4, 1.5, 0.2],
+    [4.4, 2.9, 1.4, 0.2], [4.9, 3.1, 1.5, 0.1], [7.0, 3.2, 4.7, 1.4], [6.4, 3.2, 4.5, 1.5],
+    [6.9, 3.1, 4.9, 1.5], [5.5, 2.3, 4.0, 1.3], [6.5, 2.8, 4.6, 1.5], [5.7, 2.8, 4.5, 1.3],
+    [6.3, 3.3, 4.7, 1.6], [4.9, 2.4, 3.3, 1.0], [6.6, 2.9, 4.6, 1.3], [5.2, 2.7, 3.9, 1.4]])
+
+
+@pytest.mark.parametrize("metric,om", [(S.SC_EUCLIDIAN, O.EUCLIDIAN), (S.SC_MANHATTAN, O.MANHATTAN)])
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_radius_matches_oracle(metric, om, dtype):
+    xtr, _, xq, _ = synth.knn_mixture(seed=7, n=5_000, q=64, d=16)
+    xtr, xq = xtr.astype(dtype), xq.astype(dtype)
+    ix = S.Index(xtr)
+    # radius near the 2% quantile of the first query's distances
+    d0 = np.array([O.distance(om, xq[0], xtr[i]) for i in range(500)])
+    radius = float(np.quantile(d0, 0.02))
+    got = ix.radius(xq, radius, metric)
+    for r in range(xq.shape[0]):
+        oi, od = O.linear_find_radius(xtr, xq[r], radius, om)
+        np.testing.assert_array_equal(got[r][0], oi)
+        np.testing.assert_array_equal(got[r][1].view(np.uint64), od.view(np.uint64))
+
+
+def test_radius_kat_and_errors():
+    ref = np.arange(1, 11, dtype=np.float64)[:, None]
+    ix = S.Index(ref)
+    got = ix.radius(np.array([[5.0]]), 3.0, S.SC_MANHATTAN)
+    # linear_search.rs:166-174: values 2..8 within radius 3 of 5
+    np.testing.assert_array_equal(ref[got[0][0], 0], [2, 3, 4, 5, 6, 7, 8])
+    with pytest.raises(S.Failed) as e:
+        ix.radius(np.array([[5.0]]), 0.0)
+    assert str(e.value) == "Find failed: radius should be > 0"
+
+
+def test_kmeans_step_iris_kat():
+    """bbd_tree.rs:324-364: distortion 10.68 +- 1e-2, sums[0][0] 48.6,
+    sums[1][3] 13.8, membership[17] == 1."""
+    ix = S.Index(IRIS20)
+    c = np.array([[4.86, 3.22, 1.61, 0.29], [6.23, 2.92, 4.48, 1.42]])
+    dist, sums, counts, memb = ix.kmeans_step(c)
+    assert abs(dist - 10.68) < 1e-2
+    assert abs(sums[0][0] - 48.6) < 1e-2
+    assert abs(sums[1][3] - 13.8) < 1e-2
+    assert memb[17] == 1
+    od, osums, ocounts, omemb = O.BBDTree(IRIS20).clustering(c)
+    np.testing.assert_array_equal(memb, omemb)
+    np.testing.assert_array_equal(counts, ocounts)
+    np.testing.assert_allclose(sums, osums, rtol=1e-14)
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_kmeans_step_blobs_vs_bbd(dtype):
+    x, _, init = synth.kmeans_blobs(seed=9, n=30_000, d=16, blobs=10, k=12, dtype=dtype)
+    ix = S.Index(x)
+    dist, sums, counts, memb = ix.kmeans_step(init)
+    tree = O.BBDTree(x)
+    od, osums, ocounts, omemb = tree.clustering(init)
+    np.testing.assert_array_equal(memb, omemb)
+    np.testing.assert_array_equal(counts, ocounts)
+    np.testing.assert_allclose(sums, osums, rtol=1e-11, atol=1e-9)
+    assert abs(dist - od) <= 1e-11 * abs(od)
+    bd, bsums, bcounts, bmemb = O.lloyd_step_brute(x, init)
+    np.testing.assert_array_equal(memb, bmemb)
+
+
+def test_kmeans_run_vs_oracle_fit():
+    x, _, init = synth.kmeans_blobs(seed=10, n=40_000, d=24, blobs=16, k=16)
+    ix = S.Index(x)
+    cent, labels, size, dist, iters = ix.kmeans_run(init, 30)
+    ocent, olabels, osize, odist, oiters = O.kmeans_fit(x, init, 30, use_bbd=True)
+    np.testing.assert_array_equal(labels, olabels)
+    np.testing.assert_array_equal(size, osize)
+    assert iters == oiters
+    rel = np.max(np.abs(cent - ocent)) / np.max(np.abs(ocent))
+    assert rel < 1e-12
+    assert abs(dist - odist) <= 1e-10 * abs(odist)
+
+
+def test_kmeans_run_errors():
+    ix = S.Index(np.ones((4, 3)))
+    with pytest.raises(S.Failed) as e:
+        ix.kmeans_run(np.ones((1, 3)), 10)
+    assert str(e.value) == "Fit failed: invalid number of clusters: 1"
+    with pytest.raises(S.Failed) as e:
+        ix.kmeans_run(np.ones((2, 3)), 0)
+    assert str(e.value) == "Fit failed: invalid maximum number of iterations: 0"
+
+
+def test_kmeans_predict_matches_oracle():
+    x, _, init = synth.kmeans_blobs(seed=12, n=20_000, d=20, blobs=8, k=70)
+    got = S.kmeans_predict(x, init)
+    np.testing.assert_array_equal(got, O.kmeans_predict(x, init))
