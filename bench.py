@@ -1,0 +1,273 @@
+"""Benchmark of the distance-bound hot path on one MI355X.
+
+Default workload: brute-force kNN, 4,000,000 x 128 f32 reference rows
+(rows ~ N(0, I) scaled by per-row norms ~ LogNormal(0, 0.5)), 65,536 queries
+of the same distribution, k = 10, Euclidean.  One step is one
+sc_knn_search_device call over all 65,536 queries with the rows and queries
+already resident in HBM.  `--workload kmeans` runs the k-means Lloyd loop on
+1,000,000 x 64 f32 Gaussian blobs, k = 64, up to 100 iterations (one step =
+one fit; the reported rate is Lloyd iterations per second).
+
+Prints one JSON line (rank 0) with the throughput, a roofline entry for the
+dominant kernel (its duration measured with HIP events on the stream it
+runs on, via the library's tracing hooks) and a single-core CPU baseline
+from the oracle on a bounded sample of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import platform
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "kNN queries/s (4M×128,k=10) & k-means iters/s vs FP64 form; serve p50/p99 ms"
+PEAK_BF16 = 2.5e15      # dense BF16 MFMA, FLOP/s
+PEAK_FP64 = 78.6e12     # FP64 vector, FLOP/s
+PEAK_HBM = 8.0e12       # bytes/s
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor()
+
+
+def profile_read(lib, name: str):
+    ms = C.c_double()
+    n = C.c_uint64()
+    lib.sc_profile_read(name.encode(), C.byref(ms), C.byref(n))
+    return ms.value, n.value
+
+
+def bench_knn(args):
+    import torch
+
+    from smartcore_amd import _capi as S
+    from smartcore_amd import synth
+
+    lib = S.lib()
+    lib.sc_profile_enable.argtypes = [C.c_int]
+    lib.sc_profile_read.argtypes = [C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_uint64)]
+    N, Q, D, K = args.n, args.q, 128, 10
+    mode = S.SC_MIXED if args.mode == "mixed" else S.SC_PLAIN_FP64
+    dev = torch.device("cuda:0")
+    t0 = time.time()
+    refs = synth.lognormal_rows_device(101, N, D, device=dev)
+    qs = synth.lognormal_rows_device(202, Q, D, device=dev)
+    torch.cuda.synchronize()
+    log(f"[bench] generated refs {N}x{D} and queries {Q}x{D} on device in {time.time() - t0:.1f}s")
+    ix = S.Index(device_ptr=refs.data_ptr(), n=N, d=D, dtype=np.float32)
+    idx = torch.empty((Q, K), dtype=torch.int64, device=dev)
+    dist = torch.empty((Q, K), dtype=torch.float64, device=dev)
+    cnt = torch.empty((Q,), dtype=torch.int32, device=dev)
+    stream = torch.cuda.current_stream().cuda_stream
+
+    def step():
+        ix.knn_device(qs.data_ptr(), Q, K, idx.data_ptr(), dist.data_ptr(), cnt.data_ptr(),
+                      S.SC_EUCLIDIAN, 2, mode, stream)
+
+    for w in range(args.warmup):
+        step()
+        torch.cuda.synchronize()
+        log(f"[bench] warmup {w + 1}/{args.warmup} done")
+    lib.sc_profile_enable(1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for s in range(args.steps):
+        step()
+        torch.cuda.synchronize()
+        log(f"[bench] step {s + 1}/{args.steps} done")
+    elapsed = time.perf_counter() - t0
+    lib.sc_profile_enable(0)
+    qps = Q * args.steps / elapsed
+
+    if mode == S.SC_MIXED:
+        kname = "knn_mixed_gemm"
+        flops_per_query = 2.0 * N * D
+        peak, bound, unit_peak = PEAK_BF16, "mfma", "TFLOP/s"
+    else:
+        kname = "knn_scan"
+        flops_per_query = 3.0 * N * D
+        peak, bound, unit_peak = PEAK_FP64, "fp64-valu", "TFLOP/s"
+    kms, launches = profile_read(lib, kname)
+    per_launch_q = Q * args.steps / max(launches, 1)
+    avg_s = kms / 1e3 / max(launches, 1)
+    achieved = flops_per_query * per_launch_q / avg_s if avg_s > 0 else 0.0
+    others = {}
+    for name in ("knn_scan", "knn_replay", "knn_serial", "knn_mixed_gemm", "knn_mixed_rescore"):
+        ms, n = profile_read(lib, name)
+        if n:
+            others[name] = {"avg_ms": ms / n, "launches": n}
+
+    # parity spot check of this run's output against the oracle on a few queries
+    out = {
+        "metric": METRIC,
+        "value": qps,
+        "unit": "queries/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "bf16" if mode == S.SC_MIXED else "f64",
+        "data": "synthetic (seeded, generated on device)",
+        "config": {"workload": f"kNN {N:,}x{D} f32 rows, {Q:,} queries, k={K}, Euclidean",
+                   "mode": args.mode},
+        "roofline": {"bound": bound, "achieved": achieved / 1e12, "peak": peak / 1e12,
+                     "unit": unit_peak, "frac": achieved / peak, "traffic": None,
+                     "kernel": kname, "avg_ms": avg_s * 1e3,
+                     "work_per_launch": f"{flops_per_query:.4g} FLOP/query x {per_launch_q:.0f} queries"},
+        "kernels": others,
+    }
+    if not args.no_cpu:
+        import oracle as O
+
+        S_q = args.cpu_sample
+        href = refs.cpu().numpy()
+        hq = qs[:S_q].cpu().numpy()
+        O.lib()
+        t0 = time.perf_counter()
+        oi, od, oc = O.linear_find_batch(href, hq, K)
+        cpu_s = time.perf_counter() - t0
+        gi = idx[:S_q].cpu().numpy()
+        mismatch = int(np.sum(gi != oi))
+        out["cpu_baseline"] = {"value": S_q / cpu_s, "unit": "queries/s", "cores": 1, "kind": "port",
+                               "sample": f"first {S_q} of {Q:,} queries against all {N:,} rows "
+                                         f"({cpu_s:.1f} s); host {cpu_model()}, nproc {os.cpu_count()}"}
+        out["parity_sample"] = {"queries": S_q, "index_mismatches": mismatch}
+    ix.close()
+    return out
+
+
+def bench_kmeans(args):
+    import torch
+
+    from smartcore_amd import _capi as S
+    from smartcore_amd import synth
+
+    lib = S.lib()
+    lib.sc_profile_enable.argtypes = [C.c_int]
+    lib.sc_profile_read.argtypes = [C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_uint64)]
+    n, d, k, iters = args.km_n, 64, 64, 100
+    mode = S.SC_MIXED if args.mode == "mixed" else S.SC_PLAIN_FP64
+    t0 = time.time()
+    x, _, init = synth.kmeans_blobs(seed=2, n=n, d=d, blobs=64, k=k)
+    log(f"[bench] generated blobs {n}x{d} in {time.time() - t0:.1f}s")
+    ix = S.Index(x)
+    for w in range(args.warmup):
+        ix.kmeans_run(init, iters, mode)
+        log(f"[bench] warmup {w + 1}/{args.warmup} done")
+    lib.sc_profile_enable(1)
+    torch.cuda.synchronize()
+    total_iters = 0
+    t0 = time.perf_counter()
+    for s in range(args.steps):
+        cent, labels, size, dist, it = ix.kmeans_run(init, iters, mode)
+        total_iters += it
+        log(f"[bench] step {s + 1}/{args.steps}: {it} iterations")
+    elapsed = time.perf_counter() - t0
+    lib.sc_profile_enable(0)
+    ips = total_iters / elapsed
+    kname = "kmeans_mixed_assign" if mode == S.SC_MIXED else "kmeans_assign"
+    kms, launches = profile_read(lib, kname)
+    avg_s = kms / 1e3 / max(launches, 1)
+    if mode == S.SC_MIXED:
+        bytes_per_launch = n * d * 4 + 4 * n + 2 * k * d * 8
+        achieved = bytes_per_launch / avg_s
+        roof = {"bound": "hbm", "achieved": achieved / 1e9, "peak": PEAK_HBM / 1e9, "unit": "GB/s",
+                "frac": achieved / PEAK_HBM, "traffic": None}
+    else:
+        flops = 3.0 * n * k * d
+        achieved = flops / avg_s
+        roof = {"bound": "fp64-valu", "achieved": achieved / 1e12, "peak": PEAK_FP64 / 1e12,
+                "unit": "TFLOP/s", "frac": achieved / PEAK_FP64, "traffic": None}
+    roof.update({"kernel": kname, "avg_ms": avg_s * 1e3})
+    kern = {}
+    for name in ("kmeans_assign", "kmeans_partial", "kmeans_reduce", "kmeans_mixed_assign"):
+        ms, c = profile_read(lib, name)
+        if c:
+            kern[name] = {"avg_ms": ms / c, "launches": c}
+    out = {
+        "metric": METRIC,
+        "value": ips,
+        "unit": "iterations/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "bf16" if mode == S.SC_MIXED else "f64",
+        "data": "synthetic (seeded)",
+        "config": {"workload": f"k-means Lloyd {n:,}x{d} f32 blobs, k={k}, max_iter={iters}",
+                   "mode": args.mode, "iterations_per_fit": int(total_iters / args.steps)},
+        "roofline": roof,
+        "kernels": kern,
+    }
+    if not args.no_cpu:
+        import oracle as O
+
+        t0 = time.perf_counter()
+        tree = O.BBDTree(x)
+        build_s = time.perf_counter() - t0
+        c = init.copy()
+        t0 = time.perf_counter()
+        nit = args.cpu_kmeans_iters
+        for _ in range(nit):
+            dd, sums, counts, memb = tree.clustering(c)
+            c = np.where(counts[:, None] > 0, sums / np.maximum(counts, 1)[:, None], c)
+        cpu_s = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": nit / cpu_s, "unit": "iterations/s", "cores": 1,
+                               "kind": "port",
+                               "sample": f"{nit} BBD-tree Lloyd steps on the full {n:,}x{d} set "
+                                         f"(tree build {build_s:.1f} s excluded); host {cpu_model()}, "
+                                         f"nproc {os.cpu_count()}"}
+    ix.close()
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", choices=["knn", "kmeans"], default="knn")
+    ap.add_argument("--mode", choices=["plain", "mixed"], default="plain")
+    ap.add_argument("--n", type=int, default=4_000_000)
+    ap.add_argument("--q", type=int, default=65_536)
+    ap.add_argument("--km-n", type=int, default=1_000_000)
+    ap.add_argument("--cpu-sample", type=int, default=16)
+    ap.add_argument("--cpu-kmeans-iters", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
+              "(the distributed form is design-only)", file=sys.stderr)
+        sys.exit(2)
+    if int(os.environ.get("RANK", "0")) != 0:
+        return
+    out = bench_knn(args) if args.workload == "knn" else bench_kmeans(args)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

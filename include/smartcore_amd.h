@@ -149,6 +149,13 @@ int sc_kmeans_step(const sc_index* index, const double* centroids, size_t k,
 int sc_kmeans_run(const sc_index* index, size_t k, size_t max_iter,
                   sc_mode mode, double* centroids, uint32_t* labels,
                   uint64_t* size, double* distortion, uint32_t* iters);
+/* KMeans::kmeans_plus_plus (kmeans.rs:354-413): initial labels (n) of the
+ * index's rows for k clusters.  The random stream is rand 0.8.5's SmallRng
+ * seeded by get_rng_impl (rand_custom.rs:8-33: has_seed ? seed : 0).  The
+ * per-round distance updates run on the device in the rows' own type; the
+ * sequential cost sum and cumulative scan run on the host in row order. */
+int sc_kmeans_plus_plus(const sc_index* index, size_t k, int has_seed, uint64_t seed,
+                        uint32_t* labels);
 /* KMeans::predict (kmeans.rs:327-352): labels of n rows of x (host,
  * row-major, dtype) against k centroids (k*d f64). */
 int sc_kmeans_predict(const void* x, sc_dtype dtype, size_t n, size_t d,
@@ -159,6 +166,17 @@ int sc_kmeans_predict(const void* x, sc_dtype dtype, size_t n, size_t d,
 size_t sc_last_error(char* buf, size_t len);
 /* Counters of the calling thread's last sc_knn_search* call. */
 int sc_last_stats(sc_stats* out);
+
+/* Kernel tracing.  The reference has no tracing of its own (profiling is
+ * done out of tree, .gitignore:22-24); these hooks time the device kernels.
+ * While enabled, every launch of a kernel family is bracketed by a pair of
+ * HIP events on the stream it runs on.  sc_profile_enable(1) clears earlier
+ * records.  sc_profile_read waits for the recorded events and returns the
+ * summed duration and the launch count of one family: "knn_scan",
+ * "knn_replay", "knn_serial", "knn_mixed_gemm", "knn_mixed_rescore",
+ * "kmeans_assign", "kmeans_partial", "kmeans_reduce", "kmeans_mixed_assign". */
+int sc_profile_enable(int on);
+int sc_profile_read(const char* kernel, double* total_ms, uint64_t* launches);
 
 #ifdef __cplusplus
 }

@@ -74,7 +74,33 @@ def build(verbose: bool = False) -> str:
     os.replace(tmp, OUT)
     if verbose:
         print(f"built {OUT}")
+    mod = build_bindings()
+    if verbose:
+        print(f"built {mod}")
     return OUT
+
+
+def build_bindings() -> str:
+    """Compiles the pybind11 module over the C++ estimator mirror.  It links
+    libsmartcore_amd.so from the same directory (rpath $ORIGIN)."""
+    import sysconfig
+
+    import pybind11
+
+    suffix = sysconfig.get_config_var("EXT_SUFFIX")
+    out = os.path.join(HERE, "smartcore" + suffix)
+    src = os.path.join(HERE, "host", "bindings.cpp")
+    cmd = [
+        os.environ.get("CXX", "g++"), "-O2", "-std=c++17", "-shared", "-fPIC", "-Wall",
+        "-fvisibility=hidden",
+        f"-I{pybind11.get_include()}", f"-I{sysconfig.get_paths()['include']}",
+        src, "-o", out + ".tmp", f"-L{HERE}", "-lsmartcore_amd", "-Wl,-rpath,$ORIGIN",
+    ]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"bindings build failed:\n{r.stderr}")
+    os.replace(out + ".tmp", out)
+    return out
 
 
 if __name__ == "__main__":

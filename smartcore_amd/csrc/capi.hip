@@ -2,9 +2,11 @@
 // points, error and stats reporting.
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <vector>
 
 #include "common.hpp"
 
@@ -82,6 +84,46 @@ int copy_padded_device(const void* src, sc_dtype dtype, size_t n, size_t d, size
   return SC_OK;
 }
 
+namespace {
+struct ProfRecord {
+  std::string name;
+  hipEvent_t start;
+  hipEvent_t stop;
+};
+std::mutex g_prof_mu;
+std::atomic<bool> g_prof_on{false};
+std::vector<ProfRecord> g_prof;
+}  // namespace
+
+ProfScope::ProfScope(const char* name, hipStream_t stream) : name_(name), stream_(stream) {
+  if (!g_prof_on.load(std::memory_order_relaxed)) return;
+  if (hipEventCreate(&start_) != hipSuccess) {
+    start_ = nullptr;
+    return;
+  }
+  (void)hipEventRecord(start_, stream_);
+}
+
+ProfScope::~ProfScope() {
+  if (!start_) return;
+  hipEvent_t stop = nullptr;
+  if (hipEventCreate(&stop) != hipSuccess) {
+    (void)hipEventDestroy(start_);
+    return;
+  }
+  (void)hipEventRecord(stop, stream_);
+  std::lock_guard<std::mutex> lk(g_prof_mu);
+  g_prof.push_back(ProfRecord{name_, start_, stop});
+}
+
+void prof_clear_locked() {
+  for (auto& r : g_prof) {
+    (void)hipEventDestroy(r.start);
+    (void)hipEventDestroy(r.stop);
+  }
+  g_prof.clear();
+}
+
 hipStream_t stream_for(const sc_index* index) { return thread_stream(index->device); }
 hipStream_t stream_for_device(int device) { return thread_stream(device); }
 
@@ -154,13 +196,13 @@ int validate_search(const sc_index* index, size_t k, sc_metric metric, uint16_t 
 
 int search_device_impl(const sc_index* index, const void* d_padded_queries, size_t q, size_t k,
                        sc_metric metric, uint16_t p, sc_mode mode, int64_t* d_idx, double* d_dist,
-                       uint32_t* d_count, hipStream_t s) {
+                       uint32_t* d_count, hipStream_t s, bool with_stats) {
   sc_stats& st = last_stats();
   st = sc_stats{};
+  sc_stats* sp = with_stats ? &st : nullptr;
   if (mode == SC_MIXED && index->dtype == SC_F32)
-    return knn_mixed_device(index, d_padded_queries, q, k, d_idx, d_dist, d_count, s, &st);
-  return knn_exact_device(index, d_padded_queries, q, k, metric, p, d_idx, d_dist, d_count, s,
-                          &st);
+    return knn_mixed_device(index, d_padded_queries, q, k, d_idx, d_dist, d_count, s, sp);
+  return knn_exact_device(index, d_padded_queries, q, k, metric, p, d_idx, d_dist, d_count, s, sp);
 }
 
 }  // namespace
@@ -216,7 +258,7 @@ int sc_knn_search(const sc_index* index, const void* queries, size_t q, size_t k
   SC_HIP(dc.alloc(q * sizeof(uint32_t), s));
   if (int rc = upload_padded(queries, index->dtype, q, index->d, index->ld, dq.p, s)) return rc;
   if (int rc = search_device_impl(index, dq.p, q, k, metric, p, mode, di.as<int64_t>(),
-                                  dd.as<double>(), dc.as<uint32_t>(), s))
+                                  dd.as<double>(), dc.as<uint32_t>(), s, true))
     return rc;
   SC_HIP(hipMemcpyAsync(idx, di.p, q * k * sizeof(int64_t), hipMemcpyDeviceToHost, s));
   SC_HIP(hipMemcpyAsync(dist, dd.p, q * k * sizeof(double), hipMemcpyDeviceToHost, s));
@@ -238,13 +280,14 @@ int sc_knn_search_device(const sc_index* index, const void* d_queries, size_t q,
   SC_HIP(hipSetDevice(index->device));
   hipStream_t s = static_cast<hipStream_t>(stream);
   if (index->ld == index->d) {
-    return search_device_impl(index, d_queries, q, k, metric, p, mode, d_idx, d_dist, d_count, s);
+    return search_device_impl(index, d_queries, q, k, metric, p, mode, d_idx, d_dist, d_count, s,
+                              false);
   }
   DevBuf dq;
   SC_HIP(dq.alloc(q * index->ld * elem_size(index->dtype), s));
   if (int rc = copy_padded_device(d_queries, index->dtype, q, index->d, index->ld, dq.p, s))
     return rc;
-  return search_device_impl(index, dq.p, q, k, metric, p, mode, d_idx, d_dist, d_count, s);
+  return search_device_impl(index, dq.p, q, k, metric, p, mode, d_idx, d_dist, d_count, s, false);
 }
 
 size_t sc_last_error(char* buf, size_t len) {
@@ -260,6 +303,32 @@ size_t sc_last_error(char* buf, size_t len) {
 int sc_last_stats(sc_stats* out) {
   if (!out) return SC_PARAMETERS_ERROR;
   *out = last_stats();
+  return SC_OK;
+}
+
+int sc_profile_enable(int on) {
+  std::lock_guard<std::mutex> lk(g_prof_mu);
+  if (on) prof_clear_locked();
+  g_prof_on.store(on != 0);
+  return SC_OK;
+}
+
+int sc_profile_read(const char* kernel, double* total_ms, uint64_t* launches) {
+  clear_error();
+  if (!kernel || !total_ms || !launches) return fail(SC_PARAMETERS_ERROR, "null argument");
+  std::lock_guard<std::mutex> lk(g_prof_mu);
+  double sum = 0.0;
+  uint64_t cnt = 0;
+  for (auto& r : g_prof) {
+    if (r.name != kernel) continue;
+    SC_HIP(hipEventSynchronize(r.stop));
+    float ms = 0.0f;
+    SC_HIP(hipEventElapsedTime(&ms, r.start, r.stop));
+    sum += ms;
+    ++cnt;
+  }
+  *total_ms = sum;
+  *launches = cnt;
   return SC_OK;
 }
 

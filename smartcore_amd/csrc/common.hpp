@@ -92,6 +92,22 @@ int upload_padded(const void* host, sc_dtype dtype, size_t n, size_t d, size_t l
 int copy_padded_device(const void* src, sc_dtype dtype, size_t n, size_t d, size_t ld, void* dev,
                        hipStream_t stream);
 
+// Kernel timing for the tracing hooks (sc_profile_*): while enabled, a
+// ProfScope records a HIP event pair on `stream` around the launches made in
+// its lifetime and files them under `name`.
+class ProfScope {
+ public:
+  ProfScope(const char* name, hipStream_t stream);
+  ~ProfScope();
+  ProfScope(const ProfScope&) = delete;
+  ProfScope& operator=(const ProfScope&) = delete;
+
+ private:
+  const char* name_;
+  hipStream_t stream_;
+  hipEvent_t start_ = nullptr;
+};
+
 // The calling host thread's stream for the index's device.
 hipStream_t stream_for(const sc_index* index);
 hipStream_t stream_for_device(int device);

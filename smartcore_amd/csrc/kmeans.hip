@@ -28,6 +28,7 @@
 #include "common.hpp"
 #include "kmeans.hpp"
 #include "metric.hpp"
+#include "smallrng.hpp"
 
 namespace sc {
 namespace {
@@ -247,6 +248,30 @@ __global__ void kmeans_state_init_kernel(KmState* st) {
   st->fallback = 0;
 }
 
+
+// One k-means++ round (kmeans.rs:367-379 and the final pass :399-410):
+// d_i = min(d_i, squared_distance(row_i, centroid)) in the rows' own type
+// (the reference calls Euclidian::squared_distance on TX rows here), and the
+// label of a row that improves becomes `label`.
+template <typename T>
+__global__ void kmeanspp_round_kernel(const T* __restrict__ rows, int64_t n, int ld,
+                                      const T* __restrict__ centroid, double* __restrict__ dmin,
+                                      uint32_t* __restrict__ y, uint32_t label) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const T* r = rows + i * ld;
+  double s = 0.0;
+  for (int j = 0; j < ld; ++j) s = s + Term<T, kEuclid>::f(r[j], centroid[j], 2.0);
+  if (s < dmin[i]) {
+    dmin[i] = s;
+    y[i] = label;
+  }
+}
+
+__global__ void fill_kernel(double* p, int64_t n, double v) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
 }  // namespace
 
 int kmeans_assign_launch(const sc_index* ix, const void* rows, int64_t n, const double* d_cent,
@@ -254,6 +279,7 @@ int kmeans_assign_launch(const sc_index* ix, const void* rows, int64_t n, const 
                          hipStream_t s) {
   const unsigned grid = static_cast<unsigned>((n + PT - 1) / PT);
   if (grid == 0) return SC_OK;
+  ProfScope ps("kmeans_assign", s);
   if (ix->dtype == SC_F64)
     hipLaunchKernelGGL(kmeans_assign_kernel<double>, dim3(grid), dim3(NT), 0, s,
                        static_cast<const double*>(rows), n, static_cast<int>(ix->ld), d_cent, k,
@@ -272,6 +298,7 @@ int kmeans_update_launch(const sc_index* ix, KmWork& w, int k, KmState* st, doub
   const int d = static_cast<int>(ix->d);
   const int64_t nb = (n + kBlockRows - 1) / kBlockRows;
   if (nb > 0) {
+    ProfScope ps("kmeans_partial", s);
     if (ix->dtype == SC_F64)
       hipLaunchKernelGGL(kmeans_partial_kernel<double>, dim3(static_cast<unsigned>(nb)), dim3(NT),
                          0, s, static_cast<const double*>(ix->rows), n, static_cast<int>(ix->ld),
@@ -283,6 +310,7 @@ int kmeans_update_launch(const sc_index* ix, KmWork& w, int k, KmState* st, doub
     SC_HIP(hipGetLastError());
   }
   const int64_t threads = std::max<int64_t>(1, static_cast<int64_t>(k) * d);
+  ProfScope ps("kmeans_reduce", s);
   hipLaunchKernelGGL(kmeans_reduce_kernel, dim3(static_cast<unsigned>((threads + 255) / 256)),
                      dim3(256), 0, s, nb, k, d, static_cast<int>(ix->ld), w.psum, w.pcount,
                      w.pdist, d_sums, d_counts, d_cent, st, update ? 1 : 0);
@@ -453,6 +481,70 @@ int sc_kmeans_predict(const void* x, sc_dtype dtype, size_t n, size_t d, const d
                                     static_cast<int>(k), lab.as<uint32_t>(), nullptr, nullptr, s))
     return rc;
   SC_HIP(hipMemcpyAsync(labels, lab.p, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, s));
+  SC_HIP(hipStreamSynchronize(s));
+  return SC_OK;
+}
+
+int sc_kmeans_plus_plus(const sc_index* index, size_t k, int has_seed, uint64_t seed,
+                        uint32_t* labels) {
+  clear_error();
+  if (!index || !labels) return fail(SC_PARAMETERS_ERROR, "null argument");
+  const int64_t n = static_cast<int64_t>(index->n);
+  if (n == 0) return fail(SC_PARAMETERS_ERROR, "no rows");
+  if (k < 1) return fail(SC_PARAMETERS_ERROR, "k must be >= 1");
+  SC_HIP(hipSetDevice(index->device));
+  hipStream_t s = stream_for(index);
+  // rand_custom.rs:8-33: Some(seed) -> seed_from_u64(seed); None -> 0
+  SmallRng rng(has_seed ? seed : 0);
+  const size_t es = elem_size(index->dtype);
+  const size_t row_bytes = index->ld * es;
+  DevBuf cen, dmin, dy;
+  SC_HIP(cen.alloc(row_bytes, s));
+  SC_HIP(dmin.alloc(sizeof(double) * n, s));
+  SC_HIP(dy.alloc(sizeof(uint32_t) * n, s));
+  SC_HIP(hipMemsetAsync(dy.p, 0, sizeof(uint32_t) * n, s));
+  const unsigned grid = static_cast<unsigned>((n + 255) / 256);
+  hipLaunchKernelGGL(fill_kernel, dim3(grid), dim3(256), 0, s, dmin.as<double>(), n, DBL_MAX);
+  SC_HIP(hipGetLastError());
+  const char* base = static_cast<const char*>(index->rows);
+  // kmeans.rs:358-362: first centre is a uniformly drawn row
+  uint64_t pick = rng.gen_range(static_cast<uint64_t>(n));
+  SC_HIP(hipMemcpyAsync(cen.p, base + pick * row_bytes, row_bytes, hipMemcpyDeviceToDevice, s));
+  std::vector<double> hd(static_cast<size_t>(n));
+  auto round = [&](uint32_t label) -> int {
+    if (index->dtype == SC_F64)
+      hipLaunchKernelGGL(kmeanspp_round_kernel<double>, dim3(grid), dim3(256), 0, s,
+                         static_cast<const double*>(index->rows), n, static_cast<int>(index->ld),
+                         cen.as<double>(), dmin.as<double>(), dy.as<uint32_t>(), label);
+    else
+      hipLaunchKernelGGL(kmeanspp_round_kernel<float>, dim3(grid), dim3(256), 0, s,
+                         static_cast<const float*>(index->rows), n, static_cast<int>(index->ld),
+                         cen.as<float>(), dmin.as<double>(), dy.as<uint32_t>(), label);
+    SC_HIP(hipGetLastError());
+    return SC_OK;
+  };
+  for (size_t j = 1; j < k; ++j) {
+    if (int rc = round(static_cast<uint32_t>(j - 1))) return rc;
+    SC_HIP(hipMemcpyAsync(hd.data(), dmin.p, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+    SC_HIP(hipStreamSynchronize(s));
+    // kmeans.rs:381-394: sequential sum, cutoff, linear cumulative scan
+    double sum = 0.0;
+    for (int64_t i = 0; i < n; ++i) sum += hd[i];
+    const double cutoff = rng.gen_f64() * sum;
+    double cost = 0.0;
+    int64_t index_i = 0;
+    while (index_i < n) {
+      cost += hd[index_i];
+      if (cost >= cutoff) break;
+      index_i += 1;
+    }
+    // the reference would index past the end here (only reachable with a
+    // NaN sum); stay on the last row instead
+    if (index_i >= n) index_i = n - 1;
+    SC_HIP(hipMemcpyAsync(cen.p, base + index_i * row_bytes, row_bytes, hipMemcpyDeviceToDevice, s));
+  }
+  if (int rc = round(static_cast<uint32_t>(k - 1))) return rc;
+  SC_HIP(hipMemcpyAsync(labels, dy.p, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, s));
   SC_HIP(hipStreamSynchronize(s));
   return SC_OK;
 }

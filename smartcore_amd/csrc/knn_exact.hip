@@ -329,21 +329,30 @@ int run_exact(const sc_index* index, const T* d_queries, int64_t q, int k, doubl
     for (int64_t b0 = 0; b0 < q; b0 += qb) {
       const int64_t qn = std::min(qb, q - b0);
       dim3 grid(static_cast<unsigned>((qn + QT - 1) / QT), static_cast<unsigned>(plan.nseg));
-      hipLaunchKernelGGL((knn_scan_kernel<T, M>), grid, dim3(NT), 0, stream, refs, n, ld,
-                         d_queries + b0 * ld, qn, k, p, plan.seg_len, plan.nseg,
-                         cidx.as<uint32_t>(), cs.as<double>(), ccnt.as<uint32_t>(), plan.cap);
-      SC_HIP(hipGetLastError());
-      // overflow list indices are absolute query ids: replay writes qi in
-      // batch space, so shift the outputs and fix ids in the serial pass.
-      hipLaunchKernelGGL((knn_replay_kernel<M>), dim3(static_cast<unsigned>((qn + 127) / 128)),
-                         dim3(128), 0, stream, qn, k, p, plan.nseg, plan.cap, cidx.as<uint32_t>(),
-                         cs.as<double>(), ccnt.as<uint32_t>(), d_idx + b0 * k, d_dist + b0 * k,
-                         d_count + b0, olist.as<uint32_t>(), overflow_n, total_cand);
-      SC_HIP(hipGetLastError());
-      hipLaunchKernelGGL((knn_serial_kernel<T, M>), dim3(512), dim3(NT), 0, stream, refs, n, ld,
-                         d_queries + b0 * ld, k, p, olist.as<uint32_t>(), overflow_n,
-                         d_idx + b0 * k, d_dist + b0 * k, d_count + b0);
-      SC_HIP(hipGetLastError());
+      {
+        ProfScope ps("knn_scan", stream);
+        hipLaunchKernelGGL((knn_scan_kernel<T, M>), grid, dim3(NT), 0, stream, refs, n, ld,
+                           d_queries + b0 * ld, qn, k, p, plan.seg_len, plan.nseg,
+                           cidx.as<uint32_t>(), cs.as<double>(), ccnt.as<uint32_t>(), plan.cap);
+        SC_HIP(hipGetLastError());
+      }
+      {
+        ProfScope ps("knn_replay", stream);
+        hipLaunchKernelGGL((knn_replay_kernel<M>), dim3(static_cast<unsigned>((qn + 127) / 128)),
+                           dim3(128), 0, stream, qn, k, p, plan.nseg, plan.cap,
+                           cidx.as<uint32_t>(), cs.as<double>(), ccnt.as<uint32_t>(),
+                           d_idx + b0 * k, d_dist + b0 * k, d_count + b0, olist.as<uint32_t>(),
+                           overflow_n, total_cand);
+        SC_HIP(hipGetLastError());
+      }
+      {
+        // batch-relative query ids in the overflow list
+        ProfScope ps("knn_serial", stream);
+        hipLaunchKernelGGL((knn_serial_kernel<T, M>), dim3(512), dim3(NT), 0, stream, refs, n,
+                           ld, d_queries + b0 * ld, k, p, olist.as<uint32_t>(), overflow_n,
+                           d_idx + b0 * k, d_dist + b0 * k, d_count + b0);
+        SC_HIP(hipGetLastError());
+      }
       if (stats) {
         uint32_t ov = 0;
         SC_HIP(hipMemcpyAsync(&ov, overflow_n, 4, hipMemcpyDeviceToHost, stream));

@@ -1,0 +1,311 @@
+// Python bindings of the C++ estimator mirror (include/smartcore_amd.hpp).
+//
+// Exposes smartcore's names -- KNNClassifier, KNNRegressor, KMeans, their
+// parameter structs with the with_* builders, Distances, KNNAlgorithmName,
+// KNNWeightFunction, LinearKNNSearch, KMeansSearchParameters -- for f64 and
+// f32 rows, so the parity tests can read like the reference's own.
+#include <pybind11/numpy.h>
+#include <pybind11/pybind11.h>
+#include <pybind11/stl.h>
+
+#include <variant>
+
+#include "../../include/smartcore_amd.hpp"
+
+namespace py = pybind11;
+using namespace smartcore;
+
+namespace {
+
+struct PyDistance {
+  int metric = SC_EUCLIDIAN;
+  uint16_t p = 2;
+};
+
+struct PyKNNParams {
+  PyDistance distance{};
+  KNNAlgorithmName algorithm = KNNAlgorithmName::CoverTree;
+  KNNWeightFunction weight = KNNWeightFunction::Uniform;
+  size_t k = 3;
+  SearchMode mode = SearchMode::PlainFp64;
+};
+
+template <typename T>
+DenseMatrix<T> to_matrix(py::array_t<T, py::array::c_style | py::array::forcecast> a) {
+  if (a.ndim() != 2) throw Failed::input("x must be a 2-D array");
+  return DenseMatrix<T>::from_row_major(a.shape(0), a.shape(1), a.data());
+}
+
+template <typename T, typename D>
+KNNClassifierParameters<T, D> params_for(const PyKNNParams& p, D d) {
+  KNNClassifierParameters<T, D> out{std::move(d)};
+  out.algorithm = p.algorithm;
+  out.weight = p.weight;
+  out.k = p.k;
+  out.mode = p.mode;
+  return out;
+}
+
+bool is_f32(const py::array& a) { return py::dtype::of<float>().is(a.dtype()); }
+
+template <template <typename, typename, typename> class Est, typename TY>
+class PyKNN {
+ public:
+  using V = std::variant<Est<double, TY, Euclidian<double>>, Est<double, TY, Manhattan<double>>,
+                         Est<double, TY, Minkowski<double>>, Est<float, TY, Euclidian<float>>,
+                         Est<float, TY, Manhattan<float>>, Est<float, TY, Minkowski<float>>>;
+
+  static PyKNN fit(py::array x, py::array_t<TY, py::array::c_style | py::array::forcecast> y,
+                   const PyKNNParams& p) {
+    std::vector<TY> yv(y.data(), y.data() + y.size());
+    if (is_f32(x)) return PyKNN(fit_t<float>(x, yv, p), true);
+    return PyKNN(fit_t<double>(x, yv, p), false);
+  }
+
+  py::array_t<TY> predict(py::array x) const {
+    std::vector<TY> out = std::visit(
+        [&](const auto& m) -> std::vector<TY> {
+          using TX = typename std::decay_t<decltype(m)>::value_type;
+          auto a = py::array_t<TX, py::array::c_style | py::array::forcecast>::ensure(x);
+          return m.predict(to_matrix<TX>(a));
+        },
+        m_);
+    py::array_t<TY> r(out.size());
+    std::copy(out.begin(), out.end(), r.mutable_data());
+    return r;
+  }
+
+ private:
+  PyKNN(V m, bool f32) : m_(std::move(m)), f32_(f32) {}
+
+  template <typename T>
+  static V fit_t(py::array x, const std::vector<TY>& y, const PyKNNParams& p) {
+    auto mx = to_matrix<T>(py::array_t<T, py::array::c_style | py::array::forcecast>::ensure(x));
+    switch (p.distance.metric) {
+      case SC_EUCLIDIAN:
+        return Est<T, TY, Euclidian<T>>::fit(mx, y, params_for<T>(p, Euclidian<T>{}));
+      case SC_MANHATTAN:
+        return Est<T, TY, Manhattan<T>>::fit(mx, y, params_for<T>(p, Manhattan<T>{}));
+      default:
+        return Est<T, TY, Minkowski<T>>::fit(mx, y, params_for<T>(p, Minkowski<T>(p.distance.p)));
+    }
+  }
+
+  V m_;
+  bool f32_;
+};
+
+class PyKMeans {
+ public:
+  using V = std::variant<KMeans<double, uint32_t>, KMeans<float, uint32_t>>;
+
+  static PyKMeans fit(py::array x, const KMeansParameters& p) {
+    if (is_f32(x))
+      return PyKMeans(KMeans<float, uint32_t>::fit(to_matrix<float>(ensure<float>(x)), p));
+    return PyKMeans(KMeans<double, uint32_t>::fit(to_matrix<double>(ensure<double>(x)), p));
+  }
+  static PyKMeans fit_with_centroids(py::array x, const KMeansParameters& p,
+                                     py::array_t<double, py::array::c_style | py::array::forcecast> c) {
+    std::vector<double> cv(c.data(), c.data() + c.size());
+    if (is_f32(x))
+      return PyKMeans(KMeans<float, uint32_t>::fit_with_centroids(to_matrix<float>(ensure<float>(x)), p, cv));
+    return PyKMeans(KMeans<double, uint32_t>::fit_with_centroids(to_matrix<double>(ensure<double>(x)), p, cv));
+  }
+  py::array_t<uint32_t> predict(py::array x) const {
+    std::vector<uint32_t> out = std::visit(
+        [&](const auto& m) {
+          using M = std::decay_t<decltype(m)>;
+          if constexpr (std::is_same<M, KMeans<float, uint32_t>>::value)
+            return m.predict(to_matrix<float>(ensure<float>(x)));
+          else
+            return m.predict(to_matrix<double>(ensure<double>(x)));
+        },
+        m_);
+    return vec(out);
+  }
+  py::array_t<uint32_t> labels() const {
+    return std::visit([](const auto& m) { return vec(m.labels()); }, m_);
+  }
+  py::array_t<uint64_t> size() const {
+    return std::visit([](const auto& m) { return vec(m.size()); }, m_);
+  }
+  py::array_t<double> centroids() const {
+    return std::visit(
+        [](const auto& m) {
+          auto v = vec(m.centroids());
+          v.resize({static_cast<py::ssize_t>(m.k()), static_cast<py::ssize_t>(m.centroids().size() / m.k())});
+          return v;
+        },
+        m_);
+  }
+  double distortion() const {
+    return std::visit([](const auto& m) { return m.distortion(); }, m_);
+  }
+  uint32_t iterations() const {
+    return std::visit([](const auto& m) { return m.iterations(); }, m_);
+  }
+
+ private:
+  explicit PyKMeans(V m) : m_(std::move(m)) {}
+  template <typename T>
+  static py::array_t<T, py::array::c_style | py::array::forcecast> ensure(py::array x) {
+    return py::array_t<T, py::array::c_style | py::array::forcecast>::ensure(x);
+  }
+  template <typename T>
+  static py::array_t<T> vec(const std::vector<T>& v) {
+    py::array_t<T> r(v.size());
+    std::copy(v.begin(), v.end(), r.mutable_data());
+    return r;
+  }
+  V m_;
+};
+
+// LinearKNNSearch over f64 or f32 rows with a chosen metric.
+class PyLinearKNNSearch {
+ public:
+  PyLinearKNNSearch(py::array data, const PyDistance& dist) {
+    if (is_f32(data)) {
+      init<float>(data, dist);
+    } else {
+      init<double>(data, dist);
+    }
+  }
+  py::list find(py::array from, size_t k) const { return f_find(from, k); }
+  py::list find_radius(py::array from, double radius) const { return f_radius(from, radius); }
+
+ private:
+  template <typename T>
+  void init(py::array data, const PyDistance& dist) {
+    auto a = py::array_t<T, py::array::c_style | py::array::forcecast>::ensure(data);
+    const size_t n = a.shape(0), d = a.shape(1);
+    std::vector<std::vector<T>> rows(n, std::vector<T>(d));
+    for (size_t i = 0; i < n; ++i) std::copy(a.data() + i * d, a.data() + (i + 1) * d, rows[i].begin());
+    auto bind = [&](auto metric) {
+      using D = decltype(metric);
+      auto s = std::make_shared<LinearKNNSearch<T, D>>(rows, metric);
+      f_find = [s](py::array from, size_t k) {
+        auto q = py::array_t<T, py::array::c_style | py::array::forcecast>::ensure(from);
+        std::vector<T> qv(q.data(), q.data() + q.size());
+        py::list out;
+        for (auto& t : s->find(qv, k)) out.append(py::make_tuple(std::get<0>(t), std::get<1>(t)));
+        return out;
+      };
+      f_radius = [s](py::array from, double radius) {
+        auto q = py::array_t<T, py::array::c_style | py::array::forcecast>::ensure(from);
+        std::vector<T> qv(q.data(), q.data() + q.size());
+        py::list out;
+        for (auto& t : s->find_radius(qv, radius)) out.append(py::make_tuple(std::get<0>(t), std::get<1>(t)));
+        return out;
+      };
+    };
+    if (dist.metric == SC_EUCLIDIAN)
+      bind(Euclidian<T>{});
+    else if (dist.metric == SC_MANHATTAN)
+      bind(Manhattan<T>{});
+    else
+      bind(Minkowski<T>(dist.p));
+  }
+  std::function<py::list(py::array, size_t)> f_find;
+  std::function<py::list(py::array, double)> f_radius;
+};
+
+}  // namespace
+
+PYBIND11_MODULE(smartcore, m) {
+  m.doc() = "smartcore's kNN / k-means estimator API on the MI355X C ABI";
+  static py::exception<Failed> failed_exc(m, "Failed");
+  static py::exception<Panic> panic_exc(m, "Panic");
+  py::register_exception_translator([](std::exception_ptr p) {
+    try {
+      if (p) std::rethrow_exception(p);
+    } catch (const Failed& e) {
+      py::object err = failed_exc;
+      py::object inst = err(e.what());
+      inst.attr("code") = static_cast<int>(e.error());
+      inst.attr("msg") = e.message();
+      PyErr_SetObject(failed_exc.ptr(), inst.ptr());
+    } catch (const Panic& e) {
+      PyErr_SetString(panic_exc.ptr(), e.what());
+    }
+  });
+
+  py::class_<PyDistance>(m, "Distance").def_readonly("metric", &PyDistance::metric).def_readonly("p", &PyDistance::p);
+  m.def("Euclidian", []() { return PyDistance{SC_EUCLIDIAN, 2}; });
+  m.def("Manhattan", []() { return PyDistance{SC_MANHATTAN, 1}; });
+  m.def("Minkowski", [](uint16_t p) { return PyDistance{SC_MINKOWSKI, p}; });
+  py::class_<Distances>(m, "Distances")
+      .def_static("euclidian", []() { return PyDistance{SC_EUCLIDIAN, 2}; })
+      .def_static("manhattan", []() { return PyDistance{SC_MANHATTAN, 1}; })
+      .def_static("minkowski", [](uint16_t p) { return PyDistance{SC_MINKOWSKI, p}; });
+
+  py::enum_<KNNAlgorithmName>(m, "KNNAlgorithmName")
+      .value("LinearSearch", KNNAlgorithmName::LinearSearch)
+      .value("CoverTree", KNNAlgorithmName::CoverTree);
+  py::enum_<KNNWeightFunction>(m, "KNNWeightFunction")
+      .value("Uniform", KNNWeightFunction::Uniform)
+      .value("Distance", KNNWeightFunction::Distance);
+  py::enum_<SearchMode>(m, "SearchMode")
+      .value("PlainFp64", SearchMode::PlainFp64)
+      .value("Mixed", SearchMode::Mixed);
+
+  auto params = [&](const char* name) {
+    py::class_<PyKNNParams>(m, name)
+        .def(py::init<>())
+        .def_readwrite("distance", &PyKNNParams::distance)
+        .def_readwrite("algorithm", &PyKNNParams::algorithm)
+        .def_readwrite("weight", &PyKNNParams::weight)
+        .def_readwrite("k", &PyKNNParams::k)
+        .def_readwrite("mode", &PyKNNParams::mode)
+        .def("with_k", [](PyKNNParams p, size_t k) { p.k = k; return p; })
+        .def("with_distance", [](PyKNNParams p, PyDistance d) { p.distance = d; return p; })
+        .def("with_algorithm", [](PyKNNParams p, KNNAlgorithmName a) { p.algorithm = a; return p; })
+        .def("with_weight", [](PyKNNParams p, KNNWeightFunction w) { p.weight = w; return p; })
+        .def("with_mode", [](PyKNNParams p, SearchMode md) { p.mode = md; return p; });
+  };
+  params("KNNClassifierParameters");
+  m.attr("KNNRegressorParameters") = m.attr("KNNClassifierParameters");
+
+  using Cls = PyKNN<KNNClassifier, int32_t>;
+  py::class_<Cls>(m, "KNNClassifier")
+      .def_static("fit", &Cls::fit, py::arg("x"), py::arg("y"),
+                  py::arg("parameters") = PyKNNParams{})
+      .def("predict", &Cls::predict);
+  using Reg = PyKNN<KNNRegressor, double>;
+  py::class_<Reg>(m, "KNNRegressor")
+      .def_static("fit", &Reg::fit, py::arg("x"), py::arg("y"),
+                  py::arg("parameters") = PyKNNParams{})
+      .def("predict", &Reg::predict);
+
+  py::class_<KMeansParameters>(m, "KMeansParameters")
+      .def(py::init<>())
+      .def_readwrite("k", &KMeansParameters::k)
+      .def_readwrite("max_iter", &KMeansParameters::max_iter)
+      .def_readwrite("seed", &KMeansParameters::seed)
+      .def_readwrite("mode", &KMeansParameters::mode)
+      .def("with_k", &KMeansParameters::with_k)
+      .def("with_max_iter", &KMeansParameters::with_max_iter)
+      .def("with_mode", &KMeansParameters::with_mode);
+  py::class_<KMeansSearchParameters>(m, "KMeansSearchParameters")
+      .def(py::init<>())
+      .def_readwrite("k", &KMeansSearchParameters::k)
+      .def_readwrite("max_iter", &KMeansSearchParameters::max_iter)
+      .def_readwrite("seed", &KMeansSearchParameters::seed)
+      .def("__iter__", [](const KMeansSearchParameters& s) {
+        return py::iter(py::cast(s.grid()));
+      });
+  py::class_<PyKMeans>(m, "KMeans")
+      .def_static("fit", &PyKMeans::fit, py::arg("x"), py::arg("parameters") = KMeansParameters{})
+      .def_static("fit_with_centroids", &PyKMeans::fit_with_centroids)
+      .def("predict", &PyKMeans::predict)
+      .def_property_readonly("labels", &PyKMeans::labels)
+      .def_property_readonly("size", &PyKMeans::size)
+      .def_property_readonly("centroids", &PyKMeans::centroids)
+      .def_property_readonly("distortion", &PyKMeans::distortion)
+      .def_property_readonly("iterations", &PyKMeans::iterations);
+
+  py::class_<PyLinearKNNSearch>(m, "LinearKNNSearch")
+      .def(py::init<py::array, const PyDistance&>(), py::arg("data"),
+           py::arg("distance") = PyDistance{})
+      .def("find", &PyLinearKNNSearch::find)
+      .def("find_radius", &PyLinearKNNSearch::find_radius);
+}

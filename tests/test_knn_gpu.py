@@ -114,13 +114,14 @@ def test_overflow_falls_back_to_sequential_kernel():
     candidate lists overflow and the sequential kernel takes the queries."""
     n = 60_000
     ref = np.arange(n, 0, -1, dtype=np.float64)[:, None]
-    qs = np.array([[0.0], [-1.0]])
+    # many queries -> few, long segments whose candidate lists overflow
+    qs = -np.arange(4_096, dtype=np.float64)[:, None] * 1e-3
     ix = S.Index(ref)
     gi, gd, gc = ix.knn(qs, 10)
     st = S.last_stats()
     oi, od, oc = O.linear_find_batch(ref, qs, 10)
     assert_exact((gi, gd, gc), (oi, od, oc))
-    assert st["overflow_queries"] == 2
+    assert st["overflow_queries"] == 4_096
 
 
 def test_nan_and_inf_rows_are_never_kept():
