@@ -401,6 +401,18 @@ int dispatch_metric(const sc_index* index, const void* d_queries, size_t q, size
 
 }  // namespace
 
+int launch_serial_exact_f32(const sc_index* index, const float* d_queries, int k,
+                            const uint32_t* list, const uint32_t* list_n, int64_t* d_idx,
+                            double* d_dist, uint32_t* d_count, hipStream_t stream) {
+  ProfScope ps("knn_serial", stream);
+  hipLaunchKernelGGL((knn_serial_kernel<float, kEuclid>), dim3(512), dim3(NT), 0, stream,
+                     static_cast<const float*>(index->rows), static_cast<int64_t>(index->n),
+                     static_cast<int>(index->ld), d_queries, k, 2.0, list, list_n, d_idx, d_dist,
+                     d_count);
+  SC_HIP(hipGetLastError());
+  return SC_OK;
+}
+
 int knn_exact_device(const sc_index* index, const void* d_queries, size_t q, size_t k,
                      sc_metric metric, int p, int64_t* d_idx, double* d_dist, uint32_t* d_count,
                      hipStream_t stream, sc_stats* stats) {

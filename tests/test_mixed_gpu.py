@@ -1,0 +1,84 @@
+"""SC_MIXED kNN: BF16 MFMA candidates + FP64 replay must equal the oracle bit
+for bit (indices in heap-array order and distance bits), i.e. zero index
+mismatches against the FP64 brute-force form on every input."""
+import numpy as np
+import pytest
+
+import oracle as O
+from smartcore_amd import _capi as S
+from smartcore_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+
+def check(ref, qs, k):
+    ix = S.Index(ref)
+    gi, gd, gc = ix.knn(qs, k, S.SC_EUCLIDIAN, 2, S.SC_MIXED)
+    st = S.last_stats()
+    oi, od, oc = O.linear_find_batch(ref, qs, k)
+    np.testing.assert_array_equal(gc, oc)
+    mism = 0
+    for r in range(qs.shape[0]):
+        c = int(oc[r])
+        mism += int(np.sum(gi[r, :c] != oi[r, :c]))
+        np.testing.assert_array_equal(gd[r, :c].view(np.uint64), od[r, :c].view(np.uint64))
+    assert mism == 0
+    return st
+
+
+def test_mixed_lognormal_128():
+    ref = synth.lognormal_rows(61, 30_000, 128)
+    qs = synth.lognormal_rows(62, 600, 128)  # two query tiles of 512
+    st = check(ref, qs, 10)
+    assert st["uncertified"] == 0
+    assert st["shortlist_rescored"] >= 600 * 10
+
+
+@pytest.mark.parametrize("d,k", [(96, 20), (32, 5), (8, 1), (17, 3)])
+def test_mixed_shapes(d, k):
+    ref = synth.lognormal_rows(70 + d, 12_000, d)
+    qs = synth.lognormal_rows(71 + d, 130, d)
+    check(ref, qs, k)
+
+
+def test_mixed_gaussian_96_k20():
+    rng = np.random.default_rng(80)
+    ref = rng.normal(size=(40_000, 96)).astype(np.float32)
+    qs = rng.normal(size=(64, 96)).astype(np.float32)
+    check(ref, qs, 20)
+
+
+def test_mixed_tie_heavy_keeps_heap_order():
+    rng = np.random.default_rng(81)
+    ref = rng.integers(0, 3, size=(5_000, 16)).astype(np.float32)
+    qs = rng.integers(0, 3, size=(100, 16)).astype(np.float32)
+    for k in (1, 5, 10):
+        check(ref, qs, k)
+
+
+def test_mixed_falls_back_for_non_finite_rows_and_queries():
+    rng = np.random.default_rng(82)
+    ref = rng.normal(size=(3_000, 24)).astype(np.float32)
+    ref[17, 3] = np.nan
+    qs = rng.normal(size=(40, 24)).astype(np.float32)
+    check(ref, qs, 6)
+    ref2 = rng.normal(size=(3_000, 24)).astype(np.float32)
+    qs2 = rng.normal(size=(40, 24)).astype(np.float32)
+    qs2[5, 0] = np.inf
+    qs2[9, 1] = 1e20
+    st = check(ref2, qs2, 6)
+    assert st["uncertified"] == 2
+
+
+def test_mixed_large_k_and_f64_index_use_exact_form():
+    rng = np.random.default_rng(83)
+    ref = rng.normal(size=(2_000, 16)).astype(np.float32)
+    qs = rng.normal(size=(20, 16)).astype(np.float32)
+    check(ref, qs, 40)
+    check(ref.astype(np.float64), qs.astype(np.float64), 7)
+
+
+def test_mixed_rejects_other_metrics():
+    ix = S.Index(np.ones((4, 3), np.float32))
+    with pytest.raises(S.Failed):
+        ix.knn(np.ones((1, 3), np.float32), 2, S.SC_MANHATTAN, 1, S.SC_MIXED)
