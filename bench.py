@@ -115,6 +115,27 @@ def bench_knn(args):
         if n:
             others[name] = {"avg_ms": ms / n, "launches": n}
 
+    plain = None
+    if mode == S.SC_MIXED and not args.no_compare:
+        # the FP64 form on the same inputs: one untimed and one timed step
+        step_plain = lambda: ix.knn_device(qs.data_ptr(), Q, K, idx.data_ptr(), dist.data_ptr(),
+                                           cnt.data_ptr(), S.SC_EUCLIDIAN, 2, S.SC_PLAIN_FP64,
+                                           stream)
+        gi_mixed = idx.clone()
+        gd_mixed = dist.clone()
+        step_plain()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        step_plain()
+        torch.cuda.synchronize()
+        pt = time.perf_counter() - t1
+        mism = int((idx != gi_mixed).sum().item())
+        dbits = int((dist.view(torch.int64) != gd_mixed.view(torch.int64)).sum().item())
+        plain = {"fp64_form_queries_per_s": Q / pt, "speedup_vs_fp64_form": qps / (Q / pt),
+                 "index_mismatches_vs_fp64_form": mism, "distance_bit_mismatches": dbits}
+        log(f"[bench] FP64 form {Q / pt:.0f} q/s, mixed/FP64 = {qps / (Q / pt):.2f}, "
+            f"mismatches {mism}")
+        idx.copy_(gi_mixed)
     # parity spot check of this run's output against the oracle on a few queries
     out = {
         "metric": METRIC,
@@ -137,6 +158,8 @@ def bench_knn(args):
                      "work_per_launch": f"{flops_per_query:.4g} FLOP/query x {per_launch_q:.0f} queries"},
         "kernels": others,
     }
+    if plain:
+        out["vs_fp64_form"] = plain
     if not args.no_cpu:
         import oracle as O
 
@@ -155,6 +178,21 @@ def bench_knn(args):
         out["parity_sample"] = {"queries": S_q, "index_mismatches": mismatch}
     ix.close()
     return out
+
+
+def bench_copy(args):
+    """A plain 1 GiB device copy (torch), the reference point that scales the
+    FETCH_SIZE / WRITE_SIZE counters into bytes."""
+    import torch
+
+    n = 1 << 28  # float32 elements = 1 GiB
+    a = torch.ones(n, dtype=torch.float32, device="cuda:0")
+    b = torch.empty_like(a)
+    for _ in range(max(args.steps, 1)):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    return {"workload": "copy", "bytes_read": n * 4, "bytes_written": n * 4,
+            "launches": max(args.steps, 1)}
 
 
 def bench_kmeans(args):
@@ -250,12 +288,14 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", choices=["knn", "kmeans"], default="knn")
-    ap.add_argument("--mode", choices=["plain", "mixed"], default="plain")
+    ap.add_argument("--workload", choices=["knn", "kmeans", "copy"], default="knn")
+    ap.add_argument("--mode", choices=["plain", "mixed"], default="mixed")
+    ap.add_argument("--no-compare", action="store_true",
+                    help="skip the FP64-form comparison step of the mixed kNN bench")
     ap.add_argument("--n", type=int, default=4_000_000)
     ap.add_argument("--q", type=int, default=65_536)
     ap.add_argument("--km-n", type=int, default=1_000_000)
-    ap.add_argument("--cpu-sample", type=int, default=16)
+    ap.add_argument("--cpu-sample", type=int, default=64)
     ap.add_argument("--cpu-kmeans-iters", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
@@ -265,7 +305,10 @@ def main():
         sys.exit(2)
     if int(os.environ.get("RANK", "0")) != 0:
         return
-    out = bench_knn(args) if args.workload == "knn" else bench_kmeans(args)
+    if args.workload == "copy":
+        out = bench_copy(args)
+    else:
+        out = bench_knn(args) if args.workload == "knn" else bench_kmeans(args)
     print(json.dumps(out), flush=True)
 
 
