@@ -1,23 +1,26 @@
 // k-means Lloyd step (BBDTree::clustering, bbd_tree.rs:62-87) and the loop of
-// KMeans::fit (kmeans.rs:294-310), FP64 form.
+// KMeans::fit (kmeans.rs:294-310), FP64 form, plus the reductions the MFMA
+// form shares.
 //
-//   assign   Tiled FP64 squared distances of every row (widened to f64 as
-//            KMeans::predict does, kmeans.rs:338-342) to every centroid, in the
-//            reference's element order; argmin with strict < so the lowest
-//            index wins ties (kmeans.rs:343-346).  64 rows x 64 centroids per
-//            tile, 4x4 register micro-tiles, LDS-staged dimension chunks.
-//   partial  One workgroup per block of kBlockRows rows: per-cluster sums of
-//            the block's rows in row order (one wave per cluster, lane =
-//            dimension, ballot over labels), per-cluster counts, and the
-//            block's distortion over a fixed-shape tree.
-//   reduce   Sums the block partials in block order and applies
-//            centroid_i = sums_i / size_i when size_i > 0 (kmeans.rs:297-303).
-//   control  The stop rule `if distortion <= dist break` (kmeans.rs:305-309)
-//            evaluated on the device; every kernel of later iterations
-//            returns at once when it has fired, so a whole fit is enqueued
-//            without host round trips.
-// The reduction order is fixed by kBlockRows alone, so results do not
-// depend on launch configuration.
+//   assign    Tiled FP64 squared distances of every row (widened to f64 as
+//             KMeans::predict does, kmeans.rs:338-342) to every centroid, in
+//             the reference's element order; argmin with strict < so the
+//             lowest index wins ties (kmeans.rs:343-346).  64 rows x 64
+//             centroids per tile, 4x4 register micro-tiles.
+//   partial   One workgroup per block of kBlockRows rows.  Rows are staged
+//             through LDS 128 at a time; wave w owns the clusters c with
+//             c % 4 == w and adds each owned row into an f64 LDS accumulator
+//             (lane = dimension, ds_add_f64), in row order.  One owner per
+//             cluster makes the sums deterministic.
+//   finish    Block sums added in groups of 16 blocks, groups in order;
+//             distortion from the aggregates
+//               D = sum_i ||x_i||^2 - 2 sum_c c.S_c + sum_c n_c ||c||^2
+//             (the cost BBDTree::clustering returns, bbd_tree.rs:162 and
+//             :297-305, is the same quantity); centroid_i = sums_i / size_i
+//             when size_i > 0 (kmeans.rs:297-303).
+//   control   The stop rule `if distortion <= dist break` (kmeans.rs:305-309)
+//             on the device; later kernels return at once, so a fit needs
+//             one host synchronisation.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -49,7 +52,6 @@ template <typename T>
 __global__ __launch_bounds__(NT) void kmeans_assign_kernel(const T* __restrict__ rows, int64_t n,
                                                            int ld, const double* __restrict__ cent,
                                                            int k, uint32_t* __restrict__ labels,
-                                                           double* __restrict__ mind,
                                                            const KmState* __restrict__ st) {
   if (st && st->done) return;
   __shared__ double Ps[DCK][PT + 1];
@@ -126,24 +128,109 @@ __global__ __launch_bounds__(NT) void kmeans_assign_kernel(const T* __restrict__
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       const int64_t row = p0 + tp * 4 + i;
-      if (row < n) {
-        // kmeans.rs:334-347: min starts at f64::MAX; nothing below it keeps 0
-        const bool hit = bv[i] < DBL_MAX;
-        labels[row] = hit ? bi[i] : 0u;
-        if (mind) mind[row] = hit ? bv[i] : DBL_MAX;
-      }
+      // kmeans.rs:334-347: min starts at f64::MAX; nothing below it keeps 0
+      if (row < n) labels[row] = bv[i] < DBL_MAX ? bi[i] : 0u;
     }
   }
 }
 
+// Per block: sum of ||x||^2 over its rows (each row a sequential f64 fold,
+// 8 consecutive rows per thread, then a fixed tree).
 template <typename T>
-__global__ __launch_bounds__(NT) void kmeans_partial_kernel(
+__global__ __launch_bounds__(NT) void x2_block_kernel(const T* __restrict__ rows, int64_t n, int ld,
+                                                      double* __restrict__ x2block) {
+  __shared__ double red[NT];
+  const int tid = threadIdx.x;
+  const int64_t r0 = static_cast<int64_t>(blockIdx.x) * kBlockRows;
+  double s = 0.0;
+  for (int j = 0; j < kBlockRows / NT; ++j) {
+    const int64_t r = r0 + tid * (kBlockRows / NT) + j;
+    if (r < n) {
+      double t = 0.0;
+      for (int c = 0; c < ld; ++c) {
+        const double v = static_cast<double>(rows[r * ld + c]);
+        t = t + v * v;
+      }
+      s = s + t;
+    }
+  }
+  red[tid] = s;
+  __syncthreads();
+  for (int w = NT / 2; w >= 1; w >>= 1) {
+    if (tid < w) red[tid] = red[tid] + red[tid + w];
+    __syncthreads();
+  }
+  if (tid == 0) x2block[blockIdx.x] = red[0];
+}
+
+__global__ void x2_total_kernel(const double* __restrict__ x2block, int64_t nb, KmState* st) {
+  double s = 0.0;
+  for (int64_t b = 0; b < nb; ++b) s = s + x2block[b];
+  st->x2total = s;
+}
+
+// Partial sums with one owner wave per cluster (see the file comment).
+// Dynamic LDS: acc[k*d] f64 | stage[kChunkRows*ld] T | lab[kChunkRows] | cnt[k].
+template <typename T>
+__global__ __launch_bounds__(NT) void kmeans_partial_lds_kernel(
     const T* __restrict__ rows, int64_t n, int ld, int d, const uint32_t* __restrict__ labels,
-    const double* __restrict__ mind, int k, double* __restrict__ psum, uint32_t* __restrict__ pcount,
-    double* __restrict__ pdist, const KmState* __restrict__ st) {
+    int k, double* __restrict__ psum, uint32_t* __restrict__ pcount,
+    const KmState* __restrict__ st) {
+  if (st && st->done) return;
+  extern __shared__ __align__(16) unsigned char smem[];
+  double* acc = reinterpret_cast<double*>(smem);
+  T* stage = reinterpret_cast<T*>(acc + (static_cast<size_t>(k) * d + 1) / 2 * 2);
+  uint32_t* lab = reinterpret_cast<uint32_t*>(stage + static_cast<size_t>(kChunkRows) * ld);
+  uint32_t* cnt = lab + kChunkRows;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int64_t b = blockIdx.x;
+  const int64_t r0 = b * kBlockRows;
+  const int rows_here = static_cast<int>(min<int64_t>(kBlockRows, n - r0));
+  for (int e = tid; e < k * d; e += NT) acc[e] = 0.0;
+  for (int e = tid; e < k; e += NT) cnt[e] = 0u;
+  for (int c0 = 0; c0 < rows_here; c0 += kChunkRows) {
+    const int nr = min(kChunkRows, rows_here - c0);
+    __syncthreads();
+    const T* src = rows + (r0 + c0) * ld;
+    for (int e = tid; e < nr * ld; e += NT) stage[e] = src[e];
+    for (int e = tid; e < kChunkRows; e += NT)
+      lab[e] = e < nr ? labels[r0 + c0 + e] : 0xffffffffu;
+    __syncthreads();
+    for (int i0 = 0; i0 < nr; i0 += 64) {
+      const uint32_t li = lab[i0 + lane];
+      const bool mine = i0 + lane < nr && (li & 3u) == static_cast<uint32_t>(wave);
+      uint64_t m = __ballot(mine);
+      while (m) {
+        const int i = i0 + __builtin_ctzll(m);
+        m &= m - 1;
+        const uint32_t c = lab[i];
+        for (int dc = 0; dc < d; dc += 64) {
+          const int dim = dc + lane;
+          if (dim < d)
+            atomicAdd(&acc[static_cast<size_t>(c) * d + dim],
+                      static_cast<double>(stage[static_cast<size_t>(i) * ld + dim]));
+        }
+        if (lane == 0) cnt[c] += 1u;
+      }
+    }
+  }
+  __syncthreads();
+  double* out = psum + b * static_cast<int64_t>(k) * d;
+  for (int e = tid; e < k * d; e += NT) out[e] = acc[e];
+  for (int e = tid; e < k; e += NT) pcount[b * k + e] = cnt[e];
+}
+
+// Fallback when the accumulators do not fit LDS: one wave per cluster,
+// ballots over the labels, rows added in row order.
+template <typename T>
+__global__ __launch_bounds__(NT) void kmeans_partial_ballot_kernel(
+    const T* __restrict__ rows, int64_t n, int ld, int d, const uint32_t* __restrict__ labels,
+    int k, double* __restrict__ psum, uint32_t* __restrict__ pcount,
+    const KmState* __restrict__ st) {
   if (st && st->done) return;
   __shared__ uint32_t lab[kBlockRows];
-  __shared__ double red[NT];
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = tid >> 6;
@@ -151,24 +238,7 @@ __global__ __launch_bounds__(NT) void kmeans_partial_kernel(
   const int64_t r0 = b * kBlockRows;
   const int rows_here = static_cast<int>(min<int64_t>(kBlockRows, n - r0));
   for (int i = tid; i < kBlockRows; i += NT) lab[i] = i < rows_here ? labels[r0 + i] : 0xffffffffu;
-
-  // block distortion: 8 consecutive rows per thread, then a fixed tree
-  {
-    double s = 0.0;
-    const int per = kBlockRows / NT;
-    for (int j = 0; j < per; ++j) {
-      const int i = tid * per + j;
-      if (i < rows_here) s = s + mind[r0 + i];
-    }
-    red[tid] = s;
-  }
   __syncthreads();
-  for (int w = NT / 2; w >= 1; w >>= 1) {
-    if (tid < w) red[tid] = red[tid] + red[tid + w];
-    __syncthreads();
-  }
-  if (tid == 0) pdist[b] = red[0];
-
   double* my_psum = psum + b * static_cast<int64_t>(k) * d;
   for (int c = wave; c < k; c += NT / 64) {
     uint32_t cnt = 0;
@@ -182,20 +252,9 @@ __global__ __launch_bounds__(NT) void kmeans_partial_kernel(
         uint64_t m = __ballot(mine);
         if (dc0 == 0) cnt += static_cast<uint32_t>(__popcll(m));
         while (m) {
-          // up to 8 rows in flight, added in row order
-          int idx[8];
-          int nb = 0;
-          while (m && nb < 8) {
-            idx[nb++] = __builtin_ctzll(m);
-            m &= m - 1;
-          }
-          double v[8];
-#pragma unroll
-          for (int t = 0; t < 8; ++t)
-            v[t] = (t < nb && live) ? static_cast<double>(rows[(r0 + i0 + idx[t]) * ld + dim]) : 0.0;
-#pragma unroll
-          for (int t = 0; t < 8; ++t)
-            if (t < nb) acc = acc + v[t];
+          const int r = __builtin_ctzll(m);
+          m &= m - 1;
+          if (live) acc = acc + static_cast<double>(rows[(r0 + i0 + r) * ld + dim]);
         }
       }
       if (live) my_psum[static_cast<int64_t>(c) * d + dim] = acc;
@@ -204,30 +263,77 @@ __global__ __launch_bounds__(NT) void kmeans_partial_kernel(
   }
 }
 
-__global__ void kmeans_reduce_kernel(int64_t nb, int k, int d, int ld,
-                                     const double* __restrict__ psum,
-                                     const uint32_t* __restrict__ pcount,
-                                     const double* __restrict__ pdist, double* __restrict__ sums,
-                                     uint64_t* __restrict__ counts, double* __restrict__ cent,
-                                     KmState* __restrict__ st, int update) {
+// gsum[g][c][j] = sum of psum over blocks 16g .. 16g+15, in block order.
+__global__ void kmeans_group_kernel(int64_t nb, int64_t ng, int k, int d,
+                                    const double* __restrict__ psum,
+                                    const uint32_t* __restrict__ pcount, double* __restrict__ gsum,
+                                    uint64_t* __restrict__ gcount, const KmState* __restrict__ st) {
   if (st && st->done) return;
+  const int64_t kd = static_cast<int64_t>(k) * d;
   const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (t == 0) {
-    double s = 0.0;
-    for (int64_t b = 0; b < nb; ++b) s = s + pdist[b];
+  if (t >= ng * kd) return;
+  const int64_t g = t / kd, e = t % kd;
+  const int64_t b0 = g * kGroupBlocks, b1 = min<int64_t>(nb, b0 + kGroupBlocks);
+  double s = 0.0;
+  for (int64_t b = b0; b < b1; ++b) s = s + psum[b * kd + e];
+  gsum[g * kd + e] = s;
+  if (e % d == 0) {
+    const int64_t c = e / d;
+    uint64_t cnt = 0;
+    for (int64_t b = b0; b < b1; ++b) cnt += pcount[b * k + c];
+    gcount[g * k + c] = cnt;
+  }
+}
+
+__global__ void kmeans_sum_kernel(int64_t ng, int k, int d, const double* __restrict__ gsum,
+                                  const uint64_t* __restrict__ gcount, double* __restrict__ sums,
+                                  uint64_t* __restrict__ counts, const KmState* __restrict__ st) {
+  if (st && st->done) return;
+  const int64_t kd = static_cast<int64_t>(k) * d;
+  const int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e >= kd) return;
+  double s = 0.0;
+  for (int64_t g = 0; g < ng; ++g) s = s + gsum[g * kd + e];
+  sums[e] = s;
+  if (e % d == 0) {
+    const int64_t c = e / d;
+    uint64_t cnt = 0;
+    for (int64_t g = 0; g < ng; ++g) cnt += gcount[g * k + c];
+    counts[c] = cnt;
+  }
+}
+
+// One workgroup: distortion from the aggregates with the centroids of this
+// step, then (update) the new centroids.
+__global__ __launch_bounds__(NT) void kmeans_distortion_update_kernel(
+    int k, int d, int ld, const double* __restrict__ sums, const uint64_t* __restrict__ counts,
+    double* __restrict__ cent, double* __restrict__ term, KmState* __restrict__ st, int update) {
+  if (st->done) return;
+  const int tid = threadIdx.x;
+  for (int c = tid; c < k; c += NT) {
+    const double* cc = cent + static_cast<int64_t>(c) * ld;
+    const double* sc_ = sums + static_cast<int64_t>(c) * d;
+    double dot = 0.0, c2 = 0.0;
+    for (int j = 0; j < d; ++j) {
+      dot = dot + cc[j] * sc_[j];
+      c2 = c2 + cc[j] * cc[j];
+    }
+    term[c] = counts[c] > 0 ? static_cast<double>(counts[c]) * c2 - 2.0 * dot : 0.0;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    double s = st->x2total;
+    for (int c = 0; c < k; ++c) s = s + term[c];
     st->dist = s;
   }
-  if (t >= static_cast<int64_t>(k) * d) return;
-  const int c = static_cast<int>(t / d), j = static_cast<int>(t % d);
-  double s = 0.0;
-  uint64_t cnt = 0;
-  for (int64_t b = 0; b < nb; ++b) {
-    s = s + psum[(b * k + c) * static_cast<int64_t>(d) + j];
-    cnt += pcount[b * k + c];
+  __syncthreads();
+  if (update) {
+    for (int e = tid; e < k * d; e += NT) {
+      const int c = e / d, j = e % d;
+      if (counts[c] > 0)
+        cent[static_cast<int64_t>(c) * ld + j] = sums[e] / static_cast<double>(counts[c]);
+    }
   }
-  if (sums) sums[t] = s;
-  if (j == 0 && counts) counts[c] = cnt;
-  if (update && cnt > 0) cent[static_cast<int64_t>(c) * ld + j] = s / static_cast<double>(cnt);
 }
 
 __global__ void kmeans_control_kernel(KmState* st) {
@@ -243,11 +349,11 @@ __global__ void kmeans_control_kernel(KmState* st) {
 __global__ void kmeans_state_init_kernel(KmState* st) {
   st->distortion = DBL_MAX;
   st->dist = 0.0;
+  st->x2total = 0.0;
   st->iters = 0;
   st->done = 0;
   st->fallback = 0;
 }
-
 
 // One k-means++ round (kmeans.rs:367-379 and the final pass :399-410):
 // d_i = min(d_i, squared_distance(row_i, centroid)) in the rows' own type
@@ -272,66 +378,126 @@ __global__ void fill_kernel(double* p, int64_t n, double v) {
   const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
 }
+
+size_t partial_lds_bytes(const sc_index* ix, int k) {
+  return sizeof(double) * ((static_cast<size_t>(k) * ix->d + 1) / 2 * 2) +
+         elem_size(ix->dtype) * static_cast<size_t>(kChunkRows) * ix->ld +
+         sizeof(uint32_t) * (kChunkRows + static_cast<size_t>(k)) + 16;
+}
+
 }  // namespace
 
 int kmeans_assign_launch(const sc_index* ix, const void* rows, int64_t n, const double* d_cent,
-                         int k, uint32_t* d_labels, double* d_mind, const KmState* st,
-                         hipStream_t s) {
+                         int k, uint32_t* d_labels, const KmState* st, hipStream_t s) {
   const unsigned grid = static_cast<unsigned>((n + PT - 1) / PT);
   if (grid == 0) return SC_OK;
   ProfScope ps("kmeans_assign", s);
   if (ix->dtype == SC_F64)
     hipLaunchKernelGGL(kmeans_assign_kernel<double>, dim3(grid), dim3(NT), 0, s,
                        static_cast<const double*>(rows), n, static_cast<int>(ix->ld), d_cent, k,
-                       d_labels, d_mind, st);
+                       d_labels, st);
   else
     hipLaunchKernelGGL(kmeans_assign_kernel<float>, dim3(grid), dim3(NT), 0, s,
                        static_cast<const float*>(rows), n, static_cast<int>(ix->ld), d_cent, k,
-                       d_labels, d_mind, st);
+                       d_labels, st);
   SC_HIP(hipGetLastError());
   return SC_OK;
 }
 
-int kmeans_update_launch(const sc_index* ix, KmWork& w, int k, KmState* st, double* d_sums,
-                         uint64_t* d_counts, double* d_cent, bool update, hipStream_t s) {
+int kmeans_x2_launch(const sc_index* ix, KmWork& w, hipStream_t s) {
   const int64_t n = static_cast<int64_t>(ix->n);
-  const int d = static_cast<int>(ix->d);
-  const int64_t nb = (n + kBlockRows - 1) / kBlockRows;
-  if (nb > 0) {
-    ProfScope ps("kmeans_partial", s);
+  if (w.nblocks > 0) {
     if (ix->dtype == SC_F64)
-      hipLaunchKernelGGL(kmeans_partial_kernel<double>, dim3(static_cast<unsigned>(nb)), dim3(NT),
+      hipLaunchKernelGGL(x2_block_kernel<double>, dim3(static_cast<unsigned>(w.nblocks)), dim3(NT),
                          0, s, static_cast<const double*>(ix->rows), n, static_cast<int>(ix->ld),
-                         d, w.labels, w.mind, k, w.psum, w.pcount, w.pdist, st);
+                         w.x2block);
     else
-      hipLaunchKernelGGL(kmeans_partial_kernel<float>, dim3(static_cast<unsigned>(nb)), dim3(NT),
-                         0, s, static_cast<const float*>(ix->rows), n, static_cast<int>(ix->ld), d,
-                         w.labels, w.mind, k, w.psum, w.pcount, w.pdist, st);
+      hipLaunchKernelGGL(x2_block_kernel<float>, dim3(static_cast<unsigned>(w.nblocks)), dim3(NT),
+                         0, s, static_cast<const float*>(ix->rows), n, static_cast<int>(ix->ld),
+                         w.x2block);
     SC_HIP(hipGetLastError());
   }
-  const int64_t threads = std::max<int64_t>(1, static_cast<int64_t>(k) * d);
+  hipLaunchKernelGGL(x2_total_kernel, dim3(1), dim3(1), 0, s, w.x2block, w.nblocks, w.state);
+  SC_HIP(hipGetLastError());
+  return SC_OK;
+}
+
+int kmeans_partial_launch(const sc_index* ix, KmWork& w, int k, const KmState* st, hipStream_t s) {
+  const int64_t n = static_cast<int64_t>(ix->n);
+  const int d = static_cast<int>(ix->d);
+  const int ld = static_cast<int>(ix->ld);
+  if (w.nblocks == 0) return SC_OK;
+  ProfScope ps("kmeans_partial", s);
+  const size_t lds = partial_lds_bytes(ix, k);
+  const dim3 grid(static_cast<unsigned>(w.nblocks));
+  if (static_cast<int64_t>(k) * d <= kMaxLdsAcc && lds <= 96 * 1024) {
+    if (ix->dtype == SC_F64)
+      hipLaunchKernelGGL(kmeans_partial_lds_kernel<double>, grid, dim3(NT), lds, s,
+                         static_cast<const double*>(ix->rows), n, ld, d, w.labels, k, w.psum,
+                         w.pcount, st);
+    else
+      hipLaunchKernelGGL(kmeans_partial_lds_kernel<float>, grid, dim3(NT), lds, s,
+                         static_cast<const float*>(ix->rows), n, ld, d, w.labels, k, w.psum,
+                         w.pcount, st);
+  } else {
+    if (ix->dtype == SC_F64)
+      hipLaunchKernelGGL(kmeans_partial_ballot_kernel<double>, grid, dim3(NT), 0, s,
+                         static_cast<const double*>(ix->rows), n, ld, d, w.labels, k, w.psum,
+                         w.pcount, st);
+    else
+      hipLaunchKernelGGL(kmeans_partial_ballot_kernel<float>, grid, dim3(NT), 0, s,
+                         static_cast<const float*>(ix->rows), n, ld, d, w.labels, k, w.psum,
+                         w.pcount, st);
+  }
+  SC_HIP(hipGetLastError());
+  return SC_OK;
+}
+
+int kmeans_finish_launch(const sc_index* ix, KmWork& w, int k, KmState* st, double* d_cent,
+                         bool update, hipStream_t s) {
+  const int d = static_cast<int>(ix->d);
+  const int64_t kd = std::max<int64_t>(1, static_cast<int64_t>(k) * d);
   ProfScope ps("kmeans_reduce", s);
-  hipLaunchKernelGGL(kmeans_reduce_kernel, dim3(static_cast<unsigned>((threads + 255) / 256)),
-                     dim3(256), 0, s, nb, k, d, static_cast<int>(ix->ld), w.psum, w.pcount,
-                     w.pdist, d_sums, d_counts, d_cent, st, update ? 1 : 0);
+  if (w.ngroups > 0) {
+    const int64_t t = w.ngroups * kd;
+    hipLaunchKernelGGL(kmeans_group_kernel, dim3(static_cast<unsigned>((t + 255) / 256)), dim3(256),
+                       0, s, w.nblocks, w.ngroups, k, d, w.psum, w.pcount, w.gsum, w.gcount, st);
+    SC_HIP(hipGetLastError());
+  }
+  hipLaunchKernelGGL(kmeans_sum_kernel, dim3(static_cast<unsigned>((kd + 255) / 256)), dim3(256), 0,
+                     s, w.ngroups, k, d, w.gsum, w.gcount, w.sums, w.counts, st);
+  SC_HIP(hipGetLastError());
+  hipLaunchKernelGGL(kmeans_distortion_update_kernel, dim3(1), dim3(NT), 0, s, k, d,
+                     static_cast<int>(ix->ld), w.sums, w.counts, d_cent,
+                     w.x2block + std::max<int64_t>(w.nblocks, 1), st, update ? 1 : 0);
   SC_HIP(hipGetLastError());
   return SC_OK;
 }
 
 int KmWork::alloc(const sc_index* ix, int k, hipStream_t s) {
   const int64_t n = static_cast<int64_t>(ix->n);
-  const int64_t nb = (n + kBlockRows - 1) / kBlockRows;
+  const int64_t d = std::max<int64_t>(static_cast<int64_t>(ix->d), 1);
+  nblocks = (n + kBlockRows - 1) / kBlockRows;
+  ngroups = (nblocks + kGroupBlocks - 1) / kGroupBlocks;
+  const int64_t nb1 = std::max<int64_t>(nblocks, 1), ng1 = std::max<int64_t>(ngroups, 1);
   SC_HIP(b_labels.alloc(sizeof(uint32_t) * std::max<int64_t>(n, 1), s));
-  SC_HIP(b_mind.alloc(sizeof(double) * std::max<int64_t>(n, 1), s));
-  SC_HIP(b_psum.alloc(sizeof(double) * std::max<int64_t>(nb, 1) * k * std::max<size_t>(ix->d, 1), s));
-  SC_HIP(b_pcount.alloc(sizeof(uint32_t) * std::max<int64_t>(nb, 1) * k, s));
-  SC_HIP(b_pdist.alloc(sizeof(double) * std::max<int64_t>(nb, 1), s));
+  SC_HIP(b_psum.alloc(sizeof(double) * nb1 * k * d, s));
+  SC_HIP(b_pcount.alloc(sizeof(uint32_t) * nb1 * k, s));
+  SC_HIP(b_gsum.alloc(sizeof(double) * ng1 * k * d, s));
+  SC_HIP(b_gcount.alloc(sizeof(uint64_t) * ng1 * k, s));
+  SC_HIP(b_sums.alloc(sizeof(double) * k * d, s));
+  SC_HIP(b_counts.alloc(sizeof(uint64_t) * k, s));
+  // x2 per block, then k distortion terms
+  SC_HIP(b_x2.alloc(sizeof(double) * (nb1 + k), s));
   SC_HIP(b_state.alloc(sizeof(KmState), s));
   labels = b_labels.as<uint32_t>();
-  mind = b_mind.as<double>();
   psum = b_psum.as<double>();
   pcount = b_pcount.as<uint32_t>();
-  pdist = b_pdist.as<double>();
+  gsum = b_gsum.as<double>();
+  gcount = b_gcount.as<uint64_t>();
+  sums = b_sums.as<double>();
+  counts = b_counts.as<uint64_t>();
+  x2block = b_x2.as<double>();
   state = b_state.as<KmState>();
   return SC_OK;
 }
@@ -362,6 +528,49 @@ int upload_centroids(const double* host, int k, size_t d, size_t ld, double* dev
   return SC_OK;
 }
 
+// One Lloyd step of the chosen form into w (labels, sums, counts, st->dist).
+int lloyd_step(const sc_index* ix, KmWork& w, int k, double* cent, bool mixed, bool update,
+               hipStream_t s) {
+  if (mixed) {
+    if (int rc = kmeans_mixed_step_launch(ix, w, k, cent, w.state, s)) return rc;
+  } else {
+    if (int rc = kmeans_assign_launch(ix, ix->rows, static_cast<int64_t>(ix->n), cent, k, w.labels,
+                                      w.state, s))
+      return rc;
+    if (int rc = kmeans_partial_launch(ix, w, k, w.state, s)) return rc;
+  }
+  return kmeans_finish_launch(ix, w, k, w.state, cent, update, s);
+}
+
+int kmeans_run_impl(const sc_index* index, int k, size_t max_iter, bool mixed, double* centroids,
+                    uint32_t* labels, uint64_t* size, double* distortion, uint32_t* iters,
+                    hipStream_t s, uint64_t* fallback) {
+  KmWork w;
+  if (int rc = w.alloc(index, k, s)) return rc;
+  DevBuf cent;
+  SC_HIP(cent.alloc(sizeof(double) * k * index->ld, s));
+  if (int rc = upload_centroids(centroids, k, index->d, index->ld, cent.as<double>(), s)) return rc;
+  if (int rc = kmeans_state_init(w.state, s)) return rc;
+  if (int rc = kmeans_x2_launch(index, w, s)) return rc;
+  for (size_t it = 0; it < max_iter; ++it) {
+    if (int rc = lloyd_step(index, w, k, cent.as<double>(), mixed, true, s)) return rc;
+    if (int rc = kmeans_control_launch(w.state, s)) return rc;
+  }
+  KmState hs{};
+  if (index->d > 0)
+    SC_HIP(hipMemcpy2DAsync(centroids, index->d * sizeof(double), cent.p,
+                            index->ld * sizeof(double), index->d * sizeof(double), k,
+                            hipMemcpyDeviceToHost, s));
+  SC_HIP(hipMemcpyAsync(labels, w.labels, sizeof(uint32_t) * index->n, hipMemcpyDeviceToHost, s));
+  SC_HIP(hipMemcpyAsync(size, w.counts, sizeof(uint64_t) * k, hipMemcpyDeviceToHost, s));
+  SC_HIP(hipMemcpyAsync(&hs, w.state, sizeof(KmState), hipMemcpyDeviceToHost, s));
+  SC_HIP(hipStreamSynchronize(s));
+  *distortion = hs.distortion;
+  *iters = hs.iters;
+  if (fallback) *fallback = hs.fallback;
+  return SC_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -382,24 +591,19 @@ int sc_kmeans_step(const sc_index* index, const double* centroids, size_t k, dou
   hipStream_t s = stream_for(index);
   KmWork w;
   if (int rc = w.alloc(index, static_cast<int>(k), s)) return rc;
-  DevBuf cent, dsums, dcounts;
+  DevBuf cent;
   SC_HIP(cent.alloc(sizeof(double) * k * index->ld, s));
-  SC_HIP(dsums.alloc(sizeof(double) * k * std::max<size_t>(index->d, 1), s));
-  SC_HIP(dcounts.alloc(sizeof(uint64_t) * k, s));
   if (int rc = upload_centroids(centroids, static_cast<int>(k), index->d, index->ld,
                                 cent.as<double>(), s))
     return rc;
   if (int rc = kmeans_state_init(w.state, s)) return rc;
-  if (int rc = kmeans_assign_launch(index, index->rows, static_cast<int64_t>(index->n),
-                                    cent.as<double>(), static_cast<int>(k), w.labels, w.mind,
-                                    w.state, s))
-    return rc;
-  if (int rc = kmeans_update_launch(index, w, static_cast<int>(k), w.state, dsums.as<double>(),
-                                    dcounts.as<uint64_t>(), cent.as<double>(), false, s))
+  if (int rc = kmeans_x2_launch(index, w, s)) return rc;
+  if (int rc = lloyd_step(index, w, static_cast<int>(k), cent.as<double>(), false, false, s))
     return rc;
   KmState hs{};
-  SC_HIP(hipMemcpyAsync(sums, dsums.p, sizeof(double) * k * index->d, hipMemcpyDeviceToHost, s));
-  SC_HIP(hipMemcpyAsync(counts, dcounts.p, sizeof(uint64_t) * k, hipMemcpyDeviceToHost, s));
+  if (index->d > 0)
+    SC_HIP(hipMemcpyAsync(sums, w.sums, sizeof(double) * k * index->d, hipMemcpyDeviceToHost, s));
+  SC_HIP(hipMemcpyAsync(counts, w.counts, sizeof(uint64_t) * k, hipMemcpyDeviceToHost, s));
   SC_HIP(hipMemcpyAsync(membership, w.labels, sizeof(uint32_t) * index->n, hipMemcpyDeviceToHost, s));
   SC_HIP(hipMemcpyAsync(&hs, w.state, sizeof(KmState), hipMemcpyDeviceToHost, s));
   SC_HIP(hipStreamSynchronize(s));
@@ -418,40 +622,18 @@ int sc_kmeans_run(const sc_index* index, size_t k, size_t max_iter, sc_mode mode
   if (!index || !centroids || !labels || !size || !distortion || !iters)
     return fail(SC_PARAMETERS_ERROR, "null argument");
   if (mode != SC_PLAIN_FP64 && mode != SC_MIXED) return fail(SC_ERR_UNSUPPORTED, "unknown mode");
+  if (index->n == 0) return fail(SC_FIT_FAILED, "no rows");
   SC_HIP(hipSetDevice(index->device));
   hipStream_t s = stream_for(index);
-  if (mode == SC_MIXED)
-    return kmeans_run_mixed(index, static_cast<int>(k), max_iter, centroids, labels, size,
-                            distortion, iters, s);
-  KmWork w;
-  if (int rc = w.alloc(index, static_cast<int>(k), s)) return rc;
-  DevBuf cent, dcounts;
-  SC_HIP(cent.alloc(sizeof(double) * k * index->ld, s));
-  SC_HIP(dcounts.alloc(sizeof(uint64_t) * k, s));
-  if (int rc = upload_centroids(centroids, static_cast<int>(k), index->d, index->ld,
-                                cent.as<double>(), s))
-    return rc;
-  if (int rc = kmeans_state_init(w.state, s)) return rc;
-  for (size_t it = 0; it < max_iter; ++it) {
-    if (int rc = kmeans_assign_launch(index, index->rows, static_cast<int64_t>(index->n),
-                                      cent.as<double>(), static_cast<int>(k), w.labels, w.mind,
-                                      w.state, s))
-      return rc;
-    if (int rc = kmeans_update_launch(index, w, static_cast<int>(k), w.state, nullptr,
-                                      dcounts.as<uint64_t>(), cent.as<double>(), true, s))
-      return rc;
-    if (int rc = kmeans_control_launch(w.state, s)) return rc;
-  }
-  KmState hs{};
-  SC_HIP(hipMemcpy2DAsync(centroids, index->d * sizeof(double), cent.p, index->ld * sizeof(double),
-                          index->d * sizeof(double), k, hipMemcpyDeviceToHost, s));
-  SC_HIP(hipMemcpyAsync(labels, w.labels, sizeof(uint32_t) * index->n, hipMemcpyDeviceToHost, s));
-  SC_HIP(hipMemcpyAsync(size, dcounts.p, sizeof(uint64_t) * k, hipMemcpyDeviceToHost, s));
-  SC_HIP(hipMemcpyAsync(&hs, w.state, sizeof(KmState), hipMemcpyDeviceToHost, s));
-  SC_HIP(hipStreamSynchronize(s));
-  *distortion = hs.distortion;
-  *iters = hs.iters;
-  return SC_OK;
+  const bool mixed = mode == SC_MIXED && kmeans_mixed_supported(index, static_cast<int>(k));
+  uint64_t fb = 0;
+  const int rc = kmeans_run_impl(index, static_cast<int>(k), max_iter, mixed, centroids, labels,
+                                 size, distortion, iters, s, &fb);
+  sc_stats& st = last_stats();
+  st = sc_stats{};
+  st.queries = index->n;
+  st.uncertified = fb;
+  return rc;
 }
 
 int sc_kmeans_predict(const void* x, sc_dtype dtype, size_t n, size_t d, const double* centroids,
@@ -478,7 +660,7 @@ int sc_kmeans_predict(const void* x, sc_dtype dtype, size_t n, size_t d, const d
   if (int rc = upload_centroids(centroids, static_cast<int>(k), d, tmp.ld, cent.as<double>(), s))
     return rc;
   if (int rc = kmeans_assign_launch(&tmp, rows.p, static_cast<int64_t>(n), cent.as<double>(),
-                                    static_cast<int>(k), lab.as<uint32_t>(), nullptr, nullptr, s))
+                                    static_cast<int>(k), lab.as<uint32_t>(), nullptr, s))
     return rc;
   SC_HIP(hipMemcpyAsync(labels, lab.p, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, s));
   SC_HIP(hipStreamSynchronize(s));
@@ -508,7 +690,7 @@ int sc_kmeans_plus_plus(const sc_index* index, size_t k, int has_seed, uint64_t 
   SC_HIP(hipGetLastError());
   const char* base = static_cast<const char*>(index->rows);
   // kmeans.rs:358-362: first centre is a uniformly drawn row
-  uint64_t pick = rng.gen_range(static_cast<uint64_t>(n));
+  const uint64_t pick = rng.gen_range(static_cast<uint64_t>(n));
   SC_HIP(hipMemcpyAsync(cen.p, base + pick * row_bytes, row_bytes, hipMemcpyDeviceToDevice, s));
   std::vector<double> hd(static_cast<size_t>(n));
   auto round = [&](uint32_t label) -> int {

@@ -1,0 +1,77 @@
+"""SC_MIXED k-means (certified BF16 MFMA assignment fused with the FP64 sums)
+must reproduce the FP64 form bit for bit -- labels, sizes, iteration count,
+centroid bits, distortion bits -- and the BBD-tree oracle's labels."""
+import numpy as np
+import pytest
+
+import oracle as O
+from smartcore_amd import _capi as S
+from smartcore_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+
+def run_both(x, init, iters):
+    ix = S.Index(x)
+    a = ix.kmeans_run(init, iters, S.SC_PLAIN_FP64)
+    b = ix.kmeans_run(init, iters, S.SC_MIXED)
+    st = S.last_stats()
+    return a, b, st
+
+
+def assert_same(a, b):
+    ca, la, sa, da, ia = a
+    cb, lb, sb, db, ib = b
+    np.testing.assert_array_equal(la, lb)
+    np.testing.assert_array_equal(sa, sb)
+    assert ia == ib
+    np.testing.assert_array_equal(ca.view(np.uint64), cb.view(np.uint64))
+    assert np.float64(da).view(np.uint64) == np.float64(db).view(np.uint64)
+
+
+@pytest.mark.parametrize("n,d,k", [(50_000, 64, 64), (20_000, 16, 12), (30_000, 24, 70),
+                                   (9_000, 100, 5), (4_097, 8, 128)])
+def test_mixed_equals_fp64_form(n, d, k):
+    x, _, init = synth.kmeans_blobs(seed=20 + k, n=n, d=d, blobs=max(2, k // 2), k=k)
+    a, b, st = run_both(x, init, 15)
+    assert_same(a, b)
+
+
+def test_mixed_labels_match_bbd_oracle():
+    x, _, init = synth.kmeans_blobs(seed=31, n=40_000, d=32, blobs=16, k=16)
+    ix = S.Index(x)
+    cent, labels, size, dist, iters = ix.kmeans_run(init, 20, S.SC_MIXED)
+    ocent, olabels, osize, odist, oiters = O.kmeans_fit(x, init, 20, use_bbd=True)
+    np.testing.assert_array_equal(labels, olabels)
+    np.testing.assert_array_equal(size, osize)
+    assert iters == oiters
+    assert np.max(np.abs(cent - ocent)) <= 1e-12 * np.max(np.abs(ocent))
+
+
+def test_mixed_near_ties_use_fp64_fallback():
+    """Centroids placed almost on top of each other cannot be separated by
+    the BF16 bound, so those rows go through the FP64 path."""
+    rng = np.random.default_rng(40)
+    x = rng.normal(size=(20_000, 32)).astype(np.float32)
+    base = x[:8].astype(np.float64)
+    init = np.concatenate([base, base + 1e-9]).astype(np.float64)
+    a, b, st = run_both(x, init, 5)
+    assert_same(a, b)
+    assert st["uncertified"] > 0
+
+
+def test_mixed_handles_non_finite_rows():
+    rng = np.random.default_rng(41)
+    x = rng.normal(size=(5_000, 16)).astype(np.float32)
+    x[7, 3] = np.nan
+    x[100, 0] = np.inf
+    init = x[200:204].astype(np.float64)
+    a, b, st = run_both(x, init, 3)
+    np.testing.assert_array_equal(a[1], b[1])
+    np.testing.assert_array_equal(a[2], b[2])
+
+
+def test_mixed_on_f64_rows_serves_fp64_form():
+    x, _, init = synth.kmeans_blobs(seed=42, n=10_000, d=16, blobs=8, k=8, dtype=np.float64)
+    a, b, st = run_both(x, init, 10)
+    assert_same(a, b)
