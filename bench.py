@@ -180,6 +180,107 @@ def bench_knn(args):
     return out
 
 
+def bench_serve(args):
+    """Continuous-batching server over a resident 2,000,000 x 96 f32 index
+    (rows ~ N(0, I)), k = 20.  Requests arrive by a Poisson process at fixed
+    rates, each with round(exp(U[0, ln 4096])) query rows; 80% Euclidean on
+    the mixed form, 20% Manhattan on the FP64 form.  Reports per rate the
+    query throughput and the p50 / p99 submit-to-result latency, and checks
+    sampled requests against the same search run alone."""
+    import threading
+
+    import torch
+
+    from smartcore_amd import _capi as S
+    from smartcore_amd import synth  # noqa: F401
+
+    N, D, K = args.serve_n, 96, 20
+    dev = torch.device("cuda:0")
+    g = torch.Generator(device=dev)
+    g.manual_seed(303)
+    refs = torch.randn((N, D), generator=g, device=dev, dtype=torch.float32)
+    ix = S.Index(device_ptr=refs.data_ptr(), n=N, d=D, dtype=np.float32)
+    rng = np.random.default_rng(404)
+    pool = rng.normal(size=(65_536, D)).astype(np.float32)
+    srv = S.Server(ix, max_batch_queries=4096, max_inflight=3, batch_window_us=200)
+
+    def make_request():
+        q = int(round(np.exp(rng.uniform(0.0, np.log(4096.0)))))
+        s0 = int(rng.integers(0, pool.shape[0] - q + 1))
+        euclid = rng.random() < 0.8
+        metric = S.SC_EUCLIDIAN if euclid else S.SC_MANHATTAN
+        mode = S.SC_MIXED if euclid else S.SC_PLAIN_FP64
+        return pool[s0:s0 + q], metric, mode
+
+    # warm-up: both forms compiled and the BF16 copy built
+    for metric, mode in ((S.SC_EUCLIDIAN, S.SC_MIXED), (S.SC_MANHATTAN, S.SC_PLAIN_FP64)):
+        srv.wait(srv.submit(pool[:64], K, metric, 2, mode))
+    levels = []
+    for rate in args.serve_rates:
+        n_req = max(20, int(rate * args.serve_seconds))
+        reqs = [make_request() for _ in range(n_req)]
+        gaps = rng.exponential(1.0 / rate, size=n_req)
+        tickets = [None] * n_req
+        lat = [0.0] * n_req
+        outs = {}
+        sample = set(range(0, n_req, max(1, n_req // 12)))
+
+        def waiter():
+            for j in range(n_req):
+                while tickets[j] is None:
+                    time.sleep(0.0002)
+                gi, gd, gc, lt = srv.wait(tickets[j])
+                lat[j] = lt
+                if j in sample:
+                    outs[j] = (gi, gd)
+
+        th = threading.Thread(target=waiter)
+        th.start()
+        t0 = time.perf_counter()
+        t_next = t0
+        for j, (qs, metric, mode) in enumerate(reqs):
+            t_next += gaps[j]
+            dt = t_next - time.perf_counter()
+            if dt > 0:
+                time.sleep(dt)
+            tickets[j] = srv.submit(qs, K, metric, 2, mode)
+        th.join()
+        wall = time.perf_counter() - t0
+        mism = 0
+        for j in sample:
+            qs, metric, mode = reqs[j]
+            ai, ad, ac = ix.knn(qs, K, metric, 2, mode)
+            mism += int(np.sum(outs[j][0] != ai))
+        lat_ms = np.array(lat)
+        nq = sum(r[0].shape[0] for r in reqs)
+        levels.append({"offered_requests_per_s": rate, "requests": n_req,
+                       "queries_per_s": nq / wall, "p50_ms": float(np.percentile(lat_ms, 50)),
+                       "p99_ms": float(np.percentile(lat_ms, 99)),
+                       "checked_requests": len(sample), "index_mismatches_vs_alone": mism})
+        log(f"[bench] serve {rate} req/s: {nq / wall:.0f} q/s p50 {np.percentile(lat_ms, 50):.1f} ms "
+            f"p99 {np.percentile(lat_ms, 99):.1f} ms")
+    st = srv.stats()
+    srv.close()
+    # the oracle on a few rows of one Euclidean and one Manhattan request
+    oracle_mism = None
+    if not args.no_cpu:
+        import oracle as O
+
+        href = refs.cpu().numpy()
+        qs = pool[:2]
+        ai, _, _ = ix.knn(qs, K, S.SC_EUCLIDIAN, 2, S.SC_MIXED)
+        bi, _, _ = ix.knn(qs, K, S.SC_MANHATTAN, 1, S.SC_PLAIN_FP64)
+        oi, _, _ = O.linear_find_batch(href, qs, K, O.EUCLIDIAN)
+        mi, _, _ = O.linear_find_batch(href, qs, K, O.MANHATTAN)
+        oracle_mism = int(np.sum(ai != oi) + np.sum(bi != mi))
+    ix.close()
+    return {"workload": f"kNN server, {N:,}x{D} f32 resident rows, k={K}, requests of 1-4096 "
+                        "rows (log-uniform), Poisson arrivals, 80% Euclidean mixed / 20% "
+                        "Manhattan FP64",
+            "levels": levels, "batches": st["batches"], "hbm_peak_bytes": st["hbm_peak"],
+            "oracle_index_mismatches_4_queries": oracle_mism}
+
+
 def bench_copy(args):
     """A plain 1 GiB device copy (torch), the reference point that scales the
     FETCH_SIZE / WRITE_SIZE counters into bytes."""
@@ -224,6 +325,16 @@ def bench_kmeans(args):
     elapsed = time.perf_counter() - t0
     lib.sc_profile_enable(0)
     ips = total_iters / elapsed
+    vs_fp64 = None
+    if mode == S.SC_MIXED:
+        t1 = time.perf_counter()
+        pc, pl, psz, pd, pit = ix.kmeans_run(init, iters, S.SC_PLAIN_FP64)
+        pt = time.perf_counter() - t1
+        vs_fp64 = {"fp64_form_iterations_per_s": pit / pt, "speedup_vs_fp64_form": ips / (pit / pt),
+                   "label_mismatches": int(np.sum(pl != labels)),
+                   "centroid_bit_mismatches": int(np.sum(pc.view(np.uint64) != cent.view(np.uint64))),
+                   "iterations_equal": bool(pit == it)}
+        log(f"[bench] k-means FP64 form {pit / pt:.0f} it/s; mixed/FP64 = {ips / (pit / pt):.2f}")
     kname = "kmeans_mixed_assign" if mode == S.SC_MIXED else "kmeans_assign"
     kms, launches = profile_read(lib, kname)
     avg_s = kms / 1e3 / max(launches, 1)
@@ -261,6 +372,8 @@ def bench_kmeans(args):
         "roofline": roof,
         "kernels": kern,
     }
+    if vs_fp64:
+        out["vs_fp64_form"] = vs_fp64
     if not args.no_cpu:
         import oracle as O
 
@@ -288,7 +401,12 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", choices=["knn", "kmeans", "copy"], default="knn")
+    ap.add_argument("--workload", choices=["all", "knn", "kmeans", "serve", "copy"],
+                    default="all",
+                    help="all = the kNN line plus the k-means and serving figures")
+    ap.add_argument("--serve-n", type=int, default=2_000_000)
+    ap.add_argument("--serve-rates", type=float, nargs="+", default=[25.0, 100.0, 200.0])
+    ap.add_argument("--serve-seconds", type=float, default=5.0)
     ap.add_argument("--mode", choices=["plain", "mixed"], default="mixed")
     ap.add_argument("--no-compare", action="store_true",
                     help="skip the FP64-form comparison step of the mixed kNN bench")
@@ -307,8 +425,21 @@ def main():
         return
     if args.workload == "copy":
         out = bench_copy(args)
+    elif args.workload == "kmeans":
+        out = bench_kmeans(args)
+    elif args.workload == "serve":
+        out = bench_serve(args)
     else:
-        out = bench_knn(args) if args.workload == "knn" else bench_kmeans(args)
+        out = bench_knn(args)
+        if args.workload == "all":
+            # the other two parts of the metric, reported beside the kNN line
+            km_args = argparse.Namespace(**vars(args))
+            km_args.steps, km_args.warmup = 1, 1
+            km = bench_kmeans(km_args)
+            out["kmeans"] = {k: km[k] for k in ("value", "unit", "ms_per_step", "dtype", "config",
+                                                "roofline", "kernels", "cpu_baseline",
+                                                "vs_fp64_form") if k in km}
+            out["serve"] = bench_serve(args)
     print(json.dumps(out), flush=True)
 
 

@@ -141,14 +141,38 @@ int sc_kmeans_step(const sc_index* index, const double* centroids, size_t k,
 /* The Lloyd loop of KMeans::fit (kmeans.rs:294-310) from caller-supplied
  * initial centroids (k*d f64, updated in place): at most max_iter steps,
  * centroid_i = sums_i / size_i when size_i > 0, stop when the previous
- * distortion <= the new one.  mode SC_MIXED runs the assignment on BF16x3
- * MFMA cross terms certified against the FP64 rule (uncertain rows are
- * reassigned in FP64), so labels equal SC_PLAIN_FP64's.  Errors:
+ * distortion <= the new one.  mode SC_MIXED (f32 rows, k <= 128, d <= 128)
+ * runs the assignment on BF16 MFMA cross terms certified against the FP64
+ * rule (rows the bound cannot separate are reassigned in FP64) fused with the
+ * FP64 sums, so its labels, sizes, centroids, distortion and iteration count
+ * equal SC_PLAIN_FP64's bit for bit; other shapes run SC_PLAIN_FP64.  Errors:
  * SC_FIT_FAILED "invalid number of clusters: k" / "invalid maximum number of
  * iterations: m" (kmeans.rs:257-269). */
 int sc_kmeans_run(const sc_index* index, size_t k, size_t max_iter,
                   sc_mode mode, double* centroids, uint32_t* labels,
                   uint64_t* size, double* distortion, uint32_t* iters);
+/* Many independent KMeans fits at once (no reference counterpart: smartcore
+ * fits one KMeans at a time, kmeans.rs:254-323; this batches that loop).
+ * Each problem gives host rows (n*d row-major, `dtype`), k and k*d initial
+ * centroids.  Outputs are concatenated in problem order: centroids
+ * (sum k_i*d_i), labels (sum n_i), sizes (sum k_i), one distortion and one
+ * iteration count per problem.  SC_MIXED with f32 rows runs every problem's
+ * certified MFMA Lloyd step in one persistent kernel per iteration, all
+ * iterations captured in one hipGraph; each problem's results are
+ * bit-identical to sc_kmeans_run(SC_MIXED) on it alone.  Other modes and
+ * shapes outside the batched kernel's limits (k <= 128, d <= 128,
+ * k*d <= 8192) run the problems one by one. */
+typedef struct {
+  const void* rows;
+  size_t n;
+  size_t d;
+  size_t k;
+  const double* init_centroids;
+} sc_kmeans_problem;
+int sc_kmeans_batch_run(const sc_kmeans_problem* problems, size_t count, sc_dtype dtype,
+                        size_t max_iter, sc_mode mode, double* centroids_out,
+                        uint32_t* labels_out, uint64_t* sizes_out, double* distortion_out,
+                        uint32_t* iters_out);
 /* KMeans::kmeans_plus_plus (kmeans.rs:354-413): initial labels (n) of the
  * index's rows for k clusters.  The random stream is rand 0.8.5's SmallRng
  * seeded by get_rng_impl (rand_custom.rs:8-33: has_seed ? seed : 0).  The
@@ -160,6 +184,46 @@ int sc_kmeans_plus_plus(const sc_index* index, size_t k, int has_seed, uint64_t 
  * row-major, dtype) against k centroids (k*d f64). */
 int sc_kmeans_predict(const void* x, sc_dtype dtype, size_t n, size_t d,
                       const double* centroids, size_t k, uint32_t* labels);
+
+/* ---- kNN serving loop (no reference counterpart: smartcore answers one
+ * find() at a time; this serves LinearKNNSearch::find, linear_search.rs:52-84,
+ * to concurrent callers) ----
+ * One index stays resident.  sc_server_submit queues a request of q query
+ * rows (host memory, copied) and returns a ticket at once; sc_server_wait
+ * blocks until that request's rows are answered (same layout as
+ * sc_knn_search) and gives its submit-to-result latency.  A scheduler thread
+ * admits queued requests while the HBM their batch needs fits the budget,
+ * coalesces requests with the same (metric, p, mode, k) into batches of up
+ * to max_batch_queries rows, and keeps up to max_inflight batches running
+ * concurrently on their own streams (continuous batching).  Results do not
+ * depend on which requests share a batch. */
+typedef struct sc_server sc_server;
+typedef struct {
+  size_t max_batch_queries; /* rows per device batch (0: 4096) */
+  size_t max_inflight;      /* batches on the device at once (0: 3) */
+  size_t hbm_budget_bytes;  /* admission budget (0: 80% of free HBM) */
+  uint32_t batch_window_us; /* wait for company when a lone small request is queued */
+} sc_server_config;
+typedef struct {
+  uint64_t completed;  /* requests answered */
+  uint64_t batches;    /* device batches launched */
+  uint64_t refused;    /* requests larger than the whole budget */
+  uint64_t queries;    /* query rows answered */
+  uint64_t pending;    /* requests waiting for admission */
+  uint64_t hbm_in_use; /* admitted bytes now */
+  uint64_t hbm_peak;   /* largest admitted bytes */
+  uint64_t hbm_budget;
+} sc_server_info;
+int sc_server_create(const sc_index* index, const sc_server_config* cfg, sc_server** out);
+/* Errors as sc_knn_search (SC_FIND_FAILED for k outside 1..n, ...);
+ * SC_ERR_OUT_OF_MEMORY when the request alone exceeds the budget. */
+int sc_server_submit(sc_server* server, const void* queries, size_t q, size_t k,
+                     sc_metric metric, uint16_t p, sc_mode mode, uint64_t* ticket);
+int sc_server_wait(sc_server* server, uint64_t ticket, int64_t* idx, double* dist,
+                   uint32_t* count, double* latency_ms);
+int sc_server_stats(sc_server* server, sc_server_info* out);
+/* Waits for launched batches; queued requests fail with SC_INVALID_STATE. */
+int sc_server_destroy(sc_server* server);
 
 /* Message of the calling thread's last failure; returns its full length
  * (writes at most len-1 bytes plus a terminating NUL). */

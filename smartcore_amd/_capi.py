@@ -266,6 +266,123 @@ class Index:
         return cent, labels[: self.n], size[:k], dist.value, iters.value
 
 
+class sc_server_config(C.Structure):
+    _fields_ = [("max_batch_queries", C.c_size_t), ("max_inflight", C.c_size_t),
+                ("hbm_budget_bytes", C.c_size_t), ("batch_window_us", C.c_uint32)]
+
+
+class sc_server_info(C.Structure):
+    _fields_ = [(f, C.c_uint64) for f in ("completed", "batches", "refused", "queries", "pending",
+                                          "hbm_in_use", "hbm_peak", "hbm_budget")]
+
+
+class Server:
+    """sc_server_*: continuous-batching kNN server over one resident index."""
+
+    def __init__(self, index: Index, max_batch_queries: int = 0, max_inflight: int = 0,
+                 hbm_budget_bytes: int = 0, batch_window_us: int = 200):
+        L = lib()
+        L.sc_server_create.argtypes = [C.c_void_p, C.POINTER(sc_server_config),
+                                       C.POINTER(C.c_void_p)]
+        L.sc_server_submit.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_size_t, C.c_int,
+                                       C.c_uint16, C.c_int, C.POINTER(C.c_uint64)]
+        L.sc_server_wait.argtypes = [C.c_void_p, C.c_uint64, C.c_void_p, C.c_void_p, C.c_void_p,
+                                     C.POINTER(C.c_double)]
+        L.sc_server_stats.argtypes = [C.c_void_p, C.POINTER(sc_server_info)]
+        L.sc_server_destroy.argtypes = [C.c_void_p]
+        self.index = index
+        cfg = sc_server_config(max_batch_queries, max_inflight, hbm_budget_bytes, batch_window_us)
+        h = C.c_void_p()
+        check(L.sc_server_create(index.handle, C.byref(cfg), C.byref(h)))
+        self._h = h
+        self._meta = {}
+
+    def submit(self, queries: np.ndarray, k: int, metric: int = SC_EUCLIDIAN, p: int = 2,
+               mode: int = SC_PLAIN_FP64) -> int:
+        queries = np.ascontiguousarray(queries, dtype=self.index.dtype)
+        if queries.ndim == 1:
+            queries = queries[None, :]
+        t = C.c_uint64()
+        check(lib().sc_server_submit(self._h, _ptr(queries), queries.shape[0], k, metric, p, mode,
+                                     C.byref(t)))
+        self._meta[t.value] = (queries.shape[0], k)
+        return t.value
+
+    def wait(self, ticket: int):
+        q, k = self._meta.pop(ticket)
+        idx = np.empty((q, k), np.int64)
+        dist = np.empty((q, k), np.float64)
+        cnt = np.empty(q, np.uint32)
+        lat = C.c_double()
+        check(lib().sc_server_wait(self._h, ticket, _ptr(idx), _ptr(dist), _ptr(cnt),
+                                   C.byref(lat)))
+        return idx, dist, cnt, lat.value
+
+    def stats(self) -> dict:
+        info = sc_server_info()
+        lib().sc_server_stats(self._h, C.byref(info))
+        return {f: getattr(info, f) for f, _ in sc_server_info._fields_}
+
+    def close(self) -> None:
+        if getattr(self, "_h", None) and self._h.value:
+            lib().sc_server_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class sc_kmeans_problem(C.Structure):
+    _fields_ = [("rows", C.c_void_p), ("n", C.c_size_t), ("d", C.c_size_t), ("k", C.c_size_t),
+                ("init_centroids", C.c_void_p)]
+
+
+def kmeans_batch_run(problems, max_iter: int, mode: int = SC_MIXED):
+    """sc_kmeans_batch_run over [(rows, init_centroids), ...] (all rows of one
+    dtype).  Returns a list of (centroids, labels, size, distortion, iters)."""
+    keep = []
+    arr = (sc_kmeans_problem * max(len(problems), 1))()
+    dtype = None
+    for i, (x, init) in enumerate(problems):
+        x = np.ascontiguousarray(x)
+        init = np.ascontiguousarray(init, dtype=np.float64)
+        dtype = x.dtype if dtype is None else dtype
+        if x.dtype != dtype:
+            raise TypeError("all problems must share one dtype")
+        keep += [x, init]
+        arr[i] = sc_kmeans_problem(x.ctypes.data, x.shape[0], x.shape[1], init.shape[0],
+                                   init.ctypes.data)
+    nk = sum(init.shape[0] * init.shape[1] for _, init in problems)
+    nl = sum(x.shape[0] for x, _ in problems)
+    ns = sum(init.shape[0] for _, init in problems)
+    cent = np.empty(max(nk, 1))
+    labels = np.empty(max(nl, 1), np.uint32)
+    sizes = np.empty(max(ns, 1), np.uint64)
+    dist = np.empty(max(len(problems), 1))
+    iters = np.empty(max(len(problems), 1), np.uint32)
+    lib().sc_kmeans_batch_run.argtypes = [C.c_void_p, C.c_size_t, C.c_int, C.c_size_t, C.c_int,
+                                          C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                          C.c_void_p]
+    check(lib().sc_kmeans_batch_run(C.cast(arr, C.c_void_p), len(problems),
+                                    dtype_code(dtype) if dtype is not None else SC_F32, max_iter,
+                                    mode, _ptr(cent), _ptr(labels), _ptr(sizes), _ptr(dist),
+                                    _ptr(iters)))
+    out = []
+    co = lo = so = 0
+    for i, (x, init) in enumerate(problems):
+        k, d = init.shape
+        n = x.shape[0]
+        out.append((cent[co:co + k * d].reshape(k, d).copy(), labels[lo:lo + n].copy(),
+                    sizes[so:so + k].copy(), float(dist[i]), int(iters[i])))
+        co += k * d
+        lo += n
+        so += k
+    return out
+
+
 def kmeans_predict(x: np.ndarray, centroids: np.ndarray) -> np.ndarray:
     x = np.ascontiguousarray(x)
     centroids = np.ascontiguousarray(centroids, dtype=np.float64)

@@ -74,3 +74,24 @@ def lognormal_rows_device(seed: int, n: int, d: int, device="cuda"):
         r = torch.exp(0.5 * torch.randn((e - s, 1), generator=g, device=device, dtype=torch.float32))
         x[s:e] = z * r
     return x
+
+
+def kmeans_batch_problems(seed: int = 4, count: int = 512, n_lo: int = 1_000, n_hi: int = 200_000,
+                          dims=(8, 16, 32, 64), k_lo: int = 4, k_hi: int = 128, dtype=np.float32):
+    """Uneven k-means problems: n ~ log-uniform[n_lo, n_hi], d from `dims`,
+    k ~ U{k_lo..k_hi}; rows are Gaussian blobs as in knn_mixture (k centres
+    ~ N(0, 4^2 I), unit noise, centre drawn uniformly per row); initial
+    centroids are k distinct rows chosen by the seed."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(count):
+        n = int(round(np.exp(rng.uniform(np.log(n_lo), np.log(n_hi)))))
+        d = int(rng.choice(dims))
+        k = int(rng.integers(k_lo, k_hi + 1))
+        k = min(k, n)
+        centres = rng.normal(0.0, 4.0, size=(k, d))
+        lab = rng.integers(0, k, size=n)
+        x = (centres[lab] + rng.normal(size=(n, d))).astype(dtype)
+        init = x[np.sort(rng.choice(n, size=k, replace=False))].astype(np.float64)
+        out.append((x, init))
+    return out

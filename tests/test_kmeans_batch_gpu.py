@@ -1,0 +1,57 @@
+"""Batched k-means (sc_kmeans_batch_run): many uneven problems in one
+persistent kernel per iteration, captured in a hipGraph.  Every problem's
+result must equal the single-problem SC_MIXED fit bit for bit, and its labels
+the FP64 form's."""
+import numpy as np
+import pytest
+
+from smartcore_amd import _capi as S
+from smartcore_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+
+def single(x, init, iters, mode):
+    ix = S.Index(x)
+    return ix.kmeans_run(init, iters, mode)
+
+
+def test_batch_equals_single_problem_runs():
+    probs = synth.kmeans_batch_problems(seed=50, count=24, n_lo=500, n_hi=40_000)
+    got = S.kmeans_batch_run(probs, 8, S.SC_MIXED)
+    for (x, init), (c, lab, size, dist, it) in zip(probs, got):
+        c1, l1, s1, d1, i1 = single(x, init, 8, S.SC_MIXED)
+        np.testing.assert_array_equal(lab, l1)
+        np.testing.assert_array_equal(size, s1)
+        assert it == i1
+        np.testing.assert_array_equal(c.view(np.uint64), c1.view(np.uint64))
+        assert np.float64(dist).view(np.uint64) == np.float64(d1).view(np.uint64)
+        c2, l2, s2, d2, i2 = single(x, init, 8, S.SC_PLAIN_FP64)
+        np.testing.assert_array_equal(lab, l2)
+
+
+def test_batch_second_call_uses_tuned_grid_and_agrees():
+    probs = synth.kmeans_batch_problems(seed=51, count=10, n_lo=300, n_hi=9_000)
+    a = S.kmeans_batch_run(probs, 6, S.SC_MIXED)
+    b = S.kmeans_batch_run(probs, 6, S.SC_MIXED)
+    for (ca, la, sa, da, ia), (cb, lb, sb, db, ib) in zip(a, b):
+        np.testing.assert_array_equal(la, lb)
+        np.testing.assert_array_equal(ca.view(np.uint64), cb.view(np.uint64))
+        assert ia == ib
+
+
+def test_batch_fp64_mode_runs_problems_one_by_one():
+    probs = synth.kmeans_batch_problems(seed=52, count=4, n_lo=300, n_hi=3_000)
+    got = S.kmeans_batch_run(probs, 5, S.SC_PLAIN_FP64)
+    for (x, init), (c, lab, size, dist, it) in zip(probs, got):
+        c1, l1, s1, d1, i1 = single(x, init, 5, S.SC_PLAIN_FP64)
+        np.testing.assert_array_equal(lab, l1)
+        np.testing.assert_array_equal(c.view(np.uint64), c1.view(np.uint64))
+
+
+def test_batch_errors():
+    x = np.ones((10, 4), np.float32)
+    with pytest.raises(S.Failed):
+        S.kmeans_batch_run([(x, np.ones((1, 4)))], 5)
+    with pytest.raises(S.Failed):
+        S.kmeans_batch_run([(x, np.ones((2, 4)))], 0)
