@@ -6,7 +6,10 @@ of the same distribution, k = 10, Euclidean.  One step is one
 sc_knn_search_device call over all 65,536 queries with the rows and queries
 already resident in HBM.  `--workload kmeans` runs the k-means Lloyd loop on
 1,000,000 x 64 f32 Gaussian blobs, k = 64, up to 100 iterations (one step =
-one fit; the reported rate is Lloyd iterations per second).
+one fit; the reported rate is Lloyd iterations per second).  `--workload
+batch` runs 512 uneven k-means problems in one batched call, `--workload
+serve` the continuous-batching server at fixed Poisson arrival rates, and the
+default `all` reports both beside the kNN and k-means figures.
 
 Prints one JSON line (rank 0) with the throughput, a roofline entry for the
 dominant kernel (its duration measured with HIP events on the stream it
@@ -396,12 +399,70 @@ def bench_kmeans(args):
     return out
 
 
+def bench_batch(args):
+    """512 independent k-means problems of uneven size (n ~ log-uniform
+    [1,000, 200,000], d in {8, 16, 32, 64}, k ~ U{4..128}, f32 blobs), 20
+    Lloyd iterations each, in one sc_kmeans_batch_run call: one persistent
+    kernel per iteration over every problem's row blocks, the iterations
+    replayed as one hipGraph.  The warmup call picks the grid size by timing;
+    the timed calls reuse it, so their graph covers all iterations."""
+    from smartcore_amd import _capi as S
+    from smartcore_amd import synth
+
+    lib = S.lib()
+    lib.sc_profile_enable.argtypes = [C.c_int]
+    lib.sc_profile_read.argtypes = [C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_uint64)]
+    iters = 20
+    t0 = time.time()
+    probs = synth.kmeans_batch_problems(seed=4, count=args.batch_count)
+    rows = sum(x.shape[0] for x, _ in probs)
+    log(f"[bench] generated {len(probs)} problems, {rows:,} rows in {time.time() - t0:.1f}s")
+    for w in range(max(args.warmup, 1)):
+        S.kmeans_batch_run(probs, iters, S.SC_MIXED)
+        log(f"[bench] batch warmup {w + 1} done")
+    lib.sc_profile_enable(1)
+    t0 = time.perf_counter()
+    for s in range(args.steps):
+        got = S.kmeans_batch_run(probs, iters, S.SC_MIXED)
+    wall = (time.perf_counter() - t0) / args.steps
+    lib.sc_profile_enable(0)
+    gms, launches = profile_read(lib, "kmeans_batch")
+    graph_s = gms / 1e3 / max(launches, 1)
+    inst_iters = sum(g[4] for g in got)
+    # algorithmic HBM bytes: every active problem's rows (f32, unpadded) and
+    # labels once per iteration it runs
+    byts = sum(g[4] * (x.shape[0] * x.shape[1] * 4 + 4 * x.shape[0]) for (x, _), g in zip(probs, got))
+    t1 = time.perf_counter()
+    ref = S.kmeans_batch_run(probs, iters, S.SC_PLAIN_FP64)
+    fp64_s = time.perf_counter() - t1
+    bad = [i for i, (a, b) in enumerate(zip(got, ref)) if not np.array_equal(a[1], b[1])]
+    it_bad = sum(1 for a, b in zip(got, ref) if a[4] != b[4])
+    log(f"[bench] batch graph {graph_s * 1e3:.1f} ms, call {wall * 1e3:.1f} ms; "
+        f"FP64 form {fp64_s * 1e3:.0f} ms; label-mismatched problems {len(bad)}")
+    return {
+        "value": inst_iters / graph_s,
+        "unit": "problem-iterations/s",
+        "ms_per_call_device": graph_s * 1e3,
+        "ms_per_call_with_transfers": wall * 1e3,
+        "config": {"workload": f"{len(probs)} k-means problems, n log-uniform[1e3, 2e5] "
+                               f"({rows:,} rows), d in {{8,16,32,64}}, k in 4..128, "
+                               f"max_iter={iters}, one persistent launch per iteration in a hipGraph"},
+        "roofline": {"bound": "hbm", "achieved": byts / graph_s / 1e9, "peak": PEAK_HBM / 1e9,
+                     "unit": "GB/s", "frac": byts / graph_s / PEAK_HBM, "traffic": None,
+                     "kernel": "kmb_step_kernel (graph of all iterations)"},
+        "vs_fp64_form": {"fp64_form_ms_per_call": fp64_s * 1e3,
+                         "speedup_vs_fp64_form": fp64_s / wall,
+                         "problems_with_label_mismatch": len(bad),
+                         "problems_with_iteration_mismatch": it_bad},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", choices=["all", "knn", "kmeans", "serve", "copy"],
+    ap.add_argument("--workload", choices=["all", "knn", "kmeans", "batch", "serve", "copy"],
                     default="all",
                     help="all = the kNN line plus the k-means and serving figures")
     ap.add_argument("--serve-n", type=int, default=2_000_000)
@@ -413,6 +474,7 @@ def main():
     ap.add_argument("--n", type=int, default=4_000_000)
     ap.add_argument("--q", type=int, default=65_536)
     ap.add_argument("--km-n", type=int, default=1_000_000)
+    ap.add_argument("--batch-count", type=int, default=512)
     ap.add_argument("--cpu-sample", type=int, default=64)
     ap.add_argument("--cpu-kmeans-iters", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
@@ -429,6 +491,11 @@ def main():
         out = bench_kmeans(args)
     elif args.workload == "serve":
         out = bench_serve(args)
+    elif args.workload == "batch":
+        out = bench_batch(args)
+        out.update({"metric": METRIC, "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+                    "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+                    "dtype": "bf16", "data": "synthetic (seeded)"})
     else:
         out = bench_knn(args)
         if args.workload == "all":
@@ -439,6 +506,9 @@ def main():
             out["kmeans"] = {k: km[k] for k in ("value", "unit", "ms_per_step", "dtype", "config",
                                                 "roofline", "kernels", "cpu_baseline",
                                                 "vs_fp64_form") if k in km}
+            b_args = argparse.Namespace(**vars(args))
+            b_args.steps, b_args.warmup = 1, 1
+            out["kmeans_batch"] = bench_batch(b_args)
             out["serve"] = bench_serve(args)
     print(json.dumps(out), flush=True)
 

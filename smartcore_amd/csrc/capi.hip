@@ -97,6 +97,9 @@ std::vector<ProfRecord> g_prof;
 
 ProfScope::ProfScope(const char* name, hipStream_t stream) : name_(name), stream_(stream) {
   if (!g_prof_on.load(std::memory_order_relaxed)) return;
+  // launches recorded into a graph are timed by the scope around the graph
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(stream_, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) return;
   if (hipEventCreate(&start_) != hipSuccess) {
     start_ = nullptr;
     return;

@@ -12,15 +12,19 @@
 //             c % 4 == w and adds each owned row into an f64 LDS accumulator
 //             (lane = dimension, ds_add_f64), in row order.  One owner per
 //             cluster makes the sums deterministic.
-//   finish    Block sums added in groups of 16 blocks, groups in order;
-//             distortion from the aggregates
+//   reduce    Two kernels shared by every form and by the batched form.
+//             group: block sums added in groups of 16 blocks, in block
+//             order.  final: one workgroup per cluster adds the groups in
+//             order, computes the cluster's distortion term and (update)
+//             centroid_i = sums_i / size_i when size_i > 0
+//             (kmeans.rs:297-303); the last workgroup of a problem adds the
+//             terms into the distortion
 //               D = sum_i ||x_i||^2 - 2 sum_c c.S_c + sum_c n_c ||c||^2
 //             (the cost BBDTree::clustering returns, bbd_tree.rs:162 and
-//             :297-305, is the same quantity); centroid_i = sums_i / size_i
-//             when size_i > 0 (kmeans.rs:297-303).
-//   control   The stop rule `if distortion <= dist break` (kmeans.rs:305-309)
-//             on the device; later kernels return at once, so a fit needs
-//             one host synchronisation.
+//             :297-305, is the same quantity) and runs the stop rule
+//             `if distortion <= dist break` (kmeans.rs:305-309) on the
+//             device; later kernels return at once, so a fit needs one host
+//             synchronisation.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -263,86 +267,190 @@ __global__ __launch_bounds__(NT) void kmeans_partial_ballot_kernel(
   }
 }
 
-// gsum[g][c][j] = sum of psum over blocks 16g .. 16g+15, in block order.
-__global__ void kmeans_group_kernel(int64_t nb, int64_t ng, int k, int d,
-                                    const double* __restrict__ psum,
-                                    const uint32_t* __restrict__ pcount, double* __restrict__ gsum,
-                                    uint64_t* __restrict__ gcount, const KmState* __restrict__ st) {
-  if (st && st->done) return;
-  const int64_t kd = static_cast<int64_t>(k) * d;
-  const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (t >= ng * kd) return;
-  const int64_t g = t / kd, e = t % kd;
-  const int64_t b0 = g * kGroupBlocks, b1 = min<int64_t>(nb, b0 + kGroupBlocks);
-  double s = 0.0;
-  for (int64_t b = b0; b < b1; ++b) s = s + psum[b * kd + e];
-  gsum[g * kd + e] = s;
-  if (e % d == 0) {
-    const int64_t c = e / d;
-    uint64_t cnt = 0;
-    for (int64_t b = b0; b < b1; ++b) cnt += pcount[b * k + c];
-    gcount[g * k + c] = cnt;
-  }
-}
-
-__global__ void kmeans_sum_kernel(int64_t ng, int k, int d, const double* __restrict__ gsum,
-                                  const uint64_t* __restrict__ gcount, double* __restrict__ sums,
-                                  uint64_t* __restrict__ counts, const KmState* __restrict__ st) {
-  if (st && st->done) return;
-  const int64_t kd = static_cast<int64_t>(k) * d;
-  const int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+// Group sums.  Item (problem, group * slices + slice): for 256 consecutive
+// elements e of k x d, gsum[g][e] = psum over blocks 16g .. 16g+15 added in
+// block order (and the counts of the clusters whose first element is in the
+// slice).  All sixteen loads are issued before the adds.
+__global__ __launch_bounds__(kReduceChunk) void kmeans_group_kernel(
+    const KmProb* __restrict__ P, const uint2* __restrict__ items) {
+  const uint2 it = items[blockIdx.x];
+  const KmProb p = P[it.x];
+  if (p.st->done) return;
+  const int64_t kd = static_cast<int64_t>(p.k) * p.d;
+  const int64_t slices = (kd + kReduceChunk - 1) / kReduceChunk;
+  const int64_t g = it.y / slices;
+  const int64_t e = (it.y % slices) * kReduceChunk + threadIdx.x;
   if (e >= kd) return;
+  const int64_t b0 = g * kGroupBlocks;
+  const int nbk = static_cast<int>(min<int64_t>(kGroupBlocks, p.nb - b0));
+  double v[kGroupBlocks];
+#pragma unroll
+  for (int i = 0; i < kGroupBlocks; ++i) v[i] = i < nbk ? p.psum[(b0 + i) * kd + e] : 0.0;
   double s = 0.0;
-  for (int64_t g = 0; g < ng; ++g) s = s + gsum[g * kd + e];
-  sums[e] = s;
-  if (e % d == 0) {
-    const int64_t c = e / d;
+#pragma unroll
+  for (int i = 0; i < kGroupBlocks; ++i)
+    if (i < nbk) s = s + v[i];
+  p.gsum[g * kd + e] = s;
+  if (e % p.d == 0) {
+    const int64_t c = e / p.d;
     uint64_t cnt = 0;
-    for (int64_t g = 0; g < ng; ++g) cnt += gcount[g * k + c];
-    counts[c] = cnt;
+    for (int i = 0; i < nbk; ++i) cnt += p.pcount[(b0 + i) * p.k + c];
+    p.gcount[g * p.k + c] = cnt;
   }
 }
 
-// One workgroup: distortion from the aggregates with the centroids of this
-// step, then (update) the new centroids.
-__global__ __launch_bounds__(NT) void kmeans_distortion_update_kernel(
-    int k, int d, int ld, const double* __restrict__ sums, const uint64_t* __restrict__ counts,
-    double* __restrict__ cent, double* __restrict__ term, KmState* __restrict__ st, int update) {
-  if (st->done) return;
+__device__ __forceinline__ uint16_t bf16_rne_bits(float f) {
+  uint32_t u = __float_as_uint(f);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return static_cast<uint16_t>(u >> 16);
+}
+
+// BF16 split of -2c (c rounded to f32 first): hi = RN(-2c), lo = RN(-2c - hi).
+__device__ __forceinline__ void split_m2(double c, uint16_t& hi, uint16_t& lo) {
+  const float v = -2.0f * static_cast<float>(c);
+  hi = bf16_rne_bits(v);
+  lo = bf16_rne_bits(v - __uint_as_float(static_cast<uint32_t>(hi) << 16));
+}
+
+// Largest finite ||c||^2 of a problem, rounded up, and its square root.
+__device__ __forceinline__ void set_cmax(const KmProb& p) {
+  const volatile double* c2 = p.c2d;
+  double mx = 0.0;
+  for (int c = 0; c < p.k; ++c) {
+    const double v = c2[c];
+    if (v < 1e300) mx = fmax(mx, v);  // NaN and inf centroids never count
+  }
+  p.cmax[0] = __double2float_ru(sqrt(mx) * (1.0 + 9.5367431640625e-07));
+  p.cmax[1] = __double2float_ru(mx * (1.0 + 9.5367431640625e-07));
+}
+
+// Final sums.  Item (problem, cluster c), 128 threads: sums_c = group sums
+// in group order, the count, the distortion term of c with this step's
+// centroid (dot and ||c||^2 added in dimension order), then (update)
+// c = sums_c / n_c when n_c > 0, and the MFMA terms of the resulting
+// centroid when the problem has them.  The last workgroup of a problem adds
+// the terms in cluster order, applies the stop rule (control) and sets the
+// centroid-norm maxima.
+__global__ __launch_bounds__(128) void kmeans_final_kernel(const KmProb* __restrict__ P,
+                                                           const uint2* __restrict__ items,
+                                                           int update, int control) {
+  const uint2 it = items[blockIdx.x];
+  const KmProb p = P[it.x];
+  if (p.st->done) return;
+  const int c = static_cast<int>(it.y);
   const int tid = threadIdx.x;
-  for (int c = tid; c < k; c += NT) {
-    const double* cc = cent + static_cast<int64_t>(c) * ld;
-    const double* sc_ = sums + static_cast<int64_t>(c) * d;
-    double dot = 0.0, c2 = 0.0;
-    for (int j = 0; j < d; ++j) {
-      dot = dot + cc[j] * sc_[j];
-      c2 = c2 + cc[j] * cc[j];
-    }
-    term[c] = counts[c] > 0 ? static_cast<double>(counts[c]) * c2 - 2.0 * dot : 0.0;
-  }
-  __syncthreads();
+  const int d = p.d;
+  const int64_t kd = static_cast<int64_t>(p.k) * d;
+  const bool tables = p.bhi != nullptr;
+  __shared__ double s_dot[128], s_c2[128], s_n2[128];
+  __shared__ uint64_t s_n;
+  __shared__ int s_last;
   if (tid == 0) {
-    double s = st->x2total;
-    for (int c = 0; c < k; ++c) s = s + term[c];
-    st->dist = s;
+    uint64_t n = 0;
+    for (int64_t g = 0; g < p.ng; ++g) n += p.gcount[g * p.k + c];
+    s_n = n;
+    p.counts[c] = n;
   }
   __syncthreads();
-  if (update) {
-    for (int e = tid; e < k * d; e += NT) {
-      const int c = e / d, j = e % d;
-      if (counts[c] > 0)
-        cent[static_cast<int64_t>(c) * ld + j] = sums[e] / static_cast<double>(counts[c]);
+  const uint64_t n = s_n;
+  const bool upd = update && n > 0;
+  double dot = 0.0, c2 = 0.0, n2 = 0.0;
+  for (int j0 = 0; j0 < d; j0 += 128) {
+    const int j = j0 + tid;
+    if (j < d) {
+      const double* gs = p.gsum + static_cast<int64_t>(c) * d + j;
+      double S = 0.0;
+      int64_t g = 0;
+      for (; g + 8 <= p.ng; g += 8) {
+        double v[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) v[i] = gs[(g + i) * kd];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) S = S + v[i];
+      }
+      for (; g < p.ng; ++g) S = S + gs[g * kd];
+      p.sums[static_cast<int64_t>(c) * d + j] = S;
+      const double cj = p.cent[static_cast<int64_t>(c) * p.ld + j];
+      const double nc = upd ? S / static_cast<double>(n) : cj;
+      s_dot[tid] = cj * S;
+      s_c2[tid] = cj * cj;
+      s_n2[tid] = nc * nc;
+      if (upd) p.cent[static_cast<int64_t>(c) * p.ld + j] = nc;
+      if (tables) {
+        uint16_t h, l;
+        split_m2(nc, h, l);
+        p.bhi[static_cast<int64_t>(c) * p.ld + j] = h;
+        p.blo[static_cast<int64_t>(c) * p.ld + j] = l;
+      }
+    }
+    __syncthreads();
+    if (tid == 0) {
+      const int m = min(128, d - j0);
+      for (int jj = 0; jj < m; ++jj) {
+        dot = dot + s_dot[jj];
+        c2 = c2 + s_c2[jj];
+        n2 = n2 + s_n2[jj];
+      }
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    p.term[c] = n > 0 ? static_cast<double>(n) * c2 - 2.0 * dot : 0.0;
+    if (tables) {
+      p.c2f[c] = static_cast<float>(n2);
+      p.c2d[c] = n2;
+    }
+    __threadfence();
+    const uint32_t old = atomicAdd(p.arrive, 1u);
+    s_last = old == static_cast<uint32_t>(p.k - 1) ? 1 : 0;
+  }
+  __syncthreads();
+  if (!s_last || tid != 0) return;
+  __threadfence();
+  const volatile double* tv = p.term;
+  double s = p.st->x2total;
+  for (int cc = 0; cc < p.k; ++cc) s = s + tv[cc];
+  p.st->dist = s;
+  if (control) {
+    p.st->iters += 1;
+    if (p.st->distortion <= s) {
+      p.st->done = 1;
+    } else {
+      p.st->distortion = s;
     }
   }
+  if (tables) set_cmax(p);
+  *p.arrive = 0u;
 }
 
-__global__ void kmeans_control_kernel(KmState* st) {
-  if (st->done) return;
-  st->iters += 1;
-  if (st->distortion <= st->dist) {
-    st->done = 1;
-  } else {
-    st->distortion = st->dist;
+// MFMA terms of the current centroids: one workgroup per problem.  Padding
+// rows (k .. kpad) and padding dimensions get zeros and c2f = +inf.
+__global__ __launch_bounds__(NT) void kmeans_tables_kernel(const KmProb* __restrict__ P) {
+  const KmProb p = P[blockIdx.x];
+  if (p.bhi == nullptr) return;
+  for (int64_t e = threadIdx.x; e < static_cast<int64_t>(p.kpad) * p.ld; e += NT) {
+    const int c = static_cast<int>(e / p.ld), j = static_cast<int>(e % p.ld);
+    uint16_t h = 0, l = 0;
+    if (c < p.k && j < p.d) split_m2(p.cent[static_cast<int64_t>(c) * p.ld + j], h, l);
+    p.bhi[e] = h;
+    p.blo[e] = l;
+  }
+  for (int c = threadIdx.x; c < p.kpad; c += NT) {
+    if (c < p.k) {
+      const double* cc = p.cent + static_cast<int64_t>(c) * p.ld;
+      double s = 0.0;
+      for (int j = 0; j < p.d; ++j) s = s + cc[j] * cc[j];
+      p.c2f[c] = static_cast<float>(s);
+      p.c2d[c] = s;
+    } else {
+      p.c2f[c] = INFINITY;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence();
+    set_cmax(p);
+    *p.arrive = 0u;
   }
 }
 
@@ -406,18 +514,18 @@ int kmeans_assign_launch(const sc_index* ix, const void* rows, int64_t n, const 
 
 int kmeans_x2_launch(const sc_index* ix, KmWork& w, hipStream_t s) {
   const int64_t n = static_cast<int64_t>(ix->n);
-  if (w.nblocks > 0) {
+  if (w.p.nb > 0) {
     if (ix->dtype == SC_F64)
-      hipLaunchKernelGGL(x2_block_kernel<double>, dim3(static_cast<unsigned>(w.nblocks)), dim3(NT),
+      hipLaunchKernelGGL(x2_block_kernel<double>, dim3(static_cast<unsigned>(w.p.nb)), dim3(NT),
                          0, s, static_cast<const double*>(ix->rows), n, static_cast<int>(ix->ld),
-                         w.x2block);
+                         w.p.x2block);
     else
-      hipLaunchKernelGGL(x2_block_kernel<float>, dim3(static_cast<unsigned>(w.nblocks)), dim3(NT),
+      hipLaunchKernelGGL(x2_block_kernel<float>, dim3(static_cast<unsigned>(w.p.nb)), dim3(NT),
                          0, s, static_cast<const float*>(ix->rows), n, static_cast<int>(ix->ld),
-                         w.x2block);
+                         w.p.x2block);
     SC_HIP(hipGetLastError());
   }
-  hipLaunchKernelGGL(x2_total_kernel, dim3(1), dim3(1), 0, s, w.x2block, w.nblocks, w.state);
+  hipLaunchKernelGGL(x2_total_kernel, dim3(1), dim3(1), 0, s, w.p.x2block, w.p.nb, w.p.st);
   SC_HIP(hipGetLastError());
   return SC_OK;
 }
@@ -426,90 +534,140 @@ int kmeans_partial_launch(const sc_index* ix, KmWork& w, int k, const KmState* s
   const int64_t n = static_cast<int64_t>(ix->n);
   const int d = static_cast<int>(ix->d);
   const int ld = static_cast<int>(ix->ld);
-  if (w.nblocks == 0) return SC_OK;
+  if (w.p.nb == 0) return SC_OK;
   ProfScope ps("kmeans_partial", s);
   const size_t lds = partial_lds_bytes(ix, k);
-  const dim3 grid(static_cast<unsigned>(w.nblocks));
+  const dim3 grid(static_cast<unsigned>(w.p.nb));
   if (static_cast<int64_t>(k) * d <= kMaxLdsAcc && lds <= 96 * 1024) {
     if (ix->dtype == SC_F64)
       hipLaunchKernelGGL(kmeans_partial_lds_kernel<double>, grid, dim3(NT), lds, s,
-                         static_cast<const double*>(ix->rows), n, ld, d, w.labels, k, w.psum,
-                         w.pcount, st);
+                         static_cast<const double*>(ix->rows), n, ld, d, w.p.labels, k, w.p.psum,
+                         w.p.pcount, st);
     else
       hipLaunchKernelGGL(kmeans_partial_lds_kernel<float>, grid, dim3(NT), lds, s,
-                         static_cast<const float*>(ix->rows), n, ld, d, w.labels, k, w.psum,
-                         w.pcount, st);
+                         static_cast<const float*>(ix->rows), n, ld, d, w.p.labels, k, w.p.psum,
+                         w.p.pcount, st);
   } else {
     if (ix->dtype == SC_F64)
       hipLaunchKernelGGL(kmeans_partial_ballot_kernel<double>, grid, dim3(NT), 0, s,
-                         static_cast<const double*>(ix->rows), n, ld, d, w.labels, k, w.psum,
-                         w.pcount, st);
+                         static_cast<const double*>(ix->rows), n, ld, d, w.p.labels, k, w.p.psum,
+                         w.p.pcount, st);
     else
       hipLaunchKernelGGL(kmeans_partial_ballot_kernel<float>, grid, dim3(NT), 0, s,
-                         static_cast<const float*>(ix->rows), n, ld, d, w.labels, k, w.psum,
-                         w.pcount, st);
+                         static_cast<const float*>(ix->rows), n, ld, d, w.p.labels, k, w.p.psum,
+                         w.p.pcount, st);
   }
   SC_HIP(hipGetLastError());
   return SC_OK;
 }
 
-int kmeans_finish_launch(const sc_index* ix, KmWork& w, int k, KmState* st, double* d_cent,
-                         bool update, hipStream_t s) {
-  const int d = static_cast<int>(ix->d);
-  const int64_t kd = std::max<int64_t>(1, static_cast<int64_t>(k) * d);
+int kmeans_reduce_launch(const KmProb* d_probs, const KmItems& items, bool update, bool control,
+                         hipStream_t s) {
   ProfScope ps("kmeans_reduce", s);
-  if (w.ngroups > 0) {
-    const int64_t t = w.ngroups * kd;
-    hipLaunchKernelGGL(kmeans_group_kernel, dim3(static_cast<unsigned>((t + 255) / 256)), dim3(256),
-                       0, s, w.nblocks, w.ngroups, k, d, w.psum, w.pcount, w.gsum, w.gcount, st);
+  if (items.ngroup > 0) {
+    hipLaunchKernelGGL(kmeans_group_kernel, dim3(static_cast<unsigned>(items.ngroup)),
+                       dim3(kReduceChunk), 0, s, d_probs, items.group);
     SC_HIP(hipGetLastError());
   }
-  hipLaunchKernelGGL(kmeans_sum_kernel, dim3(static_cast<unsigned>((kd + 255) / 256)), dim3(256), 0,
-                     s, w.ngroups, k, d, w.gsum, w.gcount, w.sums, w.counts, st);
-  SC_HIP(hipGetLastError());
-  hipLaunchKernelGGL(kmeans_distortion_update_kernel, dim3(1), dim3(NT), 0, s, k, d,
-                     static_cast<int>(ix->ld), w.sums, w.counts, d_cent,
-                     w.x2block + std::max<int64_t>(w.nblocks, 1), st, update ? 1 : 0);
+  if (items.nfinal > 0) {
+    hipLaunchKernelGGL(kmeans_final_kernel, dim3(static_cast<unsigned>(items.nfinal)), dim3(128), 0,
+                       s, d_probs, items.fin, update ? 1 : 0, control ? 1 : 0);
+    SC_HIP(hipGetLastError());
+  }
+  return SC_OK;
+}
+
+int kmeans_tables_launch(const KmProb* d_probs, size_t count, hipStream_t s) {
+  if (count == 0) return SC_OK;
+  hipLaunchKernelGGL(kmeans_tables_kernel, dim3(static_cast<unsigned>(count)), dim3(NT), 0, s,
+                     d_probs);
   SC_HIP(hipGetLastError());
   return SC_OK;
 }
 
-int KmWork::alloc(const sc_index* ix, int k, hipStream_t s) {
-  const int64_t n = static_cast<int64_t>(ix->n);
-  const int64_t d = std::max<int64_t>(static_cast<int64_t>(ix->d), 1);
-  nblocks = (n + kBlockRows - 1) / kBlockRows;
-  ngroups = (nblocks + kGroupBlocks - 1) / kGroupBlocks;
-  const int64_t nb1 = std::max<int64_t>(nblocks, 1), ng1 = std::max<int64_t>(ngroups, 1);
-  SC_HIP(b_labels.alloc(sizeof(uint32_t) * std::max<int64_t>(n, 1), s));
-  SC_HIP(b_psum.alloc(sizeof(double) * nb1 * k * d, s));
-  SC_HIP(b_pcount.alloc(sizeof(uint32_t) * nb1 * k, s));
-  SC_HIP(b_gsum.alloc(sizeof(double) * ng1 * k * d, s));
-  SC_HIP(b_gcount.alloc(sizeof(uint64_t) * ng1 * k, s));
-  SC_HIP(b_sums.alloc(sizeof(double) * k * d, s));
-  SC_HIP(b_counts.alloc(sizeof(uint64_t) * k, s));
-  // x2 per block, then k distortion terms
-  SC_HIP(b_x2.alloc(sizeof(double) * (nb1 + k), s));
-  SC_HIP(b_state.alloc(sizeof(KmState), s));
-  labels = b_labels.as<uint32_t>();
-  psum = b_psum.as<double>();
-  pcount = b_pcount.as<uint32_t>();
-  gsum = b_gsum.as<double>();
-  gcount = b_gcount.as<uint64_t>();
-  sums = b_sums.as<double>();
-  counts = b_counts.as<uint64_t>();
-  x2block = b_x2.as<double>();
-  state = b_state.as<KmState>();
+int KmItems::build(const KmProb* probs, size_t count, hipStream_t s) {
+  std::vector<uint2> g, f;
+  for (size_t i = 0; i < count; ++i) {
+    const KmProb& p = probs[i];
+    const int64_t kd = static_cast<int64_t>(p.k) * p.d;
+    const int64_t slices = (kd + kReduceChunk - 1) / kReduceChunk;
+    for (int64_t t = 0; t < p.ng * slices; ++t)
+      g.push_back(make_uint2(static_cast<unsigned>(i), static_cast<unsigned>(t)));
+    for (int c = 0; c < p.k; ++c) f.push_back(make_uint2(static_cast<unsigned>(i), static_cast<unsigned>(c)));
+  }
+  ngroup = static_cast<int64_t>(g.size());
+  nfinal = static_cast<int64_t>(f.size());
+  if (ngroup) {
+    SC_HIP(b_group.alloc(sizeof(uint2) * ngroup, s));
+    SC_HIP(hipMemcpyAsync(b_group.p, g.data(), sizeof(uint2) * ngroup, hipMemcpyHostToDevice, s));
+    group = b_group.as<uint2>();
+  }
+  if (nfinal) {
+    SC_HIP(b_final.alloc(sizeof(uint2) * nfinal, s));
+    SC_HIP(hipMemcpyAsync(b_final.p, f.data(), sizeof(uint2) * nfinal, hipMemcpyHostToDevice, s));
+    fin = b_final.as<uint2>();
+  }
+  // the host vectors die here: make the copies complete first
+  SC_HIP(hipStreamSynchronize(s));
   return SC_OK;
+}
+
+int KmWork::alloc(const sc_index* ix, int k, const double* d_cent, bool mixed, hipStream_t s) {
+  const int64_t n = static_cast<int64_t>(ix->n);
+  const int64_t d = static_cast<int64_t>(ix->d);
+  const int64_t d1 = std::max<int64_t>(d, 1);
+  p = KmProb{};
+  p.n = n;
+  p.d = static_cast<int>(d);
+  p.ld = static_cast<int>(ix->ld);
+  p.k = k;
+  p.kpad = (k + 31) / 32 * 32;
+  p.nb = (n + kBlockRows - 1) / kBlockRows;
+  p.ng = (p.nb + kGroupBlocks - 1) / kGroupBlocks;
+  const int64_t nb1 = std::max<int64_t>(p.nb, 1), ng1 = std::max<int64_t>(p.ng, 1);
+  SC_HIP(b_labels.alloc(sizeof(uint32_t) * std::max<int64_t>(n, 1), s));
+  SC_HIP(b_psum.alloc(sizeof(double) * nb1 * k * d1, s));
+  SC_HIP(b_pcount.alloc(sizeof(uint32_t) * nb1 * k, s));
+  SC_HIP(b_gsum.alloc(sizeof(double) * ng1 * k * d1, s));
+  SC_HIP(b_gcount.alloc(sizeof(uint64_t) * ng1 * k, s));
+  SC_HIP(b_sums.alloc(sizeof(double) * k * d1, s));
+  SC_HIP(b_counts.alloc(sizeof(uint64_t) * k, s));
+  // x2 per block | k distortion terms | k centroid norms (MFMA form)
+  SC_HIP(b_x2.alloc(sizeof(double) * (nb1 + 2 * static_cast<int64_t>(k)), s));
+  // the state, then the arrival counter
+  SC_HIP(b_state.alloc(sizeof(KmState) + 16, s));
+  SC_HIP(b_prob.alloc(sizeof(KmProb), s));
+  p.rows = ix->rows;
+  p.cent = const_cast<double*>(d_cent);
+  p.labels = b_labels.as<uint32_t>();
+  p.psum = b_psum.as<double>();
+  p.pcount = b_pcount.as<uint32_t>();
+  p.gsum = b_gsum.as<double>();
+  p.gcount = b_gcount.as<uint64_t>();
+  p.sums = b_sums.as<double>();
+  p.counts = b_counts.as<uint64_t>();
+  p.x2block = b_x2.as<double>();
+  p.term = p.x2block + nb1;
+  p.st = b_state.as<KmState>();
+  p.arrive = reinterpret_cast<uint32_t*>(b_state.as<char>() + sizeof(KmState));
+  SC_HIP(hipMemsetAsync(p.arrive, 0, 4, s));
+  if (mixed) {
+    const int64_t tab = static_cast<int64_t>(p.kpad) * p.ld;
+    SC_HIP(b_tab.alloc(sizeof(uint16_t) * 2 * tab + sizeof(float) * (p.kpad + 4), s));
+    p.bhi = b_tab.as<uint16_t>();
+    p.blo = p.bhi + tab;
+    p.c2f = reinterpret_cast<float*>(p.blo + tab);
+    p.cmax = p.c2f + p.kpad;
+    p.c2d = p.term + k;
+  }
+  SC_HIP(hipMemcpyAsync(b_prob.p, &p, sizeof(KmProb), hipMemcpyHostToDevice, s));
+  dprob = b_prob.as<KmProb>();
+  // builds the item lists and waits for the stream (p's copy included)
+  return items.build(&p, 1, s);
 }
 
 int kmeans_state_init(KmState* st, hipStream_t s) {
   hipLaunchKernelGGL(kmeans_state_init_kernel, dim3(1), dim3(1), 0, s, st);
-  SC_HIP(hipGetLastError());
-  return SC_OK;
-}
-
-int kmeans_control_launch(KmState* st, hipStream_t s) {
-  hipLaunchKernelGGL(kmeans_control_kernel, dim3(1), dim3(1), 0, s, st);
   SC_HIP(hipGetLastError());
   return SC_OK;
 }
@@ -528,42 +686,43 @@ int upload_centroids(const double* host, int k, size_t d, size_t ld, double* dev
   return SC_OK;
 }
 
-// One Lloyd step of the chosen form into w (labels, sums, counts, st->dist).
-int lloyd_step(const sc_index* ix, KmWork& w, int k, double* cent, bool mixed, bool update,
+// One Lloyd step of the chosen form into w (labels, sums, counts, st->dist;
+// with `update` the new centroids, with `control` the stop rule).
+int lloyd_step(const sc_index* ix, KmWork& w, bool mixed, bool update, bool control,
                hipStream_t s) {
   if (mixed) {
-    if (int rc = kmeans_mixed_step_launch(ix, w, k, cent, w.state, s)) return rc;
+    if (int rc = kmeans_mixed_step_launch(ix, w, s)) return rc;
   } else {
-    if (int rc = kmeans_assign_launch(ix, ix->rows, static_cast<int64_t>(ix->n), cent, k, w.labels,
-                                      w.state, s))
+    if (int rc = kmeans_assign_launch(ix, ix->rows, static_cast<int64_t>(ix->n), w.p.cent, w.p.k,
+                                      w.p.labels, w.p.st, s))
       return rc;
-    if (int rc = kmeans_partial_launch(ix, w, k, w.state, s)) return rc;
+    if (int rc = kmeans_partial_launch(ix, w, w.p.k, w.p.st, s)) return rc;
   }
-  return kmeans_finish_launch(ix, w, k, w.state, cent, update, s);
+  return kmeans_reduce_launch(w.dprob, w.items, update, control, s);
 }
 
 int kmeans_run_impl(const sc_index* index, int k, size_t max_iter, bool mixed, double* centroids,
                     uint32_t* labels, uint64_t* size, double* distortion, uint32_t* iters,
                     hipStream_t s, uint64_t* fallback) {
-  KmWork w;
-  if (int rc = w.alloc(index, k, s)) return rc;
   DevBuf cent;
   SC_HIP(cent.alloc(sizeof(double) * k * index->ld, s));
   if (int rc = upload_centroids(centroids, k, index->d, index->ld, cent.as<double>(), s)) return rc;
-  if (int rc = kmeans_state_init(w.state, s)) return rc;
+  KmWork w;
+  if (int rc = w.alloc(index, k, cent.as<double>(), mixed, s)) return rc;
+  if (int rc = kmeans_state_init(w.p.st, s)) return rc;
   if (int rc = kmeans_x2_launch(index, w, s)) return rc;
-  for (size_t it = 0; it < max_iter; ++it) {
-    if (int rc = lloyd_step(index, w, k, cent.as<double>(), mixed, true, s)) return rc;
-    if (int rc = kmeans_control_launch(w.state, s)) return rc;
-  }
+  if (mixed)
+    if (int rc = kmeans_tables_launch(w.dprob, 1, s)) return rc;
+  for (size_t it = 0; it < max_iter; ++it)
+    if (int rc = lloyd_step(index, w, mixed, true, true, s)) return rc;
   KmState hs{};
   if (index->d > 0)
     SC_HIP(hipMemcpy2DAsync(centroids, index->d * sizeof(double), cent.p,
                             index->ld * sizeof(double), index->d * sizeof(double), k,
                             hipMemcpyDeviceToHost, s));
-  SC_HIP(hipMemcpyAsync(labels, w.labels, sizeof(uint32_t) * index->n, hipMemcpyDeviceToHost, s));
-  SC_HIP(hipMemcpyAsync(size, w.counts, sizeof(uint64_t) * k, hipMemcpyDeviceToHost, s));
-  SC_HIP(hipMemcpyAsync(&hs, w.state, sizeof(KmState), hipMemcpyDeviceToHost, s));
+  SC_HIP(hipMemcpyAsync(labels, w.p.labels, sizeof(uint32_t) * index->n, hipMemcpyDeviceToHost, s));
+  SC_HIP(hipMemcpyAsync(size, w.p.counts, sizeof(uint64_t) * k, hipMemcpyDeviceToHost, s));
+  SC_HIP(hipMemcpyAsync(&hs, w.p.st, sizeof(KmState), hipMemcpyDeviceToHost, s));
   SC_HIP(hipStreamSynchronize(s));
   *distortion = hs.distortion;
   *iters = hs.iters;
@@ -589,23 +748,22 @@ int sc_kmeans_step(const sc_index* index, const double* centroids, size_t k, dou
   }
   SC_HIP(hipSetDevice(index->device));
   hipStream_t s = stream_for(index);
-  KmWork w;
-  if (int rc = w.alloc(index, static_cast<int>(k), s)) return rc;
   DevBuf cent;
   SC_HIP(cent.alloc(sizeof(double) * k * index->ld, s));
   if (int rc = upload_centroids(centroids, static_cast<int>(k), index->d, index->ld,
                                 cent.as<double>(), s))
     return rc;
-  if (int rc = kmeans_state_init(w.state, s)) return rc;
+  KmWork w;
+  if (int rc = w.alloc(index, static_cast<int>(k), cent.as<double>(), false, s)) return rc;
+  if (int rc = kmeans_state_init(w.p.st, s)) return rc;
   if (int rc = kmeans_x2_launch(index, w, s)) return rc;
-  if (int rc = lloyd_step(index, w, static_cast<int>(k), cent.as<double>(), false, false, s))
-    return rc;
+  if (int rc = lloyd_step(index, w, false, false, false, s)) return rc;
   KmState hs{};
   if (index->d > 0)
-    SC_HIP(hipMemcpyAsync(sums, w.sums, sizeof(double) * k * index->d, hipMemcpyDeviceToHost, s));
-  SC_HIP(hipMemcpyAsync(counts, w.counts, sizeof(uint64_t) * k, hipMemcpyDeviceToHost, s));
-  SC_HIP(hipMemcpyAsync(membership, w.labels, sizeof(uint32_t) * index->n, hipMemcpyDeviceToHost, s));
-  SC_HIP(hipMemcpyAsync(&hs, w.state, sizeof(KmState), hipMemcpyDeviceToHost, s));
+    SC_HIP(hipMemcpyAsync(sums, w.p.sums, sizeof(double) * k * index->d, hipMemcpyDeviceToHost, s));
+  SC_HIP(hipMemcpyAsync(counts, w.p.counts, sizeof(uint64_t) * k, hipMemcpyDeviceToHost, s));
+  SC_HIP(hipMemcpyAsync(membership, w.p.labels, sizeof(uint32_t) * index->n, hipMemcpyDeviceToHost, s));
+  SC_HIP(hipMemcpyAsync(&hs, w.p.st, sizeof(KmState), hipMemcpyDeviceToHost, s));
   SC_HIP(hipStreamSynchronize(s));
   *distortion = hs.dist;
   return SC_OK;

@@ -1,4 +1,5 @@
-// k-means device state and launch helpers shared by the FP64 and MFMA forms.
+// k-means device state and launch helpers shared by the FP64 and MFMA forms
+// and by the batched form.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -20,6 +21,8 @@ constexpr int kGroupBlocks = 16;
 constexpr int kChunkRows = 128;
 // Largest k * d whose f64 accumulators fit the LDS partial kernels.
 constexpr int kMaxLdsAcc = 8192;
+// Elements of one cluster's sum vector handled by one reduce workgroup.
+constexpr int kReduceChunk = 256;
 
 // Device-side Lloyd loop state (kmeans.rs:273, :305-309).
 struct KmState {
@@ -31,19 +34,53 @@ struct KmState {
   unsigned long long fallback;  // rows reassigned in FP64 by the certified MFMA form
 };
 
+// Everything the kernels of one k-means problem touch.  The batched form
+// keeps an array of these in device memory; the single-problem form passes
+// one by value.
+struct KmProb {
+  int64_t n, nb, ng;
+  int d, ld, k, kpad;
+  const void* rows;     // n x ld, f32 (or f64 in the FP64 form)
+  double* cent;         // k x ld, the centroids of the current step
+  uint32_t* labels;     // n
+  double* psum;         // nb x k x d
+  uint32_t* pcount;     // nb x k
+  double* gsum;         // ng x k x d
+  uint64_t* gcount;     // ng x k
+  double* sums;         // k x d
+  uint64_t* counts;     // k
+  double* term;         // k distortion terms of the last step
+  double* x2block;      // nb
+  KmState* st;
+  uint32_t* arrive;     // arrival counter of the final reduce kernel
+  // MFMA form only (null otherwise): the terms of the current centroids
+  uint16_t* bhi;        // kpad x ld, BF16 RN(-2 c)
+  uint16_t* blo;        // kpad x ld, BF16 RN(-2 c - bhi)
+  float* c2f;           // kpad, RN(||c||^2); +inf for padding rows
+  double* c2d;          // k, ||c||^2 in f64
+  float* cmax;          // RU(max ||c||), RU(max ||c||^2)
+};
+
+// Work items of the reduce kernels over a set of problems.
+//   group items: (problem, group * slices + slice), a slice being 256
+//                consecutive elements of the k x d sums
+//   final items: (problem, cluster)
+struct KmItems {
+  DevBuf b_group, b_final;
+  uint2* group = nullptr;
+  uint2* fin = nullptr;
+  int64_t ngroup = 0, nfinal = 0;
+  int build(const KmProb* probs, size_t count, hipStream_t s);
+};
+
+// Buffers of one problem: p points into them; dprob is p in device memory.
 struct KmWork {
-  DevBuf b_labels, b_psum, b_pcount, b_gsum, b_gcount, b_sums, b_counts, b_state, b_x2;
-  uint32_t* labels = nullptr;
-  double* psum = nullptr;      // nblocks x k x d
-  uint32_t* pcount = nullptr;  // nblocks x k
-  double* gsum = nullptr;      // ngroups x k x d
-  uint64_t* gcount = nullptr;  // ngroups x k
-  double* sums = nullptr;      // k x d
-  uint64_t* counts = nullptr;  // k
-  double* x2block = nullptr;   // nblocks
-  KmState* state = nullptr;
-  int64_t nblocks = 0, ngroups = 0;
-  int alloc(const sc_index* ix, int k, hipStream_t s);
+  DevBuf b_labels, b_psum, b_pcount, b_gsum, b_gcount, b_sums, b_counts, b_state, b_x2, b_tab,
+      b_prob;
+  KmProb p{};
+  KmProb* dprob = nullptr;
+  KmItems items;
+  int alloc(const sc_index* ix, int k, const double* d_cent, bool mixed, hipStream_t s);
 };
 
 int kmeans_state_init(KmState* st, hipStream_t s);
@@ -53,16 +90,20 @@ int kmeans_assign_launch(const sc_index* ix, const void* rows, int64_t n, const 
                          int k, uint32_t* d_labels, const KmState* st, hipStream_t s);
 // Block partial sums of the rows by label (FP64 form).
 int kmeans_partial_launch(const sc_index* ix, KmWork& w, int k, const KmState* st, hipStream_t s);
-// Group and final sums, counts, distortion from the aggregates, and (when
-// `update`) centroid_i = sums_i / size_i for size_i > 0.
-int kmeans_finish_launch(const sc_index* ix, KmWork& w, int k, KmState* st, double* d_cent,
-                         bool update, hipStream_t s);
-int kmeans_control_launch(KmState* st, hipStream_t s);
+// Group and final sums, counts, distortion from the aggregates
+//   D = sum_i ||x_i||^2 - 2 sum_c c.S_c + sum_c n_c ||c||^2,
+// and (update) centroid_i = sums_i / size_i for size_i > 0, (control) the
+// stop rule, and the MFMA terms of the new centroids when the problem has
+// them.  `probs` is a device array of `count` problems.
+int kmeans_reduce_launch(const KmProb* d_probs, const KmItems& items, bool update, bool control,
+                         hipStream_t s);
+// MFMA terms of the current centroids (before the first step).
+int kmeans_tables_launch(const KmProb* d_probs, size_t count, hipStream_t s);
 
-// One certified MFMA assignment + fused partial sums (same outputs as
-// assign + partial of the FP64 form).
-int kmeans_mixed_step_launch(const sc_index* ix, KmWork& w, int k, const double* d_cent,
-                             KmState* st, hipStream_t s);
+// One certified MFMA assignment fused with the partial sums, for every
+// block of a single problem (same labels and partials as assign + partial of
+// the FP64 form).
+int kmeans_mixed_step_launch(const sc_index* ix, KmWork& w, hipStream_t s);
 bool kmeans_mixed_supported(const sc_index* ix, int k);
 
 }  // namespace sc

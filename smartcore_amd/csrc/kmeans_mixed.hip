@@ -1,34 +1,50 @@
-// Certified BF16-MFMA Lloyd step for f32 rows (sc_kmeans_run, SC_MIXED).
+// Certified BF16-MFMA Lloyd step for f32 rows (sc_kmeans_run and
+// sc_kmeans_batch_run, SC_MIXED), fused with the centroid partial sums.
 //
-// One fused pass over the rows per iteration:
-//   * a workgroup owns a block of kBlockRows rows and stages 128 rows at a
-//     time through LDS (one HBM read of the f32 rows per iteration);
-//   * each wave converts its 32 rows to BF16 B-fragments in registers and
-//     multiplies them with the BF16 centroids (A-fragments, loaded once per
-//     iteration) on v_mfma_f32_32x32x16_bf16, giving approximate
-//     a_c = ||c||^2 - 2 x.c for every centroid;
-//   * the label is certified: with E bounding |(s_c - ||x||^2) - a_c| for every
-//     c (s_c the f64 squared distance KMeans::predict computes,
-//     kmeans.rs:338-346), a row whose only centroid with a_c <= min a + 2E is
-//     c* has s_c > s_{c*} for every other c, so c* is the exact argmin.  Rows
-//     with more than one such centroid (or non-finite values) are assigned in
-//     FP64 by the wave, lanes = centroids, the same arithmetic and
-//     lowest-index tie rule as the FP64 assign kernel;
-//   * the block's per-cluster f64 sums are accumulated exactly as the FP64
-//     form's partial kernel does (owner wave per cluster, rows in order), so
-//     both forms produce bit-identical partials, sums, centroids and
-//     distortions whenever their labels agree -- which certification
-//     guarantees.
+// A workgroup owns one block of kBlockRows rows and streams it through LDS
+// 128 rows at a time; the next chunk is loaded into registers while the
+// current one is processed, so each row is read from HBM once per step.
 //
-// Bound: x.c - C <= c1 ||x|| ||c||, c1 = 2^-7 + 2^-16 + 2^-20 + 128 * 2^-22
-// (BF16 rounding of both operands -- the centroid's through f32 --, FP32
-// accumulation at four ulps per step), and every other rounding (f32 norms,
-// the epilogue FMA, the f64 sum behind s_c) below 2^-20 (||x||^2 + max ||c||^2).
+// Labels.  Each wave takes 32 rows of the chunk and splits them into BF16
+// hi/lo terms (x = xh + xl + r, |r| <= 2^-16 |x|).  The centroid tables hold
+// the same split of -2c.  Three v_mfma_f32_32x32x16_bf16 products per K step
+// (lo(c) hi(x), hi(c) lo(x), hi(c) hi(x)) plus ||c||^2 give, for every
+// centroid, a_c ~ s_c - ||x||^2, where s_c is the f64 squared distance that
+// KMeans::predict computes (kmeans.rs:338-346: x widened to f64, differences
+// and squares in f64, summed in dimension order).  With
+//   c1 = 3.01 * 2^-16 + (3 ld + 2) * 2.04 * 2^-24
+// (dropped product and representation errors of both operands, FP32
+// accumulation over 3 ld products at two ulps each) and
+//   E = 2 c1 ||x|| max||c|| + 2^-20 (||x||^2 + max||c||^2)
+// (the f32 norms, the final add and the f64 rounding of s_c), every c has
+// |a_c - (s_c - ||x||^2)| <= E.  A row whose minimum a is the only value
+// <= min a + 2E is certified: every other centroid has s_c strictly above
+// the winner's, so the winner is the exact argmin.  Any other centroid with
+// a_c <= min a + 2E is a candidate, and no non-candidate can be the argmin
+// or tie with it; an uncertified row is assigned by computing s_c exactly
+// for its candidates only and taking the lowest (value, index) -- the rule
+// of the FP64 assign kernel (strict < in index order).  Rows that are not
+// finite or whose norm is too large for the bound compute all k distances.
+// So the labels equal the FP64 form's on every input.
+//
+// Sums.  After the labels, wave w adds the chunk's rows of its clusters
+// (c % 4 == w) into f64 registers (lane = dimension) in row order.  That is
+// the order of the FP64 form's partial kernel, so both forms write the same
+// block partials bit for bit, and the shared reduce kernels give the same
+// sums, centroids and distortions.
+//
+// Batched form.  Many independent problems (uneven n, d, k) run per Lloyd
+// iteration as one persistent launch per shape class over (problem, block)
+// items, then the shared reduce kernels over every problem; the iterations
+// are captured in one hipGraph.  The grid size is picked by timing the first
+// iterations.  Every problem gets bit-identical results to sc_kmeans_run on
+// it alone.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <cfloat>
 #include <cmath>
+#include <map>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -45,24 +61,42 @@ typedef float f32x16 __attribute__((ext_vector_type(16)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
-constexpr int NT = 256;
-constexpr int kMaxCB = 4;  // centroid blocks of 32 -> k <= 128
-constexpr int kMaxKS = 8;  // K steps of 16 -> dimension <= 128
-constexpr double kC1 = 0.0078125 + 1.52587890625e-05 + 9.5367431640625e-07 + 128.0 * 2.384185791015625e-07;
-constexpr double kKappa = 9.5367431640625e-07;  // 2^-20
+constexpr int NT = 256;  // threads of the helper kernels
 constexpr uint32_t kUncert = 0xffffffffu;
+constexpr double kKappa = 9.5367431640625e-07;   // 2^-20
+constexpr size_t kLdsBudget = 150 * 1024;         // dynamic LDS one workgroup may take
 
-__device__ __forceinline__ uint32_t bf16_bits_rne(float f) {
+// Shape class of the step kernel: k <= 32 KBM, padded dimension <= 16 KSM.
+template <int KBM, int KSM>
+struct Cfg {
+  static constexpr int LD = 16 * KSM;                    // largest padded dimension
+  static constexpr int LDP = LD + 4;                     // stage row stride (floats)
+  static constexpr int KP = 32 * KBM;                    // largest padded k
+  static constexpr int WAVES = KSM == 8 ? 4 : 8;         // waves per workgroup
+  static constexpr int NTH = 64 * WAVES;                 // threads per workgroup
+  static constexpr int CR = 32 * WAVES;                  // rows per chunk
+  static constexpr int SLOTS = (KP + WAVES - 1) / WAVES; // clusters one wave owns
+  static constexpr int DPL = (LD + 63) / 64;             // dimensions per lane
+  static constexpr int PFV = CR * LD / 4 / NTH;          // f32x4 loads per thread per chunk
+  // LDS: stage | lab | cnt | c2 | A table (hi | lo) | [f64 centroids]
+  static constexpr size_t STAGE = sizeof(float) * CR * LDP;
+  static constexpr size_t ATAB = sizeof(uint16_t) * KP * LD;
+  static constexpr size_t FIXED = STAGE + sizeof(uint32_t) * CR + sizeof(uint32_t) * KP +
+                                  sizeof(float) * KP + 2 * ATAB;
+  static_assert(kBlockRows % CR == 0, "a block is a whole number of chunks");
+  static_assert(CR % 64 == 0, "chunks are whole ballots");
+};
+
+__device__ __forceinline__ uint32_t bf16_bits(float f) {
   uint32_t u = __float_as_uint(f);
   u += 0x7fffu + ((u >> 16) & 1u);
   return u >> 16;
 }
 
-__device__ __forceinline__ void lexmin_f(float& v, uint32_t& i, float v2, uint32_t i2) {
-  if (v2 < v || (v2 == v && i2 < i)) {
-    v = v2;
-    i = i2;
-  }
+__device__ __forceinline__ bf16x8 as_frag(u32x4 v) { return __builtin_bit_cast(bf16x8, v); }
+
+__device__ __forceinline__ bf16x8 load_frag(const uint16_t* p) {
+  return as_frag(*reinterpret_cast<const u32x4*>(p));
 }
 
 __device__ __forceinline__ void lexmin_d(double& v, uint32_t& i, double v2, uint32_t i2) {
@@ -72,352 +106,483 @@ __device__ __forceinline__ void lexmin_d(double& v, uint32_t& i, double v2, uint
   }
 }
 
-// Centroid terms for one iteration: BF16 copy (kpad x ld), c2 = RN(||c||^2)
-// (+inf for padding rows), and two scalars: RU(max ||c||), RU(max ||c||^2).
-__device__ __forceinline__ void km_prep_centroids_body(const double* __restrict__ cent, int k,
-                                                       int kpad, int ld,
-                                                       uint16_t* __restrict__ cb16,
-                                                       float* __restrict__ c2,
-                                                       float* __restrict__ cmax, double* red) {
-  const int tid = threadIdx.x;
-  double mx = 0.0;
-  for (int c = tid; c < kpad; c += NT) {
-    double s = 0.0;
-    for (int j = 0; j < ld; ++j) {
-      const double v = c < k ? cent[static_cast<int64_t>(c) * ld + j] : 0.0;
-      s += v * v;
-      cb16[static_cast<int64_t>(c) * ld + j] =
-          static_cast<uint16_t>(bf16_bits_rne(static_cast<float>(v)));
-    }
-    c2[c] = c < k ? static_cast<float>(s) : INFINITY;
-    if (c < k && s < 1e300) mx = fmax(mx, s);  // NaN centroids never count
-  }
-  red[tid] = mx;
-  __syncthreads();
-  for (int w = NT / 2; w >= 1; w >>= 1) {
-    if (tid < w) red[tid] = fmax(red[tid], red[tid + w]);
-    __syncthreads();
-  }
-  if (tid == 0) {
-    cmax[0] = __double2float_ru(sqrt(red[0]) * (1.0 + 9.5367431640625e-07));
-    cmax[1] = __double2float_ru(red[0] * (1.0 + 9.5367431640625e-07));
-  }
-  __syncthreads();
+// s_c of KMeans::predict for one row (f32, stride 1) and one centroid (f64):
+// the reference's sequential fold.
+__device__ __forceinline__ double exact_s(const float* x, const double* c, int ld) {
+  double s = 0.0;
+  for (int j = 0; j < ld; ++j) s = s + Term<double, kEuclid>::f(static_cast<double>(x[j]), c[j], 2.0);
+  return s;
 }
 
-__global__ __launch_bounds__(NT) void km_prep_centroids_kernel(const double* __restrict__ cent,
-                                                               int k, int kpad, int ld,
-                                                               uint16_t* __restrict__ cb16,
-                                                               float* __restrict__ c2,
-                                                               float* __restrict__ cmax,
-                                                               const KmState* __restrict__ st) {
-  if (st->done) return;
-  __shared__ double red[NT];
-  km_prep_centroids_body(cent, k, kpad, ld, cb16, c2, cmax, red);
+// Centroid index of bit `sl` (16 cb + v) of a lane in row half `half`.
+__device__ __forceinline__ uint32_t slot_centroid(int sl, int half) {
+  return static_cast<uint32_t>(32 * (sl >> 4) + 8 * ((sl & 15) >> 2) + 4 * half + (sl & 3));
 }
 
-__device__ __forceinline__ bf16x8 load_frag(const uint16_t* p) {
-  const u32x4 v = *reinterpret_cast<const u32x4*>(p);
-  return __builtin_bit_cast(bf16x8, v);
-}
-
-// Dynamic LDS: acc[k*d] f64 | stage[kChunkRows*ld] f32 | lab[kChunkRows] | cnt[k].
-// One block b (kBlockRows rows) of one problem: certified labels and the
-// block's per-cluster partial sums.  Ends with a workgroup barrier so LDS
-// can be reused for the next block.
-__device__ __forceinline__ void km_block_step(
-    int64_t b, unsigned char* smem, const float* __restrict__ rows, int64_t n, int ld, int d,
-    const double* __restrict__ cent, const uint16_t* __restrict__ cb16,
-    const float* __restrict__ c2g, const float* __restrict__ cmaxg, int k,
-    uint32_t* __restrict__ labels, double* __restrict__ psum, uint32_t* __restrict__ pcount,
-    KmState* __restrict__ st) {
-  double* acc = reinterpret_cast<double*>(smem);
-  // stage starts on a 16-byte boundary (k * d rounded up to even)
-  float* stage = reinterpret_cast<float*>(acc + (static_cast<size_t>(k) * d + 1) / 2 * 2);
-  uint32_t* lab = reinterpret_cast<uint32_t*>(stage + static_cast<size_t>(kChunkRows) * ld);
-  uint32_t* cnt = lab + kChunkRows;
+// One block b of problem p: labels of its rows and its partial sums.
+// EXACT: the problem has exactly KBM centroid blocks and KSM K steps.
+template <int KBM, int KSM, bool EXACT>
+__device__ __forceinline__ void km_block(const KmProb& p, int64_t b, unsigned char* smem,
+                                         bool ctab_lds, unsigned long long& fallback) {
+  using C = Cfg<KBM, KSM>;
+  constexpr int NTH = C::NTH;
+  constexpr int CR = C::CR;
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = tid >> 6;
-  const int col = lane & 31;   // point inside the wave's 32 rows
+  const int col = lane & 31;
   const int half = lane >> 5;
-  const int ks_n = ld / 16;
-  const int kb = (k + 31) / 32;
-  const int64_t r0 = b * kBlockRows;
-  const int rows_here = static_cast<int>(min<int64_t>(kBlockRows, n - r0));
+  const int k = p.k;
+  const int ld = p.ld;
+  const int d = p.d;
+  const int kb = EXACT ? KBM : (k + 31) / 32;
+  const int ks_n = EXACT ? KSM : ld / 16;
+  const int ldq = ld / 4;
 
-  for (int e = tid; e < k * d; e += NT) acc[e] = 0.0;
-  for (int e = tid; e < k; e += NT) cnt[e] = 0u;
+  float* stage = reinterpret_cast<float*>(smem);
+  uint32_t* lab = reinterpret_cast<uint32_t*>(smem + C::STAGE);
+  uint32_t* cnt = lab + CR;
+  float* c2s = reinterpret_cast<float*>(cnt + C::KP);
+  uint16_t* ahi = reinterpret_cast<uint16_t*>(c2s + C::KP);  // KP x LD
+  uint16_t* alo = ahi + C::KP * C::LD;
+  double* ctab_s = reinterpret_cast<double*>(smem + C::FIXED);
+  const double* ctab = ctab_lds ? ctab_s : p.cent;
 
-  // A fragments: centroid 32 cb + col, dims 16 ks + 8 half .. +7
-  bf16x8 ca[kMaxCB][kMaxKS];
-  float c2v[kMaxCB][16];
-#pragma unroll
-  for (int cb = 0; cb < kMaxCB; ++cb) {
-#pragma unroll
-    for (int ks = 0; ks < kMaxKS; ++ks)
-      ca[cb][ks] = (cb < kb && ks < ks_n)
-                       ? load_frag(cb16 + static_cast<int64_t>(32 * cb + col) * ld + 16 * ks + 8 * half)
-                       : bf16x8{};
-    // output rows of this lane: centroid 32 cb + 8 (v / 4) + 4 half + v % 4
-#pragma unroll
-    for (int v = 0; v < 16; ++v)
-      c2v[cb][v] = cb < kb ? c2g[32 * cb + 8 * (v >> 2) + 4 * half + (v & 3)] : INFINITY;
+  // ---- per-block tables ----
+  for (int e = tid; e < C::KP; e += NTH) {
+    cnt[e] = 0u;
+    c2s[e] = e < p.kpad ? p.c2f[e] : INFINITY;
   }
-  const float cmax = cmaxg[0];
-  const float c2max = cmaxg[1];
+  for (int e = tid; e < C::KP * C::LD / 8; e += NTH) {
+    const int r = e / (C::LD / 8), j8 = e % (C::LD / 8);
+    const bool in = r < p.kpad && 8 * j8 < ld;
+    const int64_t src = static_cast<int64_t>(r) * ld + 8 * j8;
+    const u32x4 z = {0u, 0u, 0u, 0u};
+    *reinterpret_cast<u32x4*>(ahi + 8 * e) = in ? *reinterpret_cast<const u32x4*>(p.bhi + src) : z;
+    *reinterpret_cast<u32x4*>(alo + 8 * e) = in ? *reinterpret_cast<const u32x4*>(p.blo + src) : z;
+  }
+  if (ctab_lds)
+    for (int64_t e = tid; e < static_cast<int64_t>(k) * ld; e += NTH) ctab_s[e] = p.cent[e];
+  auto frag = [&](const uint16_t* tab, int cb, int ks) -> bf16x8 {
+    return load_frag(tab + (32 * cb + col) * C::LD + 16 * ks + 8 * half);
+  };
 
-  for (int c0 = 0; c0 < rows_here; c0 += kChunkRows) {
-    const int nr = min(kChunkRows, rows_here - c0);
-    __syncthreads();
-    {
-      const float* src = rows + (r0 + c0) * ld;
-      const int nvec = nr * ld / 4;
-      const f32x4* s4 = reinterpret_cast<const f32x4*>(src);
-      f32x4* d4 = reinterpret_cast<f32x4*>(stage);
-      for (int e = tid; e < nvec; e += NT) d4[e] = s4[e];
+  const float cmax = p.cmax[0];
+  const float c2max = p.cmax[1];
+  // cross-term error constant: representation and dropped-product errors
+  // (3 * 2^-16), FP32 accumulation at two ulps per add of the hi x hi phase
+  // (ld adds at |x||c|) and of the lo phases (2 ld adds at 2^-7 |x||c|)
+  const double c1 = 3.01 * 1.52587890625e-05 +
+                    (ld + 1.0 + 2.0 * ld * 0.0078125) * 2.04 * 5.9604644775390625e-08;
+  // tree vs sequential f64 fold of the same ld non-negative terms
+  const double dr = 2.0004 * (ld + 8.0) * 1.1102230246251565e-16;
+
+  double sacc[C::SLOTS][C::DPL];
+#pragma unroll
+  for (int s = 0; s < C::SLOTS; ++s)
+#pragma unroll
+    for (int dd = 0; dd < C::DPL; ++dd) sacc[s][dd] = 0.0;
+
+  const float* rows = static_cast<const float*>(p.rows);
+  const int64_t r0 = b * kBlockRows;
+  const int rows_here = static_cast<int>(min<int64_t>(kBlockRows, p.n - r0));
+  f32x4 pf[C::PFV];
+  auto load_chunk = [&](int c0) {
+    const int nr = min(CR, rows_here - c0);
+    const f32x4* src = reinterpret_cast<const f32x4*>(rows + (r0 + c0) * ld);
+    const int nvec = nr * ldq;
+#pragma unroll
+    for (int i = 0; i < C::PFV; ++i) {
+      const int e = tid + NTH * i;
+      pf[i] = e < nvec ? src[e] : f32x4{0.0f, 0.0f, 0.0f, 0.0f};
     }
-    __syncthreads();
+  };
+  load_chunk(0);
+  unsigned long long fb = 0;
 
-    // ---- certified assignment of this wave's 32 rows ----
-    const int pr = 32 * wave + col;  // row inside the chunk
+  for (int c0 = 0; c0 < rows_here; c0 += CR) {
+    const int nr = min(CR, rows_here - c0);
+    __syncthreads();  // the previous chunk is consumed; tables are written
+#pragma unroll
+    for (int i = 0; i < C::PFV; ++i) {
+      const int e = tid + NTH * i;
+      if (e < nr * ldq) {
+        const int r = e / ldq, q4 = e - r * ldq;
+        *reinterpret_cast<f32x4*>(stage + r * C::LDP + 4 * q4) = pf[i];
+      }
+    }
+    if (tid < CR && tid >= nr) lab[tid] = kUncert;
+    __syncthreads();
+    if (c0 + CR < rows_here) load_chunk(c0 + CR);
+
+    // ---- labels of this wave's 32 rows ----
+    const int pr = 32 * wave + col;
     const bool pvalid = pr < nr;
-    const float* xr = stage + static_cast<size_t>(pvalid ? pr : 0) * ld;
-    bf16x8 xb[kMaxKS];
+    const float* xr = stage + (pvalid ? pr : 0) * C::LDP;
+    bf16x8 xh[KSM], xl[KSM];
     float x2p = 0.0f;
-    bool finite = true;
+    bool fin = true;
 #pragma unroll
-    for (int ks = 0; ks < kMaxKS; ++ks) {
-      if (ks < ks_n) {
-        const f32x4 lo = *reinterpret_cast<const f32x4*>(xr + 16 * ks + 8 * half);
-        const f32x4 hi = *reinterpret_cast<const f32x4*>(xr + 16 * ks + 8 * half + 4);
-        u32x4 pk;
-        pk[0] = bf16_bits_rne(lo[0]) | (bf16_bits_rne(lo[1]) << 16);
-        pk[1] = bf16_bits_rne(lo[2]) | (bf16_bits_rne(lo[3]) << 16);
-        pk[2] = bf16_bits_rne(hi[0]) | (bf16_bits_rne(hi[1]) << 16);
-        pk[3] = bf16_bits_rne(hi[2]) | (bf16_bits_rne(hi[3]) << 16);
-        xb[ks] = __builtin_bit_cast(bf16x8, pk);
+    for (int ks = 0; ks < KSM; ++ks) {
+      if (EXACT || ks < ks_n) {
+        const f32x4 v0 = *reinterpret_cast<const f32x4*>(xr + 16 * ks + 8 * half);
+        const f32x4 v1 = *reinterpret_cast<const f32x4*>(xr + 16 * ks + 8 * half + 4);
+        const float v[8] = {v0[0], v0[1], v0[2], v0[3], v1[0], v1[1], v1[2], v1[3]};
+        uint32_t h[8], l[8];
 #pragma unroll
-        for (int t = 0; t < 4; ++t) {
-          x2p = fmaf(lo[t], lo[t], x2p);
-          x2p = fmaf(hi[t], hi[t], x2p);
-          finite = finite && isfinite(lo[t]) && isfinite(hi[t]);
+        for (int t = 0; t < 8; ++t) {
+          h[t] = bf16_bits(v[t]);
+          l[t] = bf16_bits(v[t] - __uint_as_float(h[t] << 16));
+          x2p = fmaf(v[t], v[t], x2p);
+          fin = fin && isfinite(v[t]);
         }
+        xh[ks] = as_frag(u32x4{h[0] | (h[1] << 16), h[2] | (h[3] << 16), h[4] | (h[5] << 16),
+                               h[6] | (h[7] << 16)});
+        xl[ks] = as_frag(u32x4{l[0] | (l[1] << 16), l[2] | (l[3] << 16), l[4] | (l[5] << 16),
+                               l[6] | (l[7] << 16)});
       } else {
-        xb[ks] = bf16x8{};
+        xh[ks] = bf16x8{};
+        xl[ks] = bf16x8{};
       }
     }
     const float x2 = x2p + __shfl_xor(x2p, 32);
-    finite = finite && (__shfl_xor(finite ? 1 : 0, 32) != 0);
+    fin = fin && (__shfl_xor(fin ? 1 : 0, 32) != 0);
 
-    f32x16 cacc[kMaxCB];
+    // lo(c) x hi(x) and hi(c) x lo(x) first, hi(c) x hi(x) last
+    f32x16 acc[KBM];
 #pragma unroll
-    for (int cb = 0; cb < kMaxCB; ++cb) cacc[cb] = f32x16{};
+    for (int cb = 0; cb < KBM; ++cb) acc[cb] = f32x16{};
 #pragma unroll
-    for (int ks = 0; ks < kMaxKS; ++ks) {
-      if (ks < ks_n) {
+    for (int ks = 0; ks < KSM; ++ks)
 #pragma unroll
-        for (int cb = 0; cb < kMaxCB; ++cb)
-          if (cb < kb) cacc[cb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ca[cb][ks], xb[ks], cacc[cb], 0, 0, 0);
-      }
-    }
+      for (int cb = 0; cb < KBM; ++cb)
+        if (EXACT || (ks < ks_n && cb < kb)) {
+          acc[cb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(frag(alo, cb, ks), xh[ks], acc[cb], 0, 0, 0);
+          acc[cb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(frag(ahi, cb, ks), xl[ks], acc[cb], 0, 0, 0);
+        }
+#pragma unroll
+    for (int ks = 0; ks < KSM; ++ks)
+#pragma unroll
+      for (int cb = 0; cb < KBM; ++cb)
+        if (EXACT || (ks < ks_n && cb < kb))
+          acc[cb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(frag(ahi, cb, ks), xh[ks], acc[cb], 0, 0, 0);
+
+    // a_c = ||c||^2 + acc; lane (col, half) holds centroids
+    // 32 cb + 8 g + 4 half + r for v = 4 g + r, in increasing index order
     float amin = INFINITY;
-    uint32_t aidx = kUncert;
+    int aslot = 0;  // 16 cb + v of the lane's minimum
 #pragma unroll
-    for (int cb = 0; cb < kMaxCB; ++cb) {
-      if (cb < kb) {
+    for (int cb = 0; cb < KBM; ++cb) {
+      if (EXACT || cb < kb) {
 #pragma unroll
-        for (int v = 0; v < 16; ++v) {
-          const float a = fmaf(-2.0f, cacc[cb][v], c2v[cb][v]);
-          cacc[cb][v] = a;
-          lexmin_f(amin, aidx, a, static_cast<uint32_t>(32 * cb + 8 * (v >> 2) + 4 * half + (v & 3)));
+        for (int g = 0; g < 4; ++g) {
+          const f32x4 c2q = *reinterpret_cast<const f32x4*>(c2s + 32 * cb + 8 * g + 4 * half);
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const float a = acc[cb][4 * g + r] + c2q[r];
+            acc[cb][4 * g + r] = a;
+            if (a < amin) {
+              amin = a;
+              aslot = 16 * cb + 4 * g + r;
+            }
+          }
         }
       }
     }
+    uint32_t aidx = amin < INFINITY ? slot_centroid(aslot, half) : kUncert;
     {
       const float v2 = __shfl_xor(amin, 32);
       const uint32_t i2 = __shfl_xor(aidx, 32);
-      lexmin_f(amin, aidx, v2, i2);
+      if (v2 < amin || (v2 == amin && i2 < aidx)) {
+        amin = v2;
+        aidx = i2;
+      }
     }
-    // E >= |(s_c - ||x||^2) - a_c| for every c
-    const double x2u = static_cast<double>(x2) * (1.0 + 3.814697265625e-06) + 1e-300;
-    const double E = 2.0 * kC1 * sqrt(x2u) * static_cast<double>(cmax) * (1.0 + 2.384185791015625e-07) +
+    const double x2u = static_cast<double>(x2) * (1.0 + 6.103515625e-05) + 1e-300;
+    const double E = 2.0 * c1 * sqrt(x2u) * static_cast<double>(cmax) * (1.0 + 9.5367431640625e-07) +
                      kKappa * (x2u + static_cast<double>(c2max));
     const float thr = __double2float_ru(static_cast<double>(amin) + 2.0 * E * (1.0 + 9.765625e-04));
-    int ncand = 0;
+    uint64_t cm = 0;  // this lane's candidates: bit 16 cb + v
 #pragma unroll
-    for (int cb = 0; cb < kMaxCB; ++cb) {
-      if (cb < kb) {
+    for (int cb = 0; cb < KBM; ++cb)
+      if (EXACT || cb < kb) {
+        uint32_t m16 = 0;
 #pragma unroll
-        for (int v = 0; v < 16; ++v) ncand += cacc[cb][v] <= thr ? 1 : 0;
+        for (int v = 0; v < 16; ++v) m16 |= static_cast<uint32_t>(acc[cb][v] <= thr) << v;
+        cm |= static_cast<uint64_t>(m16) << (16 * cb);
+      }
+    const int ncand = __popcll(cm) + __shfl_xor(__popcll(cm), 32);
+    // a finite threshold keeps padding centroids (a = +inf) out of the
+    // candidates; anything else takes the all-centroid path below
+    const bool sane = fin && x2 < 1e30f && amin < INFINITY && thr < INFINITY;
+    const bool certified = sane && ncand == 1;
+    const bool cand = pvalid && sane && !certified;
+    if (!cand) cm = 0;
+
+    // Level 2: s of each candidate with lanes = dimensions and a fixed tree
+    // over the lanes; the lane owning the candidate keeps its best and
+    // second-best value.  The tree differs from the reference's sequential
+    // fold of the same terms by at most (ld + 6) ulps of the sum, so a best
+    // value below the runner-up by more than twice that is the exact argmin.
+    double b1 = INFINITY, b2 = INFINITY;
+    uint32_t bc = kUncert;
+    {
+      uint64_t work = cm;
+      for (;;) {
+        const uint64_t act = __ballot(work != 0);
+        if (act == 0) break;
+        const int L = __builtin_ctzll(act);
+        const uint32_t wlo = __builtin_amdgcn_readlane(static_cast<uint32_t>(work), L);
+        const uint32_t whi = __builtin_amdgcn_readlane(static_cast<uint32_t>(work >> 32), L);
+        const int sl = __builtin_ctzll((static_cast<uint64_t>(whi) << 32) | wlo);
+        if (lane == L) work &= work - 1;
+        const uint32_t c = slot_centroid(sl, L >> 5);
+        if (c >= static_cast<uint32_t>(k)) continue;  // never: padding has a = +inf
+        const float* xs = stage + (32 * wave + (L & 31)) * C::LDP;
+        const double* cc = ctab + static_cast<int64_t>(c) * ld;
+        double t = 0.0;
+#pragma unroll
+        for (int dd = 0; dd < C::DPL; ++dd) {
+          const int j = lane + 64 * dd;
+          if (j < ld) t = t + Term<double, kEuclid>::f(static_cast<double>(xs[j]), cc[j], 2.0);
+        }
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) t = t + __shfl_xor(t, off);
+        if (lane == L) {
+          if (t < b1 || (t == b1 && c < bc)) {
+            b2 = b1;
+            b1 = t;
+            bc = c;
+          } else if (t < b2) {
+            b2 = t;
+          }
+        }
       }
     }
-    ncand += __shfl_xor(ncand, 32);
-    const bool certified = finite && ncand == 1 && amin < INFINITY && x2 < 1e30f;
-    if (half == 0 && pvalid) lab[pr] = certified ? aidx : kUncert;
-    // ---- FP64 assignment of uncertified rows: lanes = centroids ----
-    uint64_t um = __ballot(half == 0 && pvalid && !certified);
-    while (um) {
-      const int p = __builtin_ctzll(um);
-      um &= um - 1;
-      const float* x = stage + static_cast<size_t>(32 * wave + p) * ld;
-      double bv = INFINITY;
-      uint32_t bi = kUncert;
-      for (int c = lane; c < k; c += 64) {
-        const double* cc = cent + static_cast<int64_t>(c) * ld;
-        double s = 0.0;
-        for (int j = 0; j < ld; ++j) s = s + Term<double, kEuclid>::f(static_cast<double>(x[j]), cc[j], 2.0);
-        lexmin_d(bv, bi, s, static_cast<uint32_t>(c));
+    bool ok2;
+    {
+      const double pb1 = __shfl_xor(b1, 32);
+      const double pb2 = __shfl_xor(b2, 32);
+      const uint32_t pc = __shfl_xor(bc, 32);
+      double second;
+      if (pb1 < b1 || (pb1 == b1 && pc < bc)) {
+        second = fmin(b1, pb2);
+        b1 = pb1;
+        bc = pc;
+      } else {
+        second = fmin(pb1, b2);
       }
+      ok2 = b1 < INFINITY && second * (1.0 - dr) > b1;
+    }
+    // Level 3 (near-exact ties): the reference fold for each candidate
+    uint64_t cm3 = ok2 ? 0 : cm;
+    double bs = INFINITY;
+    uint32_t bi = kUncert;
+    while (__ballot(cm3 != 0)) {
+      if (cm3 != 0) {
+        const int sl = __builtin_ctzll(cm3);
+        cm3 &= cm3 - 1;
+        const uint32_t c = slot_centroid(sl, half);
+        if (c < static_cast<uint32_t>(k))
+          lexmin_d(bs, bi, exact_s(xr, ctab + static_cast<int64_t>(c) * ld, ld), c);
+      }
+    }
+    {
+      const double v2 = __shfl_xor(bs, 32);
+      const uint32_t i2 = __shfl_xor(bi, 32);
+      lexmin_d(bs, bi, v2, i2);
+    }
+    if (half == 0 && pvalid) {
+      uint32_t lb = kUncert;
+      if (certified)
+        lb = aidx;
+      else if (cand)
+        lb = ok2 ? bc : (bs < DBL_MAX ? bi : 0u);
+      lab[pr] = lb;
+    }
+    // rows outside the bound's domain: all k centroids, lanes = centroids
+    uint64_t um = __ballot(half == 0 && pvalid && !sane);
+    fb += static_cast<unsigned long long>(__popcll(__ballot(half == 0 && pvalid && !certified)));
+    while (um) {
+      const int q = __builtin_ctzll(um);
+      um &= um - 1;
+      const float* x = stage + (32 * wave + q) * C::LDP;
+      double bv = INFINITY;
+      uint32_t bj = kUncert;
+      for (int c = lane; c < k; c += 64)
+        lexmin_d(bv, bj, exact_s(x, ctab + static_cast<int64_t>(c) * ld, ld), static_cast<uint32_t>(c));
 #pragma unroll
       for (int off = 32; off >= 1; off >>= 1) {
         const double v2 = __shfl_xor(bv, off);
-        const uint32_t i2 = __shfl_xor(bi, off);
-        lexmin_d(bv, bi, v2, i2);
+        const uint32_t i2 = __shfl_xor(bj, off);
+        lexmin_d(bv, bj, v2, i2);
       }
-      if (lane == 0) {
-        // kmeans.rs:334-347: min starts at f64::MAX; nothing below it keeps 0
-        lab[32 * wave + p] = bv < DBL_MAX ? bi : 0u;
-        atomicAdd(&st->fallback, 1ull);
-      }
+      // kmeans.rs:334-347: min starts at f64::MAX; nothing below it keeps 0
+      if (lane == 0) lab[32 * wave + q] = bv < DBL_MAX ? bj : 0u;
     }
-    if (half == 0 && pvalid) labels[r0 + c0 + pr] = lab[pr];
     __syncthreads();
+    if (tid < nr) p.labels[r0 + c0 + tid] = lab[tid];
 
-    // ---- per-cluster sums, owner wave per cluster, rows in order ----
-    for (int i0 = 0; i0 < nr; i0 += 64) {
-      const uint32_t li = lab[i0 + lane];
-      const bool mine = i0 + lane < nr && (li & 3u) == static_cast<uint32_t>(wave);
-      uint64_t m = __ballot(mine);
-      while (m) {
-        const int i = i0 + __builtin_ctzll(m);
-        m &= m - 1;
-        const uint32_t c = lab[i];
-        for (int dc = 0; dc < d; dc += 64) {
-          const int dim = dc + lane;
-          if (dim < d)
-            atomicAdd(&acc[static_cast<size_t>(c) * d + dim],
-                      static_cast<double>(stage[static_cast<size_t>(i) * ld + dim]));
+    // ---- sums: wave owns clusters WAVES s + wave, rows in order ----
+    uint32_t lrow[CR / 64];
+#pragma unroll
+    for (int g = 0; g < CR / 64; ++g) lrow[g] = lab[64 * g + lane];
+#pragma unroll
+    for (int s = 0; s < C::SLOTS; ++s) {
+      const uint32_t c = static_cast<uint32_t>(C::WAVES * s + wave);
+      if (c >= static_cast<uint32_t>(k)) break;
+#pragma unroll
+      for (int g = 0; g < CR / 64; ++g) {
+        uint64_t m = __ballot(lrow[g] == c);
+        if (lane == 0 && m) atomicAdd(&cnt[c], static_cast<uint32_t>(__popcll(m)));
+        while (m) {
+          const int i = 64 * g + __builtin_ctzll(m);
+          m &= m - 1;
+#pragma unroll
+          for (int dd = 0; dd < C::DPL; ++dd)
+            if (lane + 64 * dd < ld)
+              sacc[s][dd] = sacc[s][dd] + static_cast<double>(stage[i * C::LDP + lane + 64 * dd]);
         }
-        if (lane == 0) cnt[c] += 1u;
       }
     }
   }
   __syncthreads();
-  double* out = psum + b * static_cast<int64_t>(k) * d;
-  for (int e = tid; e < k * d; e += NT) out[e] = acc[e];
-  for (int e = tid; e < k; e += NT) pcount[b * k + e] = cnt[e];
-  __syncthreads();
+  // ---- block partials ----
+  const int64_t kd = static_cast<int64_t>(k) * d;
+#pragma unroll
+  for (int s = 0; s < C::SLOTS; ++s) {
+    const int c = C::WAVES * s + wave;
+    if (c >= k) break;
+#pragma unroll
+    for (int dd = 0; dd < C::DPL; ++dd) {
+      const int dim = lane + 64 * dd;
+      if (dim < d) p.psum[b * kd + static_cast<int64_t>(c) * d + dim] = sacc[s][dd];
+    }
+  }
+  for (int c = tid; c < k; c += NTH) p.pcount[b * k + c] = cnt[c];
+  fallback += fb;
+  __syncthreads();  // LDS is rewritten by the next block
 }
 
-__global__ __launch_bounds__(NT) void km_mixed_step_kernel(
-    const float* __restrict__ rows, int64_t n, int ld, int d, const double* __restrict__ cent,
-    const uint16_t* __restrict__ cb16, const float* __restrict__ c2g,
-    const float* __restrict__ cmaxg, int k, uint32_t* __restrict__ labels,
-    double* __restrict__ psum, uint32_t* __restrict__ pcount, KmState* __restrict__ st) {
-  if (st->done) return;
+// Persistent step kernel: items are (problem, block) pairs of one shape
+// class; with items == nullptr the launch covers every block of problem 0.
+template <int KBM, int KSM, bool EXACT>
+__global__ __launch_bounds__((Cfg<KBM, KSM>::NTH), (Cfg<KBM, KSM>::WAVES / 4)) void km_step_kernel(
+    const KmProb* __restrict__ P, const uint2* __restrict__ items, int64_t nitems, int ctab_lds) {
   extern __shared__ __align__(16) unsigned char smem[];
-  km_block_step(blockIdx.x, smem, rows, n, ld, d, cent, cb16, c2g, cmaxg, k, labels, psum, pcount,
-                st);
+  unsigned long long fb = 0;
+  int cur = -1;
+  KmState* st = nullptr;
+  for (int64_t it = blockIdx.x; it < nitems; it += gridDim.x) {
+    const int i = items ? static_cast<int>(items[it].x) : 0;
+    const int64_t b = items ? static_cast<int64_t>(items[it].y) : it;
+    const KmProb p = P[i];
+    // every workgroup of this launch sees the same value: the flag only
+    // changes in the final reduce kernel
+    if (*reinterpret_cast<volatile uint32_t*>(&p.st->done)) continue;
+    if (i != cur) {
+      if (fb && (threadIdx.x & 63) == 0) atomicAdd(&st->fallback, fb);
+      fb = 0;
+      cur = i;
+      st = p.st;
+    }
+    km_block<KBM, KSM, EXACT>(p, b, smem, ctab_lds != 0, fb);
+  }
+  // each wave adds its own count
+  if (st && fb && (threadIdx.x & 63) == 0) atomicAdd(&st->fallback, fb);
 }
 
-size_t mixed_lds_bytes(const sc_index* ix, int k) {
-  return sizeof(double) * ((static_cast<size_t>(k) * ix->d + 1) / 2 * 2) +
-         sizeof(float) * static_cast<size_t>(kChunkRows) * ix->ld +
-         sizeof(uint32_t) * (kChunkRows + static_cast<size_t>(k)) + 16;
+template <int KBM, int KSM>
+size_t lds_bytes(int k, int ld, bool* ctab_lds) {
+  const size_t fixed = Cfg<KBM, KSM>::FIXED;
+  const size_t ctab = sizeof(double) * static_cast<size_t>(k) * ld;
+  *ctab_lds = fixed + ctab <= kLdsBudget;
+  return fixed + (*ctab_lds ? ctab : 0);
+}
+
+// Shape classes: KBM in {1, 2, 4}, KSM in {1, 2, 4, 8}.
+int kbm_of(int k) { return k <= 32 ? 1 : (k <= 64 ? 2 : 4); }
+int ksm_of(int ld) { return ld <= 16 ? 1 : (ld <= 32 ? 2 : (ld <= 64 ? 4 : 8)); }
+
+template <int KBM, int KSM>
+int launch_class(const KmProb* dP, const uint2* items, int64_t nitems, unsigned grid, bool exact,
+                 int kmax, int ldmax, hipStream_t s) {
+  bool ctl = false;
+  const size_t lds = lds_bytes<KBM, KSM>(kmax, ldmax, &ctl);
+  const dim3 block(Cfg<KBM, KSM>::NTH);
+  // allow the largest dynamic LDS this class can ask for (once per kernel)
+  static const bool attr = [] {
+    const int mx = static_cast<int>(kLdsBudget);
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&km_step_kernel<KBM, KSM, true>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, mx);
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&km_step_kernel<KBM, KSM, false>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, mx);
+    return true;
+  }();
+  (void)attr;
+  if (exact)
+    hipLaunchKernelGGL((km_step_kernel<KBM, KSM, true>), dim3(grid), block, lds, s, dP, items,
+                       nitems, ctl ? 1 : 0);
+  else
+    hipLaunchKernelGGL((km_step_kernel<KBM, KSM, false>), dim3(grid), block, lds, s, dP, items,
+                       nitems, ctl ? 1 : 0);
+  SC_HIP(hipGetLastError());
+  return SC_OK;
+}
+
+int launch_step(int kbm, int ksm, const KmProb* dP, const uint2* items, int64_t nitems,
+                unsigned grid, bool exact, int kmax, int ldmax, hipStream_t s) {
+#define SC_KM_CASE(A, B)                                                                     \
+  if (kbm == A && ksm == B)                                                                  \
+    return launch_class<A, B>(dP, items, nitems, grid, exact, kmax, ldmax, s);
+  SC_KM_CASE(1, 1) SC_KM_CASE(1, 2) SC_KM_CASE(1, 4) SC_KM_CASE(1, 8)
+  SC_KM_CASE(2, 1) SC_KM_CASE(2, 2) SC_KM_CASE(2, 4) SC_KM_CASE(2, 8)
+  SC_KM_CASE(4, 1) SC_KM_CASE(4, 2) SC_KM_CASE(4, 4) SC_KM_CASE(4, 8)
+#undef SC_KM_CASE
+  return fail(SC_ERR_UNSUPPORTED, "k-means MFMA form: unsupported shape");
 }
 
 }  // namespace
 
 bool kmeans_mixed_supported(const sc_index* ix, int k) {
-  return ix->dtype == SC_F32 && k >= 1 && k <= 32 * kMaxCB && ix->ld <= 16 * kMaxKS &&
-         static_cast<int64_t>(k) * static_cast<int64_t>(ix->d) <= kMaxLdsAcc &&
-         mixed_lds_bytes(ix, k) <= 96 * 1024;
+  return ix->dtype == SC_F32 && k >= 1 && k <= 128 && ix->ld <= 128;
 }
 
-int kmeans_mixed_step_launch(const sc_index* ix, KmWork& w, int k, const double* d_cent,
-                             KmState* st, hipStream_t s) {
-  const int ld = static_cast<int>(ix->ld);
-  const int kpad = (k + 31) / 32 * 32;
-  // per-iteration centroid terms (stream-ordered scratch)
-  DevBuf terms;
-  SC_HIP(terms.alloc(sizeof(uint16_t) * kpad * ld + sizeof(float) * (kpad + 4), s));
-  uint16_t* cb16 = terms.as<uint16_t>();
-  float* c2 = reinterpret_cast<float*>(cb16 + static_cast<size_t>(kpad) * ld);
-  float* cmax = c2 + kpad;
-  hipLaunchKernelGGL(km_prep_centroids_kernel, dim3(1), dim3(NT), 0, s, d_cent, k, kpad, ld, cb16,
-                     c2, cmax, st);
-  SC_HIP(hipGetLastError());
-  if (w.nblocks == 0) return SC_OK;
+int kmeans_mixed_step_launch(const sc_index* ix, KmWork& w, hipStream_t s) {
+  if (w.p.nb == 0) return SC_OK;
+  const int k = w.p.k, ld = w.p.ld;
+  const int kbm = kbm_of(k), ksm = ksm_of(ld);
+  const bool exact = (k + 31) / 32 == kbm && ld / 16 == ksm;
   ProfScope ps("kmeans_mixed_assign", s);
-  hipLaunchKernelGGL(km_mixed_step_kernel, dim3(static_cast<unsigned>(w.nblocks)), dim3(NT),
-                     mixed_lds_bytes(ix, k), s, static_cast<const float*>(ix->rows),
-                     static_cast<int64_t>(ix->n), ld, static_cast<int>(ix->d), d_cent, cb16, c2,
-                     cmax, k, w.labels, w.psum, w.pcount, st);
-  SC_HIP(hipGetLastError());
-  return SC_OK;
+  return launch_step(kbm, ksm, w.dprob, nullptr, w.p.nb, static_cast<unsigned>(w.p.nb), exact, k,
+                     ld, s);
 }
 
 // ------------------------------------------------------------------ batch
-// Many independent problems (uneven n, d, k) in one persistent kernel per
-// Lloyd iteration.  Work items are (problem, block) pairs; a fixed grid of
-// workgroups strides over them.  The workgroup that finishes the last block
-// of a problem (a per-problem arrival counter) reduces that problem's block
-// partials in the same fixed order as the single-problem path, applies the
-// distortion, centroid update and stop rule, and prepares the next
-// iteration's BF16 centroids.  Every problem therefore gets bit-identical
-// results to sc_kmeans_run(SC_MIXED) on it alone.  The iterations are
-// captured in one hipGraph; the grid size is picked by timing the first
-// iterations (any choice gives the same results).
 namespace {
 
-struct KmbProblem {
-  int64_t n;
-  int d, ld, k, kpad;
-  int64_t nb, ng;
-  int64_t row_off, cent_off, cb_off, c2_off, lab_off, ps_off, pc_off, gs_off, sum_off, cnt_off,
-      x2_off;
-};
-
-struct KmbArena {
-  float* rows;
-  double* cent;
-  uint16_t* cb;
-  float* c2;  // per problem: c2[kpad] | max ||c|| | max ||c||^2
-  uint32_t* lab;
-  double* ps;
-  uint32_t* pc;
-  double* gs;
-  double* sums;
-  uint64_t* cnt;
-  double* x2;  // per problem: x2 per block
-  KmState* st;
-  uint32_t* arrivals;
-};
-
-__global__ void kmb_init_kernel(KmbArena a, int count) {
+__global__ void kmb_init_kernel(const KmProb* __restrict__ P, int count) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= count) return;
-  KmState* st = a.st + i;
+  KmState* st = P[i].st;
   st->distortion = DBL_MAX;
   st->dist = 0.0;
   st->x2total = 0.0;
   st->iters = 0;
   st->done = 0;
   st->fallback = 0;
-  a.arrivals[i] = 0u;
+  *P[i].arrive = 0u;
 }
 
 // ||x||^2 per (problem, block): same order as the single-problem x2 kernel.
-__global__ __launch_bounds__(NT) void kmb_x2_kernel(const KmbProblem* __restrict__ P,
-                                                    const uint2* __restrict__ items, int64_t nitems,
-                                                    KmbArena a) {
+__global__ __launch_bounds__(NT) void kmb_x2_kernel(const KmProb* __restrict__ P,
+                                                    const uint2* __restrict__ items, int64_t nitems) {
   __shared__ double red[NT];
   const int tid = threadIdx.x;
   for (int64_t it = blockIdx.x; it < nitems; it += gridDim.x) {
-    const KmbProblem p = P[items[it].x];
+    const KmProb p = P[items[it].x];
     const int64_t b = items[it].y;
-    const float* rows = a.rows + p.row_off;
+    const float* rows = static_cast<const float*>(p.rows);
     const int64_t r0 = b * kBlockRows;
     double s = 0.0;
     for (int j = 0; j < kBlockRows / NT; ++j) {
@@ -437,143 +602,31 @@ __global__ __launch_bounds__(NT) void kmb_x2_kernel(const KmbProblem* __restrict
       if (tid < w) red[tid] = red[tid] + red[tid + w];
       __syncthreads();
     }
-    if (tid == 0) a.x2[p.x2_off + b] = red[0];
+    if (tid == 0) p.x2block[b] = red[0];
     __syncthreads();
   }
 }
 
-__global__ void kmb_x2_total_kernel(const KmbProblem* __restrict__ P, int count, KmbArena a) {
+__global__ void kmb_x2_total_kernel(const KmProb* __restrict__ P, int count) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= count) return;
-  const KmbProblem p = P[i];
+  const KmProb p = P[i];
   double s = 0.0;
-  for (int64_t b = 0; b < p.nb; ++b) s = s + a.x2[p.x2_off + b];
-  a.st[i].x2total = s;
+  for (int64_t b = 0; b < p.nb; ++b) s = s + p.x2block[b];
+  p.st->x2total = s;
 }
 
-__global__ __launch_bounds__(NT) void kmb_prep_kernel(const KmbProblem* __restrict__ P, KmbArena a) {
-  __shared__ double red[NT];
-  const int i = blockIdx.x;
-  if (a.st[i].done) return;
-  const KmbProblem p = P[i];
-  km_prep_centroids_body(a.cent + p.cent_off, p.k, p.kpad, p.ld, a.cb + p.cb_off, a.c2 + p.c2_off,
-                         a.c2 + p.c2_off + p.kpad, red);
-}
-
-// Reduction, distortion, update, stop rule and next-iteration centroid terms
-// of problem i, by one workgroup, in the single-problem path's order.
-__device__ void kmb_finish(const KmbProblem& p, int i, const KmbArena& a, double* red,
-                           double* term) {
-  const int tid = threadIdx.x;
-  const int k = p.k, d = p.d;
-  const int64_t kd = static_cast<int64_t>(k) * d;
-  const volatile double* ps = a.ps + p.ps_off;
-  const volatile uint32_t* pc = a.pc + p.pc_off;
-  double* gs = a.gs + p.gs_off;
-  for (int64_t t = tid; t < p.ng * kd; t += NT) {
-    const int64_t g = t / kd, e = t % kd;
-    const int64_t b0 = g * kGroupBlocks, b1 = min<int64_t>(p.nb, b0 + kGroupBlocks);
-    double s = 0.0;
-    for (int64_t bb = b0; bb < b1; ++bb) s = s + ps[bb * kd + e];
-    gs[g * kd + e] = s;
-  }
-  __syncthreads();
-  const volatile double* gsv = gs;
-  double* sums = a.sums + p.sum_off;
-  uint64_t* cnt = a.cnt + p.cnt_off;
-  for (int64_t e = tid; e < kd; e += NT) {
-    double s = 0.0;
-    for (int64_t g = 0; g < p.ng; ++g) s = s + gsv[g * kd + e];
-    sums[e] = s;
-  }
-  for (int c = tid; c < k; c += NT) {
-    uint64_t n = 0;
-    for (int64_t bb = 0; bb < p.nb; ++bb) n += pc[bb * k + c];
-    cnt[c] = n;
-  }
-  __syncthreads();
-  const volatile double* sv = sums;
-  const volatile uint64_t* cv = cnt;
-  double* cent = a.cent + p.cent_off;
-  for (int c = tid; c < k; c += NT) {
-    const double* cc = cent + static_cast<int64_t>(c) * p.ld;
-    double dot = 0.0, c2 = 0.0;
-    for (int j = 0; j < d; ++j) {
-      dot = dot + cc[j] * sv[static_cast<int64_t>(c) * d + j];
-      c2 = c2 + cc[j] * cc[j];
-    }
-    term[c] = cv[c] > 0 ? static_cast<double>(cv[c]) * c2 - 2.0 * dot : 0.0;
-  }
-  __syncthreads();
-  KmState* st = a.st + i;
-  if (tid == 0) {
-    double s = st->x2total;
-    for (int c = 0; c < k; ++c) s = s + term[c];
-    st->dist = s;
-  }
-  __syncthreads();
-  for (int64_t e = tid; e < kd; e += NT) {
-    const int c = static_cast<int>(e / d), j = static_cast<int>(e % d);
-    if (cv[c] > 0) cent[static_cast<int64_t>(c) * p.ld + j] = sv[e] / static_cast<double>(cv[c]);
-  }
-  __shared__ int s_done;
-  if (tid == 0) {
-    st->iters += 1;
-    if (st->distortion <= st->dist) {
-      st->done = 1;
-    } else {
-      st->distortion = st->dist;
-    }
-    s_done = static_cast<int>(st->done);
-    a.arrivals[i] = 0u;
-  }
-  __syncthreads();
-  if (!s_done)
-    km_prep_centroids_body(cent, k, p.kpad, p.ld, a.cb + p.cb_off, a.c2 + p.c2_off,
-                           a.c2 + p.c2_off + p.kpad, red);
-}
-
-__global__ __launch_bounds__(NT) void kmb_step_kernel(const KmbProblem* __restrict__ P,
-                                                      const uint2* __restrict__ items,
-                                                      int64_t nitems, KmbArena a) {
-  extern __shared__ __align__(16) unsigned char smem[];
-  __shared__ double red[NT];
-  __shared__ double term[32 * kMaxCB];
-  __shared__ int s_last;
-  for (int64_t it = blockIdx.x; it < nitems; it += gridDim.x) {
-    const int i = static_cast<int>(items[it].x);
-    const int64_t b = items[it].y;
-    KmState* st = a.st + i;
-    // every block of problem i in this launch sees the same value: the
-    // flag only changes in the finisher, after all of them have arrived
-    if (*reinterpret_cast<volatile uint32_t*>(&st->done)) continue;
-    const KmbProblem p = P[i];
-    km_block_step(b, smem, a.rows + p.row_off, p.n, p.ld, p.d, a.cent + p.cent_off,
-                  a.cb + p.cb_off, a.c2 + p.c2_off, a.c2 + p.c2_off + p.kpad, p.k,
-                  a.lab + p.lab_off, a.ps + p.ps_off, a.pc + p.pc_off, st);
-    if (threadIdx.x == 0) {
-      __threadfence();
-      const uint32_t old = atomicAdd(&a.arrivals[i], 1u);
-      s_last = old == static_cast<uint32_t>(p.nb - 1) ? 1 : 0;
-    }
-    __syncthreads();
-    if (s_last) {
-      __threadfence();
-      kmb_finish(p, i, a, red, term);
-    }
-    __syncthreads();
-  }
-}
-
-size_t kmb_lds_bytes(int k, int d, int ld) {
-  return sizeof(double) * ((static_cast<size_t>(k) * d + 1) / 2 * 2) +
-         sizeof(float) * static_cast<size_t>(kChunkRows) * ld +
-         sizeof(uint32_t) * (kChunkRows + static_cast<size_t>(k)) + 16;
-}
+struct Klass {
+  int kbm, ksm;
+  bool exact;
+  int kmax = 0, ldmax = 0;
+  std::vector<uint2> items;
+  DevBuf buf;
+};
 
 struct TuneEntry {
-  int64_t key;
-  unsigned grid;
+  std::vector<int64_t> key;
+  int mult;
 };
 std::mutex g_tune_mu;
 std::vector<TuneEntry> g_tune;
@@ -606,16 +659,8 @@ extern "C" int sc_kmeans_batch_run(const sc_kmeans_problem* problems, size_t cou
   hipStream_t s = stream_for_device(dev);
 
   bool batched = mode == SC_MIXED && dtype == SC_F32;
-  size_t lds = 0;
-  for (size_t i = 0; i < count && batched; ++i) {
-    const int ld = static_cast<int>(padded_dim(problems[i].d));
-    const int k = static_cast<int>(problems[i].k);
-    lds = std::max(lds, kmb_lds_bytes(k, static_cast<int>(problems[i].d), ld));
-    if (k > 32 * kMaxCB || ld > 16 * kMaxKS ||
-        static_cast<int64_t>(k) * static_cast<int64_t>(problems[i].d) > kMaxLdsAcc)
-      batched = false;
-  }
-  if (lds > 96 * 1024) batched = false;
+  for (size_t i = 0; i < count && batched; ++i)
+    if (problems[i].k > 128 || padded_dim(problems[i].d) > 128) batched = false;
 
   if (!batched) {
     // the FP64 form, one problem at a time (the comparator of the batch)
@@ -636,14 +681,16 @@ extern "C" int sc_kmeans_batch_run(const sc_kmeans_problem* problems, size_t cou
     return SC_OK;
   }
 
-  // ---- layout ----
-  std::vector<KmbProblem> P(count);
-  int64_t rows_e = 0, cent_e = 0, cb_e = 0, c2_e = 0, lab_e = 0, ps_e = 0, pc_e = 0, gs_e = 0,
-          sum_e = 0, cnt_e = 0, x2_e = 0;
-  std::vector<uint2> items;
+  // ---- layout: one arena per buffer kind ----
+  std::vector<KmProb> P(count);
+  struct Off {
+    int64_t rows, cent, lab, ps, pc, gs, gc, sum, cnt, x2, tab, c2f;
+  };
+  std::vector<Off> off(count);
+  Off tot{};
   for (size_t i = 0; i < count; ++i) {
     const sc_kmeans_problem& pr = problems[i];
-    KmbProblem& p = P[i];
+    KmProb& p = P[i];
     p.n = static_cast<int64_t>(pr.n);
     p.d = static_cast<int>(pr.d);
     p.ld = static_cast<int>(padded_dim(pr.d));
@@ -651,92 +698,153 @@ extern "C" int sc_kmeans_batch_run(const sc_kmeans_problem* problems, size_t cou
     p.kpad = (p.k + 31) / 32 * 32;
     p.nb = (p.n + kBlockRows - 1) / kBlockRows;
     p.ng = (p.nb + kGroupBlocks - 1) / kGroupBlocks;
-    const int64_t kd = static_cast<int64_t>(p.k) * p.d;
-    p.row_off = rows_e;   rows_e += (p.n * p.ld + 3) / 4 * 4;
-    p.cent_off = cent_e;  cent_e += static_cast<int64_t>(p.k) * p.ld;
-    p.cb_off = cb_e;      cb_e += (static_cast<int64_t>(p.kpad) * p.ld + 7) / 8 * 8;
-    p.c2_off = c2_e;      c2_e += p.kpad + 4;
-    p.lab_off = lab_e;    lab_e += p.n;
-    p.ps_off = ps_e;      ps_e += p.nb * kd;
-    p.pc_off = pc_e;      pc_e += p.nb * p.k;
-    p.gs_off = gs_e;      gs_e += p.ng * kd;
-    p.sum_off = sum_e;    sum_e += kd;
-    p.cnt_off = cnt_e;    cnt_e += p.k;
-    p.x2_off = x2_e;      x2_e += p.nb;
+    const int64_t kd = static_cast<int64_t>(p.k) * std::max(p.d, 1);
+    Off& o = off[i];
+    o.rows = tot.rows;  tot.rows += p.n * p.ld;
+    o.cent = tot.cent;  tot.cent += static_cast<int64_t>(p.k) * p.ld;
+    o.lab = tot.lab;    tot.lab += p.n;
+    o.ps = tot.ps;      tot.ps += p.nb * kd;
+    o.pc = tot.pc;      tot.pc += p.nb * p.k;
+    o.gs = tot.gs;      tot.gs += p.ng * kd;
+    o.gc = tot.gc;      tot.gc += p.ng * p.k;
+    o.sum = tot.sum;    tot.sum += kd;
+    o.cnt = tot.cnt;    tot.cnt += p.k;
+    o.x2 = tot.x2;      tot.x2 += p.nb + 2 * static_cast<int64_t>(p.k);
+    o.tab = tot.tab;    tot.tab += static_cast<int64_t>(p.kpad) * p.ld;
+    o.c2f = tot.c2f;    tot.c2f += p.kpad + 4;
   }
-  // items, block-major across problems so concurrent workgroups spread over
-  // problems and each problem's last block arrives early
-  int64_t maxnb = 0;
-  for (auto& p : P) maxnb = std::max(maxnb, p.nb);
-  for (int64_t b = 0; b < maxnb; ++b)
-    for (size_t i = 0; i < count; ++i)
-      if (b < P[i].nb) items.push_back(make_uint2(static_cast<unsigned>(i), static_cast<unsigned>(b)));
-  const int64_t nitems = static_cast<int64_t>(items.size());
-
-  DevBuf bP, bI, bRows, bCent, bCb, bC2, bLab, bPs, bPc, bGs, bSum, bCnt, bX2, bSt, bArr;
-  SC_HIP(bP.alloc(sizeof(KmbProblem) * count, s));
-  SC_HIP(bI.alloc(sizeof(uint2) * nitems, s));
-  SC_HIP(bRows.alloc(sizeof(float) * rows_e, s));
-  SC_HIP(bCent.alloc(sizeof(double) * cent_e, s));
-  SC_HIP(bCb.alloc(sizeof(uint16_t) * cb_e, s));
-  SC_HIP(bC2.alloc(sizeof(float) * c2_e, s));
-  SC_HIP(bLab.alloc(sizeof(uint32_t) * lab_e, s));
-  SC_HIP(bPs.alloc(sizeof(double) * std::max<int64_t>(ps_e, 1), s));
-  SC_HIP(bPc.alloc(sizeof(uint32_t) * std::max<int64_t>(pc_e, 1), s));
-  SC_HIP(bGs.alloc(sizeof(double) * std::max<int64_t>(gs_e, 1), s));
-  SC_HIP(bSum.alloc(sizeof(double) * std::max<int64_t>(sum_e, 1), s));
-  SC_HIP(bCnt.alloc(sizeof(uint64_t) * std::max<int64_t>(cnt_e, 1), s));
-  SC_HIP(bX2.alloc(sizeof(double) * std::max<int64_t>(x2_e, 1), s));
-  SC_HIP(bSt.alloc(sizeof(KmState) * count, s));
-  SC_HIP(bArr.alloc(sizeof(uint32_t) * count, s));
-  KmbArena a{bRows.as<float>(), bCent.as<double>(), bCb.as<uint16_t>(), bC2.as<float>(),
-             bLab.as<uint32_t>(), bPs.as<double>(), bPc.as<uint32_t>(), bGs.as<double>(),
-             bSum.as<double>(), bCnt.as<uint64_t>(), bX2.as<double>(), bSt.as<KmState>(),
-             bArr.as<uint32_t>()};
-  SC_HIP(hipMemcpyAsync(bP.p, P.data(), sizeof(KmbProblem) * count, hipMemcpyHostToDevice, s));
-  SC_HIP(hipMemcpyAsync(bI.p, items.data(), sizeof(uint2) * nitems, hipMemcpyHostToDevice, s));
-  SC_HIP(hipMemsetAsync(bCent.p, 0, sizeof(double) * cent_e, s));
+  DevBuf bP, bRows, bCent, bLab, bPs, bPc, bGs, bGc, bSum, bCnt, bX2, bHi, bLo, bC2f, bSt;
+  SC_HIP(bP.alloc(sizeof(KmProb) * count, s));
+  SC_HIP(bRows.alloc(sizeof(float) * tot.rows, s));
+  SC_HIP(bCent.alloc(sizeof(double) * tot.cent, s));
+  SC_HIP(bLab.alloc(sizeof(uint32_t) * tot.lab, s));
+  SC_HIP(bPs.alloc(sizeof(double) * tot.ps, s));
+  SC_HIP(bPc.alloc(sizeof(uint32_t) * tot.pc, s));
+  SC_HIP(bGs.alloc(sizeof(double) * tot.gs, s));
+  SC_HIP(bGc.alloc(sizeof(uint64_t) * tot.gc, s));
+  SC_HIP(bSum.alloc(sizeof(double) * tot.sum, s));
+  SC_HIP(bCnt.alloc(sizeof(uint64_t) * tot.cnt, s));
+  SC_HIP(bX2.alloc(sizeof(double) * tot.x2, s));
+  SC_HIP(bHi.alloc(sizeof(uint16_t) * tot.tab, s));
+  SC_HIP(bLo.alloc(sizeof(uint16_t) * tot.tab, s));
+  SC_HIP(bC2f.alloc(sizeof(float) * tot.c2f, s));
+  // per problem: state (32 B, 8-aligned) then the arrival counter
+  const size_t st_stride = (sizeof(KmState) + 8 + 15) / 16 * 16;
+  SC_HIP(bSt.alloc(st_stride * count, s));
+  for (size_t i = 0; i < count; ++i) {
+    KmProb& p = P[i];
+    const Off& o = off[i];
+    p.rows = bRows.as<float>() + o.rows;
+    p.cent = bCent.as<double>() + o.cent;
+    p.labels = bLab.as<uint32_t>() + o.lab;
+    p.psum = bPs.as<double>() + o.ps;
+    p.pcount = bPc.as<uint32_t>() + o.pc;
+    p.gsum = bGs.as<double>() + o.gs;
+    p.gcount = bGc.as<uint64_t>() + o.gc;
+    p.sums = bSum.as<double>() + o.sum;
+    p.counts = bCnt.as<uint64_t>() + o.cnt;
+    p.x2block = bX2.as<double>() + o.x2;
+    p.term = p.x2block + p.nb;
+    p.c2d = p.term + p.k;
+    p.st = reinterpret_cast<KmState*>(bSt.as<char>() + st_stride * i);
+    p.arrive = reinterpret_cast<uint32_t*>(bSt.as<char>() + st_stride * i + sizeof(KmState));
+    p.bhi = bHi.as<uint16_t>() + o.tab;
+    p.blo = bLo.as<uint16_t>() + o.tab;
+    p.c2f = bC2f.as<float>() + o.c2f;
+    p.cmax = p.c2f + p.kpad;
+  }
+  SC_HIP(hipMemcpyAsync(bP.p, P.data(), sizeof(KmProb) * count, hipMemcpyHostToDevice, s));
+  SC_HIP(hipMemsetAsync(bCent.p, 0, sizeof(double) * tot.cent, s));
   for (size_t i = 0; i < count; ++i) {
     const sc_kmeans_problem& pr = problems[i];
-    if (int rc = upload_padded(pr.rows, SC_F32, pr.n, pr.d, P[i].ld, a.rows + P[i].row_off, s))
+    if (int rc = upload_padded(pr.rows, SC_F32, pr.n, pr.d, P[i].ld, const_cast<void*>(P[i].rows), s))
       return rc;
-    SC_HIP(hipMemcpy2DAsync(a.cent + P[i].cent_off, P[i].ld * sizeof(double), pr.init_centroids,
+    SC_HIP(hipMemcpy2DAsync(P[i].cent, P[i].ld * sizeof(double), pr.init_centroids,
                             pr.d * sizeof(double), pr.d * sizeof(double), pr.k,
                             hipMemcpyHostToDevice, s));
   }
-  const KmbProblem* dP = bP.as<KmbProblem>();
-  const uint2* dI = bI.as<uint2>();
+  const KmProb* dP = bP.as<KmProb>();
+
+  // ---- items: x2 over every block; step items per shape class ----
+  std::vector<uint2> all;
+  std::map<int, Klass> klass;
+  int64_t maxnb = 0;
+  for (auto& p : P) maxnb = std::max(maxnb, p.nb);
+  // block-major across problems so concurrent workgroups spread over problems
+  for (int64_t b = 0; b < maxnb; ++b)
+    for (size_t i = 0; i < count; ++i) {
+      const KmProb& p = P[i];
+      if (b >= p.nb) continue;
+      const uint2 it = make_uint2(static_cast<unsigned>(i), static_cast<unsigned>(b));
+      all.push_back(it);
+      const int kbm = kbm_of(p.k), ksm = ksm_of(p.ld);
+      const bool exact = (p.k + 31) / 32 == kbm && p.ld / 16 == ksm;
+      Klass& K = klass[(kbm * 16 + ksm) * 2 + (exact ? 1 : 0)];
+      K.kbm = kbm;
+      K.ksm = ksm;
+      K.exact = exact;
+      K.kmax = std::max(K.kmax, p.k);
+      K.ldmax = std::max(K.ldmax, p.ld);
+      K.items.push_back(it);
+    }
+  DevBuf bAll;
+  SC_HIP(bAll.alloc(sizeof(uint2) * all.size(), s));
+  SC_HIP(hipMemcpyAsync(bAll.p, all.data(), sizeof(uint2) * all.size(), hipMemcpyHostToDevice, s));
+  for (auto& kv : klass) {
+    Klass& K = kv.second;
+    SC_HIP(K.buf.alloc(sizeof(uint2) * K.items.size(), s));
+    SC_HIP(hipMemcpyAsync(K.buf.p, K.items.data(), sizeof(uint2) * K.items.size(),
+                          hipMemcpyHostToDevice, s));
+  }
+  KmItems red;
+  if (int rc = red.build(P.data(), count, s)) return rc;  // waits for the copies above
+
   const unsigned cgrid = static_cast<unsigned>((count + 255) / 256);
-  hipLaunchKernelGGL(kmb_init_kernel, dim3(cgrid), dim3(256), 0, s, a, static_cast<int>(count));
-  hipLaunchKernelGGL(kmb_x2_kernel, dim3(1024), dim3(NT), 0, s, dP, dI, nitems, a);
-  hipLaunchKernelGGL(kmb_x2_total_kernel, dim3(cgrid), dim3(256), 0, s, dP, static_cast<int>(count), a);
-  hipLaunchKernelGGL(kmb_prep_kernel, dim3(static_cast<unsigned>(count)), dim3(NT), 0, s, dP, a);
+  hipLaunchKernelGGL(kmb_init_kernel, dim3(cgrid), dim3(256), 0, s, dP, static_cast<int>(count));
+  hipLaunchKernelGGL(kmb_x2_kernel, dim3(1024), dim3(NT), 0, s, dP, bAll.as<uint2>(),
+                     static_cast<int64_t>(all.size()));
+  hipLaunchKernelGGL(kmb_x2_total_kernel, dim3(cgrid), dim3(256), 0, s, dP, static_cast<int>(count));
   SC_HIP(hipGetLastError());
+  if (int rc = kmeans_tables_launch(dP, count, s)) return rc;
+
+  int cus = 256;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, dev) == hipSuccess && prop.multiProcessorCount > 0)
+    cus = prop.multiProcessorCount;
+  auto iteration = [&](int mult) -> int {
+    for (auto& kv : klass) {
+      Klass& K = kv.second;
+      const int64_t ni = static_cast<int64_t>(K.items.size());
+      const unsigned grid = static_cast<unsigned>(std::min<int64_t>(ni, static_cast<int64_t>(mult) * cus));
+      if (int rc = launch_step(K.kbm, K.ksm, dP, K.buf.as<uint2>(), ni, grid, K.exact, K.kmax,
+                               K.ldmax, s))
+        return rc;
+    }
+    return kmeans_reduce_launch(dP, red, true, true, s);
+  };
 
   // ---- grid size: cached, or timed on the first iterations ----
-  const int64_t key = static_cast<int64_t>(count) * 1000003 + nitems;
-  unsigned grid = 0;
+  std::vector<int64_t> key;
+  for (auto& kv : klass) {
+    key.push_back(kv.first);
+    key.push_back(static_cast<int64_t>(kv.second.items.size()));
+  }
+  int mult = 0;
   {
     std::lock_guard<std::mutex> lk(g_tune_mu);
     for (auto& e : g_tune)
-      if (e.key == key) grid = e.grid;
+      if (e.key == key) mult = e.mult;
   }
   size_t it_done = 0;
-  if (grid == 0) {
-    int cus = 256;
-    hipDeviceProp_t prop;
-    if (hipGetDeviceProperties(&prop, dev) == hipSuccess && prop.multiProcessorCount > 0)
-      cus = prop.multiProcessorCount;
-    const unsigned cand[3] = {static_cast<unsigned>(cus), static_cast<unsigned>(2 * cus),
-                              static_cast<unsigned>(4 * cus)};
+  if (mult == 0) {
+    const int cand[3] = {1, 2, 4};
     float best = 1e30f;
     hipEvent_t e0, e1;
     SC_HIP(hipEventCreate(&e0));
     SC_HIP(hipEventCreate(&e1));
     for (int c = 0; c < 3 && it_done < max_iter; ++c) {
       SC_HIP(hipEventRecord(e0, s));
-      hipLaunchKernelGGL(kmb_step_kernel, dim3(cand[c]), dim3(NT), lds, s, dP, dI, nitems, a);
-      SC_HIP(hipGetLastError());
+      if (int rc = iteration(cand[c])) return rc;
       SC_HIP(hipEventRecord(e1, s));
       SC_HIP(hipEventSynchronize(e1));
       float ms = 0.0f;
@@ -744,13 +852,13 @@ extern "C" int sc_kmeans_batch_run(const sc_kmeans_problem* problems, size_t cou
       ++it_done;
       if (ms < best) {
         best = ms;
-        grid = cand[c];
+        mult = cand[c];
       }
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     std::lock_guard<std::mutex> lk(g_tune_mu);
-    g_tune.push_back(TuneEntry{key, grid});
+    g_tune.push_back(TuneEntry{key, mult});
   }
   // ---- remaining iterations as one graph ----
   if (it_done < max_iter) {
@@ -758,9 +866,13 @@ extern "C" int sc_kmeans_batch_run(const sc_kmeans_problem* problems, size_t cou
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     SC_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-    for (size_t it = it_done; it < max_iter; ++it)
-      hipLaunchKernelGGL(kmb_step_kernel, dim3(grid), dim3(NT), lds, s, dP, dI, nitems, a);
+    int crc = SC_OK;
+    for (size_t it = it_done; it < max_iter && crc == SC_OK; ++it) crc = iteration(mult);
     hipError_t ce = hipStreamEndCapture(s, &graph);
+    if (crc) {
+      if (graph) (void)hipGraphDestroy(graph);
+      return crc;
+    }
     if (ce != hipSuccess) return hip_fail(ce, "hipStreamEndCapture");
     ce = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
     if (ce != hipSuccess) {
@@ -776,16 +888,17 @@ extern "C" int sc_kmeans_batch_run(const sc_kmeans_problem* problems, size_t cou
   }
   // ---- results ----
   std::vector<KmState> hs(count);
-  SC_HIP(hipMemcpyAsync(hs.data(), bSt.p, sizeof(KmState) * count, hipMemcpyDeviceToHost, s));
+  for (size_t i = 0; i < count; ++i)
+    SC_HIP(hipMemcpyAsync(&hs[i], P[i].st, sizeof(KmState), hipMemcpyDeviceToHost, s));
   size_t co = 0, lo = 0, so = 0;
   for (size_t i = 0; i < count; ++i) {
-    const KmbProblem& p = P[i];
-    SC_HIP(hipMemcpy2DAsync(centroids_out + co, p.d * sizeof(double), a.cent + p.cent_off,
+    const KmProb& p = P[i];
+    SC_HIP(hipMemcpy2DAsync(centroids_out + co, p.d * sizeof(double), p.cent,
                             p.ld * sizeof(double), p.d * sizeof(double), p.k,
                             hipMemcpyDeviceToHost, s));
-    SC_HIP(hipMemcpyAsync(labels_out + lo, a.lab + p.lab_off, sizeof(uint32_t) * p.n,
+    SC_HIP(hipMemcpyAsync(labels_out + lo, p.labels, sizeof(uint32_t) * p.n,
                           hipMemcpyDeviceToHost, s));
-    SC_HIP(hipMemcpyAsync(sizes_out + so, a.cnt + p.cnt_off, sizeof(uint64_t) * p.k,
+    SC_HIP(hipMemcpyAsync(sizes_out + so, p.counts, sizeof(uint64_t) * p.k,
                           hipMemcpyDeviceToHost, s));
     co += static_cast<size_t>(p.k) * p.d;
     lo += p.n;

@@ -60,6 +60,19 @@ def test_mixed_near_ties_use_fp64_fallback():
     assert st["uncertified"] > 0
 
 
+def test_mixed_exact_ties_take_lowest_index():
+    """Duplicated centroids tie exactly for every row that prefers them, so
+    neither bound separates them and the sequential FP64 fold decides: the
+    lowest index must win, as in the FP64 form."""
+    rng = np.random.default_rng(43)
+    x = rng.normal(size=(12_000, 24)).astype(np.float32)
+    base = x[:6].astype(np.float64)
+    init = np.concatenate([base, base[:3]]).astype(np.float64)
+    a, b, st = run_both(x, init, 6)
+    assert_same(a, b)
+    assert st["uncertified"] > 0
+
+
 def test_mixed_handles_non_finite_rows():
     rng = np.random.default_rng(41)
     x = rng.normal(size=(5_000, 16)).astype(np.float32)
