@@ -15,24 +15,21 @@
 //   L = (1 - 2^-18)(x2 + y2) - 2C - 2 c1 ||x|| ||y||  <=  s  <=
 //   U = (1 + 2^-16)(x2 + y2) - 2C + 2 c1 ||x|| ||y||.
 //
-// The MFMA computes L - x2m directly: every BF16 row carries one extra
-// 16-wide K step [y2 as three BF16 terms, nyb, 0...] and every BF16 query
-// [-2x | 1, 1, 1, -cxb, 0...] (scaling by -2 is exact), with nyb >= ||y|| and
-// cxb >= 2 c1 ||x|| rounded up to BF16 so their product over-covers the
-// bound.  That step is issued last, after the cross terms, so its rounding
-// is relative to the final magnitude and inside the 2^-18 margin.
-//
-// Pass 1 (scan, MFMA).  A workgroup of four waves owns 512 queries (128 per
-// wave, held as BF16 B-fragments in registers) and one contiguous segment
-// of rows, streamed 32 rows at a time (prefetched one tile ahead) through
-// v_mfma_f32_32x32x16_bf16.  Workgroups are laid out so that the 32 CUs of
-// an XCD stream the same segment and share its tiles in their L2.  Per
-// query the wave keeps the k smallest U seen in the segment; every row whose
-// L is below that running k-th U is appended (index, L) to the query's list
-// for the segment, in index order.  A row that enters the reference heap has
-// s below the running k-th smallest s of all earlier rows, which is at most
-// the running k-th U of the segment, so the lists are a superset of the heap
-// entrants.
+// Pass 1 (scan, MFMA).  A workgroup of eight waves owns 512 queries (64 per
+// wave, held as BF16 B-fragments of -2x in registers; scaling by -2 is
+// exact) and one contiguous segment of rows, streamed 32 rows at a time
+// through a padded LDS double buffer that all eight waves read (each tile
+// is loaded into registers two tiles ahead).  v_mfma_f32_32x32x16_bf16 gives
+// acc = -2C; the epilogue forms L - x2m = y2m - cxb nyb + acc with one FMA
+// and one add (y2m <= (1 - 2^-18) y2, nyb >= ||y||, cxb >= 2 c1 ||x||, all
+// rounded the safe way; the two roundings are inside the 2^-18 margin).
+// Workgroups are laid out so that the CUs of an XCD stream the same segment
+// and share its tiles in their L2.  Per query the wave keeps the k smallest
+// U seen in the segment; every row whose L is below that running k-th U is
+// appended (index, L) to the query's list for the segment, in index order.
+// A row that enters the reference heap has s below the running k-th
+// smallest s of all earlier rows, which is at most the running k-th U of the
+// segment, so the lists are a superset of the heap entrants.
 // Pass 2 (replay, FP64).  One thread per query walks its lists in index
 // order and rescores a candidate exactly only if its L is below the s of
 // the current heap root (otherwise it cannot enter), then applies the heap
@@ -61,11 +58,13 @@ namespace {
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
-constexpr int kWaves = 4;
-constexpr int kQPW = 128;                 // queries per wave (4 column blocks)
+constexpr int kWaves = 8;                 // waves per workgroup (two per SIMD)
+constexpr int kQPW = 64;                  // queries per wave (2 column blocks)
 constexpr int kQPB = kWaves * kQPW;       // queries per workgroup
+constexpr int kNT = 64 * kWaves;          // threads per workgroup
 constexpr int kRows = 32;                 // rows per tile
 constexpr int kMaxKS = 8;                 // K steps of 16 -> dimension <= 128
 constexpr int kMaxK = 32;                 // largest k served by this form
@@ -81,27 +80,17 @@ __device__ __forceinline__ uint16_t bf16_rne(float f) {
   return static_cast<uint16_t>(u >> 16);
 }
 
-// Smallest BF16 >= f for finite f >= 0.
-__device__ __forceinline__ uint16_t bf16_ru_pos(float f) {
-  const uint32_t u = __float_as_uint(f);
-  const uint32_t t = u >> 16;
-  return static_cast<uint16_t>((u & 0xffffu) ? t + 1 : t);
-}
-
-__device__ __forceinline__ float bf16_to_f32(uint16_t b) {
-  return __uint_as_float(static_cast<uint32_t>(b) << 16);
-}
-
-// Per row: BF16 copy with the extra K step, y2m = RD((1 - 2^-18) y2) and
-// nyb = the BF16 value >= ||y||.  A row whose values are not finite or whose
-// squared norm reaches kNormLimit marks `bad` (the index then uses FP64).
-__global__ void prep_rows_kernel(const float* __restrict__ rows, int64_t n, int ld, int ldb,
+// Per row: BF16 copy, y2m = RD((1 - 2^-18) y2) and nyb = RU(||y||) (with a
+// relative allowance for the f64 sum behind y2).  A row whose values are not
+// finite or whose squared norm reaches kNormLimit marks `bad` (the index
+// then uses FP64).
+__global__ void prep_rows_kernel(const float* __restrict__ rows, int64_t n, int ld,
                                  uint16_t* __restrict__ out_b, float* __restrict__ y2m,
                                  float* __restrict__ nyb, uint32_t* __restrict__ bad) {
   const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const float* r = rows + i * ld;
-  uint16_t* o = out_b + i * ldb;
+  uint16_t* o = out_b + i * ld;
   double s = 0.0;
   bool ok = true;
   for (int j = 0; j < ld; ++j) {
@@ -110,44 +99,35 @@ __global__ void prep_rows_kernel(const float* __restrict__ rows, int64_t n, int 
     s += static_cast<double>(v) * static_cast<double>(v);
     o[j] = bf16_rne(v);
   }
-  for (int j = ld; j < ldb; ++j) o[j] = 0;
   if (!ok || !(s < kNormLimit)) {
     atomicOr(bad, 1u);
     y2m[i] = INFINITY;
-    nyb[i] = INFINITY;
+    nyb[i] = 0.0f;
     return;
   }
-  const float y2 = __double2float_rd(s * (1.0 - kLoMargin));
-  const uint16_t h = bf16_rne(y2);
-  const float r1 = y2 - bf16_to_f32(h);
-  const uint16_t m = bf16_rne(r1);
-  const float r2 = r1 - bf16_to_f32(m);
-  const uint16_t l = bf16_rne(r2);
-  const uint16_t nb = bf16_ru_pos(__double2float_ru(sqrt(s)));
-  o[ld + 0] = h;
-  o[ld + 1] = m;
-  o[ld + 2] = l;
-  o[ld + 3] = nb;
-  y2m[i] = y2;
-  nyb[i] = bf16_to_f32(nb);
+  y2m[i] = __double2float_rd(s * (1.0 - kLoMargin));
+  nyb[i] = __double2float_ru(sqrt(s) * (1.0 + 9.094947017729282e-13));
 }
 
-__global__ void fill_f32_kernel(float* p, int64_t n, float v) {
-  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = v;
+// y2m of padding rows is +inf (their L is +inf), nyb is 0.
+__global__ void fill_terms_kernel(float* y2m, float* nyb, int64_t from, int64_t npad) {
+  const int64_t i = from + static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < npad) {
+    y2m[i] = INFINITY;
+    nyb[i] = 0.0f;
+  }
 }
 
-// Per query: BF16 [-2x | 1, 1, 1, -cxb], x2m and cxb (the BF16 value
-// >= 2 c1 ||x||); a query that is not finite or too large is listed for the
-// exact kernel.
-__global__ void prep_queries_kernel(const float* __restrict__ qs, int64_t q, int ld, int ldb,
+// Per query: BF16 of -2x, x2m = RD((1 - 2^-18) x2) and cxb = RU(2 c1 ||x||);
+// a query that is not finite or too large is listed for the exact kernel.
+__global__ void prep_queries_kernel(const float* __restrict__ qs, int64_t q, int ld,
                                     uint16_t* __restrict__ out_b, float* __restrict__ x2m,
                                     float* __restrict__ cxb, uint32_t* __restrict__ list,
                                     uint32_t* __restrict__ list_n) {
   const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= q) return;
   const float* r = qs + i * ld;
-  uint16_t* o = out_b + i * ldb;
+  uint16_t* o = out_b + i * ld;
   double s = 0.0;
   bool ok = true;
   for (int j = 0; j < ld; ++j) {
@@ -156,7 +136,6 @@ __global__ void prep_queries_kernel(const float* __restrict__ qs, int64_t q, int
     s += static_cast<double>(v) * static_cast<double>(v);
     o[j] = bf16_rne(-2.0f * v);
   }
-  for (int j = ld; j < ldb; ++j) o[j] = 0;
   if (!ok || !(s < kNormLimit)) {
     // never a candidate in pass 1; the exact kernel takes it
     x2m[i] = INFINITY;
@@ -165,14 +144,8 @@ __global__ void prep_queries_kernel(const float* __restrict__ qs, int64_t q, int
     list[slot] = static_cast<uint32_t>(i);
     return;
   }
-  const uint16_t cb = bf16_ru_pos(__double2float_ru(2.0 * kC1 * sqrt(s) * (1.0 + 9.5367431640625e-07)));
-  const uint16_t one = 0x3f80;  // 1.0
-  o[ld + 0] = one;
-  o[ld + 1] = one;
-  o[ld + 2] = one;
-  o[ld + 3] = static_cast<uint16_t>(cb | 0x8000u);  // -cxb
   x2m[i] = __double2float_rd(s * (1.0 - kLoMargin));
-  cxb[i] = bf16_to_f32(cb);
+  cxb[i] = __double2float_ru(2.0 * kC1 * sqrt(s) * (1.0 + 9.5367431640625e-07));
 }
 
 __device__ __forceinline__ bf16x8 load_frag(const uint16_t* p) {
@@ -180,114 +153,166 @@ __device__ __forceinline__ bf16x8 load_frag(const uint16_t* p) {
   return __builtin_bit_cast(bf16x8, v);
 }
 
-__device__ __forceinline__ float min3f(float a, float b, float c) { return fminf(a, fminf(b, c)); }
-
 template <int KS>
-struct ScanCtx {
-  static constexpr int KT = KS + 1;  // cross-term steps + the norm step
-  static constexpr int LDB = 16 * KT;
+struct Scan {
+  static constexpr int LD = 16 * KS;                    // row length (bf16)
+  static constexpr int RS = LD + 8;                     // LDS row stride: 16 B of padding
+  static constexpr int VEC = kRows * LD / 8;            // 16-byte pieces of a tile
+  static constexpr int SLOT = kRows * RS * 2 + 2 * kRows * 4;  // rows | y2m | nyb (bytes)
+  static_assert(VEC <= kNT, "one piece per thread");
 };
 
+// Dynamic LDS of the scan: two tile slots | topu[kQPB][k] | ccount[kQPB] |
+// trans[kWaves][64][16].
 template <int KS>
-__global__ __launch_bounds__(256, 1) void mixed_scan_kernel(
+size_t scan_lds_bytes(int k) {
+  return 2 * static_cast<size_t>(Scan<KS>::SLOT) + sizeof(float) * kQPB * k +
+         sizeof(uint32_t) * kQPB + sizeof(float) * kWaves * 64 * 16;
+}
+
+template <int KS>
+__global__ __launch_bounds__(kNT, 2) void mixed_scan_kernel(
     const uint16_t* __restrict__ rows_b, const float* __restrict__ y2m,
     const float* __restrict__ nyb, int64_t n, const uint16_t* __restrict__ q_b,
     const float* __restrict__ x2m, const float* __restrict__ cxbq, int64_t q, int k,
     int64_t seg_len, int nseg, int64_t nqt, uint32_t* __restrict__ cand_idx,
     float* __restrict__ cand_L, uint32_t* __restrict__ cand_count, int cap) {
-  constexpr int KT = ScanCtx<KS>::KT;
-  constexpr int LDB = ScanCtx<KS>::LDB;
+  using S = Scan<KS>;
+  extern __shared__ __align__(16) unsigned char smem[];
   // XCD-aware layout: block b runs on XCD b % 8; the XCDs with the same
   // b % nseg stream the same segment with different query tiles.
   const int64_t bid = blockIdx.x;
   const int xcd = static_cast<int>(bid % kXcds);
-  const int64_t slot = bid / kXcds;
+  const int64_t slot_i = bid / kXcds;
   const int groups = kXcds / nseg;
   const int seg = xcd % nseg;
-  const int64_t qt = slot * groups + xcd / nseg;
+  const int64_t qt = slot_i * groups + xcd / nseg;
   if (qt >= nqt) return;  // whole workgroup leaves together
 
-  __shared__ float topu[kQPB][kMaxK];  // per query: k smallest U, ascending
-  __shared__ uint32_t ccount[kQPB];
-  const int lane = threadIdx.x & 63;
-  const int wave = threadIdx.x >> 6;
+  float* topu = reinterpret_cast<float*>(smem + 2 * S::SLOT);  // per query: k smallest U
+  uint32_t* ccount = reinterpret_cast<uint32_t*>(topu + kQPB * k);
+  float* trans = reinterpret_cast<float*>(ccount + kQPB);      // slow-path transpose
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
   const int col = lane & 31;   // query column inside a 32-wide block / row of A
   const int half = lane >> 5;  // K half / row quad
   const int64_t qw0 = qt * kQPB + wave * kQPW;
   const int64_t r_begin = static_cast<int64_t>(seg) * seg_len;
   const int64_t r_end = min(n, r_begin + seg_len);
+  const int64_t ntiles = r_end > r_begin ? (r_end - r_begin + kRows - 1) / kRows : 0;
 
-  for (int e = threadIdx.x; e < kQPB * kMaxK; e += 256) (&topu[0][0])[e] = INFINITY;
-  for (int e = threadIdx.x; e < kQPB; e += 256) ccount[e] = 0u;
-  __syncthreads();
+  for (int e = tid; e < kQPB * k; e += kNT) topu[e] = INFINITY;
+  for (int e = tid; e < kQPB; e += kNT) ccount[e] = 0u;
 
-  bf16x8 bq[4][KT];
-  float thr[4], x2v[4];
-  bool qvalid[4];
+  bf16x8 bq[2][KS];
+  float thr[2], x2v[2], cxq[2];
+  bool qvalid[2];
 #pragma unroll
-  for (int cb = 0; cb < 4; ++cb) {
+  for (int cb = 0; cb < 2; ++cb) {
     const int64_t qi = qw0 + 32 * cb + col;
     // queries past the batch, and queries prep listed for the exact kernel
     // (x2m = inf), never produce candidates
     qvalid[cb] = qi < q && x2m[qi < q ? qi : 0] < INFINITY;
     const int64_t qr = qi < q ? qi : 0;
 #pragma unroll
-    for (int ks = 0; ks < KT; ++ks) bq[cb][ks] = load_frag(q_b + qr * LDB + 16 * ks + 8 * half);
+    for (int ks = 0; ks < KS; ++ks) bq[cb][ks] = load_frag(q_b + qr * S::LD + 16 * ks + 8 * half);
     x2v[cb] = qvalid[cb] ? x2m[qr] : INFINITY;
+    cxq[cb] = qi < q ? cxbq[qr] : 0.0f;
     thr[cb] = INFINITY;
   }
-  const float cxq[4] = {qw0 + col < q ? cxbq[qw0 + col] : 0.0f,
-                        qw0 + 32 + col < q ? cxbq[qw0 + 32 + col] : 0.0f,
-                        qw0 + 64 + col < q ? cxbq[qw0 + 64 + col] : 0.0f,
-                        qw0 + 96 + col < q ? cxbq[qw0 + 96 + col] : 0.0f};
 
-  auto load_tile = [&](bf16x8 (&a)[KT], int64_t t0) {
-    const int64_t arow = min<int64_t>(t0 + col, n - 1);
-#pragma unroll
-    for (int ks = 0; ks < KT; ++ks) a[ks] = load_frag(rows_b + arow * LDB + 16 * ks + 8 * half);
+  // tile loads: thread tid < VEC carries 16 bytes of rows; threads 0..63
+  // carry the tile's y2m (0..31) and nyb (32..63)
+  const int pr_row = tid / (S::LD / 8), pr_c8 = tid % (S::LD / 8);
+  auto gload = [&](int64_t t, u32x4& v, float& tv) {
+    const int64_t t0 = r_begin + t * kRows;
+    if (tid < S::VEC) {
+      const int64_t row = min<int64_t>(t0 + pr_row, n - 1);
+      v = *reinterpret_cast<const u32x4*>(rows_b + row * S::LD + 8 * pr_c8);
+    }
+    if (tid < 64) tv = (tid < 32 ? y2m : nyb)[t0 + (tid & 31)];
+  };
+  auto swrite = [&](int s, const u32x4& v, float tv) {
+    unsigned char* base = smem + s * S::SLOT;
+    if (tid < S::VEC)
+      *reinterpret_cast<u32x4*>(base + 2 * (pr_row * S::RS + 8 * pr_c8)) = v;
+    if (tid < 64) reinterpret_cast<float*>(base + 2 * kRows * S::RS)[tid] = tv;
   };
 
-  auto process_tile = [&](const bf16x8 (&a)[KT], int64_t t0) {
-    // four independent accumulation chains, interleaved so consecutive
-    // MFMAs never depend on each other; the norm step (ks = KS) goes last
-    f32x16 acc[4];
+  auto process = [&](int s, int64_t t) {
+    const unsigned char* base = smem + s * S::SLOT;
+    const uint16_t* A = reinterpret_cast<const uint16_t*>(base);
+    const float* ty2 = reinterpret_cast<const float*>(base + 2 * kRows * S::RS);
+    const float* tny = ty2 + kRows;
+    const int64_t t0 = r_begin + t * kRows;
+    f32x16 acc[2];
+    acc[0] = f32x16{};
+    acc[1] = f32x16{};
 #pragma unroll
-    for (int cb = 0; cb < 4; ++cb)
-      acc[cb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[0], bq[cb][0], f32x16{}, 0, 0, 0);
+    for (int ks = 0; ks < KS; ++ks) {
+      const bf16x8 a = load_frag(A + col * S::RS + 16 * ks + 8 * half);
+      acc[0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, bq[0][ks], acc[0], 0, 0, 0);
+      acc[1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, bq[1][ks], acc[1], 0, 0, 0);
+    }
+    // this lane's rows: 8 g + 4 half + r for v = 4 g + r
+    f32x4 yv[4], nv[4];
 #pragma unroll
-    for (int ks = 1; ks < KT; ++ks)
+    for (int g = 0; g < 4; ++g) {
+      yv[g] = *reinterpret_cast<const f32x4*>(ty2 + 8 * g + 4 * half);
+      nv[g] = *reinterpret_cast<const f32x4*>(tny + 8 * g + 4 * half);
+    }
 #pragma unroll
-      for (int cb = 0; cb < 4; ++cb)
-        acc[cb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[ks], bq[cb][ks], acc[cb], 0, 0, 0);
+    for (int cb = 0; cb < 2; ++cb) {
+      // acc <- L - x2m for query (32 cb + col) and this lane's 16 rows
+      float m = INFINITY;
 #pragma unroll
-    for (int cb = 0; cb < 4; ++cb) {
-      // acc = L - x2m for query (32 cb + col) and this lane's 16 rows
-      float m = min3f(acc[cb][0], acc[cb][1], acc[cb][2]);
-      m = min3f(m, acc[cb][3], acc[cb][4]);
-      m = min3f(m, acc[cb][5], acc[cb][6]);
-      m = min3f(m, acc[cb][7], acc[cb][8]);
-      m = min3f(m, acc[cb][9], acc[cb][10]);
-      m = min3f(m, acc[cb][11], acc[cb][12]);
-      m = min3f(m, acc[cb][13], acc[cb][14]);
-      m = fminf(m, acc[cb][15]);
+      for (int g = 0; g < 4; ++g)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const float v = fmaf(-cxq[cb], nv[g][r], acc[cb][4 * g + r]) + yv[g][r];
+          acc[cb][4 * g + r] = v;
+          m = fminf(m, v);
+        }
       const bool any = qvalid[cb] && m < thr[cb];
       if (__ballot(any) == 0) continue;
-      // Rare path: in row order, each lane holding row r for its query
-      // inserts U into the query's running top-k and appends (row, L).
-      const int ql = wave * kQPW + 32 * cb + col;
-      const int64_t qi = qw0 + 32 * cb + col;
-      for (int r = 0; r < kRows; ++r) {
-        const int h = (r >> 2) & 1;
-        const int v = (r >> 3) * 4 + (r & 3);
-        const int64_t row = t0 + r;
-        if (half == h && qvalid[cb] && row < r_end && acc[cb][v] < thr[cb]) {
-          const double val = acc[cb][v];
-          const double e = static_cast<double>(cxq[cb]) * static_cast<double>(nyb[row]);
-          const double L = static_cast<double>(x2v[cb]) + val;
-          const double U = L + 2.0 * e +
-                           (kHiMargin + 2.0 * kLoMargin) *
-                               (static_cast<double>(x2v[cb]) + static_cast<double>(y2m[row]));
-          float* tk = topu[ql];
+      // Rare path.  Both lanes of a query publish their 16 values; the
+      // half-0 lane walks the query's passing rows in row order, inserts U
+      // into the running top-k and appends (row, L).
+      uint32_t m16 = 0;
+#pragma unroll
+      for (int v = 0; v < 16; ++v) m16 |= static_cast<uint32_t>(qvalid[cb] && acc[cb][v] < thr[cb]) << v;
+      float* tw = trans + (wave * 64 + lane) * 16;
+#pragma unroll
+      for (int g = 0; g < 4; ++g)
+        *reinterpret_cast<f32x4*>(tw + 4 * g) =
+            f32x4{acc[cb][4 * g], acc[cb][4 * g + 1], acc[cb][4 * g + 2], acc[cb][4 * g + 3]};
+      const uint32_t pm = __shfl_xor(m16, 32);
+      uint32_t M = 0;
+#pragma unroll
+      for (int g = 0; g < 4; ++g) {
+        M |= ((m16 >> (4 * g)) & 0xFu) << (8 * g);
+        M |= ((pm >> (4 * g)) & 0xFu) << (8 * g + 4);
+      }
+      float tcur = thr[cb];
+      if (half == 0 && M != 0) {
+        const int ql = wave * kQPW + 32 * cb + col;
+        const int64_t qi = qw0 + 32 * cb + col;
+        float* tk = topu + ql * k;
+        uint32_t c = ccount[ql];
+        const double xv = static_cast<double>(x2v[cb]);
+        while (M != 0) {
+          const int r = __builtin_ctz(M);
+          M &= M - 1;
+          const int64_t row = t0 + r;
+          if (row >= r_end) break;  // rows ascend: the rest are past the segment
+          const int h = (r >> 2) & 1;
+          const int v = (r >> 3) * 4 + (r & 3);
+          const float val = trans[(wave * 64 + col + 32 * h) * 16 + v];
+          if (!(val < tcur)) continue;  // the threshold fell earlier in this tile
+          const double e = static_cast<double>(cxq[cb]) * static_cast<double>(tny[r]);
+          const double L = xv + static_cast<double>(val);
+          const double U = L + 2.0 * e + (kHiMargin + 2.0 * kLoMargin) * (xv + static_cast<double>(ty2[r]));
           const float uf = __double2float_ru(U);
           if (uf < tk[k - 1]) {
             int pos = k - 1;
@@ -297,35 +322,43 @@ __global__ __launch_bounds__(256, 1) void mixed_scan_kernel(
             }
             tk[pos] = uf;
           }
-          const uint32_t c = ccount[ql];
           if (c < static_cast<uint32_t>(cap)) {
             const size_t sl = (static_cast<size_t>(qi) * nseg + seg) * cap + c;
             cand_idx[sl] = static_cast<uint32_t>(row);
             cand_L[sl] = __double2float_rd(L);
           }
-          ccount[ql] = c + 1;
+          ++c;
+          // T - x2m rounded up (the f64 difference of two f32 values is exact)
+          tcur = __double2float_ru(static_cast<double>(tk[k - 1]) - xv);
         }
+        ccount[ql] = c;
       }
-      // T - x2m rounded up (the f64 difference of two f32 values is exact)
-      thr[cb] = __double2float_ru(static_cast<double>(topu[ql][k - 1]) -
-                                  static_cast<double>(x2v[cb]));
+      // both lanes of the query take the half-0 lane's threshold
+      const float tn = __shfl_xor(tcur, 32);
+      thr[cb] = half == 0 ? tcur : tn;
     }
   };
 
-  bf16x8 a0[KT], a1[KT];
-  int64_t t0 = r_begin;
-  if (t0 < r_end) load_tile(a0, t0);
-  while (t0 < r_end) {
-    if (t0 + kRows < r_end) load_tile(a1, t0 + kRows);
-    process_tile(a0, t0);
-    t0 += kRows;
-    if (t0 >= r_end) break;
-    if (t0 + kRows < r_end) load_tile(a0, t0 + kRows);
-    process_tile(a1, t0);
-    t0 += kRows;
-  }
+  u32x4 ra = {0u, 0u, 0u, 0u}, rb = {0u, 0u, 0u, 0u};
+  float ta = 0.0f, tb = 0.0f;
+  if (ntiles > 0) gload(0, ra, ta);
+  if (ntiles > 1) gload(1, rb, tb);
+  if (ntiles > 0) swrite(0, ra, ta);
   __syncthreads();
-  for (int e = threadIdx.x; e < kQPB; e += 256) {
+  for (int64_t t = 0; t < ntiles; t += 2) {
+    // tile t is in slot 0, tile t + 1 in rb
+    if (t + 2 < ntiles) gload(t + 2, ra, ta);
+    process(0, t);
+    if (t + 1 < ntiles) swrite(1, rb, tb);
+    __syncthreads();
+    if (t + 1 >= ntiles) break;
+    // tile t + 1 is in slot 1, tile t + 2 in ra
+    if (t + 3 < ntiles) gload(t + 3, rb, tb);
+    process(1, t + 1);
+    if (t + 2 < ntiles) swrite(0, ra, ta);
+    __syncthreads();
+  }
+  for (int e = tid; e < kQPB; e += kNT) {
     const int64_t qi = qt * kQPB + e;
     if (qi < q) cand_count[static_cast<size_t>(qi) * nseg + seg] = ccount[e];
   }
@@ -419,8 +452,8 @@ __global__ void mixed_replay_kernel(const float* __restrict__ rows, const float*
 }
 
 // Builds the BF16 copy and row terms of an f32 index once.  ldb (the BF16
-// row stride, ld + 16) is recorded in the index; 0 marks an index outside
-// the bound's domain.
+// row stride, ld) is recorded in the index; 0 marks an index outside the
+// bound's domain.
 int ensure_bf16(sc_index* ix, hipStream_t s, bool* bad_out, int64_t* npad_out) {
   std::lock_guard<std::mutex> lk(*static_cast<std::mutex*>(ix->mu));
   const int64_t n = static_cast<int64_t>(ix->n);
@@ -430,7 +463,7 @@ int ensure_bf16(sc_index* ix, hipStream_t s, bool* bad_out, int64_t* npad_out) {
     *bad_out = ix->ldb == 0;
     return SC_OK;
   }
-  const int ldb = static_cast<int>(ix->ld) + 16;
+  const int ldb = static_cast<int>(ix->ld);
   void* b = nullptr;
   float* terms = nullptr;
   SC_HIP(hipMalloc(&b, std::max<size_t>(1, n) * ldb * sizeof(uint16_t)));
@@ -438,11 +471,11 @@ int ensure_bf16(sc_index* ix, hipStream_t s, bool* bad_out, int64_t* npad_out) {
   DevBuf flag;
   SC_HIP(flag.alloc(4, s));
   SC_HIP(hipMemsetAsync(flag.p, 0, 4, s));
-  hipLaunchKernelGGL(fill_f32_kernel, dim3(static_cast<unsigned>((2 * npad + 255) / 256)), dim3(256),
-                     0, s, terms, 2 * npad, INFINITY);
+  hipLaunchKernelGGL(fill_terms_kernel, dim3(static_cast<unsigned>((npad - n + 255) / 256)), dim3(256),
+                     0, s, terms, terms + npad, n, npad);
   if (n > 0)
     hipLaunchKernelGGL(prep_rows_kernel, dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0,
-                       s, static_cast<const float*>(ix->rows), n, static_cast<int>(ix->ld), ldb,
+                       s, static_cast<const float*>(ix->rows), n, static_cast<int>(ix->ld),
                        static_cast<uint16_t*>(b), terms, terms + npad, flag.as<uint32_t>());
   SC_HIP(hipGetLastError());
   uint32_t bad = 0;
@@ -460,7 +493,15 @@ void launch_scan(dim3 grid, hipStream_t stream, const uint16_t* rows_b, const fl
                  const float* nyb, int64_t n, const uint16_t* q_b, const float* x2m,
                  const float* cxb, int64_t q, int k, int64_t seg_len, int nseg, int64_t nqt,
                  uint32_t* cidx, float* cL, uint32_t* ccnt, int cap) {
-  hipLaunchKernelGGL(mixed_scan_kernel<KS>, grid, dim3(256), 0, stream, rows_b, y2m, nyb, n, q_b,
+  const size_t lds = scan_lds_bytes<KS>(k);
+  static const bool attr = [] {
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&mixed_scan_kernel<KS>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize,
+                              static_cast<int>(scan_lds_bytes<KS>(kMaxK)));
+    return true;
+  }();
+  (void)attr;
+  hipLaunchKernelGGL(mixed_scan_kernel<KS>, grid, dim3(kNT), lds, stream, rows_b, y2m, nyb, n, q_b,
                      x2m, cxb, q, k, seg_len, nseg, nqt, cidx, cL, ccnt, cap);
 }
 
@@ -482,7 +523,6 @@ int knn_mixed_device(const sc_index* cix, const void* d_queries, size_t q_, size
                             stream, stats);
   const int64_t n = static_cast<int64_t>(ix->n);
   const int ld = static_cast<int>(ix->ld);
-  const int ldb = static_cast<int>(ix->ldb);
   const float* y2m = ix->norms2;
   const float* nyb = ix->norms2 + npad;
 
@@ -501,7 +541,7 @@ int knn_mixed_device(const sc_index* cix, const void* d_queries, size_t q_, size
   cap = std::min<int64_t>(cap, seg_len);
 
   DevBuf qb, x2, cx, cidx, cL, ccnt, hs, lst, flags;
-  SC_HIP(qb.alloc(sizeof(uint16_t) * q * ldb, stream));
+  SC_HIP(qb.alloc(sizeof(uint16_t) * q * ld, stream));
   SC_HIP(x2.alloc(sizeof(float) * q, stream));
   SC_HIP(cx.alloc(sizeof(float) * q, stream));
   SC_HIP(cidx.alloc(sizeof(uint32_t) * q * nseg * cap, stream));
@@ -516,7 +556,7 @@ int knn_mixed_device(const sc_index* cix, const void* d_queries, size_t q_, size
   const float* qf = static_cast<const float*>(d_queries);
 
   hipLaunchKernelGGL(prep_queries_kernel, dim3(static_cast<unsigned>((q + 255) / 256)), dim3(256),
-                     0, stream, qf, q, ld, ldb, qb.as<uint16_t>(), x2.as<float>(), cx.as<float>(),
+                     0, stream, qf, q, ld, qb.as<uint16_t>(), x2.as<float>(), cx.as<float>(),
                      lst.as<uint32_t>(), list_n);
   SC_HIP(hipGetLastError());
   {
