@@ -51,6 +51,18 @@ def cpu_model() -> str:
     return platform.processor()
 
 
+def measured_traffic(kernel: str):
+    """HBM bytes per launch of `kernel` from the last counter runs
+    (profiles/traffic.json: FETCH_SIZE and WRITE_SIZE scaled by a plain copy),
+    or None when no measurement is on file."""
+    try:
+        with open(os.path.join(ROOT, "profiles", "traffic.json")) as f:
+            t = json.load(f).get(kernel)
+    except (OSError, ValueError):
+        return None
+    return None if not t else t["bytes"]
+
+
 def profile_read(lib, name: str):
     ms = C.c_double()
     n = C.c_uint64()
@@ -156,7 +168,8 @@ def bench_knn(args):
         "config": {"workload": f"kNN {N:,}x{D} f32 rows, {Q:,} queries, k={K}, Euclidean",
                    "mode": args.mode},
         "roofline": {"bound": bound, "achieved": achieved / 1e12, "peak": peak / 1e12,
-                     "unit": unit_peak, "frac": achieved / peak, "traffic": None,
+                     "unit": unit_peak, "frac": achieved / peak,
+                     "traffic": measured_traffic(kname),
                      "kernel": kname, "avg_ms": avg_s * 1e3,
                      "work_per_launch": f"{flops_per_query:.4g} FLOP/query x {per_launch_q:.0f} queries"},
         "kernels": others,
@@ -345,7 +358,7 @@ def bench_kmeans(args):
         bytes_per_launch = n * d * 4 + 4 * n + 2 * k * d * 8
         achieved = bytes_per_launch / avg_s
         roof = {"bound": "hbm", "achieved": achieved / 1e9, "peak": PEAK_HBM / 1e9, "unit": "GB/s",
-                "frac": achieved / PEAK_HBM, "traffic": None}
+                "frac": achieved / PEAK_HBM, "traffic": measured_traffic(kname)}
     else:
         flops = 3.0 * n * k * d
         achieved = flops / avg_s

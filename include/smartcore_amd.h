@@ -157,11 +157,12 @@ int sc_kmeans_run(const sc_index* index, size_t k, size_t max_iter,
  * centroids.  Outputs are concatenated in problem order: centroids
  * (sum k_i*d_i), labels (sum n_i), sizes (sum k_i), one distortion and one
  * iteration count per problem.  SC_MIXED with f32 rows runs every problem's
- * certified MFMA Lloyd step in one persistent kernel per iteration, all
- * iterations captured in one hipGraph; each problem's results are
- * bit-identical to sc_kmeans_run(SC_MIXED) on it alone.  Other modes and
- * shapes outside the batched kernel's limits (k <= 128, d <= 128,
- * k*d <= 8192) run the problems one by one. */
+ * certified MFMA Lloyd step in one persistent launch per shape class per
+ * iteration, followed by two reduce launches over all problems; the
+ * iterations after the grid-size trials are replayed from one hipGraph.
+ * Each problem's results are bit-identical to sc_kmeans_run on it alone.
+ * Other modes and shapes outside the batched kernel's limits (k <= 128,
+ * d <= 128) run the problems one by one. */
 typedef struct {
   const void* rows;
   size_t n;

@@ -179,14 +179,23 @@ __global__ __launch_bounds__(kNT, 2) void mixed_scan_kernel(
     float* __restrict__ cand_L, uint32_t* __restrict__ cand_count, int cap) {
   using S = Scan<KS>;
   extern __shared__ __align__(16) unsigned char smem[];
-  // XCD-aware layout: block b runs on XCD b % 8; the XCDs with the same
-  // b % nseg stream the same segment with different query tiles.
+  // XCD-aware layout: block b runs on XCD b % 8.  Up to 8 segments, the
+  // XCDs with the same b % nseg stream the same segment with different
+  // query tiles; beyond 8, each XCD takes nseg / 8 segments of its own.
   const int64_t bid = blockIdx.x;
   const int xcd = static_cast<int>(bid % kXcds);
   const int64_t slot_i = bid / kXcds;
-  const int groups = kXcds / nseg;
-  const int seg = xcd % nseg;
-  const int64_t qt = slot_i * groups + xcd / nseg;
+  int seg;
+  int64_t qt;
+  if (nseg <= kXcds) {
+    const int groups = kXcds / nseg;
+    seg = xcd % nseg;
+    qt = slot_i * groups + xcd / nseg;
+  } else {
+    const int m = nseg / kXcds;
+    seg = static_cast<int>(slot_i % m) * kXcds + xcd;
+    qt = slot_i / m;
+  }
   if (qt >= nqt) return;  // whole workgroup leaves together
 
   float* topu = reinterpret_cast<float*>(smem + 2 * S::SLOT);  // per query: k smallest U
@@ -526,16 +535,17 @@ int knn_mixed_device(const sc_index* cix, const void* d_queries, size_t q_, size
   const float* y2m = ix->norms2;
   const float* nyb = ix->norms2 + npad;
 
-  // plan: ~256 workgroups (one per CU); segments in {1, 2, 4, 8} so that
-  // whole XCDs share a segment
+  // plan: ~256 workgroups (one per CU).  Segments are a power of two: up to
+  // 8 whole XCDs share a segment; small batches split the rows further so
+  // that they still fill the chip.
   const int64_t nqt = (q + kQPB - 1) / kQPB;
   const int64_t tiles = std::max<int64_t>(1, (n + kRows - 1) / kRows);
   int nseg = 1;
-  while (nseg < kXcds && nqt * nseg < 256 && nseg * 2 <= tiles) nseg *= 2;
+  while (nseg < 64 && nqt * nseg < 256 && nseg * 2 <= tiles) nseg *= 2;
   const int64_t seg_tiles = (tiles + nseg - 1) / nseg;
   const int64_t seg_len = seg_tiles * kRows;
-  const int groups = kXcds / nseg;
-  const int64_t slots = (nqt + groups - 1) / groups;
+  const int64_t nblocks =
+      nseg <= kXcds ? (nqt + kXcds / nseg - 1) / (kXcds / nseg) * kXcds : nqt * nseg;
   const double Lseg = static_cast<double>(std::min<int64_t>(seg_len, std::max<int64_t>(n, 1)));
   int64_t cap = static_cast<int64_t>(4.0 * k * (1.0 + std::log(std::max(1.0, Lseg / k)))) + 128;
   cap = std::min<int64_t>(cap, seg_len);
@@ -561,7 +571,7 @@ int knn_mixed_device(const sc_index* cix, const void* d_queries, size_t q_, size
   SC_HIP(hipGetLastError());
   {
     ProfScope ps("knn_mixed_gemm", stream);
-    const dim3 grid(static_cast<unsigned>(slots * kXcds));
+    const dim3 grid(static_cast<unsigned>(nblocks));
     const uint16_t* rb = static_cast<const uint16_t*>(ix->rows_bf16);
     switch (ld / 16) {
 #define SC_SCAN_CASE(KS)                                                                        \
