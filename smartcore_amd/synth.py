@@ -79,9 +79,9 @@ def lognormal_rows_device(seed: int, n: int, d: int, device="cuda"):
 def kmeans_batch_problems(seed: int = 4, count: int = 512, n_lo: int = 1_000, n_hi: int = 200_000,
                           dims=(8, 16, 32, 64), k_lo: int = 4, k_hi: int = 128, dtype=np.float32):
     """Uneven k-means problems: n ~ log-uniform[n_lo, n_hi], d from `dims`,
-    k ~ U{k_lo..k_hi}; rows are Gaussian blobs as in kmeans_blobs (k centres
-    ~ U(-10, 10)^d, per-blob std ~ U(0.5, 2.0), blob sizes from
-    Dirichlet(1)); initial centroids are k distinct rows chosen by the seed."""
+    k ~ U{k_lo..k_hi}; rows are Gaussian blobs as in knn_mixture (k centres
+    ~ N(0, 4^2 I), unit noise, centre drawn uniformly per row); initial
+    centroids are k distinct rows chosen by the seed."""
     rng = np.random.default_rng(seed)
     out = []
     for _ in range(count):
@@ -89,11 +89,9 @@ def kmeans_batch_problems(seed: int = 4, count: int = 512, n_lo: int = 1_000, n_
         d = int(rng.choice(dims))
         k = int(rng.integers(k_lo, k_hi + 1))
         k = min(k, n)
-        centres = rng.uniform(-10.0, 10.0, size=(k, d))
-        std = rng.uniform(0.5, 2.0, size=k)
-        lab = np.repeat(np.arange(k), rng.multinomial(n, rng.dirichlet(np.ones(k))))
-        rng.shuffle(lab)
-        x = (centres[lab] + std[lab, None] * rng.normal(size=(n, d))).astype(dtype)
+        centres = rng.normal(0.0, 4.0, size=(k, d))
+        lab = rng.integers(0, k, size=n)
+        x = (centres[lab] + rng.normal(size=(n, d))).astype(dtype)
         init = x[np.sort(rng.choice(n, size=k, replace=False))].astype(np.float64)
         out.append((x, init))
     return out
