@@ -186,6 +186,24 @@ int sc_kmeans_plus_plus(const sc_index* index, size_t k, int has_seed, uint64_t 
 int sc_kmeans_predict(const void* x, sc_dtype dtype, size_t n, size_t d,
                       const double* centroids, size_t k, uint32_t* labels);
 
+/* ---- DBSCAN (src/cluster/dbscan.rs) ---- */
+/* DBSCAN::fit (dbscan.rs:274-354) over the index's rows with find_radius
+ * (linear_search.rs:89-108) as its neighbourhood query: labels (n) get -1
+ * for outliers and the cluster id otherwise, num_classes the number of
+ * clusters.  The neighbourhoods of all rows are computed on the device in
+ * one batched radius search; the expansion loop runs on the host in the
+ * reference's order.  Errors: SC_FIT_FAILED "Invalid minPts"
+ * (min_samples < 1), "Invalid radius: " (eps <= 0). */
+int sc_dbscan_fit(const sc_index* index, double eps, size_t min_samples, sc_metric metric,
+                  uint16_t p, int32_t* labels, size_t* num_classes);
+/* DBSCAN::predict (dbscan.rs:358-397): for q host query rows (row-major,
+ * index dtype), the first most frequent fitted label among the index rows
+ * within eps (outliers pooled last); out gets that label + 1, or 0 when the
+ * outliers win.  Error: SC_FIND_FAILED "radius should be > 0". */
+int sc_dbscan_predict(const sc_index* index, const int32_t* labels, size_t num_classes,
+                      double eps, sc_metric metric, uint16_t p, const void* queries, size_t q,
+                      int32_t* out);
+
 /* ---- kNN serving loop (no reference counterpart: smartcore answers one
  * find() at a time; this serves LinearKNNSearch::find, linear_search.rs:52-84,
  * to concurrent callers) ----

@@ -617,6 +617,123 @@ class KMeans {
   std::vector<double> centroids_;
 };
 
+// DBSCANParameters (dbscan.rs:74-131, Default :245-255)
+template <typename T, typename D = Euclidian<T>>
+struct DBSCANParameters {
+  D distance{};
+  size_t min_samples = 5;
+  double eps = 0.5;
+  KNNAlgorithmName algorithm = KNNAlgorithmName::CoverTree;
+  template <typename DD>
+  DBSCANParameters<T, DD> with_distance(DD d) const {
+    return DBSCANParameters<T, DD>{std::move(d), min_samples, eps, algorithm};
+  }
+  DBSCANParameters with_min_samples(size_t m) const { auto p = *this; p.min_samples = m; return p; }
+  DBSCANParameters with_eps(double e) const { auto p = *this; p.eps = e; return p; }
+  DBSCANParameters with_algorithm(KNNAlgorithmName a) const { auto p = *this; p.algorithm = a; return p; }
+};
+
+// DBSCANSearchParameters and its grid iterator (dbscan.rs:133-229): the
+// distance varies fastest, then min_samples, eps and algorithm.
+template <typename T, typename D = Euclidian<T>>
+struct DBSCANSearchParameters {
+  std::vector<D> distance{D{}};
+  std::vector<size_t> min_samples{5};
+  std::vector<double> eps{0.5};
+  std::vector<KNNAlgorithmName> algorithm{KNNAlgorithmName::CoverTree};
+
+  std::vector<DBSCANParameters<T, D>> grid() const {
+    std::vector<DBSCANParameters<T, D>> out;
+    size_t cd = 0, cm = 0, ce = 0, ca = 0;
+    for (;;) {
+      if (cd == distance.size() && cm == min_samples.size() && ce == eps.size() &&
+          ca == algorithm.size())
+        break;
+      DBSCANParameters<T, D> next;
+      next.distance = distance[cd];
+      next.min_samples = min_samples[cm];
+      next.eps = eps[ce];
+      next.algorithm = algorithm[ca];
+      if (cd + 1 < distance.size()) {
+        cd += 1;
+      } else if (cm + 1 < min_samples.size()) {
+        cd = 0;
+        cm += 1;
+      } else if (ce + 1 < eps.size()) {
+        cd = 0;
+        cm = 0;
+        ce += 1;
+      } else if (ca + 1 < algorithm.size()) {
+        cd = 0;
+        cm = 0;
+        ce = 0;
+        ca += 1;
+      } else {
+        cd += 1;
+        cm += 1;
+        ce += 1;
+        ca += 1;
+      }
+      out.push_back(next);
+    }
+    return out;
+  }
+};
+
+// DBSCAN (dbscan.rs:59-397).  Neighbourhoods come from the device radius
+// search (the LinearSearch result; CoverTree is served by the same search);
+// the expansion loop runs in the reference's order inside sc_dbscan_fit.
+template <typename TX, typename TY = int32_t, typename D = Euclidian<TX>>
+class DBSCAN {
+ public:
+  using value_type = TX;
+  static DBSCAN fit(const DenseMatrix<TX>& x, const DBSCANParameters<TX, D>& parameters) {
+    const auto dm = parameters.distance.device_metric();
+    if (!dm) throw Failed::because(FailedError::FindFailed, "metric has no device kernel");
+    const auto sh = x.shape();
+    const auto rows = x.row_major();
+    sc_index* ix = nullptr;
+    detail::check(sc_index_create(rows.data(), detail::dtype_of<TX>(), sh.first, sh.second, &ix));
+    DBSCAN m;
+    m.index_.reset(ix, [](sc_index* p) { sc_index_destroy(p); });
+    m.labels_.resize(std::max<size_t>(sh.first, 1));
+    detail::check(sc_dbscan_fit(ix, parameters.eps, parameters.min_samples, dm->metric, dm->p,
+                                m.labels_.data(), &m.num_classes_));
+    m.labels_.resize(sh.first);
+    m.eps_ = parameters.eps;
+    m.metric_ = dm->metric;
+    m.p_ = dm->p;
+    m.dim_ = sh.second;
+    return m;
+  }
+
+  // dbscan.rs:358-397
+  std::vector<TY> predict(const DenseMatrix<TX>& x) const {
+    const auto sh = x.shape();
+    if (sh.second != dim_) throw Panic("Input vector sizes are different.");
+    const auto rows = x.row_major();
+    std::vector<int32_t> out(std::max<size_t>(sh.first, 1));
+    detail::check(sc_dbscan_predict(index_.get(), labels_.data(), num_classes_, eps_, metric_, p_,
+                                    rows.data(), sh.first, out.data()));
+    std::vector<TY> y(sh.first);
+    for (size_t i = 0; i < sh.first; ++i) y[i] = static_cast<TY>(out[i]);
+    return y;
+  }
+
+  const std::vector<int32_t>& cluster_labels() const { return labels_; }
+  size_t num_classes() const { return num_classes_; }
+  double eps() const { return eps_; }
+
+ private:
+  std::shared_ptr<sc_index> index_;
+  std::vector<int32_t> labels_;
+  size_t num_classes_ = 0;
+  double eps_ = 0.0;
+  sc_metric metric_ = SC_EUCLIDIAN;
+  uint16_t p_ = 2;
+  size_t dim_ = 0;
+};
+
 }  // namespace smartcore
 
 #endif

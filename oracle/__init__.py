@@ -76,6 +76,11 @@ def lib():
         L.or_lloyd_step_brute.restype = dbl
         L.or_lloyd_step_brute.argtypes = [vp, i32, sz, sz, vp, sz, vp, vp, vp]
         L.or_kmeans_predict.argtypes = [vp, i32, sz, sz, vp, sz, vp]
+        L.or_dbscan_fit.restype = i32
+        L.or_dbscan_fit.argtypes = [vp, i32, sz, sz, dbl, sz, i32, u32, vp,
+                                    C.POINTER(C.c_size_t)]
+        L.or_dbscan_predict.restype = i32
+        L.or_dbscan_predict.argtypes = [vp, i32, sz, sz, vp, sz, dbl, i32, u32, vp, sz, vp]
         _lib = L
     return _lib
 
@@ -292,3 +297,32 @@ def kmeans_predict(data: np.ndarray, centroids: np.ndarray) -> np.ndarray:
     lib().or_kmeans_predict(_p(data), _dt(data), n, d, _p(centroids), centroids.shape[0],
                             _p(labels))
     return labels
+
+
+def dbscan_fit(data: np.ndarray, eps: float, min_samples: int, metric: int = EUCLIDIAN,
+               p: int = 2):
+    """DBSCAN::fit (dbscan.rs:274-354): (cluster_labels int32, num_classes)."""
+    data = np.ascontiguousarray(data)
+    n, d = data.shape
+    labels = np.zeros(max(n, 1), np.int32)
+    nc = C.c_size_t(0)
+    rc = lib().or_dbscan_fit(_p(data), _dt(data), n, d, eps, min_samples, metric, p,
+                             _p(labels), C.byref(nc))
+    if rc:
+        raise OracleError(rc)
+    return labels[:n], nc.value
+
+
+def dbscan_predict(data: np.ndarray, labels: np.ndarray, num_classes: int, eps: float,
+                   queries: np.ndarray, metric: int = EUCLIDIAN, p: int = 2) -> np.ndarray:
+    """DBSCAN::predict (dbscan.rs:358-397)."""
+    data = np.ascontiguousarray(data)
+    queries = np.ascontiguousarray(queries, dtype=data.dtype)
+    labels = np.ascontiguousarray(labels, dtype=np.int32)
+    n, d = data.shape
+    out = np.zeros(max(queries.shape[0], 1), np.int32)
+    rc = lib().or_dbscan_predict(_p(data), _dt(data), n, d, _p(labels), num_classes, eps,
+                                 metric, p, _p(queries), queries.shape[0], _p(out))
+    if rc:
+        raise OracleError(rc)
+    return out[:queries.shape[0]]

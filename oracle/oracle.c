@@ -652,3 +652,109 @@ int or_kmeans_fit(const void* data, int dtype, size_t n, size_t d, size_t k,
   or_bbd_free(t);
   return 0;
 }
+
+/* ------------------------------------------------------------------ */
+/* DBSCAN                                                               */
+/* ------------------------------------------------------------------ */
+
+/* find_radius of row `i` into a fresh array (linear_search.rs:89-108). */
+static int64_t* radius_of(const void* data, int dtype, size_t n, size_t d, const void* q,
+                          double eps, int metric, unsigned p, size_t* cnt) {
+  size_t c = 0;
+  or_linear_find_radius(data, dtype, n, d, q, eps, metric, p, NULL, NULL, 0, &c);
+  int64_t* idx = (int64_t*)malloc(sizeof(int64_t) * (c ? c : 1));
+  double* dist = (double*)malloc(sizeof(double) * (c ? c : 1));
+  size_t c2 = 0;
+  or_linear_find_radius(data, dtype, n, d, q, eps, metric, p, idx, dist, c, &c2);
+  free(dist);
+  *cnt = c2;
+  return idx;
+}
+
+/* dbscan.rs:274-354 */
+int or_dbscan_fit(const void* data, int dtype, size_t n, size_t d, double eps,
+                  size_t min_samples, int metric, unsigned p, int32_t* y,
+                  size_t* num_classes) {
+  const int32_t queued = -2, outlier = -1, undefined = -3;
+  *num_classes = 0;
+  if (min_samples < 1) return 1;
+  if (eps <= 0.0) return 1;
+  int32_t k = 0;
+  for (size_t i = 0; i < n; ++i) y[i] = undefined;
+  for (size_t i = 0; i < n; ++i) {
+    if (y[i] != undefined) continue;
+    size_t nc = 0;
+    int64_t* nb = radius_of(data, dtype, n, d, row_ptr(data, dtype, d, i), eps, metric, p, &nc);
+    if (nc < min_samples) {
+      y[i] = outlier;
+      free(nb);
+      continue;
+    }
+    y[i] = k;
+    for (size_t j = 0; j < nc; ++j)
+      if (y[nb[j]] == undefined) y[nb[j]] = queued;
+    /* `neighbors` is used as a stack: pop from the end, push at the end */
+    size_t top = nc, cap = nc ? nc : 1;
+    int64_t* st = nb;
+    while (top > 0) {
+      const int64_t index = st[--top];
+      if (y[index] == outlier) y[index] = k;
+      if (y[index] == undefined || y[index] == queued) {
+        y[index] = k;
+        size_t sc = 0;
+        int64_t* sec = radius_of(data, dtype, n, d, row_ptr(data, dtype, d, (size_t)index), eps,
+                                 metric, p, &sc);
+        if (sc >= min_samples) {
+          for (size_t j = 0; j < sc; ++j) {
+            const int32_t label = y[sec[j]];
+            if (label == undefined) y[sec[j]] = queued;
+            if (label == undefined || label == outlier) {
+              if (top == cap) {
+                cap *= 2;
+                st = (int64_t*)realloc(st, sizeof(int64_t) * cap);
+              }
+              st[top++] = sec[j];
+            }
+          }
+        }
+        free(sec);
+      }
+    }
+    free(st);
+    k += 1;
+  }
+  *num_classes = (size_t)k;
+  return 0;
+}
+
+/* dbscan.rs:358-397 with which_max (decision_tree_classifier.rs:490-502):
+ * the first maximum wins. */
+int or_dbscan_predict(const void* data, int dtype, size_t n, size_t d,
+                      const int32_t* labels, size_t num_classes, double eps,
+                      int metric, unsigned p, const void* queries, size_t q,
+                      int32_t* out) {
+  if (eps <= 0.0) return 4;
+  size_t* hist = (size_t*)malloc(sizeof(size_t) * (num_classes + 1));
+  for (size_t i = 0; i < q; ++i) {
+    size_t nc = 0;
+    int64_t* nb = radius_of(data, dtype, n, d, row_ptr(queries, dtype, d, i), eps, metric, p, &nc);
+    for (size_t c = 0; c <= num_classes; ++c) hist[c] = 0;
+    for (size_t j = 0; j < nc; ++j) {
+      const int32_t yi = labels[nb[j]];
+      if (yi < 0)
+        hist[num_classes] += 1;
+      else
+        hist[yi] += 1;
+    }
+    size_t which = 0, m = hist[0];
+    for (size_t c = 1; c <= num_classes; ++c)
+      if (hist[c] > m) {
+        m = hist[c];
+        which = c;
+      }
+    out[i] = which != num_classes ? (int32_t)(which + 1) : 0;
+    free(nb);
+  }
+  free(hist);
+  return 0;
+}

@@ -132,6 +132,22 @@ double or_lloyd_step_brute(const void* data, int dtype, size_t n, size_t d,
 void or_kmeans_predict(const void* data, int dtype, size_t n, size_t d,
                        const double* centroids, size_t k, uint32_t* labels);
 
+/* ---- DBSCAN (src/cluster/dbscan.rs) ---- */
+/* DBSCAN::fit, dbscan.rs:274-354: cluster_labels (n; -1 outlier, else the
+ * cluster id) and the number of clusters, with find_radius on the rows in
+ * index order.  Returns 0, or 1 (FitFailed: min_samples < 1, eps <= 0). */
+int or_dbscan_fit(const void* data, int dtype, size_t n, size_t d, double eps,
+                  size_t min_samples, int metric, unsigned p, int32_t* labels,
+                  size_t* num_classes);
+/* DBSCAN::predict, dbscan.rs:358-397: per query the first most frequent
+ * label among the fitted rows within eps (outliers pooled in slot
+ * num_classes); result class + 1, or 0 for the outlier slot.  Returns 0 or
+ * 4 (FindFailed: eps <= 0). */
+int or_dbscan_predict(const void* data, int dtype, size_t n, size_t d,
+                      const int32_t* labels, size_t num_classes, double eps,
+                      int metric, unsigned p, const void* queries, size_t q,
+                      int32_t* out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -113,6 +113,8 @@ def lib():
     L.sc_kmeans_step.argtypes = [vp, vp, sz, vp, vp, vp, C.POINTER(dbl)]
     L.sc_kmeans_run.argtypes = [vp, sz, sz, i32, vp, vp, vp, C.POINTER(dbl), C.POINTER(C.c_uint32)]
     L.sc_kmeans_predict.argtypes = [vp, i32, sz, sz, vp, sz, vp]
+    L.sc_dbscan_fit.argtypes = [vp, dbl, sz, i32, u16, vp, C.POINTER(sz)]
+    L.sc_dbscan_predict.argtypes = [vp, vp, sz, dbl, i32, u16, vp, sz, vp]
     L.sc_last_error.argtypes = [C.c_char_p, sz]
     L.sc_last_error.restype = sz
     L.sc_last_stats.argtypes = [C.POINTER(sc_stats)]
@@ -238,6 +240,25 @@ class Index:
         finally:
             lib().sc_radius_result_free(r)
         return [(idx[off[i]:off[i + 1]].copy(), dist[off[i]:off[i + 1]].copy()) for i in range(q)]
+
+    def dbscan_fit(self, eps: float, min_samples: int, metric: int = SC_EUCLIDIAN, p: int = 2):
+        """sc_dbscan_fit: (cluster_labels int32[n], num_classes)."""
+        labels = np.empty(max(self.n, 1), np.int32)
+        nc = C.c_size_t()
+        check(lib().sc_dbscan_fit(self._h, float(eps), min_samples, metric, p, _ptr(labels),
+                                  C.byref(nc)))
+        return labels[: self.n], nc.value
+
+    def dbscan_predict(self, labels: np.ndarray, num_classes: int, eps: float,
+                       queries: np.ndarray, metric: int = SC_EUCLIDIAN, p: int = 2):
+        """sc_dbscan_predict: int32[q] (cluster + 1, or 0)."""
+        queries = np.ascontiguousarray(queries, dtype=self.dtype)
+        labels = np.ascontiguousarray(labels, dtype=np.int32)
+        q = queries.shape[0]
+        out = np.empty(max(q, 1), np.int32)
+        check(lib().sc_dbscan_predict(self._h, _ptr(labels), num_classes, float(eps), metric, p,
+                                      _ptr(queries), q, _ptr(out)))
+        return out[:q]
 
     def kmeans_step(self, centroids: np.ndarray):
         """sc_kmeans_step: (distortion, sums[k,d], counts[k], membership[n])."""

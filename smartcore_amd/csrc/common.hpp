@@ -7,8 +7,18 @@
 #include <cstdint>
 #include <cstdio>
 #include <string>
+#include <vector>
 
 #include "../../include/smartcore_amd.h"
+
+// Result of a radius search: per query, the matching rows in index order
+// (CSR offsets over the queries).
+struct sc_radius_result {
+  size_t q = 0;
+  std::vector<uint64_t> offsets;
+  std::vector<int64_t> idx;
+  std::vector<double> dist;
+};
 
 namespace sc {
 
@@ -123,5 +133,11 @@ int knn_exact_device(const sc_index* index, const void* d_queries, size_t q, siz
 int knn_mixed_device(const sc_index* index, const void* d_queries, size_t q, size_t k,
                      int64_t* d_idx, double* d_dist, uint32_t* d_count, hipStream_t stream,
                      sc_stats* stats);
+
+// find_radius (linear_search.rs:89-108) for q padded device queries (stride
+// index->ld), all at once: res gets each query's rows in index order.
+int radius_search_device(const sc_index* index, const void* d_queries, int64_t q,
+                         sc_metric metric, int p, double radius, sc_radius_result* res,
+                         hipStream_t stream);
 
 }  // namespace sc

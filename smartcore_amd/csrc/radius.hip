@@ -18,13 +18,6 @@
 #include "metric.hpp"
 #include "tile.hpp"
 
-struct sc_radius_result {
-  size_t q = 0;
-  std::vector<uint64_t> offsets;
-  std::vector<int64_t> idx;
-  std::vector<double> dist;
-};
-
 namespace sc {
 namespace {
 
@@ -174,6 +167,20 @@ int radius_dispatch(const sc_index* ix, const void* dq, int64_t q, sc_metric met
 }
 
 }  // namespace
+
+int radius_search_device(const sc_index* index, const void* d_queries, int64_t q,
+                         sc_metric metric, int p, double radius, sc_radius_result* res,
+                         hipStream_t s) {
+  res->q = static_cast<size_t>(q);
+  res->offsets.assign(static_cast<size_t>(q) + 1, 0);
+  res->idx.clear();
+  res->dist.clear();
+  if (q == 0 || index->n == 0) return SC_OK;
+  return index->dtype == SC_F64
+             ? radius_dispatch<double>(index, d_queries, q, metric, p, radius, res, s)
+             : radius_dispatch<float>(index, d_queries, q, metric, p, radius, res, s);
+}
+
 }  // namespace sc
 
 using namespace sc;
@@ -216,11 +223,8 @@ int sc_radius_search(const sc_index* index, const void* queries, size_t q, doubl
       return hip_fail(e, "alloc queries");
     }
     rc = upload_padded(queries, index->dtype, q, index->d, index->ld, dq.p, s);
-    if (rc == SC_OK) {
-      rc = index->dtype == SC_F64
-               ? radius_dispatch<double>(index, dq.p, static_cast<int64_t>(q), metric, p, radius, res, s)
-               : radius_dispatch<float>(index, dq.p, static_cast<int64_t>(q), metric, p, radius, res, s);
-    }
+    if (rc == SC_OK)
+      rc = radius_search_device(index, dq.p, static_cast<int64_t>(q), metric, p, radius, res, s);
   }
   if (rc != SC_OK) {
     delete res;

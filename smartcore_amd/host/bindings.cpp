@@ -95,6 +95,81 @@ class PyKNN {
   bool f32_;
 };
 
+struct PyDBSCANParams {
+  PyDistance distance{};
+  size_t min_samples = 5;
+  double eps = 0.5;
+  KNNAlgorithmName algorithm = KNNAlgorithmName::CoverTree;
+};
+
+class PyDBSCAN {
+ public:
+  using V = std::variant<DBSCAN<double, int32_t, Euclidian<double>>,
+                         DBSCAN<double, int32_t, Manhattan<double>>,
+                         DBSCAN<double, int32_t, Minkowski<double>>,
+                         DBSCAN<float, int32_t, Euclidian<float>>,
+                         DBSCAN<float, int32_t, Manhattan<float>>,
+                         DBSCAN<float, int32_t, Minkowski<float>>>;
+
+  static PyDBSCAN fit(py::array x, const PyDBSCANParams& p) {
+    if (is_f32(x)) return PyDBSCAN(fit_t<float>(x, p), true);
+    return PyDBSCAN(fit_t<double>(x, p), false);
+  }
+  py::array_t<int32_t> predict(py::array x) const {
+    std::vector<int32_t> out = std::visit(
+        [&](const auto& m) -> std::vector<int32_t> {
+          if (f32_) return predict_t<float>(m, x);
+          return predict_t<double>(m, x);
+        },
+        m_);
+    py::array_t<int32_t> r(out.size());
+    std::copy(out.begin(), out.end(), r.mutable_data());
+    return r;
+  }
+  py::array_t<int32_t> cluster_labels() const {
+    return std::visit(
+        [](const auto& m) {
+          const auto& v = m.cluster_labels();
+          py::array_t<int32_t> r(v.size());
+          std::copy(v.begin(), v.end(), r.mutable_data());
+          return r;
+        },
+        m_);
+  }
+  size_t num_classes() const {
+    return std::visit([](const auto& m) { return m.num_classes(); }, m_);
+  }
+
+ private:
+  PyDBSCAN(V m, bool f32) : m_(std::move(m)), f32_(f32) {}
+
+  template <typename T, typename M>
+  static std::vector<int32_t> predict_t(const M& m, py::array x) {
+    if constexpr (std::is_same<typename M::value_type, T>::value)
+      return m.predict(to_matrix<T>(py::array_t<T, py::array::c_style | py::array::forcecast>::ensure(x)));
+    else
+      return {};
+  }
+
+  template <typename T>
+  static V fit_t(py::array x, const PyDBSCANParams& p) {
+    auto mx = to_matrix<T>(py::array_t<T, py::array::c_style | py::array::forcecast>::ensure(x));
+    auto base = DBSCANParameters<T>{}.with_min_samples(p.min_samples).with_eps(p.eps)
+                    .with_algorithm(p.algorithm);
+    switch (p.distance.metric) {
+      case SC_EUCLIDIAN:
+        return DBSCAN<T, int32_t, Euclidian<T>>::fit(mx, base);
+      case SC_MANHATTAN:
+        return DBSCAN<T, int32_t, Manhattan<T>>::fit(mx, base.with_distance(Manhattan<T>{}));
+      default:
+        return DBSCAN<T, int32_t, Minkowski<T>>::fit(mx, base.with_distance(Minkowski<T>(p.distance.p)));
+    }
+  }
+
+  V m_;
+  bool f32_;
+};
+
 class PyKMeans {
  public:
   using V = std::variant<KMeans<double, uint32_t>, KMeans<float, uint32_t>>;
@@ -302,6 +377,34 @@ PYBIND11_MODULE(smartcore, m) {
       .def_property_readonly("centroids", &PyKMeans::centroids)
       .def_property_readonly("distortion", &PyKMeans::distortion)
       .def_property_readonly("iterations", &PyKMeans::iterations);
+
+  py::class_<PyDBSCANParams>(m, "DBSCANParameters")
+      .def(py::init<>())
+      .def_readwrite("distance", &PyDBSCANParams::distance)
+      .def_readwrite("min_samples", &PyDBSCANParams::min_samples)
+      .def_readwrite("eps", &PyDBSCANParams::eps)
+      .def_readwrite("algorithm", &PyDBSCANParams::algorithm)
+      .def("with_distance", [](PyDBSCANParams p, PyDistance d) { p.distance = d; return p; })
+      .def("with_min_samples", [](PyDBSCANParams p, size_t v) { p.min_samples = v; return p; })
+      .def("with_eps", [](PyDBSCANParams p, double v) { p.eps = v; return p; })
+      .def("with_algorithm", [](PyDBSCANParams p, KNNAlgorithmName a) { p.algorithm = a; return p; });
+  using DSP = DBSCANSearchParameters<double, PyDistance>;
+  py::class_<DSP>(m, "DBSCANSearchParameters")
+      .def(py::init<>())
+      .def_readwrite("distance", &DSP::distance)
+      .def_readwrite("min_samples", &DSP::min_samples)
+      .def_readwrite("eps", &DSP::eps)
+      .def_readwrite("algorithm", &DSP::algorithm)
+      .def("__iter__", [](const DSP& s) {
+        std::vector<PyDBSCANParams> out;
+        for (const auto& g : s.grid()) out.push_back(PyDBSCANParams{g.distance, g.min_samples, g.eps, g.algorithm});
+        return py::iter(py::cast(out));
+      });
+  py::class_<PyDBSCAN>(m, "DBSCAN")
+      .def_static("fit", &PyDBSCAN::fit, py::arg("x"), py::arg("parameters") = PyDBSCANParams{})
+      .def("predict", &PyDBSCAN::predict)
+      .def_property_readonly("cluster_labels", &PyDBSCAN::cluster_labels)
+      .def_property_readonly("num_classes", &PyDBSCAN::num_classes);
 
   py::class_<PyLinearKNNSearch>(m, "LinearKNNSearch")
       .def(py::init<py::array, const PyDistance&>(), py::arg("data"),

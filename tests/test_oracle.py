@@ -245,3 +245,67 @@ def test_means_from_labels_point_order():
         rows = x[y == j]
         np.testing.assert_array_equal(c[j], np.cumsum(rows, axis=0)[-1] / len(rows))
         assert size[j] == len(rows)
+
+
+def test_dbscan_fit_predict_kat():
+    """dbscan.rs:430-470 (fit_predict_dbscan)."""
+    x = np.array([[1.0, 2.0], [1.1, 2.1], [0.9, 1.9], [1.2, 2.2], [0.8, 1.8], [2.0, 1.0],
+                  [2.1, 1.1], [1.9, 0.9], [2.2, 1.2], [1.8, 0.8], [3.0, 5.0]])
+    lab, nc = O.dbscan_fit(x, 0.5, 2)
+    assert nc == 2
+    np.testing.assert_array_equal(O.dbscan_predict(x, lab, nc, 0.5, x),
+                                  [1, 1, 1, 1, 1, 2, 2, 2, 2, 2, 0])
+    with pytest.raises(O.OracleError):
+        O.dbscan_fit(x, 0.5, 0)
+    with pytest.raises(O.OracleError):
+        O.dbscan_fit(x, 0.0, 2)
+
+
+def _dbscan_by_components(s2, eps2, min_samples):
+    """DBSCAN labels restated as graph facts: core points (at least
+    min_samples rows within eps, itself included), clusters = connected
+    components of the core-core eps graph numbered in order of their first
+    core row, a non-core row takes the lowest cluster id among its core
+    neighbours or stays an outlier.  s2 holds exact squared distances."""
+    from scipy.sparse import csr_matrix
+    from scipy.sparse.csgraph import connected_components
+    adj = s2 <= eps2
+    core = adj.sum(1) >= min_samples
+    ci = np.flatnonzero(core)
+    _, comp = connected_components(csr_matrix(adj[np.ix_(ci, ci)]), directed=False)
+    first = {}
+    for row, c in zip(ci, comp):
+        first.setdefault(c, len(first))
+    lab = np.full(len(s2), -1)
+    lab[ci] = [first[c] for c in comp]
+    for i in np.flatnonzero(~core):
+        nb = ci[adj[i, ci]]
+        if len(nb):
+            lab[i] = lab[nb].min()
+    return lab, len(first)
+
+
+@pytest.mark.parametrize("eps,ms", [(18.0, 5), (22.0, 12), (25.0, 30)])
+def test_dbscan_equals_graph_restatement_on_digits(eps, ms):
+    """On the integer-valued digits every distance is exact, so the
+    reference's stack-driven expansion must equal the graph formulation."""
+    import os
+    g = np.load(os.path.join(os.path.dirname(__file__), "golden", "digits.npz"))
+    x = g["x"][:900].astype(np.float32)
+    xi = x.astype(np.int64)
+    s2 = ((xi[:, None, :] - xi[None, :, :]) ** 2).sum(-1)
+    lab, nc = O.dbscan_fit(x, eps, ms)
+    ref, rnc = _dbscan_by_components(s2, eps * eps, ms)
+    assert nc == rnc and nc >= 2
+    np.testing.assert_array_equal(lab, ref)
+    # predict: first most frequent label among rows within eps, outliers pooled last
+    q = g["x"][900:1000].astype(np.float32)
+    qs2 = ((q.astype(np.int64)[:, None, :] - xi[None, :, :]) ** 2).sum(-1)
+    want = []
+    for r in qs2:
+        h = np.zeros(nc + 1, int)
+        for j in np.flatnonzero(r <= eps * eps):
+            h[lab[j] if lab[j] >= 0 else nc] += 1
+        w = int(np.argmax(h))  # first maximum, as which_max
+        want.append(0 if w == nc else w + 1)
+    np.testing.assert_array_equal(O.dbscan_predict(x, lab, nc, eps, q), want)
