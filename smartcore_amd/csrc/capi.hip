@@ -62,10 +62,16 @@ int upload_padded(const void* host, sc_dtype dtype, size_t n, size_t d, size_t l
     SC_HIP(hipMemcpyAsync(dev, host, n * d * es, hipMemcpyHostToDevice, stream));
     return SC_OK;
   }
-  SC_HIP(hipMemsetAsync(dev, 0, n * ld * es, stream));
-  if (d > 0)
-    SC_HIP(hipMemcpy2DAsync(dev, ld * es, host, d * es, d * es, n, hipMemcpyHostToDevice, stream));
-  return SC_OK;
+  if (d == 0) {
+    SC_HIP(hipMemsetAsync(dev, 0, n * ld * es, stream));
+    return SC_OK;
+  }
+  // one contiguous transfer, then the padding on the device (a strided
+  // copy from pageable memory moves row by row)
+  DevBuf tmp;
+  SC_HIP(tmp.alloc(n * d * es, stream));
+  SC_HIP(hipMemcpyAsync(tmp.p, host, n * d * es, hipMemcpyHostToDevice, stream));
+  return copy_padded_device(tmp.p, dtype, n, d, ld, dev, stream);
 }
 
 int copy_padded_device(const void* src, sc_dtype dtype, size_t n, size_t d, size_t ld, void* dev,

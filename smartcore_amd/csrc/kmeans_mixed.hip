@@ -80,7 +80,8 @@ struct Cfg {
   static constexpr int PFV = CR * LD / 4 / NTH;          // f32x4 loads per thread per chunk
   // LDS: stage | lab | cnt | c2 | A table (hi | lo) | [f64 centroids]
   static constexpr size_t STAGE = sizeof(float) * CR * LDP;
-  static constexpr size_t ATAB = sizeof(uint16_t) * KP * LD;
+  static constexpr int LDA = LD + 8;                     // A-table row stride: 16 B of padding
+  static constexpr size_t ATAB = sizeof(uint16_t) * KP * LDA;
   static constexpr size_t FIXED = STAGE + sizeof(uint32_t) * CR + sizeof(uint32_t) * KP +
                                   sizeof(float) * KP + 2 * ATAB;
   static_assert(kBlockRows % CR == 0, "a block is a whole number of chunks");
@@ -143,8 +144,8 @@ __device__ __forceinline__ void km_block(const KmProb& p, int64_t b, unsigned ch
   uint32_t* lab = reinterpret_cast<uint32_t*>(smem + C::STAGE);
   uint32_t* cnt = lab + CR;
   float* c2s = reinterpret_cast<float*>(cnt + C::KP);
-  uint16_t* ahi = reinterpret_cast<uint16_t*>(c2s + C::KP);  // KP x LD
-  uint16_t* alo = ahi + C::KP * C::LD;
+  uint16_t* ahi = reinterpret_cast<uint16_t*>(c2s + C::KP);  // KP x LDA
+  uint16_t* alo = ahi + C::KP * C::LDA;
   double* ctab_s = reinterpret_cast<double*>(smem + C::FIXED);
   const double* ctab = ctab_lds ? ctab_s : p.cent;
 
@@ -158,13 +159,14 @@ __device__ __forceinline__ void km_block(const KmProb& p, int64_t b, unsigned ch
     const bool in = r < p.kpad && 8 * j8 < ld;
     const int64_t src = static_cast<int64_t>(r) * ld + 8 * j8;
     const u32x4 z = {0u, 0u, 0u, 0u};
-    *reinterpret_cast<u32x4*>(ahi + 8 * e) = in ? *reinterpret_cast<const u32x4*>(p.bhi + src) : z;
-    *reinterpret_cast<u32x4*>(alo + 8 * e) = in ? *reinterpret_cast<const u32x4*>(p.blo + src) : z;
+    const int dst = r * C::LDA + 8 * j8;
+    *reinterpret_cast<u32x4*>(ahi + dst) = in ? *reinterpret_cast<const u32x4*>(p.bhi + src) : z;
+    *reinterpret_cast<u32x4*>(alo + dst) = in ? *reinterpret_cast<const u32x4*>(p.blo + src) : z;
   }
   if (ctab_lds)
     for (int64_t e = tid; e < static_cast<int64_t>(k) * ld; e += NTH) ctab_s[e] = p.cent[e];
   auto frag = [&](const uint16_t* tab, int cb, int ks) -> bf16x8 {
-    return load_frag(tab + (32 * cb + col) * C::LD + 16 * ks + 8 * half);
+    return load_frag(tab + (32 * cb + col) * C::LDA + 16 * ks + 8 * half);
   };
 
   const float cmax = p.cmax[0];
