@@ -2,8 +2,9 @@
 // sc_kmeans_batch_run, SC_MIXED), fused with the centroid partial sums.
 //
 // A workgroup owns one block of kBlockRows rows and streams it through LDS
-// 128 rows at a time; the next chunk is loaded into registers while the
-// current one is processed, so each row is read from HBM once per step.
+// 256 rows at a time (128 for the widest rows); the next chunk is loaded
+// into registers while the current one is processed, so each row is read
+// from HBM once per step.
 //
 // Labels.  Each wave takes 32 rows of the chunk and splits them into BF16
 // hi/lo terms (x = xh + xl + r, |r| <= 2^-16 |x|).  The centroid tables hold
@@ -12,23 +13,28 @@
 // centroid, a_c ~ s_c - ||x||^2, where s_c is the f64 squared distance that
 // KMeans::predict computes (kmeans.rs:338-346: x widened to f64, differences
 // and squares in f64, summed in dimension order).  With
-//   c1 = 3.01 * 2^-16 + (3 ld + 2) * 2.04 * 2^-24
-// (dropped product and representation errors of both operands, FP32
-// accumulation over 3 ld products at two ulps each) and
+//   c1 = 3.01 * 2^-16 + (ld + 1 + 2 ld * 2^-7) * 2.04 * 2^-24
+// (dropped product and representation errors of both operands; FP32
+// accumulation at two ulps per add: ld adds at |x||c| in the hi x hi phase,
+// 2 ld adds at 2^-7 |x||c| in the lo phases, which run first) and
 //   E = 2 c1 ||x|| max||c|| + 2^-20 (||x||^2 + max||c||^2)
 // (the f32 norms, the final add and the f64 rounding of s_c), every c has
 // |a_c - (s_c - ||x||^2)| <= E.  A row whose minimum a is the only value
 // <= min a + 2E is certified: every other centroid has s_c strictly above
 // the winner's, so the winner is the exact argmin.  Any other centroid with
 // a_c <= min a + 2E is a candidate, and no non-candidate can be the argmin
-// or tie with it; an uncertified row is assigned by computing s_c exactly
-// for its candidates only and taking the lowest (value, index) -- the rule
-// of the FP64 assign kernel (strict < in index order).  Rows that are not
-// finite or whose norm is too large for the bound compute all k distances.
-// So the labels equal the FP64 form's on every input.
+// or tie with it.  An uncertified row is assigned among its candidates only:
+// the wave computes each candidate's s_c with lanes = dimensions and a fixed
+// tree (it differs from the sequential fold of the same terms by at most
+// (ld + 6) ulps), and a best value below the runner-up by more than twice
+// that is the exact argmin; near-exact ties take the reference's sequential
+// fold and the lowest (value, index) -- the rule of the FP64 assign kernel
+// (strict < in index order).  Rows that are not finite or whose norm is too
+// large for the bound compute all k distances.  So the labels equal the FP64
+// form's on every input.
 //
 // Sums.  After the labels, wave w adds the chunk's rows of its clusters
-// (c % 4 == w) into f64 registers (lane = dimension) in row order.  That is
+// (c % waves == w) into f64 registers (lane = dimension) in row order.  That is
 // the order of the FP64 form's partial kernel, so both forms write the same
 // block partials bit for bit, and the shared reduce kernels give the same
 // sums, centroids and distortions.
@@ -146,8 +152,15 @@ __device__ __forceinline__ void km_block(const KmProb& p, int64_t b, unsigned ch
   float* c2s = reinterpret_cast<float*>(cnt + C::KP);
   uint16_t* ahi = reinterpret_cast<uint16_t*>(c2s + C::KP);  // KP x LDA
   uint16_t* alo = ahi + C::KP * C::LDA;
+  // f64 centroids for the exact paths: the LDS copy when it fits, else the
+  // global array.  Each use branches on the (uniform) choice so that every
+  // load is a typed LDS or global load: a generic load would make the
+  // fallback wait for the chunk prefetch in flight.
   double* ctab_s = reinterpret_cast<double*>(smem + C::FIXED);
-  const double* ctab = ctab_lds ? ctab_s : p.cent;
+  auto exact_c = [&](const float* x, uint32_t c) -> double {
+    return ctab_lds ? exact_s(x, ctab_s + static_cast<int64_t>(c) * ld, ld)
+                    : exact_s(x, p.cent + static_cast<int64_t>(c) * ld, ld);
+  };
 
   // ---- per-block tables ----
   for (int e = tid; e < C::KP; e += NTH) {
@@ -341,12 +354,15 @@ __device__ __forceinline__ void km_block(const KmProb& p, int64_t b, unsigned ch
         const uint32_t c = slot_centroid(sl, L >> 5);
         if (c >= static_cast<uint32_t>(k)) continue;  // never: padding has a = +inf
         const float* xs = stage + (32 * wave + (L & 31)) * C::LDP;
-        const double* cc = ctab + static_cast<int64_t>(c) * ld;
+        const int64_t cofs = static_cast<int64_t>(c) * ld;
         double t = 0.0;
 #pragma unroll
         for (int dd = 0; dd < C::DPL; ++dd) {
           const int j = lane + 64 * dd;
-          if (j < ld) t = t + Term<double, kEuclid>::f(static_cast<double>(xs[j]), cc[j], 2.0);
+          if (j < ld) {
+            const double cj = ctab_lds ? ctab_s[cofs + j] : p.cent[cofs + j];
+            t = t + Term<double, kEuclid>::f(static_cast<double>(xs[j]), cj, 2.0);
+          }
         }
 #pragma unroll
         for (int off = 32; off >= 1; off >>= 1) t = t + __shfl_xor(t, off);
@@ -386,7 +402,7 @@ __device__ __forceinline__ void km_block(const KmProb& p, int64_t b, unsigned ch
         cm3 &= cm3 - 1;
         const uint32_t c = slot_centroid(sl, half);
         if (c < static_cast<uint32_t>(k))
-          lexmin_d(bs, bi, exact_s(xr, ctab + static_cast<int64_t>(c) * ld, ld), c);
+          lexmin_d(bs, bi, exact_c(xr, c), c);
       }
     }
     {
@@ -412,7 +428,7 @@ __device__ __forceinline__ void km_block(const KmProb& p, int64_t b, unsigned ch
       double bv = INFINITY;
       uint32_t bj = kUncert;
       for (int c = lane; c < k; c += 64)
-        lexmin_d(bv, bj, exact_s(x, ctab + static_cast<int64_t>(c) * ld, ld), static_cast<uint32_t>(c));
+        lexmin_d(bv, bj, exact_c(x, static_cast<uint32_t>(c)), static_cast<uint32_t>(c));
 #pragma unroll
       for (int off = 32; off >= 1; off >>= 1) {
         const double v2 = __shfl_xor(bv, off);

@@ -18,8 +18,11 @@
 // Pass 1 (scan, MFMA).  A workgroup of eight waves owns 512 queries (64 per
 // wave, held as BF16 B-fragments of -2x in registers; scaling by -2 is
 // exact) and one contiguous segment of rows, streamed 32 rows at a time
-// through a padded LDS double buffer that all eight waves read (each tile
-// is loaded into registers two tiles ahead).  v_mfma_f32_32x32x16_bf16 gives
+// through three padded LDS tile slots that all eight waves read (each tile
+// is loaded into registers two tiles ahead).  The loop is software
+// pipelined: a wave issues the MFMAs of tile t + 1 before the epilogue of
+// tile t, so the matrix pipe runs under the epilogue, the rare candidate
+// path and the tile store.  v_mfma_f32_32x32x16_bf16 gives
 // acc = -2C; the epilogue forms L - x2m = y2m - cxb nyb + acc with one FMA
 // and one add (y2m <= (1 - 2^-18) y2, nyb >= ||y||, cxb >= 2 c1 ||x||, all
 // rounded the safe way; the two roundings are inside the 2^-18 margin).
@@ -162,11 +165,15 @@ struct Scan {
   static_assert(VEC <= kNT, "one piece per thread");
 };
 
-// Dynamic LDS of the scan: two tile slots | topu[kQPB][k] | ccount[kQPB] |
-// trans[kWaves][64][16].
+// Row stride of the per-query top-k: 16 (one aligned register block) up to
+// k = 16, else k.
+__host__ __device__ inline int topk_stride(int k) { return k <= 16 ? 16 : k; }
+
+// Dynamic LDS of the scan: three tile slots | topu[kQPB][stride] |
+// ccount[kQPB] | trans[kWaves][64][16].
 template <int KS>
 size_t scan_lds_bytes(int k) {
-  return 2 * static_cast<size_t>(Scan<KS>::SLOT) + sizeof(float) * kQPB * k +
+  return 3 * static_cast<size_t>(Scan<KS>::SLOT) + sizeof(float) * kQPB * topk_stride(k) +
          sizeof(uint32_t) * kQPB + sizeof(float) * kWaves * 64 * 16;
 }
 
@@ -198,8 +205,9 @@ __global__ __launch_bounds__(kNT, 2) void mixed_scan_kernel(
   }
   if (qt >= nqt) return;  // whole workgroup leaves together
 
-  float* topu = reinterpret_cast<float*>(smem + 2 * S::SLOT);  // per query: k smallest U
-  uint32_t* ccount = reinterpret_cast<uint32_t*>(topu + kQPB * k);
+  const int kst = topk_stride(k);
+  float* topu = reinterpret_cast<float*>(smem + 3 * S::SLOT);  // per query: k smallest U
+  uint32_t* ccount = reinterpret_cast<uint32_t*>(topu + kQPB * kst);
   float* trans = reinterpret_cast<float*>(ccount + kQPB);      // slow-path transpose
   const int tid = threadIdx.x;
   const int lane = tid & 63;
@@ -211,7 +219,7 @@ __global__ __launch_bounds__(kNT, 2) void mixed_scan_kernel(
   const int64_t r_end = min(n, r_begin + seg_len);
   const int64_t ntiles = r_end > r_begin ? (r_end - r_begin + kRows - 1) / kRows : 0;
 
-  for (int e = tid; e < kQPB * k; e += kNT) topu[e] = INFINITY;
+  for (int e = tid; e < kQPB * kst; e += kNT) topu[e] = INFINITY;
   for (int e = tid; e < kQPB; e += kNT) ccount[e] = 0u;
 
   bf16x8 bq[2][KS];
@@ -249,13 +257,15 @@ __global__ __launch_bounds__(kNT, 2) void mixed_scan_kernel(
     if (tid < 64) reinterpret_cast<float*>(base + 2 * kRows * S::RS)[tid] = tv;
   };
 
-  auto process = [&](int s, int64_t t) {
-    const unsigned char* base = smem + s * S::SLOT;
-    const uint16_t* A = reinterpret_cast<const uint16_t*>(base);
-    const float* ty2 = reinterpret_cast<const float*>(base + 2 * kRows * S::RS);
-    const float* tny = ty2 + kRows;
-    const int64_t t0 = r_begin + t * kRows;
-    f32x16 acc[2];
+  // Software pipeline, one barrier per tile.  LDS holds three tiles: the
+  // one whose MFMAs are issued (t + 1), the one whose epilogue runs (t; its
+  // row terms are still needed) and the one being written (t + 2).  Each
+  // wave issues the MFMAs of tile t + 1 interleaved with the VALU epilogue
+  // of tile t, so the matrix pipe keeps working while the epilogue, the
+  // rare candidate path and the tile store run.
+  auto slot_ptr = [&](int64_t t) { return smem + static_cast<int>(t % 3) * S::SLOT; };
+  auto mfma_tile = [&](int64_t t, f32x16 (&acc)[2]) {
+    const uint16_t* A = reinterpret_cast<const uint16_t*>(slot_ptr(t));
     acc[0] = f32x16{};
     acc[1] = f32x16{};
 #pragma unroll
@@ -264,7 +274,13 @@ __global__ __launch_bounds__(kNT, 2) void mixed_scan_kernel(
       acc[0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, bq[0][ks], acc[0], 0, 0, 0);
       acc[1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, bq[1][ks], acc[1], 0, 0, 0);
     }
-    // this lane's rows: 8 g + 4 half + r for v = 4 g + r
+  };
+  // fast epilogue of tile t: acc <- L - x2m; returns per column block
+  // whether any lane of the wave has a row below its query's threshold
+  auto epilogue = [&](int64_t t, f32x16 (&acc)[2], bool (&hit)[2]) {
+    const unsigned char* base = slot_ptr(t);
+    const float* ty2 = reinterpret_cast<const float*>(base + 2 * kRows * S::RS);
+    const float* tny = ty2 + kRows;
     f32x4 yv[4], nv[4];
 #pragma unroll
     for (int g = 0; g < 4; ++g) {
@@ -273,7 +289,6 @@ __global__ __launch_bounds__(kNT, 2) void mixed_scan_kernel(
     }
 #pragma unroll
     for (int cb = 0; cb < 2; ++cb) {
-      // acc <- L - x2m for query (32 cb + col) and this lane's 16 rows
       float m = INFINITY;
 #pragma unroll
       for (int g = 0; g < 4; ++g)
@@ -283,46 +298,81 @@ __global__ __launch_bounds__(kNT, 2) void mixed_scan_kernel(
           acc[cb][4 * g + r] = v;
           m = fminf(m, v);
         }
-      const bool any = qvalid[cb] && m < thr[cb];
-      if (__ballot(any) == 0) continue;
-      // Rare path.  Both lanes of a query publish their 16 values; the
-      // half-0 lane walks the query's passing rows in row order, inserts U
-      // into the running top-k and appends (row, L).
-      uint32_t m16 = 0;
+      hit[cb] = __ballot(qvalid[cb] && m < thr[cb]) != 0;
+    }
+  };
+  // Rare path for column block cb of tile t (see the file comment).
+  auto slow = [&](int64_t t, int cb, f32x16& accv) {
+    const unsigned char* base = slot_ptr(t);
+    const float* ty2 = reinterpret_cast<const float*>(base + 2 * kRows * S::RS);
+    const float* tny = ty2 + kRows;
+    const int64_t t0 = r_begin + t * kRows;
+    uint32_t m16 = 0;
 #pragma unroll
-      for (int v = 0; v < 16; ++v) m16 |= static_cast<uint32_t>(qvalid[cb] && acc[cb][v] < thr[cb]) << v;
-      float* tw = trans + (wave * 64 + lane) * 16;
+    for (int v = 0; v < 16; ++v) m16 |= static_cast<uint32_t>(qvalid[cb] && accv[v] < thr[cb]) << v;
+    float* tw = trans + (wave * 64 + lane) * 16;
 #pragma unroll
-      for (int g = 0; g < 4; ++g)
-        *reinterpret_cast<f32x4*>(tw + 4 * g) =
-            f32x4{acc[cb][4 * g], acc[cb][4 * g + 1], acc[cb][4 * g + 2], acc[cb][4 * g + 3]};
-      const uint32_t pm = __shfl_xor(m16, 32);
-      uint32_t M = 0;
+    for (int g = 0; g < 4; ++g)
+      *reinterpret_cast<f32x4*>(tw + 4 * g) =
+          f32x4{accv[4 * g], accv[4 * g + 1], accv[4 * g + 2], accv[4 * g + 3]};
+    const uint32_t pm = __shfl_xor(m16, 32);
+    uint32_t M = 0;
 #pragma unroll
-      for (int g = 0; g < 4; ++g) {
-        M |= ((m16 >> (4 * g)) & 0xFu) << (8 * g);
-        M |= ((pm >> (4 * g)) & 0xFu) << (8 * g + 4);
+    for (int g = 0; g < 4; ++g) {
+      M |= ((m16 >> (4 * g)) & 0xFu) << (8 * g);
+      M |= ((pm >> (4 * g)) & 0xFu) << (8 * g + 4);
+    }
+    float tcur = thr[cb];
+    if (half == 0 && M != 0) {
+      const int ql = wave * kQPW + 32 * cb + col;
+      const int64_t qi = qw0 + 32 * cb + col;
+      float* tk = topu + ql * kst;
+      uint32_t c = ccount[ql];
+      const double xv = static_cast<double>(x2v[cb]);
+      // k <= 16: the sorted top-k lives in registers while this tile's rows
+      // are inserted (one LDS round trip instead of a dependent chain)
+      const bool vec = k <= 16;
+      float tv[16];
+      if (vec) {
+#pragma unroll
+        for (int g = 0; g < 4; ++g) {
+          const f32x4 q4 = *reinterpret_cast<const f32x4*>(tk + 4 * g);
+          tv[4 * g] = q4[0];
+          tv[4 * g + 1] = q4[1];
+          tv[4 * g + 2] = q4[2];
+          tv[4 * g + 3] = q4[3];
+        }
       }
-      float tcur = thr[cb];
-      if (half == 0 && M != 0) {
-        const int ql = wave * kQPW + 32 * cb + col;
-        const int64_t qi = qw0 + 32 * cb + col;
-        float* tk = topu + ql * k;
-        uint32_t c = ccount[ql];
-        const double xv = static_cast<double>(x2v[cb]);
-        while (M != 0) {
-          const int r = __builtin_ctz(M);
-          M &= M - 1;
-          const int64_t row = t0 + r;
-          if (row >= r_end) break;  // rows ascend: the rest are past the segment
-          const int h = (r >> 2) & 1;
-          const int v = (r >> 3) * 4 + (r & 3);
-          const float val = trans[(wave * 64 + col + 32 * h) * 16 + v];
-          if (!(val < tcur)) continue;  // the threshold fell earlier in this tile
-          const double e = static_cast<double>(cxq[cb]) * static_cast<double>(tny[r]);
-          const double L = xv + static_cast<double>(val);
-          const double U = L + 2.0 * e + (kHiMargin + 2.0 * kLoMargin) * (xv + static_cast<double>(ty2[r]));
-          const float uf = __double2float_ru(U);
+      while (M != 0) {
+        const int r = __builtin_ctz(M);
+        M &= M - 1;
+        const int64_t row = t0 + r;
+        if (row >= r_end) break;  // rows ascend: the rest are past the segment
+        const int h = (r >> 2) & 1;
+        const int v = (r >> 3) * 4 + (r & 3);
+        const float val = trans[(wave * 64 + col + 32 * h) * 16 + v];
+        if (!(val < tcur)) continue;  // the threshold fell earlier in this tile
+        const double e = static_cast<double>(cxq[cb]) * static_cast<double>(tny[r]);
+        const double L = xv + static_cast<double>(val);
+        const double U = L + 2.0 * e + (kHiMargin + 2.0 * kLoMargin) * (xv + static_cast<double>(ty2[r]));
+        const float uf = __double2float_ru(U);
+        float kth;
+        if (vec) {
+          // insert uf into the ascending tv[0..k): slot j keeps its value if
+          // it is <= uf, else takes uf or its left neighbour
+          float nv[16];
+#pragma unroll
+          for (int j = 0; j < 16; ++j) {
+            const float left = j ? tv[j - 1] : -INFINITY;
+            nv[j] = j >= k ? INFINITY : (tv[j] <= uf ? tv[j] : (left <= uf ? uf : left));
+          }
+          kth = INFINITY;
+#pragma unroll
+          for (int j = 0; j < 16; ++j) {
+            tv[j] = nv[j];
+            if (j == k - 1) kth = nv[j];
+          }
+        } else {
           if (uf < tk[k - 1]) {
             int pos = k - 1;
             while (pos > 0 && tk[pos - 1] > uf) {
@@ -331,41 +381,62 @@ __global__ __launch_bounds__(kNT, 2) void mixed_scan_kernel(
             }
             tk[pos] = uf;
           }
-          if (c < static_cast<uint32_t>(cap)) {
-            const size_t sl = (static_cast<size_t>(qi) * nseg + seg) * cap + c;
-            cand_idx[sl] = static_cast<uint32_t>(row);
-            cand_L[sl] = __double2float_rd(L);
-          }
-          ++c;
-          // T - x2m rounded up (the f64 difference of two f32 values is exact)
-          tcur = __double2float_ru(static_cast<double>(tk[k - 1]) - xv);
+          kth = tk[k - 1];
         }
-        ccount[ql] = c;
+        if (c < static_cast<uint32_t>(cap)) {
+          const size_t sl = (static_cast<size_t>(qi) * nseg + seg) * cap + c;
+          cand_idx[sl] = static_cast<uint32_t>(row);
+          cand_L[sl] = __double2float_rd(L);
+        }
+        ++c;
+        // T - x2m rounded up (the f64 difference of two f32 values is exact)
+        tcur = __double2float_ru(static_cast<double>(kth) - xv);
       }
-      // both lanes of the query take the half-0 lane's threshold
-      const float tn = __shfl_xor(tcur, 32);
-      thr[cb] = half == 0 ? tcur : tn;
+      if (vec) {
+#pragma unroll
+        for (int g = 0; g < 4; ++g)
+          *reinterpret_cast<f32x4*>(tk + 4 * g) =
+              f32x4{tv[4 * g], tv[4 * g + 1], tv[4 * g + 2], tv[4 * g + 3]};
+      }
+      ccount[ql] = c;
     }
+    // both lanes of the query take the half-0 lane's threshold
+    const float tn = __shfl_xor(tcur, 32);
+    thr[cb] = half == 0 ? tcur : tn;
+  };
+  // one pipeline step: MFMAs of tile t + 1 into nxt, epilogue of tile t in
+  // cur, then the rare path, the store of tile t + 2 and the barrier
+  auto step = [&](int64_t t, f32x16 (&cur)[2], f32x16 (&nxt)[2], u32x4& rv, float& tv) {
+    bool hit[2];
+    // tile t + 1 may be past the segment on the last step: its slot then
+    // holds stale rows whose results are never read
+    mfma_tile(t + 1, nxt);
+    epilogue(t, cur, hit);
+    if (hit[0]) slow(t, 0, cur[0]);
+    if (hit[1]) slow(t, 1, cur[1]);
+    if (t + 2 < ntiles) {
+      swrite(static_cast<int>((t + 2) % 3), rv, tv);
+      if (t + 4 < ntiles) gload(t + 4, rv, tv);
+    }
+    __syncthreads();
   };
 
   u32x4 ra = {0u, 0u, 0u, 0u}, rb = {0u, 0u, 0u, 0u};
   float ta = 0.0f, tb = 0.0f;
+  f32x16 accA[2], accB[2];
   if (ntiles > 0) gload(0, ra, ta);
   if (ntiles > 1) gload(1, rb, tb);
   if (ntiles > 0) swrite(0, ra, ta);
+  if (ntiles > 1) swrite(1, rb, tb);
+  if (ntiles > 2) gload(2, ra, ta);
+  if (ntiles > 3) gload(3, rb, tb);
   __syncthreads();
+  if (ntiles > 0) mfma_tile(0, accA);
   for (int64_t t = 0; t < ntiles; t += 2) {
-    // tile t is in slot 0, tile t + 1 in rb
-    if (t + 2 < ntiles) gload(t + 2, ra, ta);
-    process(0, t);
-    if (t + 1 < ntiles) swrite(1, rb, tb);
-    __syncthreads();
+    // tile t: MFMAs in accA, tile t + 2 in ra; tile t + 3 in rb
+    step(t, accA, accB, ra, ta);
     if (t + 1 >= ntiles) break;
-    // tile t + 1 is in slot 1, tile t + 2 in ra
-    if (t + 3 < ntiles) gload(t + 3, rb, tb);
-    process(1, t + 1);
-    if (t + 2 < ntiles) swrite(0, ra, ta);
-    __syncthreads();
+    step(t + 1, accB, accA, rb, tb);
   }
   for (int e = tid; e < kQPB; e += kNT) {
     const int64_t qi = qt * kQPB + e;
