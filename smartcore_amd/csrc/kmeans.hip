@@ -305,11 +305,15 @@ __device__ __forceinline__ uint16_t bf16_rne_bits(float f) {
   return static_cast<uint16_t>(u >> 16);
 }
 
-// BF16 split of -2c (c rounded to f32 first): hi = RN(-2c), lo = RN(-2c - hi).
-__device__ __forceinline__ void split_m2(double c, uint16_t& hi, uint16_t& lo) {
+// Three-term BF16 split of v = -2c (c rounded to f32 first): hi = RN(v),
+// mid = RN(v - hi), lo = RN(v - hi - mid).  Both differences are exact in
+// f32, so |v - hi - mid - lo| <= 2^-24 |v|.
+__device__ __forceinline__ void split_m2(double c, uint16_t& hi, uint16_t& mid, uint16_t& lo) {
   const float v = -2.0f * static_cast<float>(c);
   hi = bf16_rne_bits(v);
-  lo = bf16_rne_bits(v - __uint_as_float(static_cast<uint32_t>(hi) << 16));
+  const float e1 = v - __uint_as_float(static_cast<uint32_t>(hi) << 16);
+  mid = bf16_rne_bits(e1);
+  lo = bf16_rne_bits(e1 - __uint_as_float(static_cast<uint32_t>(mid) << 16));
 }
 
 // Largest finite ||c||^2 of a problem, rounded up, and its square root.
@@ -377,9 +381,10 @@ __global__ __launch_bounds__(128) void kmeans_final_kernel(const KmProb* __restr
       s_n2[tid] = nc * nc;
       if (upd) p.cent[static_cast<int64_t>(c) * p.ld + j] = nc;
       if (tables) {
-        uint16_t h, l;
-        split_m2(nc, h, l);
+        uint16_t h, m, l;
+        split_m2(nc, h, m, l);
         p.bhi[static_cast<int64_t>(c) * p.ld + j] = h;
+        p.bmd[static_cast<int64_t>(c) * p.ld + j] = m;
         p.blo[static_cast<int64_t>(c) * p.ld + j] = l;
       }
     }
@@ -430,9 +435,10 @@ __global__ __launch_bounds__(NT) void kmeans_tables_kernel(const KmProb* __restr
   if (p.bhi == nullptr) return;
   for (int64_t e = threadIdx.x; e < static_cast<int64_t>(p.kpad) * p.ld; e += NT) {
     const int c = static_cast<int>(e / p.ld), j = static_cast<int>(e % p.ld);
-    uint16_t h = 0, l = 0;
-    if (c < p.k && j < p.d) split_m2(p.cent[static_cast<int64_t>(c) * p.ld + j], h, l);
+    uint16_t h = 0, m = 0, l = 0;
+    if (c < p.k && j < p.d) split_m2(p.cent[static_cast<int64_t>(c) * p.ld + j], h, m, l);
     p.bhi[e] = h;
+    p.bmd[e] = m;
     p.blo[e] = l;
   }
   for (int c = threadIdx.x; c < p.kpad; c += NT) {
@@ -653,9 +659,10 @@ int KmWork::alloc(const sc_index* ix, int k, const double* d_cent, bool mixed, h
   SC_HIP(hipMemsetAsync(p.arrive, 0, 4, s));
   if (mixed) {
     const int64_t tab = static_cast<int64_t>(p.kpad) * p.ld;
-    SC_HIP(b_tab.alloc(sizeof(uint16_t) * 2 * tab + sizeof(float) * (p.kpad + 4), s));
+    SC_HIP(b_tab.alloc(sizeof(uint16_t) * 3 * tab + sizeof(float) * (p.kpad + 4), s));
     p.bhi = b_tab.as<uint16_t>();
-    p.blo = p.bhi + tab;
+    p.bmd = p.bhi + tab;
+    p.blo = p.bmd + tab;
     p.c2f = reinterpret_cast<float*>(p.blo + tab);
     p.cmax = p.c2f + p.kpad;
     p.c2d = p.term + k;

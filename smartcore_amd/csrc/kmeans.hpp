@@ -54,8 +54,9 @@ struct KmProb {
   KmState* st;
   uint32_t* arrive;     // arrival counter of the final reduce kernel
   // MFMA form only (null otherwise): the terms of the current centroids
-  uint16_t* bhi;        // kpad x ld, BF16 RN(-2 c)
-  uint16_t* blo;        // kpad x ld, BF16 RN(-2 c - bhi)
+  uint16_t* bhi;        // kpad x ld, BF16 RN(v), v = RN_f32(-2 c)
+  uint16_t* bmd;        // kpad x ld, BF16 RN(v - bhi)
+  uint16_t* blo;        // kpad x ld, BF16 RN(v - bhi - bmd)
   float* c2f;           // kpad, RN(||c||^2); +inf for padding rows
   double* c2d;          // k, ||c||^2 in f64
   float* cmax;          // RU(max ||c||), RU(max ||c||^2)

@@ -6,17 +6,19 @@
 // into registers while the current one is processed, so each row is read
 // from HBM once per step.
 //
-// Labels.  Each wave takes 32 rows of the chunk and splits them into BF16
-// hi/lo terms (x = xh + xl + r, |r| <= 2^-16 |x|).  The centroid tables hold
-// the same split of -2c.  Three v_mfma_f32_32x32x16_bf16 products per K step
-// (lo(c) hi(x), hi(c) lo(x), hi(c) hi(x)) plus ||c||^2 give, for every
-// centroid, a_c ~ s_c - ||x||^2, where s_c is the f64 squared distance that
-// KMeans::predict computes (kmeans.rs:338-346: x widened to f64, differences
-// and squares in f64, summed in dimension order).  With
-//   c1 = 3.01 * 2^-16 + (ld + 1 + 2 ld * 2^-7) * 2.04 * 2^-24
-// (dropped product and representation errors of both operands; FP32
-// accumulation at two ulps per add: ld adds at |x||c| in the hi x hi phase,
-// 2 ld adds at 2^-7 |x||c| in the lo phases, which run first) and
+// Labels.  Each wave takes 32 rows of the chunk and splits them into three
+// BF16 terms (x = xh + xm + xl + r, |r| <= 2^-24 |x|); the centroid tables
+// hold the same split of v = RN_f32(-2c).  Six v_mfma_f32_32x32x16_bf16
+// products per K step in three phases by weight, smallest first ((lo, hi),
+// (hi, lo), (mid, mid) at 2^-16; (mid, hi), (hi, mid) at 2^-8; (hi, hi))
+// plus ||c||^2 give, for every centroid, a_c ~ s_c - ||x||^2, where s_c is
+// the f64 squared distance that KMeans::predict computes
+// (kmeans.rs:338-346: x widened to f64, differences and squares in f64,
+// summed in dimension order).  With
+//   c1 = 5.3 * 2^-24 + (ld + 1 + 2.1 ld 2^-7 + 3.1 ld 2^-14) * 2.04 * 2^-24
+// (representation errors of both operands, the f32 rounding of -2c and the
+// dropped (mid, lo), (lo, mid), (lo, lo) products; FP32 accumulation at two
+// ulps per add at each phase's scale) and
 //   E = 2 c1 ||x|| max||c|| + 2^-20 (||x||^2 + max||c||^2)
 // (the f32 norms, the final add and the f64 rounding of s_c), every c has
 // |a_c - (s_c - ||x||^2)| <= E.  A row whose minimum a is the only value
@@ -29,9 +31,9 @@
 // (ld + 6) ulps), and a best value below the runner-up by more than twice
 // that is the exact argmin; near-exact ties take the reference's sequential
 // fold and the lowest (value, index) -- the rule of the FP64 assign kernel
-// (strict < in index order).  Rows that are not finite or whose norm is too
-// large for the bound compute all k distances.  So the labels equal the FP64
-// form's on every input.
+// (strict < in index order).  Rows that are not finite, or whose squared
+// norm is at least 1e30 or below 1e-30 (but not zero), compute all k
+// distances.  So the labels equal the FP64 form's on every input.
 //
 // Sums.  After the labels, wave w adds the chunk's rows of its clusters
 // (c % waves == w) into f64 registers (lane = dimension) in row order.  That is
@@ -66,6 +68,8 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
 
 constexpr int NT = 256;  // threads of the helper kernels
 constexpr uint32_t kUncert = 0xffffffffu;
@@ -84,12 +88,12 @@ struct Cfg {
   static constexpr int SLOTS = (KP + WAVES - 1) / WAVES; // clusters one wave owns
   static constexpr int DPL = (LD + 63) / 64;             // dimensions per lane
   static constexpr int PFV = CR * LD / 4 / NTH;          // f32x4 loads per thread per chunk
-  // LDS: stage | lab | cnt | c2 | A table (hi | lo) | [f64 centroids]
+  // LDS: stage | lab | cnt | c2 | A tables (hi | mid | lo) | [f64 centroids]
   static constexpr size_t STAGE = sizeof(float) * CR * LDP;
   static constexpr int LDA = LD + 8;                     // A-table row stride: 16 B of padding
   static constexpr size_t ATAB = sizeof(uint16_t) * KP * LDA;
   static constexpr size_t FIXED = STAGE + sizeof(uint32_t) * CR + sizeof(uint32_t) * KP +
-                                  sizeof(float) * KP + 2 * ATAB;
+                                  sizeof(float) * KP + 3 * ATAB;
   static_assert(kBlockRows % CR == 0, "a block is a whole number of chunks");
   static_assert(CR % 64 == 0, "chunks are whole ballots");
 };
@@ -151,7 +155,8 @@ __device__ __forceinline__ void km_block(const KmProb& p, int64_t b, unsigned ch
   uint32_t* cnt = lab + CR;
   float* c2s = reinterpret_cast<float*>(cnt + C::KP);
   uint16_t* ahi = reinterpret_cast<uint16_t*>(c2s + C::KP);  // KP x LDA
-  uint16_t* alo = ahi + C::KP * C::LDA;
+  uint16_t* amd = ahi + C::KP * C::LDA;
+  uint16_t* alo = amd + C::KP * C::LDA;
   // f64 centroids for the exact paths: the LDS copy when it fits, else the
   // global array.  Each use branches on the (uniform) choice so that every
   // load is a typed LDS or global load: a generic load would make the
@@ -174,6 +179,7 @@ __device__ __forceinline__ void km_block(const KmProb& p, int64_t b, unsigned ch
     const u32x4 z = {0u, 0u, 0u, 0u};
     const int dst = r * C::LDA + 8 * j8;
     *reinterpret_cast<u32x4*>(ahi + dst) = in ? *reinterpret_cast<const u32x4*>(p.bhi + src) : z;
+    *reinterpret_cast<u32x4*>(amd + dst) = in ? *reinterpret_cast<const u32x4*>(p.bmd + src) : z;
     *reinterpret_cast<u32x4*>(alo + dst) = in ? *reinterpret_cast<const u32x4*>(p.blo + src) : z;
   }
   if (ctab_lds)
@@ -184,19 +190,22 @@ __device__ __forceinline__ void km_block(const KmProb& p, int64_t b, unsigned ch
 
   const float cmax = p.cmax[0];
   const float c2max = p.cmax[1];
-  // cross-term error constant: representation and dropped-product errors
-  // (3 * 2^-16), FP32 accumulation at two ulps per add of the hi x hi phase
-  // (ld adds at |x||c|) and of the lo phases (2 ld adds at 2^-7 |x||c|)
-  const double c1 = 3.01 * 1.52587890625e-05 +
-                    (ld + 1.0 + 2.0 * ld * 0.0078125) * 2.04 * 5.9604644775390625e-08;
+  // cross-term error constant, relative to sum |x_i| |v_i| <= ||x|| ||v||:
+  // representation (both three-term residuals, the f32 rounding of -2c) and
+  // the dropped mid x lo, lo x mid, lo x lo products (5.3 * 2^-24), and FP32
+  // accumulation at two ulps per add: 3 ld adds at 2^-14 in the first phase,
+  // 2 ld adds at 1.02 * 2^-7 in the second, ld + 1 adds at 1.02 in the last
+  const double c1 = 5.3 * 5.9604644775390625e-08 +
+                    (ld + 1.0 + 2.1 * ld * 0.0078125 + 3.1 * ld * 6.103515625e-05) * 2.04 *
+                        5.9604644775390625e-08;
   // tree vs sequential f64 fold of the same ld non-negative terms
   const double dr = 2.0004 * (ld + 8.0) * 1.1102230246251565e-16;
 
-  double sacc[C::SLOTS][C::DPL];
+  // f64 sums of the wave's clusters, slot-major; indexed with a uniform
+  // slot (register-relative moves, no scratch)
+  double sacc[C::SLOTS * C::DPL];
 #pragma unroll
-  for (int s = 0; s < C::SLOTS; ++s)
-#pragma unroll
-    for (int dd = 0; dd < C::DPL; ++dd) sacc[s][dd] = 0.0;
+  for (int e = 0; e < C::SLOTS * C::DPL; ++e) sacc[e] = 0.0;
 
   const float* rows = static_cast<const float*>(p.rows);
   const int64_t r0 = b * kBlockRows;
@@ -231,56 +240,94 @@ __device__ __forceinline__ void km_block(const KmProb& p, int64_t b, unsigned ch
     if (c0 + CR < rows_here) load_chunk(c0 + CR);
 
     // ---- labels of this wave's 32 rows ----
+    // Three MFMA phases by term weight, smallest first: (lo(c) hi(x),
+    // hi(c) lo(x), mid(c) mid(x)) at 2^-16, (mid(c) hi(x), hi(c) mid(x)) at
+    // 2^-8, then hi(c) hi(x).  Each phase splits the staged row again, so no
+    // split is held across phases.
     const int pr = 32 * wave + col;
     const bool pvalid = pr < nr;
     const float* xr = stage + (pvalid ? pr : 0) * C::LDP;
-    bf16x8 xh[KSM], xl[KSM];
     float x2p = 0.0f;
     bool fin = true;
-#pragma unroll
-    for (int ks = 0; ks < KSM; ++ks) {
-      if (EXACT || ks < ks_n) {
-        const f32x4 v0 = *reinterpret_cast<const f32x4*>(xr + 16 * ks + 8 * half);
-        const f32x4 v1 = *reinterpret_cast<const f32x4*>(xr + 16 * ks + 8 * half + 4);
-        const float v[8] = {v0[0], v0[1], v0[2], v0[3], v1[0], v1[1], v1[2], v1[3]};
-        uint32_t h[8], l[8];
-#pragma unroll
-        for (int t = 0; t < 8; ++t) {
-          h[t] = bf16_bits(v[t]);
-          l[t] = bf16_bits(v[t] - __uint_as_float(h[t] << 16));
-          x2p = fmaf(v[t], v[t], x2p);
-          fin = fin && isfinite(v[t]);
-        }
-        xh[ks] = as_frag(u32x4{h[0] | (h[1] << 16), h[2] | (h[3] << 16), h[4] | (h[5] << 16),
-                               h[6] | (h[7] << 16)});
-        xl[ks] = as_frag(u32x4{l[0] | (l[1] << 16), l[2] | (l[3] << 16), l[4] | (l[5] << 16),
-                               l[6] | (l[7] << 16)});
-      } else {
-        xh[ks] = bf16x8{};
-        xl[ks] = bf16x8{};
-      }
-    }
-    const float x2 = x2p + __shfl_xor(x2p, 32);
-    fin = fin && (__shfl_xor(fin ? 1 : 0, 32) != 0);
-
-    // lo(c) x hi(x) and hi(c) x lo(x) first, hi(c) x hi(x) last
     f32x16 acc[KBM];
 #pragma unroll
     for (int cb = 0; cb < KBM; ++cb) acc[cb] = f32x16{};
+    // the parts of dimensions 16 ks + 8 half .. + 7 of the lane's row, two
+    // values per v_cvt_pk_bf16_f32 (round to nearest even); the differences
+    // v - hi and (v - hi) - mid are exact in f32
+    auto split_x = [&](int ks, int parts, bf16x8& xh, bf16x8& xm, bf16x8& xl, bool norms) {
+      const f32x4 v0 = *reinterpret_cast<const f32x4*>(xr + 16 * ks + 8 * half);
+      const f32x4 v1 = *reinterpret_cast<const f32x4*>(xr + 16 * ks + 8 * half + 4);
+      const float v[8] = {v0[0], v0[1], v0[2], v0[3], v1[0], v1[1], v1[2], v1[3]};
+      u32x4 H, M = {0u, 0u, 0u, 0u}, L = {0u, 0u, 0u, 0u};
 #pragma unroll
-    for (int ks = 0; ks < KSM; ++ks)
-#pragma unroll
-      for (int cb = 0; cb < KBM; ++cb)
-        if (EXACT || (ks < ks_n && cb < kb)) {
-          acc[cb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(frag(alo, cb, ks), xh[ks], acc[cb], 0, 0, 0);
-          acc[cb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(frag(ahi, cb, ks), xl[ks], acc[cb], 0, 0, 0);
+      for (int q = 0; q < 4; ++q) {
+        const f32x2 a = {v[2 * q], v[2 * q + 1]};
+        const uint32_t h = __builtin_bit_cast(uint32_t, __builtin_convertvector(a, bf16x2));
+        H[q] = h;
+        if (parts > 1) {
+          const f32x2 e1 = {a[0] - __uint_as_float(h << 16), a[1] - __uint_as_float(h & 0xffff0000u)};
+          const uint32_t m = __builtin_bit_cast(uint32_t, __builtin_convertvector(e1, bf16x2));
+          M[q] = m;
+          if (parts > 2) {
+            const f32x2 e2 = {e1[0] - __uint_as_float(m << 16),
+                              e1[1] - __uint_as_float(m & 0xffff0000u)};
+            L[q] = __builtin_bit_cast(uint32_t, __builtin_convertvector(e2, bf16x2));
+          }
         }
+      }
+      if (norms) {
+#pragma unroll
+        for (int t = 0; t < 8; ++t) {
+          x2p = fmaf(v[t], v[t], x2p);
+          fin = fin && isfinite(v[t]);
+        }
+      }
+      xh = as_frag(H);
+      xm = as_frag(M);
+      xl = as_frag(L);
+    };
 #pragma unroll
     for (int ks = 0; ks < KSM; ++ks)
+      if (EXACT || ks < ks_n) {
+        bf16x8 xh, xm, xl;
+        split_x(ks, 3, xh, xm, xl, true);
 #pragma unroll
-      for (int cb = 0; cb < KBM; ++cb)
-        if (EXACT || (ks < ks_n && cb < kb))
-          acc[cb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(frag(ahi, cb, ks), xh[ks], acc[cb], 0, 0, 0);
+        for (int cb = 0; cb < KBM; ++cb)
+          if (EXACT || cb < kb) {
+            acc[cb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(frag(alo, cb, ks), xh, acc[cb], 0, 0, 0);
+            acc[cb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(frag(ahi, cb, ks), xl, acc[cb], 0, 0, 0);
+            acc[cb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(frag(amd, cb, ks), xm, acc[cb], 0, 0, 0);
+          }
+      }
+    // a compiler-level memory barrier: the next phase loads its operands
+    // again instead of keeping the previous phase's in registers
+    asm volatile("" ::: "memory");
+#pragma unroll
+    for (int ks = 0; ks < KSM; ++ks)
+      if (EXACT || ks < ks_n) {
+        bf16x8 xh, xm, xl;
+        split_x(ks, 2, xh, xm, xl, false);
+#pragma unroll
+        for (int cb = 0; cb < KBM; ++cb)
+          if (EXACT || cb < kb) {
+            acc[cb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(frag(amd, cb, ks), xh, acc[cb], 0, 0, 0);
+            acc[cb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(frag(ahi, cb, ks), xm, acc[cb], 0, 0, 0);
+          }
+      }
+    asm volatile("" ::: "memory");
+#pragma unroll
+    for (int ks = 0; ks < KSM; ++ks)
+      if (EXACT || ks < ks_n) {
+        bf16x8 xh, xm, xl;
+        split_x(ks, 1, xh, xm, xl, false);
+#pragma unroll
+        for (int cb = 0; cb < KBM; ++cb)
+          if (EXACT || cb < kb)
+            acc[cb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(frag(ahi, cb, ks), xh, acc[cb], 0, 0, 0);
+      }
+    const float x2 = x2p + __shfl_xor(x2p, 32);
+    fin = fin && (__shfl_xor(fin ? 1 : 0, 32) != 0);
 
     // a_c = ||c||^2 + acc; lane (col, half) holds centroids
     // 32 cb + 8 g + 4 half + r for v = 4 g + r, in increasing index order
@@ -329,7 +376,9 @@ __device__ __forceinline__ void km_block(const KmProb& p, int64_t b, unsigned ch
     const int ncand = __popcll(cm) + __shfl_xor(__popcll(cm), 32);
     // a finite threshold keeps padding centroids (a = +inf) out of the
     // candidates; anything else takes the all-centroid path below
-    const bool sane = fin && x2 < 1e30f && amin < INFINITY && thr < INFINITY;
+    // rows with 0 < ||x||^2 < 1e-30 could lose split terms to underflow
+    const bool sane = fin && x2 < 1e30f && (x2 == 0.0f || x2 >= 1e-30f) && amin < INFINITY &&
+                      thr < INFINITY;
     const bool certified = sane && ncand == 1;
     const bool cand = pvalid && sane && !certified;
     if (!cand) cm = 0;
@@ -442,24 +491,71 @@ __device__ __forceinline__ void km_block(const KmProb& p, int64_t b, unsigned ch
     if (tid < nr) p.labels[r0 + c0 + tid] = lab[tid];
 
     // ---- sums: wave owns clusters WAVES s + wave, rows in order ----
-    uint32_t lrow[CR / 64];
-#pragma unroll
-    for (int g = 0; g < CR / 64; ++g) lrow[g] = lab[64 * g + lane];
-#pragma unroll
-    for (int s = 0; s < C::SLOTS; ++s) {
-      const uint32_t c = static_cast<uint32_t>(C::WAVES * s + wave);
-      if (c >= static_cast<uint32_t>(k)) break;
+    if constexpr (C::SLOTS * C::DPL <= 8) {
+      // Per 64-row group the owned rows are taken four at a time: their
+      // values are loaded together, then added in row order into the slot of
+      // their cluster (a uniform index).
 #pragma unroll
       for (int g = 0; g < CR / 64; ++g) {
-        uint64_t m = __ballot(lrow[g] == c);
-        if (lane == 0 && m) atomicAdd(&cnt[c], static_cast<uint32_t>(__popcll(m)));
-        while (m) {
-          const int i = 64 * g + __builtin_ctzll(m);
-          m &= m - 1;
+        const uint32_t lr = lab[64 * g + lane];
+        const bool own = lr < static_cast<uint32_t>(k) &&
+                         lr % static_cast<uint32_t>(C::WAVES) == static_cast<uint32_t>(wave);
+        if (own) atomicAdd(&cnt[lr], 1u);
+        uint64_t m = __ballot(own);
+        const float* sg = stage + 64 * g * C::LDP + lane;
+        while (m != 0) {
+          int ri[4];
+          uint32_t sl[4];
+          const int nb4 = min(4, __popcll(m));
 #pragma unroll
-          for (int dd = 0; dd < C::DPL; ++dd)
-            if (lane + 64 * dd < ld)
-              sacc[s][dd] = sacc[s][dd] + static_cast<double>(stage[i * C::LDP + lane + 64 * dd]);
+          for (int j = 0; j < 4; ++j) {
+            // m != 0 here for j < nb4; entries past nb4 repeat the first row
+            ri[j] = j < nb4 ? __builtin_ctzll(m) : ri[0];
+            if (j < nb4) m &= m - 1;
+            sl[j] = __builtin_amdgcn_readlane(lr, ri[j]) / C::WAVES;
+          }
+          // Entries past nb4 and lanes past ld add +0.0, which leaves a sum
+          // unchanged: sums start at +0.0 and never become -0.0 (x + (-x) and
+          // +0.0 + -0.0 round to +0.0), so the updates need no conditions.
+          float v[4][C::DPL];
+#pragma unroll
+          for (int j = 0; j < 4; ++j)
+#pragma unroll
+            for (int dd = 0; dd < C::DPL; ++dd) {
+              const float x = sg[ri[j] * C::LDP + 64 * dd];
+              v[j][dd] = (j < nb4 && lane + 64 * dd < ld) ? x : 0.0f;
+            }
+#pragma unroll
+          for (int j = 0; j < 4; ++j)
+#pragma unroll
+            for (int dd = 0; dd < C::DPL; ++dd) {
+              const uint32_t e = sl[j] * C::DPL + dd;
+              sacc[e] = sacc[e] + static_cast<double>(v[j][dd]);
+            }
+        }
+      }
+    } else {
+      // many slots: one ballot per (slot, group), static slot indices
+      uint32_t lrow[CR / 64];
+#pragma unroll
+      for (int g = 0; g < CR / 64; ++g) lrow[g] = lab[64 * g + lane];
+#pragma unroll
+      for (int s = 0; s < C::SLOTS; ++s) {
+        const uint32_t c = static_cast<uint32_t>(C::WAVES * s + wave);
+        if (c >= static_cast<uint32_t>(k)) break;
+#pragma unroll
+        for (int g = 0; g < CR / 64; ++g) {
+          uint64_t m = __ballot(lrow[g] == c);
+          if (lane == 0 && m) atomicAdd(&cnt[c], static_cast<uint32_t>(__popcll(m)));
+          while (m) {
+            const int i = 64 * g + __builtin_ctzll(m);
+            m &= m - 1;
+#pragma unroll
+            for (int dd = 0; dd < C::DPL; ++dd)
+              if (lane + 64 * dd < ld)
+                sacc[s * C::DPL + dd] =
+                    sacc[s * C::DPL + dd] + static_cast<double>(stage[i * C::LDP + lane + 64 * dd]);
+          }
         }
       }
     }
@@ -474,7 +570,7 @@ __device__ __forceinline__ void km_block(const KmProb& p, int64_t b, unsigned ch
 #pragma unroll
     for (int dd = 0; dd < C::DPL; ++dd) {
       const int dim = lane + 64 * dd;
-      if (dim < d) p.psum[b * kd + static_cast<int64_t>(c) * d + dim] = sacc[s][dd];
+      if (dim < d) p.psum[b * kd + static_cast<int64_t>(c) * d + dim] = sacc[s * C::DPL + dd];
     }
   }
   for (int c = tid; c < k; c += NTH) p.pcount[b * k + c] = cnt[c];
@@ -731,7 +827,7 @@ extern "C" int sc_kmeans_batch_run(const sc_kmeans_problem* problems, size_t cou
     o.tab = tot.tab;    tot.tab += static_cast<int64_t>(p.kpad) * p.ld;
     o.c2f = tot.c2f;    tot.c2f += p.kpad + 4;
   }
-  DevBuf bP, bRows, bCent, bLab, bPs, bPc, bGs, bGc, bSum, bCnt, bX2, bHi, bLo, bC2f, bSt;
+  DevBuf bP, bRows, bCent, bLab, bPs, bPc, bGs, bGc, bSum, bCnt, bX2, bHi, bMd, bLo, bC2f, bSt;
   SC_HIP(bP.alloc(sizeof(KmProb) * count, s));
   SC_HIP(bRows.alloc(sizeof(float) * tot.rows, s));
   SC_HIP(bCent.alloc(sizeof(double) * tot.cent, s));
@@ -744,6 +840,7 @@ extern "C" int sc_kmeans_batch_run(const sc_kmeans_problem* problems, size_t cou
   SC_HIP(bCnt.alloc(sizeof(uint64_t) * tot.cnt, s));
   SC_HIP(bX2.alloc(sizeof(double) * tot.x2, s));
   SC_HIP(bHi.alloc(sizeof(uint16_t) * tot.tab, s));
+  SC_HIP(bMd.alloc(sizeof(uint16_t) * tot.tab, s));
   SC_HIP(bLo.alloc(sizeof(uint16_t) * tot.tab, s));
   SC_HIP(bC2f.alloc(sizeof(float) * tot.c2f, s));
   // per problem: state (32 B, 8-aligned) then the arrival counter
@@ -767,6 +864,7 @@ extern "C" int sc_kmeans_batch_run(const sc_kmeans_problem* problems, size_t cou
     p.st = reinterpret_cast<KmState*>(bSt.as<char>() + st_stride * i);
     p.arrive = reinterpret_cast<uint32_t*>(bSt.as<char>() + st_stride * i + sizeof(KmState));
     p.bhi = bHi.as<uint16_t>() + o.tab;
+    p.bmd = bMd.as<uint16_t>() + o.tab;
     p.blo = bLo.as<uint16_t>() + o.tab;
     p.c2f = bC2f.as<float>() + o.c2f;
     p.cmax = p.c2f + p.kpad;

@@ -88,3 +88,16 @@ def test_mixed_on_f64_rows_serves_fp64_form():
     x, _, init = synth.kmeans_blobs(seed=42, n=10_000, d=16, blobs=8, k=8, dtype=np.float64)
     a, b, st = run_both(x, init, 10)
     assert_same(a, b)
+
+
+def test_mixed_tiny_and_mixed_scale_rows():
+    """Rows with squared norms below 1e-30 (whose BF16 split terms could
+    underflow) take the all-centroid path; rows of ordinary scale in the same
+    set keep the certified path.  Both must match the FP64 form."""
+    rng = np.random.default_rng(44)
+    x = rng.normal(size=(8_000, 32)).astype(np.float32)
+    x[:2_000] *= np.float32(1e-17)   # ||x||^2 ~ 3e-32
+    x[2_000:2_100] = 0.0             # exact zeros stay certified
+    init = np.concatenate([x[:3], x[4_000:9_000:1_000]]).astype(np.float64)
+    a, b, st = run_both(x, init, 6)
+    assert_same(a, b)
