@@ -15,14 +15,15 @@
 //   L = (1 - 2^-18)(x2 + y2) - 2C - 2 c1 ||x|| ||y||  <=  s  <=
 //   U = (1 + 2^-16)(x2 + y2) - 2C + 2 c1 ||x|| ||y||.
 //
-// Pass 1 (scan, MFMA).  A workgroup of eight waves owns 512 queries (64 per
+// Pass 1 (scan, MFMA).  A workgroup of four waves owns 256 queries (64 per
 // wave, held as BF16 B-fragments of -2x in registers; scaling by -2 is
 // exact) and one contiguous segment of rows, streamed 32 rows at a time
-// through three padded LDS tile slots that all eight waves read (each tile
-// is loaded into registers two tiles ahead).  The loop is software
-// pipelined: a wave issues the MFMAs of tile t + 1 before the epilogue of
-// tile t, so the matrix pipe runs under the epilogue, the rare candidate
-// path and the tile store.  v_mfma_f32_32x32x16_bf16 gives
+// through four padded LDS tile slots that all four waves read; two
+// workgroups share a CU, so one can issue MFMAs while the other waits at a
+// barrier.  Tiles are loaded into registers in pairs, two steps ahead.  The
+// loop is software pipelined: a wave issues the MFMAs of tile t + 1 before
+// the epilogue of tile t, so the matrix pipe runs under the epilogue, the
+// rare candidate path and the tile stores.  v_mfma_f32_32x32x16_bf16 gives
 // acc = -2C; the epilogue forms L - x2m = y2m - cxb nyb + acc with one FMA
 // and one add (y2m <= (1 - 2^-18) y2, nyb >= ||y||, cxb >= 2 c1 ||x||, all
 // rounded the safe way; the two roundings are inside the 2^-18 margin).
@@ -64,7 +65,7 @@ typedef float f32x16 __attribute__((ext_vector_type(16)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
-constexpr int kWaves = 8;                 // waves per workgroup (two per SIMD)
+constexpr int kWaves = 4;                 // waves per workgroup (one per SIMD)
 constexpr int kQPW = 64;                  // queries per wave (2 column blocks)
 constexpr int kQPB = kWaves * kQPW;       // queries per workgroup
 constexpr int kNT = 64 * kWaves;          // threads per workgroup
@@ -72,6 +73,9 @@ constexpr int kRows = 32;                 // rows per tile
 constexpr int kMaxKS = 8;                 // K steps of 16 -> dimension <= 128
 constexpr int kMaxK = 32;                 // largest k served by this form
 constexpr int kXcds = 8;                  // MI355X: 8 XCDs, round-robin dispatch
+constexpr int kTargetBlocks = 512;        // two workgroups per CU on 256 CUs
+constexpr int64_t kPrefixRows = 65536;    // rows of the seeding launch
+constexpr int64_t kSeedMinRows = 32 * kPrefixRows;  // indexes this large are seeded
 constexpr double kC1 = 0.0078125 + 1.52587890625e-05 + 128.0 * 2.384185791015625e-07;
 constexpr double kLoMargin = 3.814697265625e-06;   // 2^-18
 constexpr double kHiMargin = 1.52587890625e-05;    // 2^-16
@@ -161,30 +165,63 @@ struct Scan {
   static constexpr int LD = 16 * KS;                    // row length (bf16)
   static constexpr int RS = LD + 8;                     // LDS row stride: 16 B of padding
   static constexpr int VEC = kRows * LD / 8;            // 16-byte pieces of a tile
+  static constexpr int NP = (VEC + kNT - 1) / kNT;       // pieces per thread
   static constexpr int SLOT = kRows * RS * 2 + 2 * kRows * 4;  // rows | y2m | nyb (bytes)
-  static_assert(VEC <= kNT, "one piece per thread");
+  static_assert(NP <= 2, "at most two pieces per thread");
 };
 
-// Row stride of the per-query top-k: 16 (one aligned register block) up to
-// k = 16, else k.
-__host__ __device__ inline int topk_stride(int k) { return k <= 16 ? 16 : k; }
-
-// Dynamic LDS of the scan: three tile slots | topu[kQPB][stride] |
-// ccount[kQPB] | trans[kWaves][64][16].
+// Dynamic LDS of the scan: four tile slots | trans[kWaves][64][32].
+constexpr int kSlots = 4;
 template <int KS>
-size_t scan_lds_bytes(int k) {
-  return 3 * static_cast<size_t>(Scan<KS>::SLOT) + sizeof(float) * kQPB * topk_stride(k) +
-         sizeof(uint32_t) * kQPB + sizeof(float) * kWaves * 64 * 16;
+size_t scan_lds_bytes() {
+  return kSlots * static_cast<size_t>(Scan<KS>::SLOT) + sizeof(float) * kWaves * 64 * 32;
 }
 
-template <int KS>
-__global__ __launch_bounds__(kNT, 2) void mixed_scan_kernel(
-    const uint16_t* __restrict__ rows_b, const float* __restrict__ y2m,
-    const float* __restrict__ nyb, int64_t n, const uint16_t* __restrict__ q_b,
-    const float* __restrict__ x2m, const float* __restrict__ cxbq, int64_t q, int k,
-    int64_t seg_len, int nseg, int64_t nqt, uint32_t* __restrict__ cand_idx,
-    float* __restrict__ cand_L, uint32_t* __restrict__ cand_count, int cap) {
+// One scan launch covers rows [row0, row_end) in nseg segments of seg_len
+// rows; its segment s writes list s + seg_base of nseg_all per query.  With
+// seed, every query starts from those k U values (U bounds of k distinct
+// rows before row0) instead of an empty top-k; with topu_out, each
+// (query, segment) writes its final k smallest U there.
+struct ScanArgs {
+  const uint16_t* rows_b;
+  const float* y2m;
+  const float* nyb;
+  int64_t n;
+  const uint16_t* q_b;
+  const float* x2m;
+  const float* cxbq;
+  int64_t q;
+  int k;
+  int64_t row0, row_end, seg_len;
+  int nseg;
+  int64_t nqt;
+  int seg_base, nseg_all;
+  uint32_t* cand_idx;
+  float* cand_L;
+  uint32_t* cand_count;
+  int cap;
+  const float* seed;  // q x k ascending, or null
+  float* topu_out;    // q x nseg x k, or null
+};
+
+// KT: registers of the per-query top-k (16 for k <= 16, else 32).
+template <int KS, bool SEEDED, int KT>
+__global__ __launch_bounds__(kNT, 2) void mixed_scan_kernel(const ScanArgs a) {
   using S = Scan<KS>;
+  const uint16_t* __restrict__ rows_b = a.rows_b;
+  const float* __restrict__ y2m = a.y2m;
+  const float* __restrict__ nyb = a.nyb;
+  const int64_t n = a.n;
+  const uint16_t* __restrict__ q_b = a.q_b;
+  const float* __restrict__ x2m = a.x2m;
+  const float* __restrict__ cxbq = a.cxbq;
+  const int64_t q = a.q;
+  const int k = a.k;
+  const int nseg = a.nseg;
+  const int64_t nqt = a.nqt;
+  uint32_t* __restrict__ cand_idx = a.cand_idx;
+  float* __restrict__ cand_L = a.cand_L;
+  const int cap = a.cap;
   extern __shared__ __align__(16) unsigned char smem[];
   // XCD-aware layout: block b runs on XCD b % 8.  Up to 8 segments, the
   // XCDs with the same b % nseg stream the same segment with different
@@ -205,22 +242,18 @@ __global__ __launch_bounds__(kNT, 2) void mixed_scan_kernel(
   }
   if (qt >= nqt) return;  // whole workgroup leaves together
 
-  const int kst = topk_stride(k);
-  float* topu = reinterpret_cast<float*>(smem + 3 * S::SLOT);  // per query: k smallest U
-  uint32_t* ccount = reinterpret_cast<uint32_t*>(topu + kQPB * kst);
-  float* trans = reinterpret_cast<float*>(ccount + kQPB);      // slow-path transpose
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = tid >> 6;
   const int col = lane & 31;   // query column inside a 32-wide block / row of A
-  const int half = lane >> 5;  // K half / row quad
+  const int half = lane >> 5;  // K half / row quad; also the column block the lane owns
   const int64_t qw0 = qt * kQPB + wave * kQPW;
-  const int64_t r_begin = static_cast<int64_t>(seg) * seg_len;
-  const int64_t r_end = min(n, r_begin + seg_len);
+  const int64_t r_begin = a.row0 + static_cast<int64_t>(seg) * a.seg_len;
+  const int64_t r_end = min(a.row_end, r_begin + a.seg_len);
+  const int lseg = a.seg_base + seg;  // this segment's list
   const int64_t ntiles = r_end > r_begin ? (r_end - r_begin + kRows - 1) / kRows : 0;
-
-  for (int e = tid; e < kQPB * kst; e += kNT) topu[e] = INFINITY;
-  for (int e = tid; e < kQPB; e += kNT) ccount[e] = 0u;
+  // rare-path transpose: each lane's 32 values of a tile (both column blocks)
+  float* trans = reinterpret_cast<float*>(smem + kSlots * S::SLOT);
 
   bf16x8 bq[2][KS];
   float thr[2], x2v[2], cxq[2];
@@ -238,41 +271,85 @@ __global__ __launch_bounds__(kNT, 2) void mixed_scan_kernel(
     cxq[cb] = qi < q ? cxbq[qr] : 0.0f;
     thr[cb] = INFINITY;
   }
+  // The lane owns query (column block `half`, column col): it keeps that
+  // query's k smallest U in registers tk[0..k), ascending, with tk[k..KT)
+  // held at -inf (the insertion below leaves them there, so the k-th
+  // smallest is the maximum of tk and no register is indexed by k), its
+  // candidate count and its threshold.
+  const int64_t qo = qw0 + 32 * half + col;
+  const double xo = static_cast<double>(half == 0 ? x2v[0] : x2v[1]);
+  const double co = static_cast<double>(half == 0 ? cxq[0] : cxq[1]);
+  float tk[KT];
+#pragma unroll
+  for (int j = 0; j < KT; ++j)
+    tk[j] = j >= k ? -INFINITY : (SEEDED && qo < q ? a.seed[qo * k + j] : INFINITY);
+  auto kth_of = [&]() {
+    float m[KT];
+#pragma unroll
+    for (int j = 0; j < KT; ++j) m[j] = tk[j];
+#pragma unroll
+    for (int w = KT / 2; w >= 1; w /= 2)
+#pragma unroll
+      for (int j = 0; j < w; ++j) m[j] = fmaxf(m[j], m[j + w]);
+    return m[0];
+  };
+  uint32_t cnt = 0;
+  // both lanes of a column hold the thresholds of both column blocks
+  auto set_thr = [&](float own) {
+    const float other = __shfl_xor(own, 32);
+    thr[0] = half == 0 ? own : other;
+    thr[1] = half == 0 ? other : own;
+  };
+  if (SEEDED) {
+    const bool ov = half == 0 ? qvalid[0] : qvalid[1];
+    set_thr(ov ? __double2float_ru(static_cast<double>(kth_of()) - xo) : INFINITY);
+  }
 
-  // tile loads: thread tid < VEC carries 16 bytes of rows; threads 0..63
-  // carry the tile's y2m (0..31) and nyb (32..63)
-  const int pr_row = tid / (S::LD / 8), pr_c8 = tid % (S::LD / 8);
-  auto gload = [&](int64_t t, u32x4& v, float& tv) {
+  // tile loads: thread tid carries pieces tid and tid + kNT (16 bytes of
+  // rows each, when below VEC); threads 0..63 carry the tile's y2m (0..31)
+  // and nyb (32..63)
+  struct TileRegs {
+    u32x4 v[S::NP];
+    float tv;
+  };
+  auto gload = [&](int64_t t, TileRegs& r) {
     const int64_t t0 = r_begin + t * kRows;
-    if (tid < S::VEC) {
-      const int64_t row = min<int64_t>(t0 + pr_row, n - 1);
-      v = *reinterpret_cast<const u32x4*>(rows_b + row * S::LD + 8 * pr_c8);
+#pragma unroll
+    for (int pc = 0; pc < S::NP; ++pc) {
+      const int e = tid + kNT * pc;
+      if (e < S::VEC) {
+        const int64_t row = min<int64_t>(t0 + e / (S::LD / 8), n - 1);
+        r.v[pc] = *reinterpret_cast<const u32x4*>(rows_b + row * S::LD + 8 * (e % (S::LD / 8)));
+      }
     }
-    if (tid < 64) tv = (tid < 32 ? y2m : nyb)[t0 + (tid & 31)];
+    if (tid < 64) r.tv = (tid < 32 ? y2m : nyb)[t0 + (tid & 31)];
   };
-  auto swrite = [&](int s, const u32x4& v, float tv) {
-    unsigned char* base = smem + s * S::SLOT;
-    if (tid < S::VEC)
-      *reinterpret_cast<u32x4*>(base + 2 * (pr_row * S::RS + 8 * pr_c8)) = v;
-    if (tid < 64) reinterpret_cast<float*>(base + 2 * kRows * S::RS)[tid] = tv;
+  auto swrite = [&](int64_t t, const TileRegs& r) {
+    unsigned char* base = smem + static_cast<int>(t & (kSlots - 1)) * S::SLOT;
+#pragma unroll
+    for (int pc = 0; pc < S::NP; ++pc) {
+      const int e = tid + kNT * pc;
+      if (e < S::VEC)
+        *reinterpret_cast<u32x4*>(base + 2 * ((e / (S::LD / 8)) * S::RS + 8 * (e % (S::LD / 8)))) =
+            r.v[pc];
+    }
+    if (tid < 64) reinterpret_cast<float*>(base + 2 * kRows * S::RS)[tid] = r.tv;
   };
 
-  // Software pipeline, one barrier per tile.  LDS holds three tiles: the
-  // one whose MFMAs are issued (t + 1), the one whose epilogue runs (t; its
-  // row terms are still needed) and the one being written (t + 2).  Each
-  // wave issues the MFMAs of tile t + 1 interleaved with the VALU epilogue
-  // of tile t, so the matrix pipe keeps working while the epilogue, the
-  // rare candidate path and the tile store run.
-  auto slot_ptr = [&](int64_t t) { return smem + static_cast<int>(t % 3) * S::SLOT; };
+  // LDS holds four tiles: the one being scanned, the next one and two being
+  // written.  Tiles are stored and loaded in pairs on even steps: the
+  // stores of tiles t + 2, t + 3 wait for loads issued two steps earlier,
+  // never for the previous step's.
+  auto slot_ptr = [&](int64_t t) { return smem + static_cast<int>(t & (kSlots - 1)) * S::SLOT; };
   auto mfma_tile = [&](int64_t t, f32x16 (&acc)[2]) {
     const uint16_t* A = reinterpret_cast<const uint16_t*>(slot_ptr(t));
     acc[0] = f32x16{};
     acc[1] = f32x16{};
 #pragma unroll
     for (int ks = 0; ks < KS; ++ks) {
-      const bf16x8 a = load_frag(A + col * S::RS + 16 * ks + 8 * half);
-      acc[0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, bq[0][ks], acc[0], 0, 0, 0);
-      acc[1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, bq[1][ks], acc[1], 0, 0, 0);
+      const bf16x8 fa = load_frag(A + col * S::RS + 16 * ks + 8 * half);
+      acc[0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa, bq[0][ks], acc[0], 0, 0, 0);
+      acc[1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa, bq[1][ks], acc[1], 0, 0, 0);
     }
   };
   // fast epilogue of tile t: acc <- L - x2m; returns per column block
@@ -301,146 +378,120 @@ __global__ __launch_bounds__(kNT, 2) void mixed_scan_kernel(
       hit[cb] = __ballot(qvalid[cb] && m < thr[cb]) != 0;
     }
   };
-  // Rare path for column block cb of tile t (see the file comment).
-  auto slow = [&](int64_t t, int cb, f32x16& accv) {
+  // Rare path of tile t (see the file comment).  Each lane handles its own
+  // query: the rows of the tile below its threshold, in row order.
+  auto slow = [&](int64_t t, f32x16 (&acc)[2]) {
     const unsigned char* base = slot_ptr(t);
     const float* ty2 = reinterpret_cast<const float*>(base + 2 * kRows * S::RS);
     const float* tny = ty2 + kRows;
     const int64_t t0 = r_begin + t * kRows;
-    uint32_t m16 = 0;
+    uint32_t m16[2];
 #pragma unroll
-    for (int v = 0; v < 16; ++v) m16 |= static_cast<uint32_t>(qvalid[cb] && accv[v] < thr[cb]) << v;
-    float* tw = trans + (wave * 64 + lane) * 16;
+    for (int cb = 0; cb < 2; ++cb) {
+      m16[cb] = 0;
 #pragma unroll
-    for (int g = 0; g < 4; ++g)
-      *reinterpret_cast<f32x4*>(tw + 4 * g) =
-          f32x4{accv[4 * g], accv[4 * g + 1], accv[4 * g + 2], accv[4 * g + 3]};
-    const uint32_t pm = __shfl_xor(m16, 32);
+      for (int v = 0; v < 16; ++v)
+        m16[cb] |= static_cast<uint32_t>(qvalid[cb] && acc[cb][v] < thr[cb]) << v;
+    }
+    float* tw = trans + (wave * 64 + lane) * 32;
+#pragma unroll
+    for (int cb = 0; cb < 2; ++cb)
+#pragma unroll
+      for (int g = 0; g < 4; ++g)
+        *reinterpret_cast<f32x4*>(tw + 16 * cb + 4 * g) =
+            f32x4{acc[cb][4 * g], acc[cb][4 * g + 1], acc[cb][4 * g + 2], acc[cb][4 * g + 3]};
+    // lane value v of column block cb is row 8 (v / 4) + 4 half + v % 4:
+    // the own lane has the rows of the own block with that half, the
+    // partner lane the others
+    const uint32_t mo = half == 0 ? m16[0] : m16[1];
+    const uint32_t mp = __shfl_xor(half == 0 ? m16[1] : m16[0], 32);
     uint32_t M = 0;
 #pragma unroll
     for (int g = 0; g < 4; ++g) {
-      M |= ((m16 >> (4 * g)) & 0xFu) << (8 * g);
-      M |= ((pm >> (4 * g)) & 0xFu) << (8 * g + 4);
+      const uint32_t o4 = (mo >> (4 * g)) & 0xFu, p4 = (mp >> (4 * g)) & 0xFu;
+      M |= (half == 0 ? o4 : p4) << (8 * g);
+      M |= (half == 0 ? p4 : o4) << (8 * g + 4);
     }
-    float tcur = thr[cb];
-    if (half == 0 && M != 0) {
-      const int ql = wave * kQPW + 32 * cb + col;
-      const int64_t qi = qw0 + 32 * cb + col;
-      float* tk = topu + ql * kst;
-      uint32_t c = ccount[ql];
-      const double xv = static_cast<double>(x2v[cb]);
-      // k <= 16: the sorted top-k lives in registers while this tile's rows
-      // are inserted (one LDS round trip instead of a dependent chain)
-      const bool vec = k <= 16;
-      float tv[16];
-      if (vec) {
+    float tcur = half == 0 ? thr[0] : thr[1];
+    while (M != 0) {
+      const int r = __builtin_ctz(M);
+      M &= M - 1;
+      const int64_t row = t0 + r;
+      if (row >= r_end) break;  // rows ascend: the rest are past the segment
+      const int h = (r >> 2) & 1;
+      const int v = (r >> 3) * 4 + (r & 3);
+      const float val = trans[(wave * 64 + col + 32 * h) * 32 + 16 * half + v];
+      if (!(val < tcur)) continue;  // the threshold fell earlier in this tile
+      const double e = co * static_cast<double>(tny[r]);
+      const double L = xo + static_cast<double>(val);
+      const double U = L + 2.0 * e + (kHiMargin + 2.0 * kLoMargin) * (xo + static_cast<double>(ty2[r]));
+      const float uf = __double2float_ru(U);
+      // insert into the ascending tk[0..k): slot j takes
+      // min(tk[j], max(tk[j-1], uf)); slots past k stay -inf
 #pragma unroll
-        for (int g = 0; g < 4; ++g) {
-          const f32x4 q4 = *reinterpret_cast<const f32x4*>(tk + 4 * g);
-          tv[4 * g] = q4[0];
-          tv[4 * g + 1] = q4[1];
-          tv[4 * g + 2] = q4[2];
-          tv[4 * g + 3] = q4[3];
-        }
+      for (int j = KT - 1; j > 0; --j) tk[j] = fminf(tk[j], fmaxf(tk[j - 1], uf));
+      tk[0] = fminf(tk[0], uf);
+      if (cnt < static_cast<uint32_t>(cap)) {
+        const size_t sl = (static_cast<size_t>(qo) * a.nseg_all + lseg) * cap + cnt;
+        cand_idx[sl] = static_cast<uint32_t>(row);
+        cand_L[sl] = __double2float_rd(L);
       }
-      while (M != 0) {
-        const int r = __builtin_ctz(M);
-        M &= M - 1;
-        const int64_t row = t0 + r;
-        if (row >= r_end) break;  // rows ascend: the rest are past the segment
-        const int h = (r >> 2) & 1;
-        const int v = (r >> 3) * 4 + (r & 3);
-        const float val = trans[(wave * 64 + col + 32 * h) * 16 + v];
-        if (!(val < tcur)) continue;  // the threshold fell earlier in this tile
-        const double e = static_cast<double>(cxq[cb]) * static_cast<double>(tny[r]);
-        const double L = xv + static_cast<double>(val);
-        const double U = L + 2.0 * e + (kHiMargin + 2.0 * kLoMargin) * (xv + static_cast<double>(ty2[r]));
-        const float uf = __double2float_ru(U);
-        float kth;
-        if (vec) {
-          // insert uf into the ascending tv[0..k): slot j keeps its value if
-          // it is <= uf, else takes uf or its left neighbour
-          float nv[16];
-#pragma unroll
-          for (int j = 0; j < 16; ++j) {
-            const float left = j ? tv[j - 1] : -INFINITY;
-            nv[j] = j >= k ? INFINITY : (tv[j] <= uf ? tv[j] : (left <= uf ? uf : left));
-          }
-          kth = INFINITY;
-#pragma unroll
-          for (int j = 0; j < 16; ++j) {
-            tv[j] = nv[j];
-            if (j == k - 1) kth = nv[j];
-          }
-        } else {
-          if (uf < tk[k - 1]) {
-            int pos = k - 1;
-            while (pos > 0 && tk[pos - 1] > uf) {
-              tk[pos] = tk[pos - 1];
-              --pos;
-            }
-            tk[pos] = uf;
-          }
-          kth = tk[k - 1];
-        }
-        if (c < static_cast<uint32_t>(cap)) {
-          const size_t sl = (static_cast<size_t>(qi) * nseg + seg) * cap + c;
-          cand_idx[sl] = static_cast<uint32_t>(row);
-          cand_L[sl] = __double2float_rd(L);
-        }
-        ++c;
-        // T - x2m rounded up (the f64 difference of two f32 values is exact)
-        tcur = __double2float_ru(static_cast<double>(kth) - xv);
-      }
-      if (vec) {
-#pragma unroll
-        for (int g = 0; g < 4; ++g)
-          *reinterpret_cast<f32x4*>(tk + 4 * g) =
-              f32x4{tv[4 * g], tv[4 * g + 1], tv[4 * g + 2], tv[4 * g + 3]};
-      }
-      ccount[ql] = c;
+      ++cnt;
+      // T - x2m rounded up (the f64 difference of two f32 values is exact)
+      tcur = __double2float_ru(static_cast<double>(kth_of()) - xo);
     }
-    // both lanes of the query take the half-0 lane's threshold
-    const float tn = __shfl_xor(tcur, 32);
-    thr[cb] = half == 0 ? tcur : tn;
-  };
-  // one pipeline step: MFMAs of tile t + 1 into nxt, epilogue of tile t in
-  // cur, then the rare path, the store of tile t + 2 and the barrier
-  auto step = [&](int64_t t, f32x16 (&cur)[2], f32x16 (&nxt)[2], u32x4& rv, float& tv) {
-    bool hit[2];
-    // tile t + 1 may be past the segment on the last step: its slot then
-    // holds stale rows whose results are never read
-    mfma_tile(t + 1, nxt);
-    epilogue(t, cur, hit);
-    if (hit[0]) slow(t, 0, cur[0]);
-    if (hit[1]) slow(t, 1, cur[1]);
-    if (t + 2 < ntiles) {
-      swrite(static_cast<int>((t + 2) % 3), rv, tv);
-      if (t + 4 < ntiles) gload(t + 4, rv, tv);
-    }
-    __syncthreads();
+    set_thr(tcur);
   };
 
-  u32x4 ra = {0u, 0u, 0u, 0u}, rb = {0u, 0u, 0u, 0u};
-  float ta = 0.0f, tb = 0.0f;
-  f32x16 accA[2], accB[2];
-  if (ntiles > 0) gload(0, ra, ta);
-  if (ntiles > 1) gload(1, rb, tb);
-  if (ntiles > 0) swrite(0, ra, ta);
-  if (ntiles > 1) swrite(1, rb, tb);
-  if (ntiles > 2) gload(2, ra, ta);
-  if (ntiles > 3) gload(3, rb, tb);
+  TileRegs ra, rb;
+  f32x16 acc[2];
+  if (ntiles > 0) gload(0, ra);
+  if (ntiles > 1) gload(1, rb);
+  if (ntiles > 0) swrite(0, ra);
+  if (ntiles > 1) swrite(1, rb);
+  if (ntiles > 2) gload(2, ra);
+  if (ntiles > 3) gload(3, rb);
   __syncthreads();
-  if (ntiles > 0) mfma_tile(0, accA);
-  for (int64_t t = 0; t < ntiles; t += 2) {
-    // tile t: MFMAs in accA, tile t + 2 in ra; tile t + 3 in rb
-    step(t, accA, accB, ra, ta);
-    if (t + 1 >= ntiles) break;
-    step(t + 1, accB, accA, rb, tb);
+  for (int64_t t = 0; t < ntiles; ++t) {
+    mfma_tile(t, acc);
+    bool hit[2];
+    epilogue(t, acc, hit);
+    if (hit[0] || hit[1]) slow(t, acc);
+    if ((t & 1) == 0) {
+      if (t + 2 < ntiles) swrite(t + 2, ra);
+      if (t + 3 < ntiles) swrite(t + 3, rb);
+      if (t + 4 < ntiles) gload(t + 4, ra);
+      if (t + 5 < ntiles) gload(t + 5, rb);
+    }
+    __syncthreads();
   }
-  for (int e = tid; e < kQPB; e += kNT) {
-    const int64_t qi = qt * kQPB + e;
-    if (qi < q) cand_count[static_cast<size_t>(qi) * nseg + seg] = ccount[e];
+  if (qo < q) a.cand_count[static_cast<size_t>(qo) * a.nseg_all + lseg] = cnt;
+  if (a.topu_out != nullptr && qo < q) {
+#pragma unroll
+    for (int j = 0; j < KT; ++j)
+      if (j < k) a.topu_out[(static_cast<size_t>(qo) * nseg + seg) * k + j] = tk[j];
+  }
+}
+
+// The k smallest of a query's nseg ascending top-k lists (U bounds of
+// distinct rows), ascending: the seed of the segments after them.
+__global__ void merge_topu_kernel(const float* __restrict__ topu, int64_t q, int nseg, int k,
+                                  float* __restrict__ seed) {
+  const int64_t qi = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (qi >= q) return;
+  const float* t = topu + static_cast<size_t>(qi) * nseg * k;
+  int pos[64];
+  for (int s = 0; s < nseg; ++s) pos[s] = 0;
+  for (int j = 0; j < k; ++j) {
+    int best = -1;
+    float bv = INFINITY;
+    for (int s = 0; s < nseg; ++s)
+      if (pos[s] < k && t[s * k + pos[s]] < bv) {
+        bv = t[s * k + pos[s]];
+        best = s;
+      }
+    if (best >= 0) ++pos[best];
+    seed[qi * k + j] = bv;
   }
 }
 
@@ -568,21 +619,53 @@ int ensure_bf16(sc_index* ix, hipStream_t s, bool* bad_out, int64_t* npad_out) {
   return SC_OK;
 }
 
-template <int KS>
-void launch_scan(dim3 grid, hipStream_t stream, const uint16_t* rows_b, const float* y2m,
-                 const float* nyb, int64_t n, const uint16_t* q_b, const float* x2m,
-                 const float* cxb, int64_t q, int k, int64_t seg_len, int nseg, int64_t nqt,
-                 uint32_t* cidx, float* cL, uint32_t* ccnt, int cap) {
-  const size_t lds = scan_lds_bytes<KS>(k);
+template <int KS, bool SEEDED, int KT>
+void launch_scan_t(dim3 grid, hipStream_t stream, const ScanArgs& a) {
+  const size_t lds = scan_lds_bytes<KS>();
   static const bool attr = [] {
-    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&mixed_scan_kernel<KS>),
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&mixed_scan_kernel<KS, SEEDED, KT>),
                               hipFuncAttributeMaxDynamicSharedMemorySize,
-                              static_cast<int>(scan_lds_bytes<KS>(kMaxK)));
+                              static_cast<int>(scan_lds_bytes<KS>()));
     return true;
   }();
   (void)attr;
-  hipLaunchKernelGGL(mixed_scan_kernel<KS>, grid, dim3(kNT), lds, stream, rows_b, y2m, nyb, n, q_b,
-                     x2m, cxb, q, k, seg_len, nseg, nqt, cidx, cL, ccnt, cap);
+  hipLaunchKernelGGL((mixed_scan_kernel<KS, SEEDED, KT>), grid, dim3(kNT), lds, stream, a);
+}
+
+int launch_scan(int ks, bool seeded, const ScanArgs& a, hipStream_t stream) {
+  const int64_t nblocks = a.nseg <= kXcds
+                              ? (a.nqt + kXcds / a.nseg - 1) / (kXcds / a.nseg) * kXcds
+                              : a.nqt * a.nseg;
+  const dim3 grid(static_cast<unsigned>(nblocks));
+  switch (ks) {
+#define SC_SCAN_CASE(KS)                                      \
+  case KS:                                                   \
+    if (a.k <= 16) {                                         \
+      if (seeded)                                            \
+        launch_scan_t<KS, true, 16>(grid, stream, a);        \
+      else                                                   \
+        launch_scan_t<KS, false, 16>(grid, stream, a);       \
+    } else {                                                 \
+      if (seeded)                                            \
+        launch_scan_t<KS, true, 32>(grid, stream, a);        \
+      else                                                   \
+        launch_scan_t<KS, false, 32>(grid, stream, a);       \
+    }                                                        \
+    break;
+    SC_SCAN_CASE(1)
+    SC_SCAN_CASE(2)
+    SC_SCAN_CASE(3)
+    SC_SCAN_CASE(4)
+    SC_SCAN_CASE(5)
+    SC_SCAN_CASE(6)
+    SC_SCAN_CASE(7)
+    SC_SCAN_CASE(8)
+#undef SC_SCAN_CASE
+    default:
+      return fail(SC_INVALID_STATE, "unsupported row stride for the mixed form");
+  }
+  SC_HIP(hipGetLastError());
+  return SC_OK;
 }
 
 }  // namespace
@@ -606,28 +689,40 @@ int knn_mixed_device(const sc_index* cix, const void* d_queries, size_t q_, size
   const float* y2m = ix->norms2;
   const float* nyb = ix->norms2 + npad;
 
-  // plan: ~256 workgroups (one per CU).  Segments are a power of two: up to
-  // 8 whole XCDs share a segment; small batches split the rows further so
-  // that they still fill the chip.
+  // plan: ~kTargetBlocks workgroups (two per CU).  Segments are a power of
+  // two: up to 8 whole XCDs share a segment; small batches split the rows
+  // further so that they still fill the chip.
   const int64_t nqt = (q + kQPB - 1) / kQPB;
   const int64_t tiles = std::max<int64_t>(1, (n + kRows - 1) / kRows);
   int nseg = 1;
-  while (nseg < 64 && nqt * nseg < 256 && nseg * 2 <= tiles) nseg *= 2;
-  const int64_t seg_tiles = (tiles + nseg - 1) / nseg;
-  const int64_t seg_len = seg_tiles * kRows;
-  const int64_t nblocks =
-      nseg <= kXcds ? (nqt + kXcds / nseg - 1) / (kXcds / nseg) * kXcds : nqt * nseg;
+  while (nseg < 64 && nqt * nseg < kTargetBlocks && nseg * 2 <= tiles) nseg *= 2;
+  // Large indexes: a first launch scans the first kPrefixRows rows (in nseg
+  // segments) and the k smallest U per query over them seed every later
+  // segment, which then starts from a tight threshold instead of an empty
+  // top-k.
+  const bool seeded = n >= kSeedMinRows;
+  const int nseg_p = seeded ? nseg : 0;
+  const int64_t P = seeded ? kPrefixRows : 0;
+  const int64_t seg_len_p =
+      seeded ? (kPrefixRows / kRows + nseg_p - 1) / nseg_p * kRows : 0;
+  const int64_t mtiles = std::max<int64_t>(1, (n - P + kRows - 1) / kRows);
+  const int64_t seg_len = (mtiles + nseg - 1) / nseg * kRows;
+  const int nseg_all = nseg_p + nseg;
   const double Lseg = static_cast<double>(std::min<int64_t>(seg_len, std::max<int64_t>(n, 1)));
   int64_t cap = static_cast<int64_t>(4.0 * k * (1.0 + std::log(std::max(1.0, Lseg / k)))) + 128;
-  cap = std::min<int64_t>(cap, seg_len);
+  cap = std::min<int64_t>(cap, std::max(seg_len, seg_len_p));
 
-  DevBuf qb, x2, cx, cidx, cL, ccnt, hs, lst, flags;
+  DevBuf qb, x2, cx, cidx, cL, ccnt, hs, lst, flags, tpu, seed;
   SC_HIP(qb.alloc(sizeof(uint16_t) * q * ld, stream));
   SC_HIP(x2.alloc(sizeof(float) * q, stream));
   SC_HIP(cx.alloc(sizeof(float) * q, stream));
-  SC_HIP(cidx.alloc(sizeof(uint32_t) * q * nseg * cap, stream));
-  SC_HIP(cL.alloc(sizeof(float) * q * nseg * cap, stream));
-  SC_HIP(ccnt.alloc(sizeof(uint32_t) * q * nseg, stream));
+  SC_HIP(cidx.alloc(sizeof(uint32_t) * q * nseg_all * cap, stream));
+  SC_HIP(cL.alloc(sizeof(float) * q * nseg_all * cap, stream));
+  SC_HIP(ccnt.alloc(sizeof(uint32_t) * q * nseg_all, stream));
+  if (seeded) {
+    SC_HIP(tpu.alloc(sizeof(float) * q * nseg_p * k, stream));
+    SC_HIP(seed.alloc(sizeof(float) * q * k, stream));
+  }
   SC_HIP(hs.alloc(sizeof(double) * q * k, stream));
   SC_HIP(lst.alloc(sizeof(uint32_t) * q, stream));
   SC_HIP(flags.alloc(64, stream));
@@ -640,35 +735,51 @@ int knn_mixed_device(const sc_index* cix, const void* d_queries, size_t q_, size
                      0, stream, qf, q, ld, qb.as<uint16_t>(), x2.as<float>(), cx.as<float>(),
                      lst.as<uint32_t>(), list_n);
   SC_HIP(hipGetLastError());
+  ScanArgs sa{};
+  sa.rows_b = static_cast<const uint16_t*>(ix->rows_bf16);
+  sa.y2m = y2m;
+  sa.nyb = nyb;
+  sa.n = n;
+  sa.q_b = qb.as<uint16_t>();
+  sa.x2m = x2.as<float>();
+  sa.cxbq = cx.as<float>();
+  sa.q = q;
+  sa.k = k;
+  sa.nqt = nqt;
+  sa.nseg_all = nseg_all;
+  sa.cand_idx = cidx.as<uint32_t>();
+  sa.cand_L = cL.as<float>();
+  sa.cand_count = ccnt.as<uint32_t>();
+  sa.cap = static_cast<int>(cap);
+  if (seeded) {
+    ProfScope ps("knn_mixed_prefix", stream);
+    ScanArgs pa = sa;
+    pa.row0 = 0;
+    pa.row_end = P;
+    pa.seg_len = seg_len_p;
+    pa.nseg = nseg_p;
+    pa.seg_base = 0;
+    pa.topu_out = tpu.as<float>();
+    if (int rc = launch_scan(ld / 16, false, pa, stream)) return rc;
+    hipLaunchKernelGGL(merge_topu_kernel, dim3(static_cast<unsigned>((q + 255) / 256)), dim3(256), 0,
+                       stream, tpu.as<float>(), q, nseg_p, k, seed.as<float>());
+    SC_HIP(hipGetLastError());
+  }
   {
     ProfScope ps("knn_mixed_gemm", stream);
-    const dim3 grid(static_cast<unsigned>(nblocks));
-    const uint16_t* rb = static_cast<const uint16_t*>(ix->rows_bf16);
-    switch (ld / 16) {
-#define SC_SCAN_CASE(KS)                                                                        \
-  case KS:                                                                                     \
-    launch_scan<KS>(grid, stream, rb, y2m, nyb, n, qb.as<uint16_t>(), x2.as<float>(),          \
-                    cx.as<float>(), q, k, seg_len, nseg, nqt, cidx.as<uint32_t>(), cL.as<float>(), \
-                    ccnt.as<uint32_t>(), static_cast<int>(cap));                               \
-    break;
-      SC_SCAN_CASE(1)
-      SC_SCAN_CASE(2)
-      SC_SCAN_CASE(3)
-      SC_SCAN_CASE(4)
-      SC_SCAN_CASE(5)
-      SC_SCAN_CASE(6)
-      SC_SCAN_CASE(7)
-      SC_SCAN_CASE(8)
-#undef SC_SCAN_CASE
-      default:
-        return fail(SC_INVALID_STATE, "unsupported row stride for the mixed form");
-    }
-    SC_HIP(hipGetLastError());
+    ScanArgs ma = sa;
+    ma.row0 = P;
+    ma.row_end = n;
+    ma.seg_len = seg_len;
+    ma.nseg = nseg;
+    ma.seg_base = nseg_p;
+    ma.seed = seeded ? seed.as<float>() : nullptr;
+    if (int rc = launch_scan(ld / 16, seeded, ma, stream)) return rc;
   }
   {
     ProfScope ps("knn_mixed_rescore", stream);
     hipLaunchKernelGGL(mixed_replay_kernel, dim3(static_cast<unsigned>((q + 63) / 64)), dim3(64), 0,
-                       stream, static_cast<const float*>(ix->rows), qf, ld, q, k, nseg,
+                       stream, static_cast<const float*>(ix->rows), qf, ld, q, k, nseg_all,
                        static_cast<int>(cap), cidx.as<uint32_t>(), cL.as<float>(),
                        ccnt.as<uint32_t>(), x2.as<float>(), hs.as<double>(), d_idx, d_dist, d_count,
                        lst.as<uint32_t>(), list_n, counters);

@@ -82,3 +82,30 @@ def test_mixed_rejects_other_metrics():
     ix = S.Index(np.ones((4, 3), np.float32))
     with pytest.raises(S.Failed):
         ix.knn(np.ones((1, 3), np.float32), 2, S.SC_MANHATTAN, 1, S.SC_MIXED)
+
+
+@pytest.mark.parametrize("d,k,kind", [(16, 10, "lognormal"), (48, 32, "ties")])
+def test_mixed_seeded_large_index(d, k, kind):
+    """Indexes of 2,097,152 rows or more scan a 65,536-row prefix first and
+    seed the later segments with its k smallest U per query.  The result must
+    still equal the FP64 form bit for bit (and the oracle on a few queries),
+    also with heavy exact ties."""
+    n, nq = 2_200_000, 3_000
+    if kind == "lognormal":
+        ref = synth.lognormal_rows(90 + d, n, d)
+        qs = synth.lognormal_rows(91 + d, nq, d)
+    else:
+        rng = np.random.default_rng(92)
+        ref = rng.integers(0, 4, size=(n, d)).astype(np.float32)
+        qs = rng.integers(0, 4, size=(nq, d)).astype(np.float32)
+    ix = S.Index(ref)
+    gi, gd, gc = ix.knn(qs, k, S.SC_EUCLIDIAN, 2, S.SC_MIXED)
+    fi, fd, fc = ix.knn(qs, k, S.SC_EUCLIDIAN, 2, S.SC_PLAIN_FP64)
+    np.testing.assert_array_equal(gc, fc)
+    np.testing.assert_array_equal(gi, fi)
+    np.testing.assert_array_equal(gd.view(np.uint64), fd.view(np.uint64))
+    oi, od, oc = O.linear_find_batch(ref, qs[:3], k)
+    for r in range(3):
+        c = int(oc[r])
+        np.testing.assert_array_equal(gi[r, :c], oi[r, :c])
+        np.testing.assert_array_equal(gd[r, :c].view(np.uint64), od[r, :c].view(np.uint64))
