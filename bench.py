@@ -113,8 +113,17 @@ def bench_knn(args):
     qps = Q * args.steps / elapsed
 
     if mode == S.SC_MIXED:
+        # the timed main scan is the last row range: on indexes of at least
+        # 2,097,152 rows the mixed form scans the first 65,536 rows, then
+        # ranges growing 8 times; the last one starts at the first
+        # 65,536 * 8^j whose 8-fold reaches n (524,288 at 4M rows)
         kname = "knn_mixed_gemm"
-        flops_per_query = 2.0 * N * D
+        start = 0
+        if N >= 32 * 65536:
+            start = 65536
+            while start * 8 < N:
+                start *= 8
+        flops_per_query = 2.0 * (N - start) * D
         peak, bound, unit_peak = PEAK_BF16, "mfma", "TFLOP/s"
     else:
         kname = "knn_scan"
@@ -125,7 +134,8 @@ def bench_knn(args):
     avg_s = kms / 1e3 / max(launches, 1)
     achieved = flops_per_query * per_launch_q / avg_s if avg_s > 0 else 0.0
     others = {}
-    for name in ("knn_scan", "knn_replay", "knn_serial", "knn_mixed_gemm", "knn_mixed_rescore"):
+    for name in ("knn_scan", "knn_replay", "knn_serial", "knn_mixed_prefix", "knn_mixed_gemm",
+                 "knn_mixed_rescore"):
         ms, n = profile_read(lib, name)
         if n:
             others[name] = {"avg_ms": ms / n, "launches": n}

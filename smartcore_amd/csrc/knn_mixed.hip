@@ -20,33 +20,42 @@
 // exact) and one contiguous segment of rows, streamed 32 rows at a time
 // through four padded LDS tile slots that all four waves read; two
 // workgroups share a CU, so one can issue MFMAs while the other waits at a
-// barrier.  Tiles are loaded into registers in pairs, two steps ahead.  The
-// loop is software pipelined: a wave issues the MFMAs of tile t + 1 before
-// the epilogue of tile t, so the matrix pipe runs under the epilogue, the
-// rare candidate path and the tile stores.  v_mfma_f32_32x32x16_bf16 gives
-// acc = -2C; the epilogue forms L - x2m = y2m - cxb nyb + acc with one FMA
-// and one add (y2m <= (1 - 2^-18) y2, nyb >= ||y||, cxb >= 2 c1 ||x||, all
-// rounded the safe way; the two roundings are inside the 2^-18 margin).
-// Workgroups are laid out so that the CUs of an XCD stream the same segment
-// and share its tiles in their L2.  Per query the wave keeps the k smallest
-// U seen in the segment; every row whose L is below that running k-th U is
-// appended (index, L) to the query's list for the segment, in index order.
+// barrier.  Tiles are loaded into registers in pairs, two steps ahead.
+// v_mfma_f32_32x32x16_bf16 gives acc = -2C; the epilogue forms
+// L - x2m = y2m - cxb nyb + acc with one FMA and one add (y2m <=
+// (1 - 2^-18) y2, nyb >= ||y||, cxb >= 2 c1 ||x||, all rounded the safe
+// way; the two roundings are inside the 2^-18 margin).  Workgroups are laid
+// out so that the CUs of an XCD stream the same segment and share its tiles
+// in their L2.  Each lane owns one query and keeps, in registers, the k
+// smallest U seen so far; every row whose L is below that running k-th U is
+// appended to the query's list for the segment, in index order.
 // A row that enters the reference heap has s below the running k-th
-// smallest s of all earlier rows, which is at most the running k-th U of the
-// segment, so the lists are a superset of the heap entrants.
-// Pass 2 (replay, FP64).  One thread per query walks its lists in index
-// order and rescores a candidate exactly only if its L is below the s of
-// the current heap root (otherwise it cannot enter), then applies the heap
-// rule.  Rows that would not enter the reference heap are no-ops, so the
-// final heap array equals the reference's.  Queries whose lists overflow,
-// or whose rows are not finite / too large for the bound, are recomputed
-// by the sequential exact kernel.
+// smallest s of all earlier rows, which is at most the running k-th U, so
+// the lists are a superset of the heap entrants.
+// Levels.  Small indexes run one such launch.  For indexes of at least
+// kSeedMinRows rows the first launch covers only the first kPrefixRows
+// rows; then fixed-threshold launches cover row ranges growing kLevelRatio
+// times.  Each tests every row against T, the k-th smallest of k U values
+// of distinct earlier rows (the prefix's top-k, then the k smallest over
+// that and the previous level's candidates), and appends the rows with
+// L < T.  The heap's threshold never exceeds T, so these lists are also a
+// superset of the entrants, and with no running threshold a lane writes
+// its own rows without waiting on other lanes.  Candidates are kept only
+// if the per-row bound, which uses the stored norms of y - bf16(y) and
+// x - bf16(x), also has L below the threshold.
+// Pass 2 (replay, FP64).  Every candidate's exact s is computed in
+// parallel (a workgroup per query); then one thread per query walks its
+// lists in index order and applies the heap rule.  Rows that would not
+// enter the reference heap are no-ops, so the final heap array equals the
+// reference's.  Queries whose lists overflow, or whose rows are not finite
+// / too large for the bound, are recomputed by the sequential exact kernel.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <cfloat>
 #include <cmath>
 #include <mutex>
+#include <vector>
 
 #include "common.hpp"
 #include "metric.hpp"
@@ -64,8 +73,10 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef int i32x2 __attribute__((ext_vector_type(2)));
 
-constexpr int kWaves = 4;                 // waves per workgroup (one per SIMD)
+constexpr int kWaves = 8;                 // waves per workgroup (two per SIMD)
 constexpr int kQPW = 64;                  // queries per wave (2 column blocks)
 constexpr int kQPB = kWaves * kQPW;       // queries per workgroup
 constexpr int kNT = 64 * kWaves;          // threads per workgroup
@@ -73,9 +84,11 @@ constexpr int kRows = 32;                 // rows per tile
 constexpr int kMaxKS = 8;                 // K steps of 16 -> dimension <= 128
 constexpr int kMaxK = 32;                 // largest k served by this form
 constexpr int kXcds = 8;                  // MI355X: 8 XCDs, round-robin dispatch
-constexpr int kTargetBlocks = 512;        // two workgroups per CU on 256 CUs
+constexpr int kTargetBlocks = 256;        // one workgroup per CU on 256 CUs
 constexpr int64_t kPrefixRows = 65536;    // rows of the seeding launch
 constexpr int64_t kSeedMinRows = 32 * kPrefixRows;  // indexes this large are seeded
+constexpr int64_t kLevelRatio = 8;        // row-range growth of the fixed-threshold levels
+constexpr int kMaxLists = 64 + 2 * 64 * 8;  // running lists + fixed pairs, any plan
 constexpr double kC1 = 0.0078125 + 1.52587890625e-05 + 128.0 * 2.384185791015625e-07;
 constexpr double kLoMargin = 3.814697265625e-06;   // 2^-18
 constexpr double kHiMargin = 1.52587890625e-05;    // 2^-16
@@ -87,72 +100,98 @@ __device__ __forceinline__ uint16_t bf16_rne(float f) {
   return static_cast<uint16_t>(u >> 16);
 }
 
-// Per row: BF16 copy, y2m = RD((1 - 2^-18) y2) and nyb = RU(||y||) (with a
-// relative allowance for the f64 sum behind y2).  A row whose values are not
-// finite or whose squared norm reaches kNormLimit marks `bad` (the index
-// then uses FP64).
+// Per row: BF16 copy, y2m = RD((1 - 2^-18) y2), nyb = RU(||y||),
+// nry = RU(||y - bf16(y)||) and nyh = RU(||y|| + ||y - bf16(y)||) >=
+// ||bf16(y)|| (with a relative allowance for the f64 sums).
+// A row whose values are not finite or whose squared norm reaches
+// kNormLimit marks `bad` (the index then uses FP64).
 __global__ void prep_rows_kernel(const float* __restrict__ rows, int64_t n, int ld,
                                  uint16_t* __restrict__ out_b, float* __restrict__ y2m,
-                                 float* __restrict__ nyb, uint32_t* __restrict__ bad) {
+                                 float* __restrict__ nyb, float* __restrict__ nry,
+                                 float* __restrict__ nyh, uint32_t* __restrict__ bad) {
   const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const float* r = rows + i * ld;
   uint16_t* o = out_b + i * ld;
-  double s = 0.0;
+  double s = 0.0, sr = 0.0;
   bool ok = true;
   for (int j = 0; j < ld; ++j) {
     const float v = r[j];
     ok = ok && isfinite(v);
     s += static_cast<double>(v) * static_cast<double>(v);
-    o[j] = bf16_rne(v);
+    const uint16_t b = bf16_rne(v);
+    o[j] = b;
+    const double e = static_cast<double>(v) - static_cast<double>(__uint_as_float(static_cast<uint32_t>(b) << 16));
+    sr += e * e;
   }
   if (!ok || !(s < kNormLimit)) {
     atomicOr(bad, 1u);
     y2m[i] = INFINITY;
     nyb[i] = 0.0f;
+    nry[i] = 0.0f;
+    nyh[i] = 0.0f;
     return;
   }
   y2m[i] = __double2float_rd(s * (1.0 - kLoMargin));
   nyb[i] = __double2float_ru(sqrt(s) * (1.0 + 9.094947017729282e-13));
+  nry[i] = __double2float_ru(sqrt(sr) * (1.0 + 9.094947017729282e-13));
+  nyh[i] = __double2float_ru((sqrt(s) + sqrt(sr)) * (1.0 + 9.094947017729282e-13));
 }
 
-// y2m of padding rows is +inf (their L is +inf), nyb is 0.
-__global__ void fill_terms_kernel(float* y2m, float* nyb, int64_t from, int64_t npad) {
+// y2m of padding rows is +inf (their L is +inf), nyb, nry and nyh are 0.
+__global__ void fill_terms_kernel(float* y2m, float* nyb, float* nry, float* nyh, int64_t from,
+                                  int64_t npad) {
   const int64_t i = from + static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i < npad) {
     y2m[i] = INFINITY;
     nyb[i] = 0.0f;
+    nry[i] = 0.0f;
+    nyh[i] = 0.0f;
   }
 }
 
-// Per query: BF16 of -2x, x2m = RD((1 - 2^-18) x2) and cxb = RU(2 c1 ||x||);
+// Per query: BF16 of -2x, x2m = RD((1 - 2^-18) x2), cxb = RU(2 c1 ||x||)
+// and the terms of the tight bound (see slow() in the scan):
+// qa1 = RU(2 ||x||), qa2 = RU(2 (||x - bf16(x)|| + 128 * 2^-22 ||bf16(x)||));
 // a query that is not finite or too large is listed for the exact kernel.
 __global__ void prep_queries_kernel(const float* __restrict__ qs, int64_t q, int ld,
                                     uint16_t* __restrict__ out_b, float* __restrict__ x2m,
-                                    float* __restrict__ cxb, uint32_t* __restrict__ list,
+                                    float* __restrict__ cxb, float* __restrict__ qa1,
+                                    float* __restrict__ qa2, uint32_t* __restrict__ list,
                                     uint32_t* __restrict__ list_n) {
   const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= q) return;
   const float* r = qs + i * ld;
   uint16_t* o = out_b + i * ld;
-  double s = 0.0;
+  double s = 0.0, sr = 0.0, sh = 0.0;
   bool ok = true;
   for (int j = 0; j < ld; ++j) {
     const float v = r[j];
     ok = ok && isfinite(v);
     s += static_cast<double>(v) * static_cast<double>(v);
-    o[j] = bf16_rne(-2.0f * v);
+    const uint16_t b = bf16_rne(-2.0f * v);
+    o[j] = b;
+    // bf16(-2v) = -2 bf16(v) exactly
+    const double h = -0.5 * static_cast<double>(__uint_as_float(static_cast<uint32_t>(b) << 16));
+    const double e = static_cast<double>(v) - h;
+    sr += e * e;
+    sh += h * h;
   }
   if (!ok || !(s < kNormLimit)) {
     // never a candidate in pass 1; the exact kernel takes it
     x2m[i] = INFINITY;
     cxb[i] = 0.0f;
+    qa1[i] = 0.0f;
+    qa2[i] = 0.0f;
     const uint32_t slot = atomicAdd(list_n, 1u);
     list[slot] = static_cast<uint32_t>(i);
     return;
   }
   x2m[i] = __double2float_rd(s * (1.0 - kLoMargin));
   cxb[i] = __double2float_ru(2.0 * kC1 * sqrt(s) * (1.0 + 9.5367431640625e-07));
+  qa1[i] = __double2float_ru(2.0 * sqrt(s) * (1.0 + 9.5367431640625e-07));
+  qa2[i] = __double2float_ru(2.0 * (sqrt(sr) + 128.0 * 2.384185791015625e-07 * sqrt(sh)) *
+                             (1.0 + 9.5367431640625e-07));
 }
 
 __device__ __forceinline__ bf16x8 load_frag(const uint16_t* p) {
@@ -166,7 +205,7 @@ struct Scan {
   static constexpr int RS = LD + 8;                     // LDS row stride: 16 B of padding
   static constexpr int VEC = kRows * LD / 8;            // 16-byte pieces of a tile
   static constexpr int NP = (VEC + kNT - 1) / kNT;       // pieces per thread
-  static constexpr int SLOT = kRows * RS * 2 + 2 * kRows * 4;  // rows | y2m | nyb (bytes)
+  static constexpr int SLOT = kRows * RS * 2 + 4 * kRows * 4;  // rows | y2m | nyb | nry | nyh (bytes)
   static_assert(NP <= 2, "at most two pieces per thread");
 };
 
@@ -177,40 +216,68 @@ size_t scan_lds_bytes() {
   return kSlots * static_cast<size_t>(Scan<KS>::SLOT) + sizeof(float) * kWaves * 64 * 32;
 }
 
+// Candidate lists.  Per query, lists are laid out back to back: first
+// `nrun` lists of `cap_run` entries (one per segment of a running launch),
+// then lists of `cap_fix` entries written by fixed-threshold launches, two
+// per segment (one per half of the tile's rows, each in row order; the
+// replay merges the pair).  Lists follow the rows' order.
+struct ListLayout {
+  int nrun, cap_run;  // running lists
+  int nfix, cap_fix;  // fixed-threshold lists (pairs)
+  __host__ __device__ int nlists() const { return nrun + nfix; }
+  __host__ __device__ int64_t per_query() const {
+    return static_cast<int64_t>(nrun) * cap_run + static_cast<int64_t>(nfix) * cap_fix;
+  }
+  __host__ __device__ int64_t offset(int l) const {
+    return l < nrun ? static_cast<int64_t>(l) * cap_run
+                    : static_cast<int64_t>(nrun) * cap_run + static_cast<int64_t>(l - nrun) * cap_fix;
+  }
+  __host__ __device__ int cap(int l) const { return l < nrun ? cap_run : cap_fix; }
+};
+
 // One scan launch covers rows [row0, row_end) in nseg segments of seg_len
-// rows; its segment s writes list s + seg_base of nseg_all per query.  With
-// seed, every query starts from those k U values (U bounds of k distinct
-// rows before row0) instead of an empty top-k; with topu_out, each
-// (query, segment) writes its final k smallest U there.
+// rows.  A running launch (MODE 0) keeps each query's k smallest U, from
+// empty, and writes segment s to list lbase + s; with topu_out, each
+// (query, segment) also writes its final k smallest U there.  A fixed
+// launch (MODE 2, or 3 to store U as well) tests every row against the
+// seed's k-th U and writes segment s to lists lbase + 2 s + half.
+// A seed holds k U values of k distinct rows before row0 per query,
+// ascending.
 struct ScanArgs {
   const uint16_t* rows_b;
   const float* y2m;
   const float* nyb;
+  const float* nry;
+  int64_t npad;  // y2m, nyb, nry, nyh: consecutive arrays of npad floats
   int64_t n;
   const uint16_t* q_b;
   const float* x2m;
   const float* cxbq;
+  const float* qa1;
+  const float* qa2;
   int64_t q;
   int k;
   int64_t row0, row_end, seg_len;
   int nseg;
   int64_t nqt;
-  int seg_base, nseg_all;
-  uint32_t* cand_idx;
-  float* cand_L;
-  uint32_t* cand_count;
-  int cap;
-  const float* seed;  // q x k ascending, or null
-  float* topu_out;    // q x nseg x k, or null
+  ListLayout lay;
+  int lbase;
+  uint32_t* cand_idx;  // q x lay.per_query()
+  float* cand_U;       // MODE 3: U of each entry
+  uint32_t* cand_count;  // q x lay.nlists()
+  const float* seed;   // q x k, or null
+  float* topu_out;     // q x nseg x k, or null
 };
 
-// KT: registers of the per-query top-k (16 for k <= 16, else 32).
-template <int KS, bool SEEDED, int KT>
+// KT: registers of the per-query top-k of a running launch (16 for
+// k <= 16, else 32).
+template <int KS, int MODE, int KT>
 __global__ __launch_bounds__(kNT, 2) void mixed_scan_kernel(const ScanArgs a) {
   using S = Scan<KS>;
+  constexpr bool RUNNING = MODE == 0;
+  constexpr bool STORE_U = MODE == 3;
   const uint16_t* __restrict__ rows_b = a.rows_b;
   const float* __restrict__ y2m = a.y2m;
-  const float* __restrict__ nyb = a.nyb;
   const int64_t n = a.n;
   const uint16_t* __restrict__ q_b = a.q_b;
   const float* __restrict__ x2m = a.x2m;
@@ -220,8 +287,6 @@ __global__ __launch_bounds__(kNT, 2) void mixed_scan_kernel(const ScanArgs a) {
   const int nseg = a.nseg;
   const int64_t nqt = a.nqt;
   uint32_t* __restrict__ cand_idx = a.cand_idx;
-  float* __restrict__ cand_L = a.cand_L;
-  const int cap = a.cap;
   extern __shared__ __align__(16) unsigned char smem[];
   // XCD-aware layout: block b runs on XCD b % 8.  Up to 8 segments, the
   // XCDs with the same b % nseg stream the same segment with different
@@ -250,14 +315,22 @@ __global__ __launch_bounds__(kNT, 2) void mixed_scan_kernel(const ScanArgs a) {
   const int64_t qw0 = qt * kQPB + wave * kQPW;
   const int64_t r_begin = a.row0 + static_cast<int64_t>(seg) * a.seg_len;
   const int64_t r_end = min(a.row_end, r_begin + a.seg_len);
-  const int lseg = a.seg_base + seg;  // this segment's list
   const int64_t ntiles = r_end > r_begin ? (r_end - r_begin + kRows - 1) / kRows : 0;
+  const int nl = a.lay.nlists();
+  const int64_t perq = a.lay.per_query();
+  // this launch's list(s) for the segment: running lbase + seg, fixed
+  // lbase + 2 seg + half
+  const int lseg = RUNNING ? a.lbase + seg : a.lbase + 2 * seg + half;
+  const int64_t loff = a.lay.offset(lseg);
+  const int lcap = a.lay.cap(lseg);
   // rare-path transpose: each lane's 32 values of a tile (both column blocks)
   float* trans = reinterpret_cast<float*>(smem + kSlots * S::SLOT);
 
+  // per column block values as two-element vectors, so that picking the
+  // lane's own block is a register select, not an indexed private array
   bf16x8 bq[2][KS];
-  float thr[2], x2v[2], cxq[2];
-  bool qvalid[2];
+  f32x2 thr, x2v, cxq, kfix;
+  i32x2 qvalid;
 #pragma unroll
   for (int cb = 0; cb < 2; ++cb) {
     const int64_t qi = qw0 + 32 * cb + col;
@@ -269,45 +342,61 @@ __global__ __launch_bounds__(kNT, 2) void mixed_scan_kernel(const ScanArgs a) {
     for (int ks = 0; ks < KS; ++ks) bq[cb][ks] = load_frag(q_b + qr * S::LD + 16 * ks + 8 * half);
     x2v[cb] = qvalid[cb] ? x2m[qr] : INFINITY;
     cxq[cb] = qi < q ? cxbq[qr] : 0.0f;
-    thr[cb] = INFINITY;
+    // a fixed launch tests against the seed's k-th U; T - x2m rounded up
+    kfix[cb] = !RUNNING && qvalid[cb] ? a.seed[qr * k + k - 1] : INFINITY;
+    thr[cb] = !RUNNING && qvalid[cb]
+                  ? __double2float_ru(static_cast<double>(kfix[cb]) - static_cast<double>(x2v[cb]))
+                  : INFINITY;
   }
-  // The lane owns query (column block `half`, column col): it keeps that
-  // query's k smallest U in registers tk[0..k), ascending, with tk[k..KT)
-  // held at -inf (the insertion below leaves them there, so the k-th
-  // smallest is the maximum of tk and no register is indexed by k), its
-  // candidate count and its threshold.
+  // Running launches: the lane owns query (column block `half`, column
+  // col) and keeps that query's k smallest U in registers tk[0..k),
+  // ascending, with tk[k..KT) held at -inf (the insertion below leaves them
+  // there, so the k-th smallest is the maximum of tk and no register is
+  // indexed by k), its candidate count and its threshold.
   const int64_t qo = qw0 + 32 * half + col;
-  const double xo = static_cast<double>(half == 0 ? x2v[0] : x2v[1]);
-  const double co = static_cast<double>(half == 0 ? cxq[0] : cxq[1]);
-  float tk[KT];
+  const double xo = static_cast<double>(half == 0 ? x2v.x : x2v.y);
+  const double co = static_cast<double>(half == 0 ? cxq.x : cxq.y);
+  const double a1o = qo < q ? static_cast<double>(a.qa1[qo]) : 0.0;
+  const double a2o = qo < q ? static_cast<double>(a.qa2[qo]) : 0.0;
+  float tk[RUNNING ? KT : 1];
+  if constexpr (RUNNING) {
 #pragma unroll
-  for (int j = 0; j < KT; ++j)
-    tk[j] = j >= k ? -INFINITY : (SEEDED && qo < q ? a.seed[qo * k + j] : INFINITY);
+    for (int j = 0; j < KT; ++j)
+      tk[j] = j >= k ? -INFINITY : INFINITY;
+  }
   auto kth_of = [&]() {
     float m[KT];
 #pragma unroll
-    for (int j = 0; j < KT; ++j) m[j] = tk[j];
+    for (int j = 0; j < KT; ++j) m[j] = tk[RUNNING ? j : 0];
 #pragma unroll
     for (int w = KT / 2; w >= 1; w /= 2)
 #pragma unroll
       for (int j = 0; j < w; ++j) m[j] = fmaxf(m[j], m[j + w]);
     return m[0];
   };
+  // fixed launches: the lane writes rows of its half for both column
+  // blocks' queries (counts cntf), with their tight-bound terms
   uint32_t cnt = 0;
+  uint32_t cntf[2] = {0u, 0u};
+  f32x2 fa1 = {0.0f, 0.0f}, fa2 = {0.0f, 0.0f};
+  if constexpr (!RUNNING) {
+#pragma unroll
+    for (int cb = 0; cb < 2; ++cb) {
+      const int64_t qi = qw0 + 32 * cb + col;
+      fa1[cb] = qi < q ? a.qa1[qi] : 0.0f;
+      fa2[cb] = qi < q ? a.qa2[qi] : 0.0f;
+    }
+  }
   // both lanes of a column hold the thresholds of both column blocks
   auto set_thr = [&](float own) {
     const float other = __shfl_xor(own, 32);
-    thr[0] = half == 0 ? own : other;
-    thr[1] = half == 0 ? other : own;
+    thr.x = half == 0 ? own : other;
+    thr.y = half == 0 ? other : own;
   };
-  if (SEEDED) {
-    const bool ov = half == 0 ? qvalid[0] : qvalid[1];
-    set_thr(ov ? __double2float_ru(static_cast<double>(kth_of()) - xo) : INFINITY);
-  }
 
   // tile loads: thread tid carries pieces tid and tid + kNT (16 bytes of
-  // rows each, when below VEC); threads 0..63 carry the tile's y2m (0..31)
-  // and nyb (32..63)
+  // rows each, when below VEC); threads 0..127 carry the tile's y2m (0..31),
+  // nyb (32..63), nry (64..95) and nyh (96..127)
   struct TileRegs {
     u32x4 v[S::NP];
     float tv;
@@ -322,7 +411,8 @@ __global__ __launch_bounds__(kNT, 2) void mixed_scan_kernel(const ScanArgs a) {
         r.v[pc] = *reinterpret_cast<const u32x4*>(rows_b + row * S::LD + 8 * (e % (S::LD / 8)));
       }
     }
-    if (tid < 64) r.tv = (tid < 32 ? y2m : nyb)[t0 + (tid & 31)];
+    // y2m, nyb, nry and nyh are consecutive arrays of a.npad floats
+    if (tid < 128) r.tv = y2m[(tid >> 5) * a.npad + t0 + (tid & 31)];
   };
   auto swrite = [&](int64_t t, const TileRegs& r) {
     unsigned char* base = smem + static_cast<int>(t & (kSlots - 1)) * S::SLOT;
@@ -333,7 +423,7 @@ __global__ __launch_bounds__(kNT, 2) void mixed_scan_kernel(const ScanArgs a) {
         *reinterpret_cast<u32x4*>(base + 2 * ((e / (S::LD / 8)) * S::RS + 8 * (e % (S::LD / 8)))) =
             r.v[pc];
     }
-    if (tid < 64) reinterpret_cast<float*>(base + 2 * kRows * S::RS)[tid] = r.tv;
+    if (tid < 128) reinterpret_cast<float*>(base + 2 * kRows * S::RS)[tid] = r.tv;
   };
 
   // LDS holds four tiles: the one being scanned, the next one and two being
@@ -353,16 +443,21 @@ __global__ __launch_bounds__(kNT, 2) void mixed_scan_kernel(const ScanArgs a) {
     }
   };
   // fast epilogue of tile t: acc <- L - x2m; returns per column block
-  // whether any lane of the wave has a row below its query's threshold
+  // whether any lane of the wave has a row below its query's threshold.
+  // Running launches use the loose bound (e = cxb nyb, one FMA), fixed
+  // launches the tight one (et = qa1 nry + qa2 nyh, two FMAs).
   auto epilogue = [&](int64_t t, f32x16 (&acc)[2], bool (&hit)[2]) {
     const unsigned char* base = slot_ptr(t);
     const float* ty2 = reinterpret_cast<const float*>(base + 2 * kRows * S::RS);
     const float* tny = ty2 + kRows;
-    f32x4 yv[4], nv[4];
+    const float* trr = tny + kRows;
+    const float* tnh = trr + kRows;
+    f32x4 yv[4], nv[4], rv[4];
 #pragma unroll
     for (int g = 0; g < 4; ++g) {
       yv[g] = *reinterpret_cast<const f32x4*>(ty2 + 8 * g + 4 * half);
-      nv[g] = *reinterpret_cast<const f32x4*>(tny + 8 * g + 4 * half);
+      nv[g] = *reinterpret_cast<const f32x4*>((RUNNING ? tny : tnh) + 8 * g + 4 * half);
+      if constexpr (!RUNNING) rv[g] = *reinterpret_cast<const f32x4*>(trr + 8 * g + 4 * half);
     }
 #pragma unroll
     for (int cb = 0; cb < 2; ++cb) {
@@ -371,19 +466,33 @@ __global__ __launch_bounds__(kNT, 2) void mixed_scan_kernel(const ScanArgs a) {
       for (int g = 0; g < 4; ++g)
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          const float v = fmaf(-cxq[cb], nv[g][r], acc[cb][4 * g + r]) + yv[g][r];
+          float v;
+          if constexpr (RUNNING)
+            v = fmaf(-cxq[cb], nv[g][r], acc[cb][4 * g + r]) + yv[g][r];
+          else
+            v = fmaf(-fa2[cb], nv[g][r], fmaf(-fa1[cb], rv[g][r], acc[cb][4 * g + r])) + yv[g][r];
           acc[cb][4 * g + r] = v;
           m = fminf(m, v);
         }
-      hit[cb] = __ballot(qvalid[cb] && m < thr[cb]) != 0;
+      hit[cb] = __ballot(qvalid[cb] != 0 && m < thr[cb]) != 0;
     }
   };
-  // Rare path of tile t (see the file comment).  Each lane handles its own
-  // query: the rows of the tile below its threshold, in row order.
+  // Tight bound of one row: with x = bf16(x) + rx and y = bf16(y) + ry,
+  // |x.y - C| <= ||x|| ||ry|| + (||rx|| + 128 * 2^-22 ||bf16(x)||) ||bf16(y)||
+  // and ||bf16(y)|| <= nyh, so 2 |x.y - C| <= et = qa1 nry + qa2 nyh.
+  auto tight_e = [&](double q1, double q2, float nr, float nh) {
+    return (q1 * static_cast<double>(nr) + q2 * static_cast<double>(nh)) *
+           (1.0 + 4.440892098500626e-16);
+  };
+  // Rare path of a running launch for tile t (see the file comment): each
+  // lane handles its own query, the rows of the tile below its threshold,
+  // in row order.
   auto slow = [&](int64_t t, f32x16 (&acc)[2]) {
     const unsigned char* base = slot_ptr(t);
     const float* ty2 = reinterpret_cast<const float*>(base + 2 * kRows * S::RS);
     const float* tny = ty2 + kRows;
+    const float* trr = tny + kRows;
+    const float* tnh = trr + kRows;
     const int64_t t0 = r_begin + t * kRows;
     uint32_t m16[2];
 #pragma unroll
@@ -391,7 +500,7 @@ __global__ __launch_bounds__(kNT, 2) void mixed_scan_kernel(const ScanArgs a) {
       m16[cb] = 0;
 #pragma unroll
       for (int v = 0; v < 16; ++v)
-        m16[cb] |= static_cast<uint32_t>(qvalid[cb] && acc[cb][v] < thr[cb]) << v;
+        m16[cb] |= static_cast<uint32_t>(qvalid[cb] != 0 && acc[cb][v] < thr[cb]) << v;
     }
     float* tw = trans + (wave * 64 + lane) * 32;
 #pragma unroll
@@ -412,7 +521,8 @@ __global__ __launch_bounds__(kNT, 2) void mixed_scan_kernel(const ScanArgs a) {
       M |= (half == 0 ? o4 : p4) << (8 * g);
       M |= (half == 0 ? p4 : o4) << (8 * g + 4);
     }
-    float tcur = half == 0 ? thr[0] : thr[1];
+    float tcur = half == 0 ? thr.x : thr.y;
+    float kcur = kth_of();
     while (M != 0) {
       const int r = __builtin_ctz(M);
       M &= M - 1;
@@ -422,25 +532,72 @@ __global__ __launch_bounds__(kNT, 2) void mixed_scan_kernel(const ScanArgs a) {
       const int v = (r >> 3) * 4 + (r & 3);
       const float val = trans[(wave * 64 + col + 32 * h) * 32 + 16 * half + v];
       if (!(val < tcur)) continue;  // the threshold fell earlier in this tile
-      const double e = co * static_cast<double>(tny[r]);
-      const double L = xo + static_cast<double>(val);
-      const double U = L + 2.0 * e + (kHiMargin + 2.0 * kLoMargin) * (xo + static_cast<double>(ty2[r]));
+      // the epilogue subtracted the loose e = cxb nyb; swap it for et
+      const double et = tight_e(a1o, a2o, trr[r], tnh[r]);
+      const double L = xo + static_cast<double>(val) + (co * static_cast<double>(tny[r]) - et);
+      if (!(L < static_cast<double>(kcur))) continue;
+      const double U = L + 2.0 * et + (kHiMargin + 2.0 * kLoMargin) * (xo + static_cast<double>(ty2[r]));
       const float uf = __double2float_ru(U);
       // insert into the ascending tk[0..k): slot j takes
       // min(tk[j], max(tk[j-1], uf)); slots past k stay -inf
+      if constexpr (RUNNING) {
 #pragma unroll
-      for (int j = KT - 1; j > 0; --j) tk[j] = fminf(tk[j], fmaxf(tk[j - 1], uf));
-      tk[0] = fminf(tk[0], uf);
-      if (cnt < static_cast<uint32_t>(cap)) {
-        const size_t sl = (static_cast<size_t>(qo) * a.nseg_all + lseg) * cap + cnt;
+        for (int j = KT - 1; j > 0; --j) tk[j] = fminf(tk[j], fmaxf(tk[j - 1], uf));
+        tk[0] = fminf(tk[0], uf);
+      }
+      if (cnt < static_cast<uint32_t>(lcap)) {
+        const size_t sl = static_cast<size_t>(qo) * perq + loff + cnt;
         cand_idx[sl] = static_cast<uint32_t>(row);
-        cand_L[sl] = __double2float_rd(L);
       }
       ++cnt;
       // T - x2m rounded up (the f64 difference of two f32 values is exact)
-      tcur = __double2float_ru(static_cast<double>(kth_of()) - xo);
+      kcur = kth_of();
+      tcur = __double2float_ru(static_cast<double>(kcur) - xo);
     }
     set_thr(tcur);
+  };
+  // Hits of a fixed launch for tile t: no threshold changes, so each lane
+  // writes its own rows (its half of the tile, ascending) of both column
+  // blocks' queries to the queries' lists of that half.
+  auto fixed_hits = [&](int64_t t, f32x16 (&acc)[2], const bool (&hit)[2]) {
+    const unsigned char* base = slot_ptr(t);
+    const float* ty2 = reinterpret_cast<const float*>(base + 2 * kRows * S::RS);
+    const float* tny = ty2 + kRows;
+    const float* trr = tny + kRows;
+    const float* tnh = trr + kRows;
+    const int64_t t0 = r_begin + t * kRows;
+#pragma unroll
+    for (int cb = 0; cb < 2; ++cb) {
+      if (!hit[cb]) continue;
+      const int64_t qi = qw0 + 32 * cb + col;
+#pragma unroll
+      for (int v = 0; v < 16; ++v) {
+        const float val = acc[cb][v];
+        if (qvalid[cb] != 0 && val < thr[cb]) {
+          const int r = 8 * (v >> 2) + 4 * half + (v & 3);
+          const int64_t row = t0 + r;
+          if (row < r_end) {
+            // the epilogue formed L - x2m with the tight bound
+            const double L = static_cast<double>(x2v[cb]) + static_cast<double>(val);
+            if (L < static_cast<double>(kfix[cb])) {
+              if (cntf[cb] < static_cast<uint32_t>(lcap)) {
+                const size_t sl = static_cast<size_t>(qi) * perq + loff + cntf[cb];
+                cand_idx[sl] = static_cast<uint32_t>(row);
+                if constexpr (STORE_U) {
+                  const double et = tight_e(static_cast<double>(fa1[cb]),
+                                            static_cast<double>(fa2[cb]), trr[r], tnh[r]);
+                  const double U = L + 2.0 * et +
+                                   (kHiMargin + 2.0 * kLoMargin) *
+                                       (static_cast<double>(x2v[cb]) + static_cast<double>(ty2[r]));
+                  a.cand_U[sl] = __double2float_ru(U);
+                }
+              }
+              ++cntf[cb];
+            }
+          }
+        }
+      }
+    }
   };
 
   TileRegs ra, rb;
@@ -456,7 +613,11 @@ __global__ __launch_bounds__(kNT, 2) void mixed_scan_kernel(const ScanArgs a) {
     mfma_tile(t, acc);
     bool hit[2];
     epilogue(t, acc, hit);
-    if (hit[0] || hit[1]) slow(t, acc);
+    if constexpr (RUNNING) {
+      if (hit[0] || hit[1]) slow(t, acc);
+    } else {
+      if (hit[0] || hit[1]) fixed_hits(t, acc, hit);
+    }
     if ((t & 1) == 0) {
       if (t + 2 < ntiles) swrite(t + 2, ra);
       if (t + 3 < ntiles) swrite(t + 3, rb);
@@ -465,16 +626,24 @@ __global__ __launch_bounds__(kNT, 2) void mixed_scan_kernel(const ScanArgs a) {
     }
     __syncthreads();
   }
-  if (qo < q) a.cand_count[static_cast<size_t>(qo) * a.nseg_all + lseg] = cnt;
-  if (a.topu_out != nullptr && qo < q) {
+  if constexpr (RUNNING) {
+    if (qo < q) a.cand_count[static_cast<size_t>(qo) * nl + lseg] = cnt;
+    if (a.topu_out != nullptr && qo < q) {
 #pragma unroll
-    for (int j = 0; j < KT; ++j)
-      if (j < k) a.topu_out[(static_cast<size_t>(qo) * nseg + seg) * k + j] = tk[j];
+      for (int j = 0; j < KT; ++j)
+        if (j < k) a.topu_out[(static_cast<size_t>(qo) * nseg + seg) * k + j] = tk[j];
+    }
+  } else {
+#pragma unroll
+    for (int cb = 0; cb < 2; ++cb) {
+      const int64_t qi = qw0 + 32 * cb + col;
+      if (qi < q) a.cand_count[static_cast<size_t>(qi) * nl + lseg] = cntf[cb];
+    }
   }
 }
 
 // The k smallest of a query's nseg ascending top-k lists (U bounds of
-// distinct rows), ascending: the seed of the segments after them.
+// distinct rows), ascending: the seed of the launches after them.
 __global__ void merge_topu_kernel(const float* __restrict__ topu, int64_t q, int nseg, int k,
                                   float* __restrict__ seed) {
   const int64_t qi = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -495,6 +664,36 @@ __global__ void merge_topu_kernel(const float* __restrict__ topu, int64_t q, int
   }
 }
 
+// The k smallest of a seed and the U values stored by a fixed launch in
+// lists [l0, l0 + nl) (every U is a bound of a distinct row after the
+// seed's rows): the next seed, ascending.  Entries past a list's cap were
+// not stored; leaving them out keeps a valid seed.
+__global__ void select_seed_kernel(const float* __restrict__ seed_in, const float* __restrict__ cand_U,
+                                   const uint32_t* __restrict__ cand_count, ListLayout lay, int l0,
+                                   int nl, int64_t q, int k, float* __restrict__ seed_out) {
+  const int64_t qi = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (qi >= q) return;
+  float best[kMaxK];
+  for (int j = 0; j < k; ++j) best[j] = seed_in[qi * k + j];
+  const int64_t perq = lay.per_query();
+  for (int l = l0; l < l0 + nl; ++l) {
+    const uint32_t c = min(cand_count[static_cast<size_t>(qi) * lay.nlists() + l],
+                           static_cast<uint32_t>(lay.cap(l)));
+    const float* u = cand_U + static_cast<size_t>(qi) * perq + lay.offset(l);
+    for (uint32_t e = 0; e < c; ++e) {
+      const float v = u[e];
+      if (!(v < best[k - 1])) continue;
+      int pos = k - 1;
+      while (pos > 0 && best[pos - 1] > v) {
+        best[pos] = best[pos - 1];
+        --pos;
+      }
+      best[pos] = v;
+    }
+  }
+  for (int j = 0; j < k; ++j) seed_out[qi * k + j] = best[j];
+}
+
 __device__ inline void sift_down_s(double* hd, int64_t* hi, double* hs, int kk, int n) {
   while (2 * kk <= n) {
     int j = 2 * kk;
@@ -513,11 +712,49 @@ __device__ inline void sift_down_s(double* hd, int64_t* hi, double* hs, int kk, 
   }
 }
 
-// Pass 2: exact replay over the candidate lists (see the file comment).
-__global__ void mixed_replay_kernel(const float* __restrict__ rows, const float* __restrict__ qs,
-                                    int ld, int64_t q, int k, int nseg, int cap,
+// Pass 2a: the exact s of every stored candidate (the reference's
+// arithmetic: f32 differences and squares, f64 sum in dimension order).
+// One workgroup per query; its threads take the candidates in turn.
+constexpr int kRescoreThreads = 128;
+__global__ __launch_bounds__(kRescoreThreads) void mixed_rescore_kernel(
+    const float* __restrict__ rows, const float* __restrict__ qs, int ld, int64_t q, ListLayout lay,
+    const uint32_t* __restrict__ cand_idx, const uint32_t* __restrict__ cand_count,
+    const float* __restrict__ x2m, double* __restrict__ cand_S) {
+  const int64_t qi = blockIdx.x;
+  if (qi >= q || !(x2m[qi] < INFINITY)) return;  // uniform: listed for the exact kernel
+  __shared__ float qrow[16 * kMaxKS];
+  __shared__ uint32_t start[kMaxLists + 1];
+  const int nl = lay.nlists();
+  for (int j = threadIdx.x; j < ld; j += kRescoreThreads) qrow[j] = qs[qi * ld + j];
+  if (threadIdx.x == 0) {
+    uint32_t acc = 0;
+    for (int l = 0; l < nl; ++l) {
+      start[l] = acc;
+      acc += min(cand_count[static_cast<size_t>(qi) * nl + l], static_cast<uint32_t>(lay.cap(l)));
+    }
+    start[nl] = acc;
+  }
+  __syncthreads();
+  const size_t qbase = static_cast<size_t>(qi) * lay.per_query();
+  for (uint32_t e = threadIdx.x; e < start[nl]; e += kRescoreThreads) {
+    int l = 0;
+    while (start[l + 1] <= e) ++l;
+    const size_t slot = qbase + lay.offset(l) + (e - start[l]);
+    const float* rrow = rows + static_cast<int64_t>(cand_idx[slot]) * ld;
+    double sum = 0.0;
+    for (int j = 0; j < ld; ++j) sum = sum + Term<float, kEuclid>::f(qrow[j], rrow[j], 2.0);
+    cand_S[slot] = sum;
+  }
+}
+
+// Pass 2b: replay of the heap rule over the candidates in row order
+// (see the file comment): running lists one by one, fixed-threshold list
+// pairs merged by row index.  A candidate that would not enter the
+// reference heap is a no-op, so the final heap array equals the
+// reference's.
+__global__ void mixed_replay_kernel(int64_t q, int k, ListLayout lay,
                                     const uint32_t* __restrict__ cand_idx,
-                                    const float* __restrict__ cand_L,
+                                    const double* __restrict__ cand_S,
                                     const uint32_t* __restrict__ cand_count,
                                     const float* __restrict__ x2m, double* __restrict__ hs_all,
                                     int64_t* __restrict__ out_idx, double* __restrict__ out_dist,
@@ -535,38 +772,46 @@ __global__ void mixed_replay_kernel(const float* __restrict__ rows, const float*
     hi[j] = -1;
     hs[j] = INFINITY;
   }
-  for (int s = 0; s < nseg; ++s) {
-    if (cand_count[static_cast<size_t>(qi) * nseg + s] > static_cast<uint32_t>(cap)) {
+  const int nl = lay.nlists();
+  const uint32_t* cc = cand_count + static_cast<size_t>(qi) * nl;
+  for (int l = 0; l < nl; ++l) {
+    if (cc[l] > static_cast<uint32_t>(lay.cap(l))) {
       const uint32_t slot = atomicAdd(list_n, 1u);
       list[slot] = static_cast<uint32_t>(qi);
       return;
     }
   }
-  const float* qrow = qs + qi * ld;
-  unsigned long long seen = 0, rescored = 0;
-  for (int s = 0; s < nseg; ++s) {
-    const uint32_t cnt = cand_count[static_cast<size_t>(qi) * nseg + s];
-    const size_t base = (static_cast<size_t>(qi) * nseg + s) * cap;
-    seen += cnt;
-    for (uint32_t c = 0; c < cnt; ++c) {
-      const float L = cand_L[base + c];
-      if (!(static_cast<double>(L) < hs[0])) continue;
-      const int64_t r = cand_idx[base + c];
-      const float* rrow = rows + r * ld;
-      double sum = 0.0;
-      for (int j = 0; j < ld; ++j) sum = sum + Term<float, kEuclid>::f(qrow[j], rrow[j], 2.0);
-      ++rescored;
-      const double d = __builtin_sqrt(sum);
-      if (d < hd[0]) {
-        hd[0] = d;
-        hi[0] = r;
-        hs[0] = sum;
-        if (k > 1) sift_down_s(hd, hi, hs, 0, k - 1);
-      }
+  const size_t qbase = static_cast<size_t>(qi) * lay.per_query();
+  unsigned long long seen = 0;
+  auto visit = [&](size_t e) {
+    const double sum = cand_S[e];
+    const double d = __builtin_sqrt(sum);
+    if (d < hd[0]) {
+      hd[0] = d;
+      hi[0] = cand_idx[e];
+      hs[0] = sum;
+      if (k > 1) sift_down_s(hd, hi, hs, 0, k - 1);
+    }
+  };
+  for (int l = 0; l < lay.nrun; ++l) {
+    const size_t base = qbase + lay.offset(l);
+    seen += cc[l];
+    for (uint32_t c = 0; c < cc[l]; ++c) visit(base + c);
+  }
+  for (int l = lay.nrun; l < nl; l += 2) {
+    const size_t b0 = qbase + lay.offset(l), b1 = qbase + lay.offset(l + 1);
+    const uint32_t n0 = cc[l], n1 = cc[l + 1];
+    seen += n0 + n1;
+    uint32_t i0 = 0, i1 = 0;
+    while (i0 < n0 || i1 < n1) {
+      if (i1 >= n1 || (i0 < n0 && cand_idx[b0 + i0] < cand_idx[b1 + i1]))
+        visit(b0 + i0++);
+      else
+        visit(b1 + i1++);
     }
   }
   atomicAdd(&counters[0], seen);
-  atomicAdd(&counters[1], rescored);
+  atomicAdd(&counters[1], seen);
   int c = 0;
   for (int j = 0; j < k; ++j) {
     if (hi[j] >= 0) {
@@ -598,41 +843,43 @@ int ensure_bf16(sc_index* ix, hipStream_t s, bool* bad_out, int64_t* npad_out) {
   void* b = nullptr;
   float* terms = nullptr;
   SC_HIP(hipMalloc(&b, std::max<size_t>(1, n) * ldb * sizeof(uint16_t)));
-  SC_HIP(hipMalloc(reinterpret_cast<void**>(&terms), sizeof(float) * 2 * npad));
+  SC_HIP(hipMalloc(reinterpret_cast<void**>(&terms), sizeof(float) * 4 * npad));
   DevBuf flag;
   SC_HIP(flag.alloc(4, s));
   SC_HIP(hipMemsetAsync(flag.p, 0, 4, s));
   hipLaunchKernelGGL(fill_terms_kernel, dim3(static_cast<unsigned>((npad - n + 255) / 256)), dim3(256),
-                     0, s, terms, terms + npad, n, npad);
+                     0, s, terms, terms + npad, terms + 2 * npad, terms + 3 * npad, n, npad);
   if (n > 0)
     hipLaunchKernelGGL(prep_rows_kernel, dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0,
                        s, static_cast<const float*>(ix->rows), n, static_cast<int>(ix->ld),
-                       static_cast<uint16_t*>(b), terms, terms + npad, flag.as<uint32_t>());
+                       static_cast<uint16_t*>(b), terms, terms + npad, terms + 2 * npad,
+                       terms + 3 * npad, flag.as<uint32_t>());
   SC_HIP(hipGetLastError());
   uint32_t bad = 0;
   SC_HIP(hipMemcpyAsync(&bad, flag.p, 4, hipMemcpyDeviceToHost, s));
   SC_HIP(hipStreamSynchronize(s));
   ix->rows_bf16 = b;
-  ix->norms2 = terms;  // [0, npad): y2m, [npad, 2 npad): nyb
+  ix->norms2 = terms;  // y2m | nyb | nry | nyh, npad floats each
   ix->ldb = bad ? 0 : static_cast<size_t>(ldb);
   *bad_out = bad != 0;
   return SC_OK;
 }
 
-template <int KS, bool SEEDED, int KT>
+template <int KS, int MODE, int KT>
 void launch_scan_t(dim3 grid, hipStream_t stream, const ScanArgs& a) {
   const size_t lds = scan_lds_bytes<KS>();
   static const bool attr = [] {
-    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&mixed_scan_kernel<KS, SEEDED, KT>),
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&mixed_scan_kernel<KS, MODE, KT>),
                               hipFuncAttributeMaxDynamicSharedMemorySize,
                               static_cast<int>(scan_lds_bytes<KS>()));
     return true;
   }();
   (void)attr;
-  hipLaunchKernelGGL((mixed_scan_kernel<KS, SEEDED, KT>), grid, dim3(kNT), lds, stream, a);
+  hipLaunchKernelGGL((mixed_scan_kernel<KS, MODE, KT>), grid, dim3(kNT), lds, stream, a);
 }
 
-int launch_scan(int ks, bool seeded, const ScanArgs& a, hipStream_t stream) {
+// mode: 0 running, 2 fixed threshold (from a seed), 3 the same storing U
+int launch_scan(int ks, int mode, const ScanArgs& a, hipStream_t stream) {
   const int64_t nblocks = a.nseg <= kXcds
                               ? (a.nqt + kXcds / a.nseg - 1) / (kXcds / a.nseg) * kXcds
                               : a.nqt * a.nseg;
@@ -640,17 +887,14 @@ int launch_scan(int ks, bool seeded, const ScanArgs& a, hipStream_t stream) {
   switch (ks) {
 #define SC_SCAN_CASE(KS)                                      \
   case KS:                                                   \
-    if (a.k <= 16) {                                         \
-      if (seeded)                                            \
-        launch_scan_t<KS, true, 16>(grid, stream, a);        \
-      else                                                   \
-        launch_scan_t<KS, false, 16>(grid, stream, a);       \
-    } else {                                                 \
-      if (seeded)                                            \
-        launch_scan_t<KS, true, 32>(grid, stream, a);        \
-      else                                                   \
-        launch_scan_t<KS, false, 32>(grid, stream, a);       \
-    }                                                        \
+    if (mode == 2)                                           \
+      launch_scan_t<KS, 2, 16>(grid, stream, a);             \
+    else if (mode == 3)                                      \
+      launch_scan_t<KS, 3, 16>(grid, stream, a);             \
+    else if (a.k <= 16)                                      \
+      launch_scan_t<KS, 0, 16>(grid, stream, a);             \
+    else                                                     \
+      launch_scan_t<KS, 0, 32>(grid, stream, a);             \
     break;
     SC_SCAN_CASE(1)
     SC_SCAN_CASE(2)
@@ -689,39 +933,71 @@ int knn_mixed_device(const sc_index* cix, const void* d_queries, size_t q_, size
   const float* y2m = ix->norms2;
   const float* nyb = ix->norms2 + npad;
 
-  // plan: ~kTargetBlocks workgroups (two per CU).  Segments are a power of
+  // plan: ~kTargetBlocks workgroups (one per CU).  Segments are a power of
   // two: up to 8 whole XCDs share a segment; small batches split the rows
   // further so that they still fill the chip.
   const int64_t nqt = (q + kQPB - 1) / kQPB;
   const int64_t tiles = std::max<int64_t>(1, (n + kRows - 1) / kRows);
   int nseg = 1;
   while (nseg < 64 && nqt * nseg < kTargetBlocks && nseg * 2 <= tiles) nseg *= 2;
-  // Large indexes: a first launch scans the first kPrefixRows rows (in nseg
-  // segments) and the k smallest U per query over them seed every later
-  // segment, which then starts from a tight threshold instead of an empty
-  // top-k.
-  const bool seeded = n >= kSeedMinRows;
-  const int nseg_p = seeded ? nseg : 0;
-  const int64_t P = seeded ? kPrefixRows : 0;
-  const int64_t seg_len_p =
-      seeded ? (kPrefixRows / kRows + nseg_p - 1) / nseg_p * kRows : 0;
-  const int64_t mtiles = std::max<int64_t>(1, (n - P + kRows - 1) / kRows);
-  const int64_t seg_len = (mtiles + nseg - 1) / nseg * kRows;
-  const int nseg_all = nseg_p + nseg;
-  const double Lseg = static_cast<double>(std::min<int64_t>(seg_len, std::max<int64_t>(n, 1)));
-  int64_t cap = static_cast<int64_t>(4.0 * k * (1.0 + std::log(std::max(1.0, Lseg / k)))) + 128;
-  cap = std::min<int64_t>(cap, std::max(seg_len, seg_len_p));
+  // Levels.  Small indexes: one running launch over all rows.  Indexes of
+  // kSeedMinRows rows or more: a running launch over the first kPrefixRows
+  // rows, then fixed-threshold launches over row ranges growing kLevelRatio
+  // times, each testing against the k-th U of all rows before it (the
+  // prefix's top-k, then the selection over the previous level's U).
+  struct Level {
+    int64_t r0, r1;
+    int mode;
+  };
+  std::vector<Level> levels;
+  if (n >= kSeedMinRows) {
+    levels.push_back({0, kPrefixRows, 0});
+    for (int64_t b = kPrefixRows; b < n;) {
+      const int64_t e = std::min<int64_t>(n, b * kLevelRatio);
+      levels.push_back({b, e, e < n ? 3 : 2});
+      b = e;
+    }
+  } else {
+    levels.push_back({0, n, 0});
+  }
+  auto seg_len_of = [&](const Level& L) {
+    const int64_t lt = std::max<int64_t>(1, (L.r1 - L.r0 + kRows - 1) / kRows);
+    return (lt + nseg - 1) / nseg * kRows;
+  };
+  const int nfix_levels = static_cast<int>(levels.size()) - 1;
+  ListLayout lay{};
+  lay.nrun = nseg;
+  {
+    const int64_t sl = seg_len_of(levels[0]);
+    const double Lseg = static_cast<double>(std::min<int64_t>(sl, std::max<int64_t>(n, 1)));
+    int64_t c = static_cast<int64_t>(4.0 * k * (1.0 + std::log(std::max(1.0, Lseg / k)))) + 128;
+    lay.cap_run = static_cast<int>(std::min<int64_t>(c, sl));
+  }
+  lay.nfix = 2 * nseg * nfix_levels;
+  lay.cap_fix = 0;
+  for (int i = 1; i <= nfix_levels; ++i) {
+    // expected entries of one half-list: rows of a segment times k over
+    // the rows before the level, doubled for the bound's band
+    const Level& L = levels[i];
+    const double per_seg = static_cast<double>(L.r1 - L.r0) / nseg;
+    const double expect = per_seg * k / static_cast<double>(L.r0);
+    const int64_t c = static_cast<int64_t>(6.0 * expect) + 128;
+    lay.cap_fix = static_cast<int>(std::max<int64_t>(lay.cap_fix, std::min<int64_t>(c, seg_len_of(L))));
+  }
+  const int64_t perq = lay.per_query();
 
-  DevBuf qb, x2, cx, cidx, cL, ccnt, hs, lst, flags, tpu, seed;
+  DevBuf qb, x2, cx, cidx, cU, cS, ccnt, hs, lst, flags, tpu, seedA, seedB;
   SC_HIP(qb.alloc(sizeof(uint16_t) * q * ld, stream));
   SC_HIP(x2.alloc(sizeof(float) * q, stream));
-  SC_HIP(cx.alloc(sizeof(float) * q, stream));
-  SC_HIP(cidx.alloc(sizeof(uint32_t) * q * nseg_all * cap, stream));
-  SC_HIP(cL.alloc(sizeof(float) * q * nseg_all * cap, stream));
-  SC_HIP(ccnt.alloc(sizeof(uint32_t) * q * nseg_all, stream));
-  if (seeded) {
-    SC_HIP(tpu.alloc(sizeof(float) * q * nseg_p * k, stream));
-    SC_HIP(seed.alloc(sizeof(float) * q * k, stream));
+  SC_HIP(cx.alloc(sizeof(float) * 3 * q, stream));  // cxb | qa1 | qa2
+  SC_HIP(cidx.alloc(sizeof(uint32_t) * q * perq, stream));
+  SC_HIP(cS.alloc(sizeof(double) * q * perq, stream));
+  SC_HIP(ccnt.alloc(sizeof(uint32_t) * q * lay.nlists(), stream));
+  if (nfix_levels > 0) {
+    SC_HIP(tpu.alloc(sizeof(float) * q * nseg * k, stream));
+    SC_HIP(seedA.alloc(sizeof(float) * q * k, stream));
+    SC_HIP(seedB.alloc(sizeof(float) * q * k, stream));
+    if (nfix_levels > 1) SC_HIP(cU.alloc(sizeof(float) * q * perq, stream));
   }
   SC_HIP(hs.alloc(sizeof(double) * q * k, stream));
   SC_HIP(lst.alloc(sizeof(uint32_t) * q, stream));
@@ -733,56 +1009,81 @@ int knn_mixed_device(const sc_index* cix, const void* d_queries, size_t q_, size
 
   hipLaunchKernelGGL(prep_queries_kernel, dim3(static_cast<unsigned>((q + 255) / 256)), dim3(256),
                      0, stream, qf, q, ld, qb.as<uint16_t>(), x2.as<float>(), cx.as<float>(),
-                     lst.as<uint32_t>(), list_n);
+                     cx.as<float>() + q, cx.as<float>() + 2 * q, lst.as<uint32_t>(), list_n);
   SC_HIP(hipGetLastError());
   ScanArgs sa{};
   sa.rows_b = static_cast<const uint16_t*>(ix->rows_bf16);
   sa.y2m = y2m;
   sa.nyb = nyb;
+  sa.nry = ix->norms2 + 2 * npad;
+  sa.npad = npad;
   sa.n = n;
   sa.q_b = qb.as<uint16_t>();
   sa.x2m = x2.as<float>();
   sa.cxbq = cx.as<float>();
+  sa.qa1 = cx.as<float>() + q;
+  sa.qa2 = cx.as<float>() + 2 * q;
   sa.q = q;
   sa.k = k;
   sa.nqt = nqt;
-  sa.nseg_all = nseg_all;
+  sa.nseg = nseg;
+  sa.lay = lay;
   sa.cand_idx = cidx.as<uint32_t>();
-  sa.cand_L = cL.as<float>();
+  sa.cand_U = nfix_levels > 1 ? cU.as<float>() : nullptr;
   sa.cand_count = ccnt.as<uint32_t>();
-  sa.cap = static_cast<int>(cap);
-  if (seeded) {
-    ProfScope ps("knn_mixed_prefix", stream);
-    ScanArgs pa = sa;
-    pa.row0 = 0;
-    pa.row_end = P;
-    pa.seg_len = seg_len_p;
-    pa.nseg = nseg_p;
-    pa.seg_base = 0;
-    pa.topu_out = tpu.as<float>();
-    if (int rc = launch_scan(ld / 16, false, pa, stream)) return rc;
-    hipLaunchKernelGGL(merge_topu_kernel, dim3(static_cast<unsigned>((q + 255) / 256)), dim3(256), 0,
-                       stream, tpu.as<float>(), q, nseg_p, k, seed.as<float>());
-    SC_HIP(hipGetLastError());
-  }
+  float* seed_cur = seedA.as<float>();
+  float* seed_next = seedB.as<float>();
   {
+    // every launch before the last one is timed as the prefix
+    ProfScope ps(nfix_levels > 0 ? "knn_mixed_prefix" : "knn_mixed_gemm", stream);
+    ScanArgs la = sa;
+    la.row0 = 0;
+    la.row_end = levels[0].r1;
+    la.seg_len = seg_len_of(levels[0]);
+    la.lbase = 0;
+    la.topu_out = nfix_levels > 0 ? tpu.as<float>() : nullptr;
+    if (int rc = launch_scan(ld / 16, 0, la, stream)) return rc;
+    if (nfix_levels > 0) {
+      hipLaunchKernelGGL(merge_topu_kernel, dim3(static_cast<unsigned>((q + 255) / 256)), dim3(256),
+                         0, stream, tpu.as<float>(), q, nseg, k, seed_cur);
+      SC_HIP(hipGetLastError());
+    }
+    for (int i = 1; i < nfix_levels; ++i) {
+      ScanArgs fa = sa;
+      fa.row0 = levels[i].r0;
+      fa.row_end = levels[i].r1;
+      fa.seg_len = seg_len_of(levels[i]);
+      fa.lbase = nseg + 2 * nseg * (i - 1);
+      fa.seed = seed_cur;
+      if (int rc = launch_scan(ld / 16, 3, fa, stream)) return rc;
+      hipLaunchKernelGGL(select_seed_kernel, dim3(static_cast<unsigned>((q + 255) / 256)),
+                         dim3(256), 0, stream, seed_cur, cU.as<float>(), ccnt.as<uint32_t>(), lay,
+                         fa.lbase, 2 * nseg, q, k, seed_next);
+      SC_HIP(hipGetLastError());
+      std::swap(seed_cur, seed_next);
+    }
+  }
+  if (nfix_levels > 0) {
     ProfScope ps("knn_mixed_gemm", stream);
-    ScanArgs ma = sa;
-    ma.row0 = P;
-    ma.row_end = n;
-    ma.seg_len = seg_len;
-    ma.nseg = nseg;
-    ma.seg_base = nseg_p;
-    ma.seed = seeded ? seed.as<float>() : nullptr;
-    if (int rc = launch_scan(ld / 16, seeded, ma, stream)) return rc;
+    const int i = nfix_levels;
+    ScanArgs fa = sa;
+    fa.row0 = levels[i].r0;
+    fa.row_end = levels[i].r1;
+    fa.seg_len = seg_len_of(levels[i]);
+    fa.lbase = nseg + 2 * nseg * (i - 1);
+    fa.seed = seed_cur;
+    if (int rc = launch_scan(ld / 16, 2, fa, stream)) return rc;
   }
   {
     ProfScope ps("knn_mixed_rescore", stream);
+    hipLaunchKernelGGL(mixed_rescore_kernel, dim3(static_cast<unsigned>(q)), dim3(kRescoreThreads),
+                       0, stream, static_cast<const float*>(ix->rows), qf, ld, q, lay,
+                       cidx.as<uint32_t>(), ccnt.as<uint32_t>(), x2.as<float>(), cS.as<double>());
+    SC_HIP(hipGetLastError());
     hipLaunchKernelGGL(mixed_replay_kernel, dim3(static_cast<unsigned>((q + 63) / 64)), dim3(64), 0,
-                       stream, static_cast<const float*>(ix->rows), qf, ld, q, k, nseg_all,
-                       static_cast<int>(cap), cidx.as<uint32_t>(), cL.as<float>(),
-                       ccnt.as<uint32_t>(), x2.as<float>(), hs.as<double>(), d_idx, d_dist, d_count,
-                       lst.as<uint32_t>(), list_n, counters);
+                       stream, q, k, lay, cidx.as<uint32_t>(), cS.as<double>(), ccnt.as<uint32_t>(),
+                       x2.as<float>(), hs.as<double>(), d_idx, d_dist, d_count, lst.as<uint32_t>(),
+                       list_n, counters);
     SC_HIP(hipGetLastError());
   }
   if (int rc = launch_serial_exact_f32(ix, qf, k, lst.as<uint32_t>(), list_n, d_idx, d_dist,
