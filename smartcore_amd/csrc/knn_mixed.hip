@@ -209,8 +209,8 @@ struct Scan {
   static_assert(NP <= 2, "at most two pieces per thread");
 };
 
-// Dynamic LDS of the scan: four tile slots | trans[kWaves][64][32].
-constexpr int kSlots = 4;
+// Dynamic LDS of the scan: eight tile slots | trans[kWaves][64][32].
+constexpr int kSlots = 8;
 template <int KS>
 size_t scan_lds_bytes() {
   return kSlots * static_cast<size_t>(Scan<KS>::SLOT) + sizeof(float) * kWaves * 64 * 32;
@@ -426,10 +426,13 @@ __global__ __launch_bounds__(kNT, 2) void mixed_scan_kernel(const ScanArgs a) {
     if (tid < 128) reinterpret_cast<float*>(base + 2 * kRows * S::RS)[tid] = r.tv;
   };
 
-  // LDS holds four tiles: the one being scanned, the next one and two being
-  // written.  Tiles are stored and loaded in pairs on even steps: the
-  // stores of tiles t + 2, t + 3 wait for loads issued two steps earlier,
-  // never for the previous step's.
+  // LDS holds eight tiles.  Every fourth step (t = 0 mod 4) stores tiles
+  // t + 4 .. t + 7, loaded into registers four steps earlier, into the
+  // slots of tiles t - 4 .. t - 1, then loads tiles t + 8 .. t + 11.  A
+  // barrier after every fourth step (t = 3 mod 4) orders both the stores
+  // after the last reads of those slots and the next reads after the
+  // stores, so waves may drift apart by up to four steps, and the wait
+  // before a store covers loads issued four steps earlier.
   auto slot_ptr = [&](int64_t t) { return smem + static_cast<int>(t & (kSlots - 1)) * S::SLOT; };
   auto mfma_tile = [&](int64_t t, f32x16 (&acc)[2]) {
     const uint16_t* A = reinterpret_cast<const uint16_t*>(slot_ptr(t));
@@ -600,14 +603,17 @@ __global__ __launch_bounds__(kNT, 2) void mixed_scan_kernel(const ScanArgs a) {
     }
   };
 
-  TileRegs ra, rb;
+  TileRegs tr[4];
   f32x16 acc[2];
-  if (ntiles > 0) gload(0, ra);
-  if (ntiles > 1) gload(1, rb);
-  if (ntiles > 0) swrite(0, ra);
-  if (ntiles > 1) swrite(1, rb);
-  if (ntiles > 2) gload(2, ra);
-  if (ntiles > 3) gload(3, rb);
+#pragma unroll
+  for (int j = 0; j < 4; ++j)
+    if (j < ntiles) gload(j, tr[j]);
+#pragma unroll
+  for (int j = 0; j < 4; ++j)
+    if (j < ntiles) swrite(j, tr[j]);
+#pragma unroll
+  for (int j = 0; j < 4; ++j)
+    if (4 + j < ntiles) gload(4 + j, tr[j]);
   __syncthreads();
   for (int64_t t = 0; t < ntiles; ++t) {
     mfma_tile(t, acc);
@@ -618,13 +624,15 @@ __global__ __launch_bounds__(kNT, 2) void mixed_scan_kernel(const ScanArgs a) {
     } else {
       if (hit[0] || hit[1]) fixed_hits(t, acc, hit);
     }
-    if ((t & 1) == 0) {
-      if (t + 2 < ntiles) swrite(t + 2, ra);
-      if (t + 3 < ntiles) swrite(t + 3, rb);
-      if (t + 4 < ntiles) gload(t + 4, ra);
-      if (t + 5 < ntiles) gload(t + 5, rb);
+    if ((t & 3) == 0) {
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        if (t + 4 + j < ntiles) swrite(t + 4 + j, tr[j]);
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        if (t + 8 + j < ntiles) gload(t + 8 + j, tr[j]);
     }
-    __syncthreads();
+    if ((t & 3) == 3) __syncthreads();
   }
   if constexpr (RUNNING) {
     if (qo < q) a.cand_count[static_cast<size_t>(qo) * nl + lseg] = cnt;
