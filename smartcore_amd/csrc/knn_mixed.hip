@@ -269,13 +269,11 @@ struct ScanArgs {
   float* topu_out;     // q x nseg x k, or null
 };
 
-// KT: registers of the per-query top-k of a running launch (16 for
-// k <= 16, else 32).
-template <int KS, int MODE, int KT>
+// Running launch.  KT: registers of the per-query top-k (16 for k <= 16,
+// else 32).
+template <int KS, int KT>
 __global__ __launch_bounds__(kNT, 2) void mixed_scan_kernel(const ScanArgs a) {
   using S = Scan<KS>;
-  constexpr bool RUNNING = MODE == 0;
-  constexpr bool STORE_U = MODE == 3;
   const uint16_t* __restrict__ rows_b = a.rows_b;
   const float* __restrict__ y2m = a.y2m;
   const int64_t n = a.n;
@@ -318,9 +316,7 @@ __global__ __launch_bounds__(kNT, 2) void mixed_scan_kernel(const ScanArgs a) {
   const int64_t ntiles = r_end > r_begin ? (r_end - r_begin + kRows - 1) / kRows : 0;
   const int nl = a.lay.nlists();
   const int64_t perq = a.lay.per_query();
-  // this launch's list(s) for the segment: running lbase + seg, fixed
-  // lbase + 2 seg + half
-  const int lseg = RUNNING ? a.lbase + seg : a.lbase + 2 * seg + half;
+  const int lseg = a.lbase + seg;  // this segment's list
   const int64_t loff = a.lay.offset(lseg);
   const int lcap = a.lay.cap(lseg);
   // rare-path transpose: each lane's 32 values of a tile (both column blocks)
@@ -329,7 +325,7 @@ __global__ __launch_bounds__(kNT, 2) void mixed_scan_kernel(const ScanArgs a) {
   // per column block values as two-element vectors, so that picking the
   // lane's own block is a register select, not an indexed private array
   bf16x8 bq[2][KS];
-  f32x2 thr, x2v, cxq, kfix;
+  f32x2 thr, x2v, cxq;
   i32x2 qvalid;
 #pragma unroll
   for (int cb = 0; cb < 2; ++cb) {
@@ -342,14 +338,10 @@ __global__ __launch_bounds__(kNT, 2) void mixed_scan_kernel(const ScanArgs a) {
     for (int ks = 0; ks < KS; ++ks) bq[cb][ks] = load_frag(q_b + qr * S::LD + 16 * ks + 8 * half);
     x2v[cb] = qvalid[cb] ? x2m[qr] : INFINITY;
     cxq[cb] = qi < q ? cxbq[qr] : 0.0f;
-    // a fixed launch tests against the seed's k-th U; T - x2m rounded up
-    kfix[cb] = !RUNNING && qvalid[cb] ? a.seed[qr * k + k - 1] : INFINITY;
-    thr[cb] = !RUNNING && qvalid[cb]
-                  ? __double2float_ru(static_cast<double>(kfix[cb]) - static_cast<double>(x2v[cb]))
-                  : INFINITY;
+    thr[cb] = INFINITY;
   }
-  // Running launches: the lane owns query (column block `half`, column
-  // col) and keeps that query's k smallest U in registers tk[0..k),
+  // The lane owns query (column block `half`, column col) and keeps that
+  // query's k smallest U in registers tk[0..k),
   // ascending, with tk[k..KT) held at -inf (the insertion below leaves them
   // there, so the k-th smallest is the maximum of tk and no register is
   // indexed by k), its candidate count and its threshold.
@@ -358,35 +350,20 @@ __global__ __launch_bounds__(kNT, 2) void mixed_scan_kernel(const ScanArgs a) {
   const double co = static_cast<double>(half == 0 ? cxq.x : cxq.y);
   const double a1o = qo < q ? static_cast<double>(a.qa1[qo]) : 0.0;
   const double a2o = qo < q ? static_cast<double>(a.qa2[qo]) : 0.0;
-  float tk[RUNNING ? KT : 1];
-  if constexpr (RUNNING) {
+  float tk[KT];
 #pragma unroll
-    for (int j = 0; j < KT; ++j)
-      tk[j] = j >= k ? -INFINITY : INFINITY;
-  }
+  for (int j = 0; j < KT; ++j) tk[j] = j >= k ? -INFINITY : INFINITY;
   auto kth_of = [&]() {
     float m[KT];
 #pragma unroll
-    for (int j = 0; j < KT; ++j) m[j] = tk[RUNNING ? j : 0];
+    for (int j = 0; j < KT; ++j) m[j] = tk[j];
 #pragma unroll
     for (int w = KT / 2; w >= 1; w /= 2)
 #pragma unroll
       for (int j = 0; j < w; ++j) m[j] = fmaxf(m[j], m[j + w]);
     return m[0];
   };
-  // fixed launches: the lane writes rows of its half for both column
-  // blocks' queries (counts cntf), with their tight-bound terms
   uint32_t cnt = 0;
-  uint32_t cntf[2] = {0u, 0u};
-  f32x2 fa1 = {0.0f, 0.0f}, fa2 = {0.0f, 0.0f};
-  if constexpr (!RUNNING) {
-#pragma unroll
-    for (int cb = 0; cb < 2; ++cb) {
-      const int64_t qi = qw0 + 32 * cb + col;
-      fa1[cb] = qi < q ? a.qa1[qi] : 0.0f;
-      fa2[cb] = qi < q ? a.qa2[qi] : 0.0f;
-    }
-  }
   // both lanes of a column hold the thresholds of both column blocks
   auto set_thr = [&](float own) {
     const float other = __shfl_xor(own, 32);
@@ -445,22 +422,18 @@ __global__ __launch_bounds__(kNT, 2) void mixed_scan_kernel(const ScanArgs a) {
       acc[1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa, bq[1][ks], acc[1], 0, 0, 0);
     }
   };
-  // fast epilogue of tile t: acc <- L - x2m; returns per column block
-  // whether any lane of the wave has a row below its query's threshold.
-  // Running launches use the loose bound (e = cxb nyb, one FMA), fixed
-  // launches the tight one (et = qa1 nry + qa2 nyh, two FMAs).
+  // fast epilogue of tile t: acc <- L - x2m with the loose bound
+  // (e = cxb nyb, one FMA and one add); returns per column block whether
+  // any lane of the wave has a row below its query's threshold
   auto epilogue = [&](int64_t t, f32x16 (&acc)[2], bool (&hit)[2]) {
     const unsigned char* base = slot_ptr(t);
     const float* ty2 = reinterpret_cast<const float*>(base + 2 * kRows * S::RS);
     const float* tny = ty2 + kRows;
-    const float* trr = tny + kRows;
-    const float* tnh = trr + kRows;
-    f32x4 yv[4], nv[4], rv[4];
+    f32x4 yv[4], nv[4];
 #pragma unroll
     for (int g = 0; g < 4; ++g) {
       yv[g] = *reinterpret_cast<const f32x4*>(ty2 + 8 * g + 4 * half);
-      nv[g] = *reinterpret_cast<const f32x4*>((RUNNING ? tny : tnh) + 8 * g + 4 * half);
-      if constexpr (!RUNNING) rv[g] = *reinterpret_cast<const f32x4*>(trr + 8 * g + 4 * half);
+      nv[g] = *reinterpret_cast<const f32x4*>(tny + 8 * g + 4 * half);
     }
 #pragma unroll
     for (int cb = 0; cb < 2; ++cb) {
@@ -469,11 +442,7 @@ __global__ __launch_bounds__(kNT, 2) void mixed_scan_kernel(const ScanArgs a) {
       for (int g = 0; g < 4; ++g)
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          float v;
-          if constexpr (RUNNING)
-            v = fmaf(-cxq[cb], nv[g][r], acc[cb][4 * g + r]) + yv[g][r];
-          else
-            v = fmaf(-fa2[cb], nv[g][r], fmaf(-fa1[cb], rv[g][r], acc[cb][4 * g + r])) + yv[g][r];
+          const float v = fmaf(-cxq[cb], nv[g][r], acc[cb][4 * g + r]) + yv[g][r];
           acc[cb][4 * g + r] = v;
           m = fminf(m, v);
         }
@@ -487,9 +456,8 @@ __global__ __launch_bounds__(kNT, 2) void mixed_scan_kernel(const ScanArgs a) {
     return (q1 * static_cast<double>(nr) + q2 * static_cast<double>(nh)) *
            (1.0 + 4.440892098500626e-16);
   };
-  // Rare path of a running launch for tile t (see the file comment): each
-  // lane handles its own query, the rows of the tile below its threshold,
-  // in row order.
+  // Rare path of tile t (see the file comment): each lane handles its own
+  // query, the rows of the tile below its threshold, in row order.
   auto slow = [&](int64_t t, f32x16 (&acc)[2]) {
     const unsigned char* base = slot_ptr(t);
     const float* ty2 = reinterpret_cast<const float*>(base + 2 * kRows * S::RS);
@@ -543,11 +511,9 @@ __global__ __launch_bounds__(kNT, 2) void mixed_scan_kernel(const ScanArgs a) {
       const float uf = __double2float_ru(U);
       // insert into the ascending tk[0..k): slot j takes
       // min(tk[j], max(tk[j-1], uf)); slots past k stay -inf
-      if constexpr (RUNNING) {
 #pragma unroll
-        for (int j = KT - 1; j > 0; --j) tk[j] = fminf(tk[j], fmaxf(tk[j - 1], uf));
-        tk[0] = fminf(tk[0], uf);
-      }
+      for (int j = KT - 1; j > 0; --j) tk[j] = fminf(tk[j], fmaxf(tk[j - 1], uf));
+      tk[0] = fminf(tk[0], uf);
       if (cnt < static_cast<uint32_t>(lcap)) {
         const size_t sl = static_cast<size_t>(qo) * perq + loff + cnt;
         cand_idx[sl] = static_cast<uint32_t>(row);
@@ -559,50 +525,6 @@ __global__ __launch_bounds__(kNT, 2) void mixed_scan_kernel(const ScanArgs a) {
     }
     set_thr(tcur);
   };
-  // Hits of a fixed launch for tile t: no threshold changes, so each lane
-  // writes its own rows (its half of the tile, ascending) of both column
-  // blocks' queries to the queries' lists of that half.
-  auto fixed_hits = [&](int64_t t, f32x16 (&acc)[2], const bool (&hit)[2]) {
-    const unsigned char* base = slot_ptr(t);
-    const float* ty2 = reinterpret_cast<const float*>(base + 2 * kRows * S::RS);
-    const float* tny = ty2 + kRows;
-    const float* trr = tny + kRows;
-    const float* tnh = trr + kRows;
-    const int64_t t0 = r_begin + t * kRows;
-#pragma unroll
-    for (int cb = 0; cb < 2; ++cb) {
-      if (!hit[cb]) continue;
-      const int64_t qi = qw0 + 32 * cb + col;
-#pragma unroll
-      for (int v = 0; v < 16; ++v) {
-        const float val = acc[cb][v];
-        if (qvalid[cb] != 0 && val < thr[cb]) {
-          const int r = 8 * (v >> 2) + 4 * half + (v & 3);
-          const int64_t row = t0 + r;
-          if (row < r_end) {
-            // the epilogue formed L - x2m with the tight bound
-            const double L = static_cast<double>(x2v[cb]) + static_cast<double>(val);
-            if (L < static_cast<double>(kfix[cb])) {
-              if (cntf[cb] < static_cast<uint32_t>(lcap)) {
-                const size_t sl = static_cast<size_t>(qi) * perq + loff + cntf[cb];
-                cand_idx[sl] = static_cast<uint32_t>(row);
-                if constexpr (STORE_U) {
-                  const double et = tight_e(static_cast<double>(fa1[cb]),
-                                            static_cast<double>(fa2[cb]), trr[r], tnh[r]);
-                  const double U = L + 2.0 * et +
-                                   (kHiMargin + 2.0 * kLoMargin) *
-                                       (static_cast<double>(x2v[cb]) + static_cast<double>(ty2[r]));
-                  a.cand_U[sl] = __double2float_ru(U);
-                }
-              }
-              ++cntf[cb];
-            }
-          }
-        }
-      }
-    }
-  };
-
   TileRegs tr[4];
   f32x16 acc[2];
 #pragma unroll
@@ -619,11 +541,7 @@ __global__ __launch_bounds__(kNT, 2) void mixed_scan_kernel(const ScanArgs a) {
     mfma_tile(t, acc);
     bool hit[2];
     epilogue(t, acc, hit);
-    if constexpr (RUNNING) {
-      if (hit[0] || hit[1]) slow(t, acc);
-    } else {
-      if (hit[0] || hit[1]) fixed_hits(t, acc, hit);
-    }
+    if (hit[0] || hit[1]) slow(t, acc);
     if ((t & 3) == 0) {
 #pragma unroll
       for (int j = 0; j < 4; ++j)
@@ -634,19 +552,275 @@ __global__ __launch_bounds__(kNT, 2) void mixed_scan_kernel(const ScanArgs a) {
     }
     if ((t & 3) == 3) __syncthreads();
   }
-  if constexpr (RUNNING) {
-    if (qo < q) a.cand_count[static_cast<size_t>(qo) * nl + lseg] = cnt;
-    if (a.topu_out != nullptr && qo < q) {
+  if (qo < q) a.cand_count[static_cast<size_t>(qo) * nl + lseg] = cnt;
+  if (a.topu_out != nullptr && qo < q) {
 #pragma unroll
-      for (int j = 0; j < KT; ++j)
-        if (j < k) a.topu_out[(static_cast<size_t>(qo) * nseg + seg) * k + j] = tk[j];
-    }
+    for (int j = 0; j < KT; ++j)
+      if (j < k) a.topu_out[(static_cast<size_t>(qo) * nseg + seg) * k + j] = tk[j];
+  }
+}
+
+// Fixed-threshold launch (see the file comment).  The bound's terms are
+// folded into one more MFMA K step: a tile row carries, after its ld
+// values, the BF16 columns [y2m hi, mid, lo, RU(nry), RU(nyh), 0 ...] and
+// the query side [1, 1, 1, -RU(qa1), -RU(qa2), 0 ...], so the MFMA leaves
+// L - x2m = y2m - 2C - e (e = RU(qa1) RU(nry) + RU(qa2) RU(nyh), exact BF16
+// products, >= the tight bound) and the epilogue is a minimum.  The FP32
+// roundings of those adds and of the y2m split stay inside the 2^-18
+// margin.  Each lane writes its own rows (its half of the tile) of both
+// column blocks' queries to the queries' lists of that half.  Eight tile
+// slots (two workgroups fit one CU): step w (w = 0 mod 4) stores tiles
+// w + 4 .. w + 7 into the slots of tiles w - 4 .. w - 1 and loads tiles
+// w + 8 .. w + 11 into registers.  The MFMAs of tile t + 1 are issued
+// before the epilogue of tile t, so the last LDS reads of the slot of tile
+// w - 1 are in step w - 2; the barrier at the start of every step w + 3
+// (w = 0 mod 4) orders them before the stores of step w + 4 and the stores
+// of step w before the reads of tile w + 4 in step w + 3.  The candidate
+// path reads the row terms from global memory, so the slots hold rows only.
+template <int KS>
+struct FixTile {
+  static constexpr int LD = 16 * KS;
+  static constexpr int RS = LD + 16 + 8;  // row | folded K step | 16 B padding (bf16)
+  static constexpr int VEC = kRows * LD / 8;
+  static constexpr int NP = (VEC + kNT - 1) / kNT;
+  static constexpr int SLOT = kRows * RS * 2;  // rows with their folded columns (bytes)
+};
+constexpr int kFixSlots = 8;
+template <int KS>
+size_t fixed_lds_bytes() {
+  return kFixSlots * static_cast<size_t>(FixTile<KS>::SLOT);
+}
+
+__device__ __forceinline__ uint16_t bf16_ru_pos(float f) {
+  // f >= 0 finite: the smallest BF16 >= f
+  const uint32_t u = __float_as_uint(f);
+  return static_cast<uint16_t>((u >> 16) + ((u & 0xffffu) != 0u ? 1u : 0u));
+}
+
+template <int KS, bool STORE_U>
+__global__ __launch_bounds__(kNT, 2) void fixed_scan_kernel(const ScanArgs a) {
+  using S = FixTile<KS>;
+  const uint16_t* __restrict__ rows_b = a.rows_b;
+  const float* __restrict__ y2m = a.y2m;
+  const int64_t n = a.n;
+  const int64_t q = a.q;
+  const int k = a.k;
+  const int nseg = a.nseg;
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int64_t bid = blockIdx.x;
+  const int xcd = static_cast<int>(bid % kXcds);
+  const int64_t slot_i = bid / kXcds;
+  int seg;
+  int64_t qt;
+  if (nseg <= kXcds) {
+    const int groups = kXcds / nseg;
+    seg = xcd % nseg;
+    qt = slot_i * groups + xcd / nseg;
   } else {
+    const int m = nseg / kXcds;
+    seg = static_cast<int>(slot_i % m) * kXcds + xcd;
+    qt = slot_i / m;
+  }
+  if (qt >= a.nqt) return;  // whole workgroup leaves together
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int col = lane & 31;
+  const int half = lane >> 5;
+  const int64_t qw0 = qt * kQPB + wave * kQPW;
+  const int64_t r_begin = a.row0 + static_cast<int64_t>(seg) * a.seg_len;
+  const int64_t r_end = min(a.row_end, r_begin + a.seg_len);
+  const int64_t ntiles = r_end > r_begin ? (r_end - r_begin + kRows - 1) / kRows : 0;
+  const int nl = a.lay.nlists();
+  const int64_t perq = a.lay.per_query();
+  const int lseg = a.lbase + 2 * seg + half;  // this lane's half-list of the segment
+  const int64_t loff = a.lay.offset(lseg);
+  const int lcap = a.lay.cap(lseg);
+
+  bf16x8 bq[2][KS + 1];
+  f32x2 thr, x2v, kfix, fq1, fq2;
+  i32x2 qvalid;
+#pragma unroll
+  for (int cb = 0; cb < 2; ++cb) {
+    const int64_t qi = qw0 + 32 * cb + col;
+    qvalid[cb] = qi < q && a.x2m[qi < q ? qi : 0] < INFINITY;
+    const int64_t qr = qi < q ? qi : 0;
+#pragma unroll
+    for (int ks = 0; ks < KS; ++ks)
+      bq[cb][ks] = load_frag(a.q_b + qr * S::LD + 16 * ks + 8 * half);
+    x2v[cb] = qvalid[cb] ? a.x2m[qr] : INFINITY;
+    // the seed's k-th U; the epilogue threshold is T - x2m rounded up
+    kfix[cb] = qvalid[cb] ? a.seed[qr * k + k - 1] : INFINITY;
+    thr[cb] = qvalid[cb]
+                  ? __double2float_ru(static_cast<double>(kfix[cb]) - static_cast<double>(x2v[cb]))
+                  : INFINITY;
+    const uint16_t b1 = bf16_ru_pos(a.qa1[qr]), b2 = bf16_ru_pos(a.qa2[qr]);
+    fq1[cb] = __uint_as_float(static_cast<uint32_t>(b1) << 16);
+    fq2[cb] = __uint_as_float(static_cast<uint32_t>(b2) << 16);
+    const uint32_t one = 0x3f80u, n1 = 0x8000u | b1, n2 = 0x8000u | b2;
+    const u32x4 fold = half == 0 ? u32x4{one | (one << 16), one | (n1 << 16), n2, 0u}
+                                 : u32x4{0u, 0u, 0u, 0u};
+    bq[cb][KS] = __builtin_bit_cast(bf16x8, fold);
+  }
+  uint32_t cntf[2] = {0u, 0u};
+
+  // tile loads: thread tid carries pieces tid and tid + kNT of the rows;
+  // threads 0..31 carry the four terms of row tid
+  struct TileRegs {
+    u32x4 v[S::NP];
+    f32x4 tm;
+  };
+  auto gload = [&](int64_t t, TileRegs& r) {
+    const int64_t t0 = r_begin + t * kRows;
+#pragma unroll
+    for (int pc = 0; pc < S::NP; ++pc) {
+      const int e = tid + kNT * pc;
+      if (e < S::VEC) {
+        const int64_t row = min<int64_t>(t0 + e / (S::LD / 8), n - 1);
+        r.v[pc] = *reinterpret_cast<const u32x4*>(rows_b + row * S::LD + 8 * (e % (S::LD / 8)));
+      }
+    }
+    // y2m, nyb, nry and nyh are consecutive arrays of a.npad floats
+    if (tid < kRows) {
+      const float* p = y2m + min<int64_t>(t0 + tid, a.npad - 1);
+      r.tm = f32x4{p[0], p[a.npad], p[2 * a.npad], p[3 * a.npad]};
+    }
+  };
+  auto swrite = [&](int sl, const TileRegs& r) {
+    unsigned char* base = smem + sl * S::SLOT;
+#pragma unroll
+    for (int pc = 0; pc < S::NP; ++pc) {
+      const int e = tid + kNT * pc;
+      if (e < S::VEC)
+        *reinterpret_cast<u32x4*>(base + 2 * ((e / (S::LD / 8)) * S::RS + 8 * (e % (S::LD / 8)))) =
+            r.v[pc];
+    }
+    if (tid < kRows) {
+      // folded columns: y2m split in three (exact up to 2^-24 y2m), the
+      // row factors rounded up
+      const float y = r.tm[0];
+      const uint32_t h = bf16_rne(y);
+      const float e1 = y - __uint_as_float(h << 16);
+      const uint32_t m = bf16_rne(e1);
+      const uint32_t l = bf16_rne(e1 - __uint_as_float(m << 16));
+      const uint32_t c1 = bf16_ru_pos(r.tm[2]), c2 = bf16_ru_pos(r.tm[3]);
+      u32x4* fc = reinterpret_cast<u32x4*>(base + 2 * (tid * S::RS + S::LD));
+      fc[0] = u32x4{h | (m << 16), l | (c1 << 16), c2, 0u};
+      fc[1] = u32x4{0u, 0u, 0u, 0u};
+    }
+  };
+  // all A fragments of the tile are read before its MFMAs, so the LDS
+  // latency is paid once per tile, not once per K step
+  auto mfma_tile = [&](int sl, f32x16 (&acc)[2]) {
+    const uint16_t* A = reinterpret_cast<const uint16_t*>(smem + sl * S::SLOT);
+    bf16x8 fa[KS + 1];
+#pragma unroll
+    for (int ks = 0; ks <= KS; ++ks) fa[ks] = load_frag(A + col * S::RS + 16 * ks + 8 * half);
+    __builtin_amdgcn_sched_barrier(0);
+    acc[0] = f32x16{};
+    acc[1] = f32x16{};
+#pragma unroll
+    for (int ks = 0; ks <= KS; ++ks) {
+      acc[0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[ks], bq[0][ks], acc[0], 0, 0, 0);
+      acc[1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[ks], bq[1][ks], acc[1], 0, 0, 0);
+    }
+  };
+  // hits of tile t: rows of this lane's half, ascending
+  auto hits = [&](int64_t t, const f32x16 (&acc)[2], const bool (&hit)[2]) {
+    const int64_t t0 = r_begin + t * kRows;
+#pragma unroll
+    for (int cb = 0; cb < 2; ++cb) {
+      if (!hit[cb]) continue;
+      const int64_t qi = qw0 + 32 * cb + col;
+#pragma unroll
+      for (int v = 0; v < 16; ++v) {
+        const float val = acc[cb][v];
+        if (qvalid[cb] != 0 && val < thr[cb]) {
+          const int r = 8 * (v >> 2) + 4 * half + (v & 3);
+          const int64_t row = t0 + r;
+          const double L = static_cast<double>(x2v[cb]) + static_cast<double>(val);
+          if (row < r_end && L < static_cast<double>(kfix[cb])) {
+            if (cntf[cb] < static_cast<uint32_t>(lcap)) {
+              const size_t e = static_cast<size_t>(qi) * perq + loff + cntf[cb];
+              a.cand_idx[e] = static_cast<uint32_t>(row);
+              if constexpr (STORE_U) {
+                // the e the MFMA subtracted: exact BF16 products
+                const double c1 = static_cast<double>(__uint_as_float(
+                    static_cast<uint32_t>(bf16_ru_pos(y2m[row + 2 * a.npad])) << 16));
+                const double c2 = static_cast<double>(__uint_as_float(
+                    static_cast<uint32_t>(bf16_ru_pos(y2m[row + 3 * a.npad])) << 16));
+                const double eu = (static_cast<double>(fq1[cb]) * c1 +
+                                   static_cast<double>(fq2[cb]) * c2) *
+                                  (1.0 + 4.440892098500626e-16);
+                const double U = L + 2.0 * eu +
+                                 (kHiMargin + 2.0 * kLoMargin) *
+                                     (static_cast<double>(x2v[cb]) + static_cast<double>(y2m[row]));
+                a.cand_U[e] = __double2float_ru(U);
+              }
+            }
+            ++cntf[cb];
+          }
+        }
+      }
+    }
+  };
+
+  if (ntiles == 0) {  // the same for the whole workgroup
 #pragma unroll
     for (int cb = 0; cb < 2; ++cb) {
       const int64_t qi = qw0 + 32 * cb + col;
-      if (qi < q) a.cand_count[static_cast<size_t>(qi) * nl + lseg] = cntf[cb];
+      if (qi < q) a.cand_count[static_cast<size_t>(qi) * nl + lseg] = 0u;
     }
+    return;
+  }
+  // Loads and stores of the ring are unconditional (tiles past the
+  // segment read clamped rows into slots nobody reads): a skipped store
+  // would leave its registers' load outstanding, and the compiler then
+  // waits for every load before each later load into them.
+  TileRegs tr[4];
+  f32x16 accA[2], accB[2];
+  // tiles 0..3 into slots 0..3, tiles 4..7 into registers
+#pragma unroll
+  for (int j = 0; j < 4; ++j) gload(j, tr[j]);
+#pragma unroll
+  for (int j = 0; j < 4; ++j) swrite(j, tr[j]);
+#pragma unroll
+  for (int j = 0; j < 4; ++j) gload(4 + j, tr[j]);
+  __syncthreads();
+  auto step = [&](int64_t t, f32x16 (&cur)[2], f32x16 (&nxt)[2]) {
+    if ((t & 3) == 3) __syncthreads();
+    if (t + 1 < ntiles) mfma_tile(static_cast<int>((t + 1) & (kFixSlots - 1)), nxt);
+    bool hit[2];
+#pragma unroll
+    for (int cb = 0; cb < 2; ++cb) {
+      float m = cur[cb][0];
+#pragma unroll
+      for (int v = 1; v < 16; ++v) m = fminf(m, cur[cb][v]);
+      hit[cb] = __ballot(qvalid[cb] != 0 && m < thr[cb]) != 0;
+    }
+    if (hit[0] || hit[1]) hits(t, cur, hit);
+    if ((t & 3) == 0) {
+      // vmcnt(0): the loads of four steps ago are done.  Said once here,
+      // on every path, it spares the compiler's own waits before each load.
+      __builtin_amdgcn_s_waitcnt(0x0F70);
+      // slots of tiles t - 4 .. t - 1 are those of t + 4 .. t + 7
+#pragma unroll
+      for (int j = 0; j < 4; ++j) swrite(static_cast<int>((t + 4 + j) & (kFixSlots - 1)), tr[j]);
+#pragma unroll
+      for (int j = 0; j < 4; ++j) gload(t + 8 + j, tr[j]);
+    }
+  };
+  mfma_tile(0, accA);
+  for (int64_t t = 0; t < ntiles; t += 2) {
+    step(t, accA, accB);
+    if (t + 1 >= ntiles) break;
+    step(t + 1, accB, accA);
+  }
+#pragma unroll
+  for (int cb = 0; cb < 2; ++cb) {
+    const int64_t qi = qw0 + 32 * cb + col;
+    if (qi < q) a.cand_count[static_cast<size_t>(qi) * nl + lseg] = cntf[cb];
   }
 }
 
@@ -873,17 +1047,30 @@ int ensure_bf16(sc_index* ix, hipStream_t s, bool* bad_out, int64_t* npad_out) {
   return SC_OK;
 }
 
-template <int KS, int MODE, int KT>
-void launch_scan_t(dim3 grid, hipStream_t stream, const ScanArgs& a) {
+template <int KS, int KT>
+void launch_running(dim3 grid, hipStream_t stream, const ScanArgs& a) {
   const size_t lds = scan_lds_bytes<KS>();
   static const bool attr = [] {
-    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&mixed_scan_kernel<KS, MODE, KT>),
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&mixed_scan_kernel<KS, KT>),
                               hipFuncAttributeMaxDynamicSharedMemorySize,
                               static_cast<int>(scan_lds_bytes<KS>()));
     return true;
   }();
   (void)attr;
-  hipLaunchKernelGGL((mixed_scan_kernel<KS, MODE, KT>), grid, dim3(kNT), lds, stream, a);
+  hipLaunchKernelGGL((mixed_scan_kernel<KS, KT>), grid, dim3(kNT), lds, stream, a);
+}
+
+template <int KS, bool STORE_U>
+void launch_fixed(dim3 grid, hipStream_t stream, const ScanArgs& a) {
+  const size_t lds = fixed_lds_bytes<KS>();
+  static const bool attr = [] {
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&fixed_scan_kernel<KS, STORE_U>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize,
+                              static_cast<int>(fixed_lds_bytes<KS>()));
+    return true;
+  }();
+  (void)attr;
+  hipLaunchKernelGGL((fixed_scan_kernel<KS, STORE_U>), grid, dim3(kNT), lds, stream, a);
 }
 
 // mode: 0 running, 2 fixed threshold (from a seed), 3 the same storing U
@@ -896,13 +1083,13 @@ int launch_scan(int ks, int mode, const ScanArgs& a, hipStream_t stream) {
 #define SC_SCAN_CASE(KS)                                      \
   case KS:                                                   \
     if (mode == 2)                                           \
-      launch_scan_t<KS, 2, 16>(grid, stream, a);             \
+      launch_fixed<KS, false>(grid, stream, a);              \
     else if (mode == 3)                                      \
-      launch_scan_t<KS, 3, 16>(grid, stream, a);             \
+      launch_fixed<KS, true>(grid, stream, a);               \
     else if (a.k <= 16)                                      \
-      launch_scan_t<KS, 0, 16>(grid, stream, a);             \
+      launch_running<KS, 16>(grid, stream, a);               \
     else                                                     \
-      launch_scan_t<KS, 0, 32>(grid, stream, a);             \
+      launch_running<KS, 32>(grid, stream, a);               \
     break;
     SC_SCAN_CASE(1)
     SC_SCAN_CASE(2)
