@@ -317,6 +317,11 @@ __device__ __forceinline__ void split_m2(double c, uint16_t& hi, uint16_t& mid, 
 }
 
 // Largest finite ||c||^2 of a problem, rounded up, and its square root.
+__device__ __forceinline__ void set_cmax_of(const KmProb& p, double mx) {
+  p.cmax[0] = __double2float_ru(sqrt(mx) * (1.0 + 9.5367431640625e-07));
+  p.cmax[1] = __double2float_ru(mx * (1.0 + 9.5367431640625e-07));
+}
+
 __device__ __forceinline__ void set_cmax(const KmProb& p) {
   const volatile double* c2 = p.c2d;
   double mx = 0.0;
@@ -324,8 +329,7 @@ __device__ __forceinline__ void set_cmax(const KmProb& p) {
     const double v = c2[c];
     if (v < 1e300) mx = fmax(mx, v);  // NaN and inf centroids never count
   }
-  p.cmax[0] = __double2float_ru(sqrt(mx) * (1.0 + 9.5367431640625e-07));
-  p.cmax[1] = __double2float_ru(mx * (1.0 + 9.5367431640625e-07));
+  set_cmax_of(p, mx);
 }
 
 // Final sums.  Item (problem, cluster c), 128 threads: sums_c = group sums
@@ -349,14 +353,17 @@ __global__ __launch_bounds__(128) void kmeans_final_kernel(const KmProb* __restr
   __shared__ double s_dot[128], s_c2[128], s_n2[128];
   __shared__ uint64_t s_n;
   __shared__ int s_last;
-  if (tid == 0) {
+  // the count: integer adds, so any order gives the same value
+  if (tid == 0) s_n = 0;
+  __syncthreads();
+  {
     uint64_t n = 0;
-    for (int64_t g = 0; g < p.ng; ++g) n += p.gcount[g * p.k + c];
-    s_n = n;
-    p.counts[c] = n;
+    for (int64_t g = tid; g < p.ng; g += 128) n += p.gcount[g * p.k + c];
+    if (n) atomicAdd(reinterpret_cast<unsigned long long*>(&s_n), static_cast<unsigned long long>(n));
   }
   __syncthreads();
   const uint64_t n = s_n;
+  if (tid == 0) p.counts[c] = n;
   const bool upd = update && n > 0;
   double dot = 0.0, c2 = 0.0, n2 = 0.0;
   for (int j0 = 0; j0 < d; j0 += 128) {
@@ -389,16 +396,24 @@ __global__ __launch_bounds__(128) void kmeans_final_kernel(const KmProb* __restr
       }
     }
     __syncthreads();
-    if (tid == 0) {
+    // the three sums in dimension order, one lane each
+    if (tid < 3) {
+      const double* arr = tid == 0 ? s_dot : (tid == 1 ? s_c2 : s_n2);
+      double a = tid == 0 ? dot : (tid == 1 ? c2 : n2);
       const int m = min(128, d - j0);
-      for (int jj = 0; jj < m; ++jj) {
-        dot = dot + s_dot[jj];
-        c2 = c2 + s_c2[jj];
-        n2 = n2 + s_n2[jj];
-      }
+      for (int jj = 0; jj < m; ++jj) a = a + arr[jj];
+      if (tid == 0) dot = a;
+      if (tid == 1) c2 = a;
+      if (tid == 2) n2 = a;
     }
     __syncthreads();
   }
+  if (tid == 1) s_c2[0] = c2;
+  if (tid == 2) s_n2[0] = n2;
+  __syncthreads();
+  c2 = s_c2[0];
+  n2 = s_n2[0];
+  __syncthreads();
   if (tid == 0) {
     p.term[c] = n > 0 ? static_cast<double>(n) * c2 - 2.0 * dot : 0.0;
     if (tables) {
@@ -410,11 +425,30 @@ __global__ __launch_bounds__(128) void kmeans_final_kernel(const KmProb* __restr
     s_last = old == static_cast<uint32_t>(p.k - 1) ? 1 : 0;
   }
   __syncthreads();
-  if (!s_last || tid != 0) return;
+  if (!s_last) return;
   __threadfence();
-  const volatile double* tv = p.term;
-  double s = p.st->x2total;
-  for (int cc = 0; cc < p.k; ++cc) s = s + tv[cc];
+  // The last workgroup: the terms (and centroid norms) of all clusters,
+  // loaded by every thread 128 at a time and added in cluster order by one.
+  double s = 0.0, mx = 0.0;
+  if (tid == 0) s = p.st->x2total;
+  for (int c0 = 0; c0 < p.k; c0 += 128) {
+    const int cc = c0 + tid;
+    if (cc < p.k) {
+      s_dot[tid] = __hip_atomic_load(p.term + cc, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (tables)
+        s_c2[tid] = __hip_atomic_load(p.c2d + cc, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __syncthreads();
+    if (tid == 0) {
+      const int m = min(128, p.k - c0);
+      for (int j = 0; j < m; ++j) s = s + s_dot[j];
+      if (tables)
+        for (int j = 0; j < m; ++j)
+          if (s_c2[j] < 1e300) mx = fmax(mx, s_c2[j]);  // NaN and inf centroids never count
+    }
+    __syncthreads();
+  }
+  if (tid != 0) return;
   p.st->dist = s;
   if (control) {
     p.st->iters += 1;
@@ -424,7 +458,7 @@ __global__ __launch_bounds__(128) void kmeans_final_kernel(const KmProb* __restr
       p.st->distortion = s;
     }
   }
-  if (tables) set_cmax(p);
+  if (tables) set_cmax_of(p, mx);
   *p.arrive = 0u;
 }
 

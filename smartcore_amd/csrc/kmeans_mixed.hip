@@ -36,8 +36,10 @@
 // distances.  So the labels equal the FP64 form's on every input.
 //
 // Sums.  After the labels, wave w adds the chunk's rows of its clusters
-// (c % waves == w) into f64 registers (lane = dimension) in row order.  That is
-// the order of the FP64 form's partial kernel, so both forms write the same
+// (c % waves == w) in row order into f64 sums, lane = dimension: LDS sums
+// updated with ds_add_f64 (one wave's LDS operations apply in program order)
+// when they fit beside the stage and tables, else registers.  That is the
+// order of the FP64 form's partial kernel, so both forms write the same
 // block partials bit for bit, and the shared reduce kernels give the same
 // sums, centroids and distortions.
 //
@@ -88,12 +90,17 @@ struct Cfg {
   static constexpr int SLOTS = (KP + WAVES - 1) / WAVES; // clusters one wave owns
   static constexpr int DPL = (LD + 63) / 64;             // dimensions per lane
   static constexpr int PFV = CR * LD / 4 / NTH;          // f32x4 loads per thread per chunk
-  // LDS: stage | lab | cnt | c2 | A tables (hi | mid | lo) | [f64 centroids]
+  // LDS: stage | lab | cnt | c2 | A tables (hi | mid | lo) | [f64 sums] | [f64 centroids]
   static constexpr size_t STAGE = sizeof(float) * CR * LDP;
   static constexpr int LDA = LD + 8;                     // A-table row stride: 16 B of padding
   static constexpr size_t ATAB = sizeof(uint16_t) * KP * LDA;
-  static constexpr size_t FIXED = STAGE + sizeof(uint32_t) * CR + sizeof(uint32_t) * KP +
-                                  sizeof(float) * KP + 3 * ATAB;
+  static constexpr size_t FIXED0 = STAGE + sizeof(uint32_t) * CR + sizeof(uint32_t) * KP +
+                                   sizeof(float) * KP + 3 * ATAB;
+  // f64 cluster sums in LDS (lane = dimension) when they fit, else in the
+  // owner waves' registers
+  static constexpr size_t SUMS = sizeof(double) * KP * LD;
+  static constexpr bool LDS_SUMS = FIXED0 + SUMS <= kLdsBudget;
+  static constexpr size_t FIXED = FIXED0 + (LDS_SUMS ? SUMS : 0);
   static_assert(kBlockRows % CR == 0, "a block is a whole number of chunks");
   static_assert(CR % 64 == 0, "chunks are whole ballots");
 };
@@ -161,6 +168,7 @@ __device__ __forceinline__ void km_block(const KmProb& p, int64_t b, unsigned ch
   // global array.  Each use branches on the (uniform) choice so that every
   // load is a typed LDS or global load: a generic load would make the
   // fallback wait for the chunk prefetch in flight.
+  double* ssum = reinterpret_cast<double*>(smem + C::FIXED0);  // KP x LD (LDS_SUMS)
   double* ctab_s = reinterpret_cast<double*>(smem + C::FIXED);
   auto exact_c = [&](const float* x, uint32_t c) -> double {
     return ctab_lds ? exact_s(x, ctab_s + static_cast<int64_t>(c) * ld, ld)
@@ -182,6 +190,8 @@ __device__ __forceinline__ void km_block(const KmProb& p, int64_t b, unsigned ch
     *reinterpret_cast<u32x4*>(amd + dst) = in ? *reinterpret_cast<const u32x4*>(p.bmd + src) : z;
     *reinterpret_cast<u32x4*>(alo + dst) = in ? *reinterpret_cast<const u32x4*>(p.blo + src) : z;
   }
+  if constexpr (C::LDS_SUMS)
+    for (int e = tid; e < C::KP * C::LD; e += NTH) ssum[e] = 0.0;
   if (ctab_lds)
     for (int64_t e = tid; e < static_cast<int64_t>(k) * ld; e += NTH) ctab_s[e] = p.cent[e];
   auto frag = [&](const uint16_t* tab, int cb, int ks) -> bf16x8 {
@@ -201,11 +211,11 @@ __device__ __forceinline__ void km_block(const KmProb& p, int64_t b, unsigned ch
   // tree vs sequential f64 fold of the same ld non-negative terms
   const double dr = 2.0004 * (ld + 8.0) * 1.1102230246251565e-16;
 
-  // f64 sums of the wave's clusters, slot-major; indexed with a uniform
-  // slot (register-relative moves, no scratch)
-  double sacc[C::SLOTS * C::DPL];
+  // f64 sums of the wave's clusters in registers (wave w owns c = WAVES s +
+  // w, slot s), slot-major, lane = dimension, when they do not fit LDS
+  double sacc[C::LDS_SUMS ? 1 : C::SLOTS * C::DPL];
 #pragma unroll
-  for (int e = 0; e < C::SLOTS * C::DPL; ++e) sacc[e] = 0.0;
+  for (int e = 0; e < (C::LDS_SUMS ? 1 : C::SLOTS * C::DPL); ++e) sacc[e] = 0.0;
 
   const float* rows = static_cast<const float*>(p.rows);
   const int64_t r0 = b * kBlockRows;
@@ -491,10 +501,12 @@ __device__ __forceinline__ void km_block(const KmProb& p, int64_t b, unsigned ch
     if (tid < nr) p.labels[r0 + c0 + tid] = lab[tid];
 
     // ---- sums: wave owns clusters WAVES s + wave, rows in order ----
-    if constexpr (C::SLOTS * C::DPL <= 8) {
-      // Per 64-row group the owned rows are taken four at a time: their
-      // values are loaded together, then added in row order into the slot of
-      // their cluster (a uniform index).
+    if constexpr (C::LDS_SUMS) {
+      // Per 64-row group the owned rows are taken eight at a time: their
+      // values are read together, then added in row order into the LDS sums
+      // of their clusters.  The adds of one wave reach LDS in program order,
+      // and each cluster has one owner wave, so every sum is the row-order
+      // fold.
 #pragma unroll
       for (int g = 0; g < CR / 64; ++g) {
         const uint32_t lr = lab[64 * g + lane];
@@ -504,34 +516,29 @@ __device__ __forceinline__ void km_block(const KmProb& p, int64_t b, unsigned ch
         uint64_t m = __ballot(own);
         const float* sg = stage + 64 * g * C::LDP + lane;
         while (m != 0) {
-          int ri[4];
-          uint32_t sl[4];
-          const int nb4 = min(4, __popcll(m));
+          constexpr int B = 8;
+          int ri[B];
+          uint32_t cl[B];
+          const int nb = min(B, __popcll(m));
 #pragma unroll
-          for (int j = 0; j < 4; ++j) {
-            // m != 0 here for j < nb4; entries past nb4 repeat the first row
-            ri[j] = j < nb4 ? __builtin_ctzll(m) : ri[0];
-            if (j < nb4) m &= m - 1;
-            sl[j] = __builtin_amdgcn_readlane(lr, ri[j]) / C::WAVES;
+          for (int j = 0; j < B; ++j) {
+            // m != 0 here for j < nb; entries past nb repeat the first row
+            ri[j] = j < nb ? __builtin_ctzll(m) : ri[0];
+            if (j < nb) m &= m - 1;
+            cl[j] = __builtin_amdgcn_readlane(lr, ri[j]);
           }
-          // Entries past nb4 and lanes past ld add +0.0, which leaves a sum
-          // unchanged: sums start at +0.0 and never become -0.0 (x + (-x) and
-          // +0.0 + -0.0 round to +0.0), so the updates need no conditions.
-          float v[4][C::DPL];
+          float v[B][C::DPL];
 #pragma unroll
-          for (int j = 0; j < 4; ++j)
+          for (int j = 0; j < B; ++j)
 #pragma unroll
-            for (int dd = 0; dd < C::DPL; ++dd) {
-              const float x = sg[ri[j] * C::LDP + 64 * dd];
-              v[j][dd] = (j < nb4 && lane + 64 * dd < ld) ? x : 0.0f;
-            }
+            for (int dd = 0; dd < C::DPL; ++dd) v[j][dd] = sg[ri[j] * C::LDP + 64 * dd];
 #pragma unroll
-          for (int j = 0; j < 4; ++j)
+          for (int j = 0; j < B; ++j)
+            if (j < nb)
 #pragma unroll
-            for (int dd = 0; dd < C::DPL; ++dd) {
-              const uint32_t e = sl[j] * C::DPL + dd;
-              sacc[e] = sacc[e] + static_cast<double>(v[j][dd]);
-            }
+              for (int dd = 0; dd < C::DPL; ++dd)
+                if (lane + 64 * dd < ld)
+                  atomicAdd(&ssum[cl[j] * C::LD + lane + 64 * dd], static_cast<double>(v[j][dd]));
         }
       }
     } else {
@@ -563,14 +570,21 @@ __device__ __forceinline__ void km_block(const KmProb& p, int64_t b, unsigned ch
   __syncthreads();
   // ---- block partials ----
   const int64_t kd = static_cast<int64_t>(k) * d;
+  if constexpr (C::LDS_SUMS) {
+    for (int e = tid; e < k * d; e += NTH) {
+      const int c = e / d;
+      p.psum[b * kd + e] = ssum[c * C::LD + (e - c * d)];
+    }
+  } else {
 #pragma unroll
-  for (int s = 0; s < C::SLOTS; ++s) {
-    const int c = C::WAVES * s + wave;
-    if (c >= k) break;
+    for (int s = 0; s < C::SLOTS; ++s) {
+      const int c = C::WAVES * s + wave;
+      if (c >= k) break;
 #pragma unroll
-    for (int dd = 0; dd < C::DPL; ++dd) {
-      const int dim = lane + 64 * dd;
-      if (dim < d) p.psum[b * kd + static_cast<int64_t>(c) * d + dim] = sacc[s * C::DPL + dd];
+      for (int dd = 0; dd < C::DPL; ++dd) {
+        const int dim = lane + 64 * dd;
+        if (dim < d) p.psum[b * kd + static_cast<int64_t>(c) * d + dim] = sacc[s * C::DPL + dd];
+      }
     }
   }
   for (int c = tid; c < k; c += NTH) p.pcount[b * k + c] = cnt[c];
@@ -618,6 +632,21 @@ size_t lds_bytes(int k, int ld, bool* ctab_lds) {
 int kbm_of(int k) { return k <= 32 ? 1 : (k <= 64 ? 2 : 4); }
 int ksm_of(int ld) { return ld <= 16 ? 1 : (ld <= 32 ? 2 : (ld <= 64 ? 4 : 8)); }
 
+// Whether a shape class's fixed LDS fits one workgroup's budget.  The widest
+// rows with k above 64 do not (three centroid tables of 128 x 136 BF16 and a
+// 128-row stage): those problems take the FP64 form.
+bool class_fits(int k, int ld) {
+  if (k < 1 || k > 128 || ld > 128) return false;
+  const int kbm = kbm_of(k), ksm = ksm_of(ld);
+#define SC_KM_FIT(A, B) \
+  if (kbm == A && ksm == B) return Cfg<A, B>::FIXED <= kLdsBudget;
+  SC_KM_FIT(1, 1) SC_KM_FIT(1, 2) SC_KM_FIT(1, 4) SC_KM_FIT(1, 8)
+  SC_KM_FIT(2, 1) SC_KM_FIT(2, 2) SC_KM_FIT(2, 4) SC_KM_FIT(2, 8)
+  SC_KM_FIT(4, 1) SC_KM_FIT(4, 2) SC_KM_FIT(4, 4) SC_KM_FIT(4, 8)
+#undef SC_KM_FIT
+  return false;
+}
+
 template <int KBM, int KSM>
 int launch_class(const KmProb* dP, const uint2* items, int64_t nitems, unsigned grid, bool exact,
                  int kmax, int ldmax, hipStream_t s) {
@@ -659,7 +688,7 @@ int launch_step(int kbm, int ksm, const KmProb* dP, const uint2* items, int64_t 
 }  // namespace
 
 bool kmeans_mixed_supported(const sc_index* ix, int k) {
-  return ix->dtype == SC_F32 && k >= 1 && k <= 128 && ix->ld <= 128;
+  return ix->dtype == SC_F32 && class_fits(k, static_cast<int>(ix->ld));
 }
 
 int kmeans_mixed_step_launch(const sc_index* ix, KmWork& w, hipStream_t s) {
@@ -774,7 +803,9 @@ extern "C" int sc_kmeans_batch_run(const sc_kmeans_problem* problems, size_t cou
 
   bool batched = mode == SC_MIXED && dtype == SC_F32;
   for (size_t i = 0; i < count && batched; ++i)
-    if (problems[i].k > 128 || padded_dim(problems[i].d) > 128) batched = false;
+    if (!class_fits(static_cast<int>(std::min<size_t>(problems[i].k, 1u << 20)),
+                    static_cast<int>(std::min<size_t>(padded_dim(problems[i].d), 1u << 20))))
+      batched = false;
 
   if (!batched) {
     // the FP64 form, one problem at a time (the comparator of the batch)

@@ -30,7 +30,7 @@ def assert_same(a, b):
 
 
 @pytest.mark.parametrize("n,d,k", [(50_000, 64, 64), (20_000, 16, 12), (30_000, 24, 70),
-                                   (9_000, 100, 5), (4_097, 8, 128)])
+                                   (9_000, 100, 5), (4_097, 8, 128), (3_000, 100, 100)])
 def test_mixed_equals_fp64_form(n, d, k):
     x, _, init = synth.kmeans_blobs(seed=20 + k, n=n, d=d, blobs=max(2, k // 2), k=k)
     a, b, st = run_both(x, init, 15)
