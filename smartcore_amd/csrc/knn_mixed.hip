@@ -15,18 +15,19 @@
 //   L = (1 - 2^-18)(x2 + y2) - 2C - 2 c1 ||x|| ||y||  <=  s  <=
 //   U = (1 + 2^-16)(x2 + y2) - 2C + 2 c1 ||x|| ||y||.
 //
-// Pass 1 (scan, MFMA).  A workgroup of four waves owns 256 queries (64 per
+// Pass 1 (scan, MFMA).  A workgroup of eight waves owns 512 queries (64 per
 // wave, held as BF16 B-fragments of -2x in registers; scaling by -2 is
 // exact) and one contiguous segment of rows, streamed 32 rows at a time
-// through four padded LDS tile slots that all four waves read; two
-// workgroups share a CU, so one can issue MFMAs while the other waits at a
-// barrier.  Tiles are loaded into registers in pairs, two steps ahead.
-// v_mfma_f32_32x32x16_bf16 gives acc = -2C; the epilogue forms
-// L - x2m = y2m - cxb nyb + acc with one FMA and one add (y2m <=
-// (1 - 2^-18) y2, nyb >= ||y||, cxb >= 2 c1 ||x||, all rounded the safe
-// way; the two roundings are inside the 2^-18 margin).  Workgroups are laid
-// out so that the CUs of an XCD stream the same segment and share its tiles
-// in their L2.  Each lane owns one query and keeps, in registers, the k
+// through an eight-slot LDS tile ring that all waves read: tiles are loaded
+// into registers four steps ahead and stored four at a time, with one
+// barrier per four steps.  v_mfma_f32_32x32x16_bf16 gives acc = -2C.  In
+// the running launch the epilogue forms L - x2m = y2m - cxb nyb + acc with
+// one FMA and one add (y2m <= (1 - 2^-18) y2, nyb >= ||y||, cxb >= 2 c1
+// ||x||, all rounded the safe way; the two roundings are inside the 2^-18
+// margin); the fixed-threshold launch folds its bound into one more MFMA K
+// step (at fixed_scan_kernel).  Workgroups are laid out so that the CUs of
+// an XCD stream the same segment and share its tiles in their L2.  In the
+// running launch each lane owns one query and keeps, in registers, the k
 // smallest U seen so far; every row whose L is below that running k-th U is
 // appended to the query's list for the segment, in index order.
 // A row that enters the reference heap has s below the running k-th
@@ -263,7 +264,7 @@ struct ScanArgs {
   ListLayout lay;
   int lbase;
   uint32_t* cand_idx;  // q x lay.per_query()
-  float* cand_U;       // MODE 3: U of each entry
+  float* cand_U;       // MODE 3: the MFMA value of each entry (the seed selection derives U)
   uint32_t* cand_count;  // q x lay.nlists()
   const float* seed;   // q x k, or null
   float* topu_out;     // q x nseg x k, or null
@@ -568,15 +569,16 @@ __global__ __launch_bounds__(kNT, 2) void mixed_scan_kernel(const ScanArgs a) {
 // products, >= the tight bound) and the epilogue is a minimum.  The FP32
 // roundings of those adds and of the y2m split stay inside the 2^-18
 // margin.  Each lane writes its own rows (its half of the tile) of both
-// column blocks' queries to the queries' lists of that half.  Eight tile
-// slots (two workgroups fit one CU): step w (w = 0 mod 4) stores tiles
-// w + 4 .. w + 7 into the slots of tiles w - 4 .. w - 1 and loads tiles
-// w + 8 .. w + 11 into registers.  The MFMAs of tile t + 1 are issued
-// before the epilogue of tile t, so the last LDS reads of the slot of tile
-// w - 1 are in step w - 2; the barrier at the start of every step w + 3
-// (w = 0 mod 4) orders them before the stores of step w + 4 and the stores
-// of step w before the reads of tile w + 4 in step w + 3.  The candidate
-// path reads the row terms from global memory, so the slots hold rows only.
+// column blocks' queries to the queries' lists of that half.  Sixteen tile
+// slots (152 KiB of LDS; the registers allow one workgroup per CU): step w
+// (w = 0 mod 4) stores tiles w + 8 .. w + 11 into the slots of tiles
+// w - 8 .. w - 5 and loads tiles w + 12 .. w + 15 into registers.  The
+// MFMAs of tile t + 1 are issued before the epilogue of tile t, so the
+// last LDS reads of the slot of tile w - 5 are in step w - 6.  A barrier
+// at the start of every step 8 m + 7 falls between those reads and the
+// stores of step w, and between the stores of step w and the reads of tile
+// w + 8 in step w + 7: one barrier per eight tiles.  The candidate path
+// reads the row terms from global memory, so the slots hold rows only.
 template <int KS>
 struct FixTile {
   static constexpr int LD = 16 * KS;
@@ -585,7 +587,8 @@ struct FixTile {
   static constexpr int NP = (VEC + kNT - 1) / kNT;
   static constexpr int SLOT = kRows * RS * 2;  // rows with their folded columns (bytes)
 };
-constexpr int kFixSlots = 8;
+constexpr int kFixSlots = 16;
+constexpr int kFixAhead = kFixSlots / 2;  // tiles between a store and the first read of its slot
 template <int KS>
 size_t fixed_lds_bytes() {
   return kFixSlots * static_cast<size_t>(FixTile<KS>::SLOT);
@@ -639,7 +642,7 @@ __global__ __launch_bounds__(kNT, 2) void fixed_scan_kernel(const ScanArgs a) {
   const int lcap = a.lay.cap(lseg);
 
   bf16x8 bq[2][KS + 1];
-  f32x2 thr, x2v, kfix, fq1, fq2;
+  f32x2 thr, x2v, kfix;
   i32x2 qvalid;
 #pragma unroll
   for (int cb = 0; cb < 2; ++cb) {
@@ -656,8 +659,6 @@ __global__ __launch_bounds__(kNT, 2) void fixed_scan_kernel(const ScanArgs a) {
                   ? __double2float_ru(static_cast<double>(kfix[cb]) - static_cast<double>(x2v[cb]))
                   : INFINITY;
     const uint16_t b1 = bf16_ru_pos(a.qa1[qr]), b2 = bf16_ru_pos(a.qa2[qr]);
-    fq1[cb] = __uint_as_float(static_cast<uint32_t>(b1) << 16);
-    fq2[cb] = __uint_as_float(static_cast<uint32_t>(b2) << 16);
     const uint32_t one = 0x3f80u, n1 = 0x8000u | b1, n2 = 0x8000u | b2;
     const u32x4 fold = half == 0 ? u32x4{one | (one << 16), one | (n1 << 16), n2, 0u}
                                  : u32x4{0u, 0u, 0u, 0u};
@@ -744,20 +745,7 @@ __global__ __launch_bounds__(kNT, 2) void fixed_scan_kernel(const ScanArgs a) {
             if (cntf[cb] < static_cast<uint32_t>(lcap)) {
               const size_t e = static_cast<size_t>(qi) * perq + loff + cntf[cb];
               a.cand_idx[e] = static_cast<uint32_t>(row);
-              if constexpr (STORE_U) {
-                // the e the MFMA subtracted: exact BF16 products
-                const double c1 = static_cast<double>(__uint_as_float(
-                    static_cast<uint32_t>(bf16_ru_pos(y2m[row + 2 * a.npad])) << 16));
-                const double c2 = static_cast<double>(__uint_as_float(
-                    static_cast<uint32_t>(bf16_ru_pos(y2m[row + 3 * a.npad])) << 16));
-                const double eu = (static_cast<double>(fq1[cb]) * c1 +
-                                   static_cast<double>(fq2[cb]) * c2) *
-                                  (1.0 + 4.440892098500626e-16);
-                const double U = L + 2.0 * eu +
-                                 (kHiMargin + 2.0 * kLoMargin) *
-                                     (static_cast<double>(x2v[cb]) + static_cast<double>(y2m[row]));
-                a.cand_U[e] = __double2float_ru(U);
-              }
+              if constexpr (STORE_U) a.cand_U[e] = val;
             }
             ++cntf[cb];
           }
@@ -780,16 +768,19 @@ __global__ __launch_bounds__(kNT, 2) void fixed_scan_kernel(const ScanArgs a) {
   // waits for every load before each later load into them.
   TileRegs tr[4];
   f32x16 accA[2], accB[2];
-  // tiles 0..3 into slots 0..3, tiles 4..7 into registers
+  // tiles 0 .. kFixAhead - 1 into their slots, the next four into registers
 #pragma unroll
-  for (int j = 0; j < 4; ++j) gload(j, tr[j]);
+  for (int g = 0; g < kFixAhead; g += 4) {
 #pragma unroll
-  for (int j = 0; j < 4; ++j) swrite(j, tr[j]);
+    for (int j = 0; j < 4; ++j) gload(g + j, tr[j]);
 #pragma unroll
-  for (int j = 0; j < 4; ++j) gload(4 + j, tr[j]);
+    for (int j = 0; j < 4; ++j) swrite(g + j, tr[j]);
+  }
+#pragma unroll
+  for (int j = 0; j < 4; ++j) gload(kFixAhead + j, tr[j]);
   __syncthreads();
   auto step = [&](int64_t t, f32x16 (&cur)[2], f32x16 (&nxt)[2]) {
-    if ((t & 3) == 3) __syncthreads();
+    if ((t & (kFixAhead - 1)) == kFixAhead - 1) __syncthreads();
     if (t + 1 < ntiles) mfma_tile(static_cast<int>((t + 1) & (kFixSlots - 1)), nxt);
     bool hit[2];
 #pragma unroll
@@ -804,11 +795,12 @@ __global__ __launch_bounds__(kNT, 2) void fixed_scan_kernel(const ScanArgs a) {
       // vmcnt(0): the loads of four steps ago are done.  Said once here,
       // on every path, it spares the compiler's own waits before each load.
       __builtin_amdgcn_s_waitcnt(0x0F70);
-      // slots of tiles t - 4 .. t - 1 are those of t + 4 .. t + 7
+      // the slots of tiles t + kFixAhead + j are those of t - kFixAhead + j
 #pragma unroll
-      for (int j = 0; j < 4; ++j) swrite(static_cast<int>((t + 4 + j) & (kFixSlots - 1)), tr[j]);
+      for (int j = 0; j < 4; ++j)
+        swrite(static_cast<int>((t + kFixAhead + j) & (kFixSlots - 1)), tr[j]);
 #pragma unroll
-      for (int j = 0; j < 4; ++j) gload(t + 8 + j, tr[j]);
+      for (int j = 0; j < 4; ++j) gload(t + kFixAhead + 4 + j, tr[j]);
     }
   };
   mfma_tile(0, accA);
@@ -846,24 +838,49 @@ __global__ void merge_topu_kernel(const float* __restrict__ topu, int64_t q, int
   }
 }
 
-// The k smallest of a seed and the U values stored by a fixed launch in
-// lists [l0, l0 + nl) (every U is a bound of a distinct row after the
-// seed's rows): the next seed, ascending.  Entries past a list's cap were
-// not stored; leaving them out keeps a valid seed.
-__global__ void select_seed_kernel(const float* __restrict__ seed_in, const float* __restrict__ cand_U,
+// U of a stored entry of a fixed launch: val = y2m - 2C - e from the MFMA
+// (the folded step's e = RU(qa1) RU(nry) + RU(qa2) RU(nyh), exact BF16
+// products), so L = x2m + val and
+//   U = L + 2 e + (2^-16 + 2 * 2^-18) (x2m + y2m),
+// rounded up: a bound above the f64 distance of the row.
+__device__ __forceinline__ float entry_u(float val, float x2m, float qa1, float qa2, const float* terms,
+                                         int64_t npad, uint32_t row) {
+  auto ru = [](float f) {  // f >= 0: the smallest BF16 >= f, as f64
+    const uint32_t u = __float_as_uint(f);
+    return static_cast<double>(__uint_as_float(((u >> 16) + ((u & 0xffffu) != 0u ? 1u : 0u)) << 16));
+  };
+  const double L = static_cast<double>(x2m) + static_cast<double>(val);
+  const double eu = (ru(qa1) * ru(terms[2 * npad + row]) + ru(qa2) * ru(terms[3 * npad + row])) *
+                    (1.0 + 4.440892098500626e-16);
+  const double U = L + 2.0 * eu +
+                   (kHiMargin + 2.0 * kLoMargin) *
+                       (static_cast<double>(x2m) + static_cast<double>(terms[row]));
+  return __double2float_ru(U);
+}
+
+// The k smallest of a seed and the U values of the entries stored by a
+// fixed launch in lists [l0, l0 + nl) (every U is a bound of a distinct row
+// after the seed's rows): the next seed, ascending.  Entries past a list's
+// cap were not stored; leaving them out keeps a valid seed.
+__global__ void select_seed_kernel(const float* __restrict__ seed_in, const float* __restrict__ cand_val,
+                                   const uint32_t* __restrict__ cand_idx,
                                    const uint32_t* __restrict__ cand_count, ListLayout lay, int l0,
-                                   int nl, int64_t q, int k, float* __restrict__ seed_out) {
+                                   int nl, int64_t q, int k, const float* __restrict__ x2m,
+                                   const float* __restrict__ qa1, const float* __restrict__ qa2,
+                                   const float* __restrict__ terms, int64_t npad,
+                                   float* __restrict__ seed_out) {
   const int64_t qi = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (qi >= q) return;
   float best[kMaxK];
   for (int j = 0; j < k; ++j) best[j] = seed_in[qi * k + j];
   const int64_t perq = lay.per_query();
+  const float xq = x2m[qi], a1 = qa1[qi], a2 = qa2[qi];
   for (int l = l0; l < l0 + nl; ++l) {
     const uint32_t c = min(cand_count[static_cast<size_t>(qi) * lay.nlists() + l],
                            static_cast<uint32_t>(lay.cap(l)));
-    const float* u = cand_U + static_cast<size_t>(qi) * perq + lay.offset(l);
+    const size_t base = static_cast<size_t>(qi) * perq + lay.offset(l);
     for (uint32_t e = 0; e < c; ++e) {
-      const float v = u[e];
+      const float v = entry_u(cand_val[base + e], xq, a1, a2, terms, npad, cand_idx[base + e]);
       if (!(v < best[k - 1])) continue;
       int pos = k - 1;
       while (pos > 0 && best[pos - 1] > v) {
@@ -1252,8 +1269,9 @@ int knn_mixed_device(const sc_index* cix, const void* d_queries, size_t q_, size
       fa.seed = seed_cur;
       if (int rc = launch_scan(ld / 16, 3, fa, stream)) return rc;
       hipLaunchKernelGGL(select_seed_kernel, dim3(static_cast<unsigned>((q + 255) / 256)),
-                         dim3(256), 0, stream, seed_cur, cU.as<float>(), ccnt.as<uint32_t>(), lay,
-                         fa.lbase, 2 * nseg, q, k, seed_next);
+                         dim3(256), 0, stream, seed_cur, cU.as<float>(), cidx.as<uint32_t>(),
+                         ccnt.as<uint32_t>(), lay, fa.lbase, 2 * nseg, q, k, sa.x2m, sa.qa1, sa.qa2,
+                         sa.y2m, sa.npad, seed_next);
       SC_HIP(hipGetLastError());
       std::swap(seed_cur, seed_next);
     }
