@@ -36,12 +36,11 @@
 // distances.  So the labels equal the FP64 form's on every input.
 //
 // Sums.  After the labels, wave w adds the chunk's rows of its clusters
-// (c % waves == w) in row order into f64 sums, lane = dimension: LDS sums
-// updated with ds_add_f64 (one wave's LDS operations apply in program order)
-// when they fit beside the stage and tables, else registers.  That is the
-// order of the FP64 form's partial kernel, so both forms write the same
-// block partials bit for bit, and the shared reduce kernels give the same
-// sums, centroids and distortions.
+// (c % waves == w) in row order into f64 registers, lane = dimension, one
+// ballot per (cluster, 64-row group).  That is the order of the FP64 form's
+// partial kernel, so both forms write the same block partials bit for bit,
+// and the shared reduce kernels give the same sums, centroids and
+// distortions.
 //
 // Batched form.  Many independent problems (uneven n, d, k) run per Lloyd
 // iteration as one persistent launch per shape class over (problem, block)
@@ -90,17 +89,12 @@ struct Cfg {
   static constexpr int SLOTS = (KP + WAVES - 1) / WAVES; // clusters one wave owns
   static constexpr int DPL = (LD + 63) / 64;             // dimensions per lane
   static constexpr int PFV = CR * LD / 4 / NTH;          // f32x4 loads per thread per chunk
-  // LDS: stage | lab | cnt | c2 | A tables (hi | mid | lo) | [f64 sums] | [f64 centroids]
+  // LDS: stage | lab | cnt | c2 | A tables (hi | mid | lo) | [f64 centroids]
   static constexpr size_t STAGE = sizeof(float) * CR * LDP;
   static constexpr int LDA = LD + 8;                     // A-table row stride: 16 B of padding
   static constexpr size_t ATAB = sizeof(uint16_t) * KP * LDA;
-  static constexpr size_t FIXED0 = STAGE + sizeof(uint32_t) * CR + sizeof(uint32_t) * KP +
-                                   sizeof(float) * KP + 3 * ATAB;
-  // f64 cluster sums in LDS (lane = dimension) when they fit, else in the
-  // owner waves' registers
-  static constexpr size_t SUMS = sizeof(double) * KP * LD;
-  static constexpr bool LDS_SUMS = FIXED0 + SUMS <= kLdsBudget;
-  static constexpr size_t FIXED = FIXED0 + (LDS_SUMS ? SUMS : 0);
+  static constexpr size_t FIXED = STAGE + sizeof(uint32_t) * CR + sizeof(uint32_t) * KP +
+                                  sizeof(float) * KP + 3 * ATAB;
   static_assert(kBlockRows % CR == 0, "a block is a whole number of chunks");
   static_assert(CR % 64 == 0, "chunks are whole ballots");
 };
@@ -168,7 +162,6 @@ __device__ __forceinline__ void km_block(const KmProb& p, int64_t b, unsigned ch
   // global array.  Each use branches on the (uniform) choice so that every
   // load is a typed LDS or global load: a generic load would make the
   // fallback wait for the chunk prefetch in flight.
-  double* ssum = reinterpret_cast<double*>(smem + C::FIXED0);  // KP x LD (LDS_SUMS)
   double* ctab_s = reinterpret_cast<double*>(smem + C::FIXED);
   auto exact_c = [&](const float* x, uint32_t c) -> double {
     return ctab_lds ? exact_s(x, ctab_s + static_cast<int64_t>(c) * ld, ld)
@@ -190,8 +183,6 @@ __device__ __forceinline__ void km_block(const KmProb& p, int64_t b, unsigned ch
     *reinterpret_cast<u32x4*>(amd + dst) = in ? *reinterpret_cast<const u32x4*>(p.bmd + src) : z;
     *reinterpret_cast<u32x4*>(alo + dst) = in ? *reinterpret_cast<const u32x4*>(p.blo + src) : z;
   }
-  if constexpr (C::LDS_SUMS)
-    for (int e = tid; e < C::KP * C::LD; e += NTH) ssum[e] = 0.0;
   if (ctab_lds)
     for (int64_t e = tid; e < static_cast<int64_t>(k) * ld; e += NTH) ctab_s[e] = p.cent[e];
   auto frag = [&](const uint16_t* tab, int cb, int ks) -> bf16x8 {
@@ -211,11 +202,11 @@ __device__ __forceinline__ void km_block(const KmProb& p, int64_t b, unsigned ch
   // tree vs sequential f64 fold of the same ld non-negative terms
   const double dr = 2.0004 * (ld + 8.0) * 1.1102230246251565e-16;
 
-  // f64 sums of the wave's clusters in registers (wave w owns c = WAVES s +
-  // w, slot s), slot-major, lane = dimension, when they do not fit LDS
-  double sacc[C::LDS_SUMS ? 1 : C::SLOTS * C::DPL];
+  // f64 sums of the wave's clusters (wave w owns c = WAVES s + w, slot s),
+  // slot-major, lane = dimension
+  double sacc[C::SLOTS * C::DPL];
 #pragma unroll
-  for (int e = 0; e < (C::LDS_SUMS ? 1 : C::SLOTS * C::DPL); ++e) sacc[e] = 0.0;
+  for (int e = 0; e < C::SLOTS * C::DPL; ++e) sacc[e] = 0.0;
 
   const float* rows = static_cast<const float*>(p.rows);
   const int64_t r0 = b * kBlockRows;
@@ -501,48 +492,8 @@ __device__ __forceinline__ void km_block(const KmProb& p, int64_t b, unsigned ch
     if (tid < nr) p.labels[r0 + c0 + tid] = lab[tid];
 
     // ---- sums: wave owns clusters WAVES s + wave, rows in order ----
-    if constexpr (C::LDS_SUMS) {
-      // Per 64-row group the owned rows are taken eight at a time: their
-      // values are read together, then added in row order into the LDS sums
-      // of their clusters.  The adds of one wave reach LDS in program order,
-      // and each cluster has one owner wave, so every sum is the row-order
-      // fold.
-#pragma unroll
-      for (int g = 0; g < CR / 64; ++g) {
-        const uint32_t lr = lab[64 * g + lane];
-        const bool own = lr < static_cast<uint32_t>(k) &&
-                         lr % static_cast<uint32_t>(C::WAVES) == static_cast<uint32_t>(wave);
-        if (own) atomicAdd(&cnt[lr], 1u);
-        uint64_t m = __ballot(own);
-        const float* sg = stage + 64 * g * C::LDP + lane;
-        while (m != 0) {
-          constexpr int B = 8;
-          int ri[B];
-          uint32_t cl[B];
-          const int nb = min(B, __popcll(m));
-#pragma unroll
-          for (int j = 0; j < B; ++j) {
-            // m != 0 here for j < nb; entries past nb repeat the first row
-            ri[j] = j < nb ? __builtin_ctzll(m) : ri[0];
-            if (j < nb) m &= m - 1;
-            cl[j] = __builtin_amdgcn_readlane(lr, ri[j]);
-          }
-          float v[B][C::DPL];
-#pragma unroll
-          for (int j = 0; j < B; ++j)
-#pragma unroll
-            for (int dd = 0; dd < C::DPL; ++dd) v[j][dd] = sg[ri[j] * C::LDP + 64 * dd];
-#pragma unroll
-          for (int j = 0; j < B; ++j)
-            if (j < nb)
-#pragma unroll
-              for (int dd = 0; dd < C::DPL; ++dd)
-                if (lane + 64 * dd < ld)
-                  atomicAdd(&ssum[cl[j] * C::LD + lane + 64 * dd], static_cast<double>(v[j][dd]));
-        }
-      }
-    } else {
-      // many slots: one ballot per (slot, group), static slot indices
+    {
+      // one ballot per (slot, group); static slot indices
       uint32_t lrow[CR / 64];
 #pragma unroll
       for (int g = 0; g < CR / 64; ++g) lrow[g] = lab[64 * g + lane];
@@ -570,21 +521,14 @@ __device__ __forceinline__ void km_block(const KmProb& p, int64_t b, unsigned ch
   __syncthreads();
   // ---- block partials ----
   const int64_t kd = static_cast<int64_t>(k) * d;
-  if constexpr (C::LDS_SUMS) {
-    for (int e = tid; e < k * d; e += NTH) {
-      const int c = e / d;
-      p.psum[b * kd + e] = ssum[c * C::LD + (e - c * d)];
-    }
-  } else {
 #pragma unroll
-    for (int s = 0; s < C::SLOTS; ++s) {
-      const int c = C::WAVES * s + wave;
-      if (c >= k) break;
+  for (int s = 0; s < C::SLOTS; ++s) {
+    const int c = C::WAVES * s + wave;
+    if (c >= k) break;
 #pragma unroll
-      for (int dd = 0; dd < C::DPL; ++dd) {
-        const int dim = lane + 64 * dd;
-        if (dim < d) p.psum[b * kd + static_cast<int64_t>(c) * d + dim] = sacc[s * C::DPL + dd];
-      }
+    for (int dd = 0; dd < C::DPL; ++dd) {
+      const int dim = lane + 64 * dd;
+      if (dim < d) p.psum[b * kd + static_cast<int64_t>(c) * d + dim] = sacc[s * C::DPL + dd];
     }
   }
   for (int c = tid; c < k; c += NTH) p.pcount[b * k + c] = cnt[c];
