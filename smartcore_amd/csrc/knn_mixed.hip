@@ -36,14 +36,19 @@
 // Levels.  Small indexes run one such launch.  For indexes of at least
 // kSeedMinRows rows the first launch covers only the first kPrefixRows
 // rows; then fixed-threshold launches cover row ranges growing kLevelRatio
-// times.  Each tests every row against T, the k-th smallest of k U values
-// of distinct earlier rows (the prefix's top-k, then the k smallest over
-// that and the previous level's candidates), and appends the rows with
-// L < T.  The heap's threshold never exceeds T, so these lists are also a
+// times.  The candidates of each level get their exact s (Pass 2a) as the
+// level ends; the next level tests every row against T, the k-th smallest
+// s among the candidates of all earlier levels rounded up to f32, and
+// appends the rows with L < T.  The candidates are a subset of the earlier
+// rows, so T is at or above the k-th smallest s of all earlier rows, which
+// is the heap's threshold when the level starts: these lists are also a
 // superset of the entrants, and with no running threshold a lane writes
-// its own rows without waiting on other lanes.  Candidates are kept only
-// if the per-row bound, which uses the stored norms of y - bf16(y) and
-// x - bf16(x), also has L below the threshold.
+// its own rows without waiting on other lanes.  (The candidates also hold
+// the k nearest earlier rows -- each had L < s below the threshold of its
+// own level -- so T is the exact k-th distance, narrower than a k-th U by
+// the width of the bound, and fewer rows pass.)
+// Candidates are kept only if the per-row bound, which uses the stored
+// norms of y - bf16(y) and x - bf16(x), also has L below the threshold.
 // Pass 2 (replay, FP64).  Every candidate's exact s is computed in
 // parallel (a workgroup per query); then one thread per query walks its
 // lists in index order and applies the heap rule.  Rows that would not
@@ -238,12 +243,11 @@ struct ListLayout {
 
 // One scan launch covers rows [row0, row_end) in nseg segments of seg_len
 // rows.  A running launch (MODE 0) keeps each query's k smallest U, from
-// empty, and writes segment s to list lbase + s; with topu_out, each
-// (query, segment) also writes its final k smallest U there.  A fixed
-// launch (MODE 2, or 3 to store U as well) tests every row against the
-// seed's k-th U and writes segment s to lists lbase + 2 s + half.
-// A seed holds k U values of k distinct rows before row0 per query,
-// ascending.
+// empty, and writes segment s to list lbase + s.  A fixed launch (MODE 2)
+// tests every row against the seed's k-th value and writes segment s to
+// lists lbase + 2 s + half.  A seed holds, per query and ascending, k
+// values each at or above the distance s of a distinct row before row0
+// (exact distances of earlier candidates, rounded up to f32).
 struct ScanArgs {
   const uint16_t* rows_b;
   const float* y2m;
@@ -264,10 +268,8 @@ struct ScanArgs {
   ListLayout lay;
   int lbase;
   uint32_t* cand_idx;  // q x lay.per_query()
-  float* cand_U;       // MODE 3: the MFMA value of each entry (the seed selection derives U)
   uint32_t* cand_count;  // q x lay.nlists()
   const float* seed;   // q x k, or null
-  float* topu_out;     // q x nseg x k, or null
 };
 
 // Running launch.  KT: registers of the per-query top-k (16 for k <= 16,
@@ -554,11 +556,6 @@ __global__ __launch_bounds__(kNT, 2) void mixed_scan_kernel(const ScanArgs a) {
     if ((t & 3) == 3) __syncthreads();
   }
   if (qo < q) a.cand_count[static_cast<size_t>(qo) * nl + lseg] = cnt;
-  if (a.topu_out != nullptr && qo < q) {
-#pragma unroll
-    for (int j = 0; j < KT; ++j)
-      if (j < k) a.topu_out[(static_cast<size_t>(qo) * nseg + seg) * k + j] = tk[j];
-  }
 }
 
 // Fixed-threshold launch (see the file comment).  The bound's terms are
@@ -600,7 +597,7 @@ __device__ __forceinline__ uint16_t bf16_ru_pos(float f) {
   return static_cast<uint16_t>((u >> 16) + ((u & 0xffffu) != 0u ? 1u : 0u));
 }
 
-template <int KS, bool STORE_U>
+template <int KS>
 __global__ __launch_bounds__(kNT, 2) void fixed_scan_kernel(const ScanArgs a) {
   using S = FixTile<KS>;
   const uint16_t* __restrict__ rows_b = a.rows_b;
@@ -653,7 +650,7 @@ __global__ __launch_bounds__(kNT, 2) void fixed_scan_kernel(const ScanArgs a) {
     for (int ks = 0; ks < KS; ++ks)
       bq[cb][ks] = load_frag(a.q_b + qr * S::LD + 16 * ks + 8 * half);
     x2v[cb] = qvalid[cb] ? a.x2m[qr] : INFINITY;
-    // the seed's k-th U; the epilogue threshold is T - x2m rounded up
+    // the seed's k-th value T; the epilogue threshold is T - x2m rounded up
     kfix[cb] = qvalid[cb] ? a.seed[qr * k + k - 1] : INFINITY;
     thr[cb] = qvalid[cb]
                   ? __double2float_ru(static_cast<double>(kfix[cb]) - static_cast<double>(x2v[cb]))
@@ -745,7 +742,6 @@ __global__ __launch_bounds__(kNT, 2) void fixed_scan_kernel(const ScanArgs a) {
             if (cntf[cb] < static_cast<uint32_t>(lcap)) {
               const size_t e = static_cast<size_t>(qi) * perq + loff + cntf[cb];
               a.cand_idx[e] = static_cast<uint32_t>(row);
-              if constexpr (STORE_U) a.cand_U[e] = val;
             }
             ++cntf[cb];
           }
@@ -816,71 +812,24 @@ __global__ __launch_bounds__(kNT, 2) void fixed_scan_kernel(const ScanArgs a) {
   }
 }
 
-// The k smallest of a query's nseg ascending top-k lists (U bounds of
-// distinct rows), ascending: the seed of the launches after them.
-__global__ void merge_topu_kernel(const float* __restrict__ topu, int64_t q, int nseg, int k,
-                                  float* __restrict__ seed) {
-  const int64_t qi = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (qi >= q) return;
-  const float* t = topu + static_cast<size_t>(qi) * nseg * k;
-  int pos[64];
-  for (int s = 0; s < nseg; ++s) pos[s] = 0;
-  for (int j = 0; j < k; ++j) {
-    int best = -1;
-    float bv = INFINITY;
-    for (int s = 0; s < nseg; ++s)
-      if (pos[s] < k && t[s * k + pos[s]] < bv) {
-        bv = t[s * k + pos[s]];
-        best = s;
-      }
-    if (best >= 0) ++pos[best];
-    seed[qi * k + j] = bv;
-  }
-}
-
-// U of a stored entry of a fixed launch: val = y2m - 2C - e from the MFMA
-// (the folded step's e = RU(qa1) RU(nry) + RU(qa2) RU(nyh), exact BF16
-// products), so L = x2m + val and
-//   U = L + 2 e + (2^-16 + 2 * 2^-18) (x2m + y2m),
-// rounded up: a bound above the f64 distance of the row.
-__device__ __forceinline__ float entry_u(float val, float x2m, float qa1, float qa2, const float* terms,
-                                         int64_t npad, uint32_t row) {
-  auto ru = [](float f) {  // f >= 0: the smallest BF16 >= f, as f64
-    const uint32_t u = __float_as_uint(f);
-    return static_cast<double>(__uint_as_float(((u >> 16) + ((u & 0xffffu) != 0u ? 1u : 0u)) << 16));
-  };
-  const double L = static_cast<double>(x2m) + static_cast<double>(val);
-  const double eu = (ru(qa1) * ru(terms[2 * npad + row]) + ru(qa2) * ru(terms[3 * npad + row])) *
-                    (1.0 + 4.440892098500626e-16);
-  const double U = L + 2.0 * eu +
-                   (kHiMargin + 2.0 * kLoMargin) *
-                       (static_cast<double>(x2m) + static_cast<double>(terms[row]));
-  return __double2float_ru(U);
-}
-
-// The k smallest of a seed and the U values of the entries stored by a
-// fixed launch in lists [l0, l0 + nl) (every U is a bound of a distinct row
-// after the seed's rows): the next seed, ascending.  Entries past a list's
-// cap were not stored; leaving them out keeps a valid seed.
-__global__ void select_seed_kernel(const float* __restrict__ seed_in, const float* __restrict__ cand_val,
-                                   const uint32_t* __restrict__ cand_idx,
+// The k smallest of a seed (null: +inf) and the exact distances, rounded
+// up to f32, of the entries in lists [l0, l0 + nl) (candidates of rows after
+// the seed's rows, distinct): the next seed, ascending.  Entries past a
+// list's cap were not stored; leaving them out keeps a valid seed.
+__global__ void select_seed_kernel(const float* __restrict__ seed_in, const double* __restrict__ cand_S,
                                    const uint32_t* __restrict__ cand_count, ListLayout lay, int l0,
-                                   int nl, int64_t q, int k, const float* __restrict__ x2m,
-                                   const float* __restrict__ qa1, const float* __restrict__ qa2,
-                                   const float* __restrict__ terms, int64_t npad,
-                                   float* __restrict__ seed_out) {
+                                   int nl, int64_t q, int k, float* __restrict__ seed_out) {
   const int64_t qi = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (qi >= q) return;
   float best[kMaxK];
-  for (int j = 0; j < k; ++j) best[j] = seed_in[qi * k + j];
+  for (int j = 0; j < k; ++j) best[j] = seed_in ? seed_in[qi * k + j] : INFINITY;
   const int64_t perq = lay.per_query();
-  const float xq = x2m[qi], a1 = qa1[qi], a2 = qa2[qi];
   for (int l = l0; l < l0 + nl; ++l) {
     const uint32_t c = min(cand_count[static_cast<size_t>(qi) * lay.nlists() + l],
                            static_cast<uint32_t>(lay.cap(l)));
-    const size_t base = static_cast<size_t>(qi) * perq + lay.offset(l);
+    const double* sv = cand_S + static_cast<size_t>(qi) * perq + lay.offset(l);
     for (uint32_t e = 0; e < c; ++e) {
-      const float v = entry_u(cand_val[base + e], xq, a1, a2, terms, npad, cand_idx[base + e]);
+      const float v = __double2float_ru(sv[e]);
       if (!(v < best[k - 1])) continue;
       int pos = k - 1;
       while (pos > 0 && best[pos - 1] > v) {
@@ -917,28 +866,31 @@ __device__ inline void sift_down_s(double* hd, int64_t* hi, double* hs, int kk, 
 constexpr int kRescoreThreads = 128;
 __global__ __launch_bounds__(kRescoreThreads) void mixed_rescore_kernel(
     const float* __restrict__ rows, const float* __restrict__ qs, int ld, int64_t q, ListLayout lay,
-    const uint32_t* __restrict__ cand_idx, const uint32_t* __restrict__ cand_count,
-    const float* __restrict__ x2m, double* __restrict__ cand_S) {
+    int l_lo, int l_hi, const uint32_t* __restrict__ cand_idx,
+    const uint32_t* __restrict__ cand_count, const float* __restrict__ x2m,
+    double* __restrict__ cand_S) {
   const int64_t qi = blockIdx.x;
   if (qi >= q || !(x2m[qi] < INFINITY)) return;  // uniform: listed for the exact kernel
   __shared__ float qrow[16 * kMaxKS];
   __shared__ uint32_t start[kMaxLists + 1];
   const int nl = lay.nlists();
   for (int j = threadIdx.x; j < ld; j += kRescoreThreads) qrow[j] = qs[qi * ld + j];
+  const int nr = l_hi - l_lo;  // lists l_lo .. l_hi - 1
   if (threadIdx.x == 0) {
     uint32_t acc = 0;
-    for (int l = 0; l < nl; ++l) {
-      start[l] = acc;
-      acc += min(cand_count[static_cast<size_t>(qi) * nl + l], static_cast<uint32_t>(lay.cap(l)));
+    for (int i = 0; i < nr; ++i) {
+      start[i] = acc;
+      acc += min(cand_count[static_cast<size_t>(qi) * nl + l_lo + i],
+                 static_cast<uint32_t>(lay.cap(l_lo + i)));
     }
-    start[nl] = acc;
+    start[nr] = acc;
   }
   __syncthreads();
   const size_t qbase = static_cast<size_t>(qi) * lay.per_query();
-  for (uint32_t e = threadIdx.x; e < start[nl]; e += kRescoreThreads) {
-    int l = 0;
-    while (start[l + 1] <= e) ++l;
-    const size_t slot = qbase + lay.offset(l) + (e - start[l]);
+  for (uint32_t e = threadIdx.x; e < start[nr]; e += kRescoreThreads) {
+    int i = 0;
+    while (start[i + 1] <= e) ++i;
+    const size_t slot = qbase + lay.offset(l_lo + i) + (e - start[i]);
     const float* rrow = rows + static_cast<int64_t>(cand_idx[slot]) * ld;
     double sum = 0.0;
     for (int j = 0; j < ld; ++j) sum = sum + Term<float, kEuclid>::f(qrow[j], rrow[j], 2.0);
@@ -1077,20 +1029,20 @@ void launch_running(dim3 grid, hipStream_t stream, const ScanArgs& a) {
   hipLaunchKernelGGL((mixed_scan_kernel<KS, KT>), grid, dim3(kNT), lds, stream, a);
 }
 
-template <int KS, bool STORE_U>
+template <int KS>
 void launch_fixed(dim3 grid, hipStream_t stream, const ScanArgs& a) {
   const size_t lds = fixed_lds_bytes<KS>();
   static const bool attr = [] {
-    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&fixed_scan_kernel<KS, STORE_U>),
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&fixed_scan_kernel<KS>),
                               hipFuncAttributeMaxDynamicSharedMemorySize,
                               static_cast<int>(fixed_lds_bytes<KS>()));
     return true;
   }();
   (void)attr;
-  hipLaunchKernelGGL((fixed_scan_kernel<KS, STORE_U>), grid, dim3(kNT), lds, stream, a);
+  hipLaunchKernelGGL((fixed_scan_kernel<KS>), grid, dim3(kNT), lds, stream, a);
 }
 
-// mode: 0 running, 2 fixed threshold (from a seed), 3 the same storing U
+// mode: 0 running, 2 fixed threshold (from a seed)
 int launch_scan(int ks, int mode, const ScanArgs& a, hipStream_t stream) {
   const int64_t nblocks = a.nseg <= kXcds
                               ? (a.nqt + kXcds / a.nseg - 1) / (kXcds / a.nseg) * kXcds
@@ -1100,9 +1052,7 @@ int launch_scan(int ks, int mode, const ScanArgs& a, hipStream_t stream) {
 #define SC_SCAN_CASE(KS)                                      \
   case KS:                                                   \
     if (mode == 2)                                           \
-      launch_fixed<KS, false>(grid, stream, a);              \
-    else if (mode == 3)                                      \
-      launch_fixed<KS, true>(grid, stream, a);               \
+      launch_fixed<KS>(grid, stream, a);                     \
     else if (a.k <= 16)                                      \
       launch_running<KS, 16>(grid, stream, a);               \
     else                                                     \
@@ -1155,22 +1105,22 @@ int knn_mixed_device(const sc_index* cix, const void* d_queries, size_t q_, size
   // Levels.  Small indexes: one running launch over all rows.  Indexes of
   // kSeedMinRows rows or more: a running launch over the first kPrefixRows
   // rows, then fixed-threshold launches over row ranges growing kLevelRatio
-  // times, each testing against the k-th U of all rows before it (the
-  // prefix's top-k, then the selection over the previous level's U).
+  // times.  The candidates of each level are rescored as it ends, and the
+  // next level tests against the k-th smallest exact distance among all
+  // earlier candidates (rounded up to f32).
   struct Level {
     int64_t r0, r1;
-    int mode;
   };
   std::vector<Level> levels;
   if (n >= kSeedMinRows) {
-    levels.push_back({0, kPrefixRows, 0});
+    levels.push_back({0, kPrefixRows});
     for (int64_t b = kPrefixRows; b < n;) {
       const int64_t e = std::min<int64_t>(n, b * kLevelRatio);
-      levels.push_back({b, e, e < n ? 3 : 2});
+      levels.push_back({b, e});
       b = e;
     }
   } else {
-    levels.push_back({0, n, 0});
+    levels.push_back({0, n});
   }
   auto seg_len_of = [&](const Level& L) {
     const int64_t lt = std::max<int64_t>(1, (L.r1 - L.r0 + kRows - 1) / kRows);
@@ -1198,7 +1148,7 @@ int knn_mixed_device(const sc_index* cix, const void* d_queries, size_t q_, size
   }
   const int64_t perq = lay.per_query();
 
-  DevBuf qb, x2, cx, cidx, cU, cS, ccnt, hs, lst, flags, tpu, seedA, seedB;
+  DevBuf qb, x2, cx, cidx, cS, ccnt, hs, lst, flags, seedA, seedB;
   SC_HIP(qb.alloc(sizeof(uint16_t) * q * ld, stream));
   SC_HIP(x2.alloc(sizeof(float) * q, stream));
   SC_HIP(cx.alloc(sizeof(float) * 3 * q, stream));  // cxb | qa1 | qa2
@@ -1206,10 +1156,8 @@ int knn_mixed_device(const sc_index* cix, const void* d_queries, size_t q_, size
   SC_HIP(cS.alloc(sizeof(double) * q * perq, stream));
   SC_HIP(ccnt.alloc(sizeof(uint32_t) * q * lay.nlists(), stream));
   if (nfix_levels > 0) {
-    SC_HIP(tpu.alloc(sizeof(float) * q * nseg * k, stream));
     SC_HIP(seedA.alloc(sizeof(float) * q * k, stream));
     SC_HIP(seedB.alloc(sizeof(float) * q * k, stream));
-    if (nfix_levels > 1) SC_HIP(cU.alloc(sizeof(float) * q * perq, stream));
   }
   SC_HIP(hs.alloc(sizeof(double) * q * k, stream));
   SC_HIP(lst.alloc(sizeof(uint32_t) * q, stream));
@@ -1241,24 +1189,38 @@ int knn_mixed_device(const sc_index* cix, const void* d_queries, size_t q_, size
   sa.nseg = nseg;
   sa.lay = lay;
   sa.cand_idx = cidx.as<uint32_t>();
-  sa.cand_U = nfix_levels > 1 ? cU.as<float>() : nullptr;
   sa.cand_count = ccnt.as<uint32_t>();
   float* seed_cur = seedA.as<float>();
   float* seed_next = seedB.as<float>();
+  // exact distances of the candidates in lists [l_lo, l_hi)
+  auto rescore = [&](int l_lo, int l_hi) -> int {
+    hipLaunchKernelGGL(mixed_rescore_kernel, dim3(static_cast<unsigned>(q)), dim3(kRescoreThreads),
+                       0, stream, static_cast<const float*>(ix->rows), qf, ld, q, lay, l_lo, l_hi,
+                       cidx.as<uint32_t>(), ccnt.as<uint32_t>(), x2.as<float>(), cS.as<double>());
+    SC_HIP(hipGetLastError());
+    return SC_OK;
+  };
+  // the next seed from the current one (null: none yet) and lists [l0, l0 + nl)
+  auto select = [&](const float* in, int l0, int nl) -> int {
+    hipLaunchKernelGGL(select_seed_kernel, dim3(static_cast<unsigned>((q + 255) / 256)), dim3(256),
+                       0, stream, in, cS.as<double>(), ccnt.as<uint32_t>(), lay, l0, nl, q, k,
+                       seed_next);
+    SC_HIP(hipGetLastError());
+    std::swap(seed_cur, seed_next);
+    return SC_OK;
+  };
   {
-    // every launch before the last one is timed as the prefix
+    // every launch before the last level's scan is timed as the prefix
     ProfScope ps(nfix_levels > 0 ? "knn_mixed_prefix" : "knn_mixed_gemm", stream);
     ScanArgs la = sa;
     la.row0 = 0;
     la.row_end = levels[0].r1;
     la.seg_len = seg_len_of(levels[0]);
     la.lbase = 0;
-    la.topu_out = nfix_levels > 0 ? tpu.as<float>() : nullptr;
     if (int rc = launch_scan(ld / 16, 0, la, stream)) return rc;
     if (nfix_levels > 0) {
-      hipLaunchKernelGGL(merge_topu_kernel, dim3(static_cast<unsigned>((q + 255) / 256)), dim3(256),
-                         0, stream, tpu.as<float>(), q, nseg, k, seed_cur);
-      SC_HIP(hipGetLastError());
+      if (int rc = rescore(0, nseg)) return rc;
+      if (int rc = select(nullptr, 0, nseg)) return rc;
     }
     for (int i = 1; i < nfix_levels; ++i) {
       ScanArgs fa = sa;
@@ -1267,15 +1229,12 @@ int knn_mixed_device(const sc_index* cix, const void* d_queries, size_t q_, size
       fa.seg_len = seg_len_of(levels[i]);
       fa.lbase = nseg + 2 * nseg * (i - 1);
       fa.seed = seed_cur;
-      if (int rc = launch_scan(ld / 16, 3, fa, stream)) return rc;
-      hipLaunchKernelGGL(select_seed_kernel, dim3(static_cast<unsigned>((q + 255) / 256)),
-                         dim3(256), 0, stream, seed_cur, cU.as<float>(), cidx.as<uint32_t>(),
-                         ccnt.as<uint32_t>(), lay, fa.lbase, 2 * nseg, q, k, sa.x2m, sa.qa1, sa.qa2,
-                         sa.y2m, sa.npad, seed_next);
-      SC_HIP(hipGetLastError());
-      std::swap(seed_cur, seed_next);
+      if (int rc = launch_scan(ld / 16, 2, fa, stream)) return rc;
+      if (int rc = rescore(fa.lbase, fa.lbase + 2 * nseg)) return rc;
+      if (int rc = select(seed_cur, fa.lbase, 2 * nseg)) return rc;
     }
   }
+  int l_last = 0;  // first list of the last scan
   if (nfix_levels > 0) {
     ProfScope ps("knn_mixed_gemm", stream);
     const int i = nfix_levels;
@@ -1285,14 +1244,12 @@ int knn_mixed_device(const sc_index* cix, const void* d_queries, size_t q_, size
     fa.seg_len = seg_len_of(levels[i]);
     fa.lbase = nseg + 2 * nseg * (i - 1);
     fa.seed = seed_cur;
+    l_last = fa.lbase;
     if (int rc = launch_scan(ld / 16, 2, fa, stream)) return rc;
   }
   {
     ProfScope ps("knn_mixed_rescore", stream);
-    hipLaunchKernelGGL(mixed_rescore_kernel, dim3(static_cast<unsigned>(q)), dim3(kRescoreThreads),
-                       0, stream, static_cast<const float*>(ix->rows), qf, ld, q, lay,
-                       cidx.as<uint32_t>(), ccnt.as<uint32_t>(), x2.as<float>(), cS.as<double>());
-    SC_HIP(hipGetLastError());
+    if (int rc = rescore(l_last, lay.nlists())) return rc;
     hipLaunchKernelGGL(mixed_replay_kernel, dim3(static_cast<unsigned>((q + 63) / 64)), dim3(64), 0,
                        stream, q, k, lay, cidx.as<uint32_t>(), cS.as<double>(), ccnt.as<uint32_t>(),
                        x2.as<float>(), hs.as<double>(), d_idx, d_dist, d_count, lst.as<uint32_t>(),
