@@ -663,8 +663,10 @@ __global__ __launch_bounds__(kNT, 2) void fixed_scan_kernel(const ScanArgs a) {
   }
   uint32_t cntf[2] = {0u, 0u};
 
-  // tile loads: thread tid carries pieces tid and tid + kNT of the rows;
-  // threads 0..31 carry the four terms of row tid
+  // tile loads: thread tid carries pieces tid and tid + kNT of the rows.
+  // The terms and folded columns of tile t (t = 4 m + j) are carried by
+  // lanes 0..15 of waves 2 j and 2 j + 1 (rows 16 (w & 1) + lane), so the
+  // work is spread over all eight waves of a group of four tiles.
   struct TileRegs {
     u32x4 v[S::NP];
     f32x4 tm;
@@ -680,9 +682,9 @@ __global__ __launch_bounds__(kNT, 2) void fixed_scan_kernel(const ScanArgs a) {
       }
     }
     // y2m, nyb, nry and nyh are consecutive arrays of a.npad floats
-    if (tid < kRows) {
-      const float* p = y2m + min<int64_t>(t0 + tid, a.npad - 1);
-      r.tm = f32x4{p[0], p[a.npad], p[2 * a.npad], p[3 * a.npad]};
+    if ((wave >> 1) == static_cast<int>(t & 3) && lane < 16) {
+      const float* p = y2m + min<int64_t>(t0 + 16 * (wave & 1) + lane, a.npad - 1);
+      r.tm = f32x4{p[0], 0.0f, p[2 * a.npad], p[3 * a.npad]};
     }
   };
   auto swrite = [&](int sl, const TileRegs& r) {
@@ -694,16 +696,18 @@ __global__ __launch_bounds__(kNT, 2) void fixed_scan_kernel(const ScanArgs a) {
         *reinterpret_cast<u32x4*>(base + 2 * ((e / (S::LD / 8)) * S::RS + 8 * (e % (S::LD / 8)))) =
             r.v[pc];
     }
-    if (tid < kRows) {
+    // slot sl holds a tile t with t = sl (mod 4)
+    if ((wave >> 1) == (sl & 3) && lane < 16) {
       // folded columns: y2m split in three (exact up to 2^-24 y2m), the
       // row factors rounded up
+      const int fr = 16 * (wave & 1) + lane;
       const float y = r.tm[0];
       const uint32_t h = bf16_rne(y);
       const float e1 = y - __uint_as_float(h << 16);
       const uint32_t m = bf16_rne(e1);
       const uint32_t l = bf16_rne(e1 - __uint_as_float(m << 16));
       const uint32_t c1 = bf16_ru_pos(r.tm[2]), c2 = bf16_ru_pos(r.tm[3]);
-      u32x4* fc = reinterpret_cast<u32x4*>(base + 2 * (tid * S::RS + S::LD));
+      u32x4* fc = reinterpret_cast<u32x4*>(base + 2 * (fr * S::RS + S::LD));
       fc[0] = u32x4{h | (m << 16), l | (c1 << 16), c2, 0u};
       fc[1] = u32x4{0u, 0u, 0u, 0u};
     }
