@@ -141,8 +141,8 @@ int sc_kmeans_step(const sc_index* index, const double* centroids, size_t k,
 /* The Lloyd loop of KMeans::fit (kmeans.rs:294-310) from caller-supplied
  * initial centroids (k*d f64, updated in place): at most max_iter steps,
  * centroid_i = sums_i / size_i when size_i > 0, stop when the previous
- * distortion <= the new one.  mode SC_MIXED (f32 rows, k <= 128, d <= 128)
- * runs the assignment on BF16 MFMA cross terms certified against the FP64
+ * distortion <= the new one.  mode SC_MIXED (f32 rows, k <= 128, d <= 128,
+ * not both k > 64 and d > 64) runs the assignment on BF16 MFMA cross terms certified against the FP64
  * rule (rows the bound cannot separate are reassigned in FP64) fused with the
  * FP64 sums, so its labels, sizes, centroids, distortion and iteration count
  * equal SC_PLAIN_FP64's bit for bit; other shapes run SC_PLAIN_FP64.  Errors:
@@ -162,7 +162,7 @@ int sc_kmeans_run(const sc_index* index, size_t k, size_t max_iter,
  * iterations after the grid-size trials are replayed from one hipGraph.
  * Each problem's results are bit-identical to sc_kmeans_run on it alone.
  * Other modes and shapes outside the batched kernel's limits (k <= 128,
- * d <= 128) run the problems one by one. */
+ * d <= 128, not both k > 64 and d > 64) run the problems one by one. */
 typedef struct {
   const void* rows;
   size_t n;
