@@ -28,6 +28,16 @@ namespace sc {
 // (r = 0 - 0 = 0, |0| = 0, 0^p = 0) and therefore leaves it bit-identical.
 constexpr int kRowPad = 16;
 
+// A pointer read from memory (a field of a problem struct) has no known
+// address space, so its loads compile to FLAT instructions.  Those count in
+// LGKM_CNT as well as VM_CNT, so every wait for an LDS result would also
+// wait for them (a prefetch would stall the next LDS read).  Device buffers
+// are global memory: this says so.
+template <typename T>
+__device__ __forceinline__ __attribute__((address_space(1))) T* to_global(T* p) {
+  return (__attribute__((address_space(1))) T*)p;
+}
+
 inline size_t padded_dim(size_t d) {
   const size_t r = (d + kRowPad - 1) / kRowPad * kRowPad;
   return r ? r : kRowPad;

@@ -275,7 +275,6 @@ __global__ __launch_bounds__(kReduceChunk) void kmeans_group_kernel(
     const KmProb* __restrict__ P, const uint2* __restrict__ items) {
   const uint2 it = items[blockIdx.x];
   const KmProb p = P[it.x];
-  if (p.st->done) return;
   const int64_t kd = static_cast<int64_t>(p.k) * p.d;
   const int64_t slices = (kd + kReduceChunk - 1) / kReduceChunk;
   const int64_t g = it.y / slices;
@@ -283,19 +282,29 @@ __global__ __launch_bounds__(kReduceChunk) void kmeans_group_kernel(
   if (e >= kd) return;
   const int64_t b0 = g * kGroupBlocks;
   const int nbk = static_cast<int>(min<int64_t>(kGroupBlocks, p.nb - b0));
+  // the partials are loaded before the stop flag is read, so both requests
+  // are in flight together
   double v[kGroupBlocks];
 #pragma unroll
-  for (int i = 0; i < kGroupBlocks; ++i) v[i] = i < nbk ? p.psum[(b0 + i) * kd + e] : 0.0;
+  for (int i = 0; i < kGroupBlocks; ++i)
+    v[i] = i < nbk ? to_global(p.psum)[(b0 + i) * kd + e] : 0.0;
+  if (*reinterpret_cast<volatile const uint32_t*>(&p.st->done)) return;
   double s = 0.0;
 #pragma unroll
   for (int i = 0; i < kGroupBlocks; ++i)
     if (i < nbk) s = s + v[i];
-  p.gsum[g * kd + e] = s;
+  to_global(p.gsum)[g * kd + e] = s;
   if (e % p.d == 0) {
+    // the count: integer adds, loaded together (any order gives the same sum)
     const int64_t c = e / p.d;
+    uint32_t w[kGroupBlocks];
+#pragma unroll
+    for (int i = 0; i < kGroupBlocks; ++i)
+      w[i] = i < nbk ? to_global(p.pcount)[(b0 + i) * p.k + c] : 0u;
     uint64_t cnt = 0;
-    for (int i = 0; i < nbk; ++i) cnt += p.pcount[(b0 + i) * p.k + c];
-    p.gcount[g * p.k + c] = cnt;
+#pragma unroll
+    for (int i = 0; i < kGroupBlocks; ++i) cnt += w[i];
+    to_global(p.gcount)[g * p.k + c] = cnt;
   }
 }
 
@@ -358,7 +367,7 @@ __global__ __launch_bounds__(128) void kmeans_final_kernel(const KmProb* __restr
   __syncthreads();
   {
     uint64_t n = 0;
-    for (int64_t g = tid; g < p.ng; g += 128) n += p.gcount[g * p.k + c];
+    for (int64_t g = tid; g < p.ng; g += 128) n += to_global(p.gcount)[g * p.k + c];
     if (n) atomicAdd(reinterpret_cast<unsigned long long*>(&s_n), static_cast<unsigned long long>(n));
   }
   __syncthreads();
@@ -369,7 +378,7 @@ __global__ __launch_bounds__(128) void kmeans_final_kernel(const KmProb* __restr
   for (int j0 = 0; j0 < d; j0 += 128) {
     const int j = j0 + tid;
     if (j < d) {
-      const double* gs = p.gsum + static_cast<int64_t>(c) * d + j;
+      const auto gs = to_global(p.gsum) + static_cast<int64_t>(c) * d + j;
       double S = 0.0;
       int64_t g = 0;
       for (; g + 8 <= p.ng; g += 8) {

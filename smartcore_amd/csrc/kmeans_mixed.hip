@@ -120,7 +120,8 @@ __device__ __forceinline__ void lexmin_d(double& v, uint32_t& i, double v2, uint
 
 // s_c of KMeans::predict for one row (f32, stride 1) and one centroid (f64):
 // the reference's sequential fold.
-__device__ __forceinline__ double exact_s(const float* x, const double* c, int ld) {
+template <typename CP>
+__device__ __forceinline__ double exact_s(const float* x, CP c, int ld) {
   double s = 0.0;
   for (int j = 0; j < ld; ++j) s = s + Term<double, kEuclid>::f(static_cast<double>(x[j]), c[j], 2.0);
   return s;
@@ -165,13 +166,13 @@ __device__ __forceinline__ void km_block(const KmProb& p, int64_t b, unsigned ch
   double* ctab_s = reinterpret_cast<double*>(smem + C::FIXED);
   auto exact_c = [&](const float* x, uint32_t c) -> double {
     return ctab_lds ? exact_s(x, ctab_s + static_cast<int64_t>(c) * ld, ld)
-                    : exact_s(x, p.cent + static_cast<int64_t>(c) * ld, ld);
+                    : exact_s(x, to_global(p.cent) + static_cast<int64_t>(c) * ld, ld);
   };
 
   // ---- per-block tables ----
   for (int e = tid; e < C::KP; e += NTH) {
     cnt[e] = 0u;
-    c2s[e] = e < p.kpad ? p.c2f[e] : INFINITY;
+    c2s[e] = e < p.kpad ? to_global(p.c2f)[e] : INFINITY;
   }
   for (int e = tid; e < C::KP * C::LD / 8; e += NTH) {
     const int r = e / (C::LD / 8), j8 = e % (C::LD / 8);
@@ -179,18 +180,21 @@ __device__ __forceinline__ void km_block(const KmProb& p, int64_t b, unsigned ch
     const int64_t src = static_cast<int64_t>(r) * ld + 8 * j8;
     const u32x4 z = {0u, 0u, 0u, 0u};
     const int dst = r * C::LDA + 8 * j8;
-    *reinterpret_cast<u32x4*>(ahi + dst) = in ? *reinterpret_cast<const u32x4*>(p.bhi + src) : z;
-    *reinterpret_cast<u32x4*>(amd + dst) = in ? *reinterpret_cast<const u32x4*>(p.bmd + src) : z;
-    *reinterpret_cast<u32x4*>(alo + dst) = in ? *reinterpret_cast<const u32x4*>(p.blo + src) : z;
+    *reinterpret_cast<u32x4*>(ahi + dst) =
+        in ? *to_global(reinterpret_cast<const u32x4*>(p.bhi + src)) : z;
+    *reinterpret_cast<u32x4*>(amd + dst) =
+        in ? *to_global(reinterpret_cast<const u32x4*>(p.bmd + src)) : z;
+    *reinterpret_cast<u32x4*>(alo + dst) =
+        in ? *to_global(reinterpret_cast<const u32x4*>(p.blo + src)) : z;
   }
   if (ctab_lds)
-    for (int64_t e = tid; e < static_cast<int64_t>(k) * ld; e += NTH) ctab_s[e] = p.cent[e];
+    for (int64_t e = tid; e < static_cast<int64_t>(k) * ld; e += NTH) ctab_s[e] = to_global(p.cent)[e];
   auto frag = [&](const uint16_t* tab, int cb, int ks) -> bf16x8 {
     return load_frag(tab + (32 * cb + col) * C::LDA + 16 * ks + 8 * half);
   };
 
-  const float cmax = p.cmax[0];
-  const float c2max = p.cmax[1];
+  const float cmax = to_global(p.cmax)[0];
+  const float c2max = to_global(p.cmax)[1];
   // cross-term error constant, relative to sum |x_i| |v_i| <= ||x|| ||v||:
   // representation (both three-term residuals, the f32 rounding of -2c) and
   // the dropped mid x lo, lo x mid, lo x lo products (5.3 * 2^-24), and FP32
@@ -214,7 +218,7 @@ __device__ __forceinline__ void km_block(const KmProb& p, int64_t b, unsigned ch
   f32x4 pf[C::PFV];
   auto load_chunk = [&](int c0) {
     const int nr = min(CR, rows_here - c0);
-    const f32x4* src = reinterpret_cast<const f32x4*>(rows + (r0 + c0) * ld);
+    const auto src = to_global(reinterpret_cast<const f32x4*>(rows + (r0 + c0) * ld));
     const int nvec = nr * ldq;
 #pragma unroll
     for (int i = 0; i < C::PFV; ++i) {
@@ -410,7 +414,7 @@ __device__ __forceinline__ void km_block(const KmProb& p, int64_t b, unsigned ch
         for (int dd = 0; dd < C::DPL; ++dd) {
           const int j = lane + 64 * dd;
           if (j < ld) {
-            const double cj = ctab_lds ? ctab_s[cofs + j] : p.cent[cofs + j];
+            const double cj = ctab_lds ? ctab_s[cofs + j] : to_global(p.cent)[cofs + j];
             t = t + Term<double, kEuclid>::f(static_cast<double>(xs[j]), cj, 2.0);
           }
         }
@@ -489,7 +493,7 @@ __device__ __forceinline__ void km_block(const KmProb& p, int64_t b, unsigned ch
       if (lane == 0) lab[32 * wave + q] = bv < DBL_MAX ? bj : 0u;
     }
     __syncthreads();
-    if (tid < nr) p.labels[r0 + c0 + tid] = lab[tid];
+    if (tid < nr) to_global(p.labels)[r0 + c0 + tid] = lab[tid];
 
     // ---- sums: wave owns clusters WAVES s + wave, rows in order ----
     {
@@ -528,10 +532,10 @@ __device__ __forceinline__ void km_block(const KmProb& p, int64_t b, unsigned ch
 #pragma unroll
     for (int dd = 0; dd < C::DPL; ++dd) {
       const int dim = lane + 64 * dd;
-      if (dim < d) p.psum[b * kd + static_cast<int64_t>(c) * d + dim] = sacc[s * C::DPL + dd];
+      if (dim < d) to_global(p.psum)[b * kd + static_cast<int64_t>(c) * d + dim] = sacc[s * C::DPL + dd];
     }
   }
-  for (int c = tid; c < k; c += NTH) p.pcount[b * k + c] = cnt[c];
+  for (int c = tid; c < k; c += NTH) to_global(p.pcount)[b * k + c] = cnt[c];
   fallback += fb;
   __syncthreads();  // LDS is rewritten by the next block
 }
