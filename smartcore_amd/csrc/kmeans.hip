@@ -372,7 +372,7 @@ __global__ __launch_bounds__(128) void kmeans_final_kernel(const KmProb* __restr
   }
   __syncthreads();
   const uint64_t n = s_n;
-  if (tid == 0) p.counts[c] = n;
+  if (tid == 0) to_global(p.counts)[c] = n;
   const bool upd = update && n > 0;
   double dot = 0.0, c2 = 0.0, n2 = 0.0;
   for (int j0 = 0; j0 < d; j0 += 128) {
@@ -389,19 +389,19 @@ __global__ __launch_bounds__(128) void kmeans_final_kernel(const KmProb* __restr
         for (int i = 0; i < 8; ++i) S = S + v[i];
       }
       for (; g < p.ng; ++g) S = S + gs[g * kd];
-      p.sums[static_cast<int64_t>(c) * d + j] = S;
-      const double cj = p.cent[static_cast<int64_t>(c) * p.ld + j];
+      to_global(p.sums)[static_cast<int64_t>(c) * d + j] = S;
+      const double cj = to_global(p.cent)[static_cast<int64_t>(c) * p.ld + j];
       const double nc = upd ? S / static_cast<double>(n) : cj;
       s_dot[tid] = cj * S;
       s_c2[tid] = cj * cj;
       s_n2[tid] = nc * nc;
-      if (upd) p.cent[static_cast<int64_t>(c) * p.ld + j] = nc;
+      if (upd) to_global(p.cent)[static_cast<int64_t>(c) * p.ld + j] = nc;
       if (tables) {
         uint16_t h, m, l;
         split_m2(nc, h, m, l);
-        p.bhi[static_cast<int64_t>(c) * p.ld + j] = h;
-        p.bmd[static_cast<int64_t>(c) * p.ld + j] = m;
-        p.blo[static_cast<int64_t>(c) * p.ld + j] = l;
+        to_global(p.bhi)[static_cast<int64_t>(c) * p.ld + j] = h;
+        to_global(p.bmd)[static_cast<int64_t>(c) * p.ld + j] = m;
+        to_global(p.blo)[static_cast<int64_t>(c) * p.ld + j] = l;
       }
     }
     __syncthreads();
@@ -424,10 +424,10 @@ __global__ __launch_bounds__(128) void kmeans_final_kernel(const KmProb* __restr
   n2 = s_n2[0];
   __syncthreads();
   if (tid == 0) {
-    p.term[c] = n > 0 ? static_cast<double>(n) * c2 - 2.0 * dot : 0.0;
+    to_global(p.term)[c] = n > 0 ? static_cast<double>(n) * c2 - 2.0 * dot : 0.0;
     if (tables) {
-      p.c2f[c] = static_cast<float>(n2);
-      p.c2d[c] = n2;
+      to_global(p.c2f)[c] = static_cast<float>(n2);
+      to_global(p.c2d)[c] = n2;
     }
     __threadfence();
     const uint32_t old = atomicAdd(p.arrive, 1u);
