@@ -169,6 +169,24 @@ __device__ __forceinline__ void km_block(const KmProb& p, int64_t b, unsigned ch
                     : exact_s(x, to_global(p.cent) + static_cast<int64_t>(c) * ld, ld);
   };
 
+  // the first chunk's rows are requested before the tables are loaded, so
+  // their latency overlaps the table loads
+  const float* rows = static_cast<const float*>(p.rows);
+  const int64_t r0 = b * kBlockRows;
+  const int rows_here = static_cast<int>(min<int64_t>(kBlockRows, p.n - r0));
+  f32x4 pf[C::PFV];
+  auto load_chunk = [&](int c0) {
+    const int nr = min(CR, rows_here - c0);
+    const auto src = to_global(reinterpret_cast<const f32x4*>(rows + (r0 + c0) * ld));
+    const int nvec = nr * ldq;
+#pragma unroll
+    for (int i = 0; i < C::PFV; ++i) {
+      const int e = tid + NTH * i;
+      pf[i] = e < nvec ? src[e] : f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    }
+  };
+  load_chunk(0);
+
   // ---- per-block tables ----
   for (int e = tid; e < C::KP; e += NTH) {
     cnt[e] = 0u;
@@ -212,21 +230,6 @@ __device__ __forceinline__ void km_block(const KmProb& p, int64_t b, unsigned ch
 #pragma unroll
   for (int e = 0; e < C::SLOTS * C::DPL; ++e) sacc[e] = 0.0;
 
-  const float* rows = static_cast<const float*>(p.rows);
-  const int64_t r0 = b * kBlockRows;
-  const int rows_here = static_cast<int>(min<int64_t>(kBlockRows, p.n - r0));
-  f32x4 pf[C::PFV];
-  auto load_chunk = [&](int c0) {
-    const int nr = min(CR, rows_here - c0);
-    const auto src = to_global(reinterpret_cast<const f32x4*>(rows + (r0 + c0) * ld));
-    const int nvec = nr * ldq;
-#pragma unroll
-    for (int i = 0; i < C::PFV; ++i) {
-      const int e = tid + NTH * i;
-      pf[i] = e < nvec ? src[e] : f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-    }
-  };
-  load_chunk(0);
   unsigned long long fb = 0;
 
   for (int c0 = 0; c0 < rows_here; c0 += CR) {
