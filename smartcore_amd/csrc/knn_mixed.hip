@@ -938,31 +938,61 @@ __global__ void mixed_replay_kernel(int64_t q, int k, ListLayout lay,
   }
   const size_t qbase = static_cast<size_t>(qi) * lay.per_query();
   unsigned long long seen = 0;
-  auto visit = [&](size_t e) {
-    const double sum = cand_S[e];
+  // hd[0] (the heap's largest distance) is kept in a register; candidates
+  // below it enter the heap (the reference's strict <)
+  double top = INFINITY;
+  auto visit = [&](double sum, uint32_t row) {
     const double d = __builtin_sqrt(sum);
-    if (d < hd[0]) {
+    if (d < top) {
       hd[0] = d;
-      hi[0] = cand_idx[e];
+      hi[0] = row;
       hs[0] = sum;
       if (k > 1) sift_down_s(hd, hi, hs, 0, k - 1);
+      top = hd[0];
     }
   };
   for (int l = 0; l < lay.nrun; ++l) {
     const size_t base = qbase + lay.offset(l);
-    seen += cc[l];
-    for (uint32_t c = 0; c < cc[l]; ++c) visit(base + c);
+    const uint32_t n = cc[l];
+    seen += n;
+    // eight entries loaded together, visited in order
+    for (uint32_t c0 = 0; c0 < n; c0 += 8) {
+      double sv[8];
+      uint32_t rv[8];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const size_t e = base + min(c0 + j, n - 1);
+        sv[j] = cand_S[e];
+        rv[j] = cand_idx[e];
+      }
+#pragma unroll
+      for (int j = 0; j < 8; ++j)
+        if (c0 + j < n) visit(sv[j], rv[j]);
+    }
   }
   for (int l = lay.nrun; l < nl; l += 2) {
     const size_t b0 = qbase + lay.offset(l), b1 = qbase + lay.offset(l + 1);
     const uint32_t n0 = cc[l], n1 = cc[l + 1];
     seen += n0 + n1;
+    // merge of the pair by row index; each stream's next row and distance
+    // are held in registers
     uint32_t i0 = 0, i1 = 0;
+    uint32_t r0 = n0 > 0 ? cand_idx[b0] : 0xffffffffu, r1 = n1 > 0 ? cand_idx[b1] : 0xffffffffu;
+    double s0 = n0 > 0 ? cand_S[b0] : 0.0, s1 = n1 > 0 ? cand_S[b1] : 0.0;
     while (i0 < n0 || i1 < n1) {
-      if (i1 >= n1 || (i0 < n0 && cand_idx[b0 + i0] < cand_idx[b1 + i1]))
-        visit(b0 + i0++);
-      else
-        visit(b1 + i1++);
+      if (i1 >= n1 || (i0 < n0 && r0 < r1)) {
+        visit(s0, r0);
+        if (++i0 < n0) {
+          r0 = cand_idx[b0 + i0];
+          s0 = cand_S[b0 + i0];
+        }
+      } else {
+        visit(s1, r1);
+        if (++i1 < n1) {
+          r1 = cand_idx[b1 + i1];
+          s1 = cand_S[b1 + i1];
+        }
+      }
     }
   }
   atomicAdd(&counters[0], seen);
