@@ -895,9 +895,20 @@ __global__ __launch_bounds__(kRescoreThreads) void mixed_rescore_kernel(
     int i = 0;
     while (start[i + 1] <= e) ++i;
     const size_t slot = qbase + lay.offset(l_lo + i) + (e - start[i]);
-    const float* rrow = rows + static_cast<int64_t>(cand_idx[slot]) * ld;
+    // the row read 16 values at a time (ld is a multiple of 16, rows are
+    // 64-byte aligned), the terms added in dimension order
+    const f32x4* r4 = reinterpret_cast<const f32x4*>(rows + static_cast<int64_t>(cand_idx[slot]) * ld);
     double sum = 0.0;
-    for (int j = 0; j < ld; ++j) sum = sum + Term<float, kEuclid>::f(qrow[j], rrow[j], 2.0);
+    for (int j = 0; j < ld; j += 16) {
+      f32x4 v[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) v[i] = r4[j / 4 + i];
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int c = 0; c < 4; ++c)
+          sum = sum + Term<float, kEuclid>::f(qrow[j + 4 * i + c], v[i][c], 2.0);
+    }
     cand_S[slot] = sum;
   }
 }
