@@ -909,15 +909,44 @@ extern "C" int sc_kmeans_batch_run(const sc_kmeans_problem* problems, size_t cou
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, dev) == hipSuccess && prop.multiProcessorCount > 0)
     cus = prop.multiProcessorCount;
+  // One side stream per shape class (default priority), forked from s and
+  // joined back in every iteration, so the classes' launches run side by
+  // side; the reduce launches follow the join.  Inside the graph capture
+  // the fork and join become parallel branches.
+  struct Sides {
+    std::vector<hipStream_t> st;
+    std::vector<hipEvent_t> done;
+    hipEvent_t fork = nullptr;
+    ~Sides() {
+      for (hipStream_t x : st) (void)hipStreamDestroy(x);
+      for (hipEvent_t e : done) (void)hipEventDestroy(e);
+      if (fork) (void)hipEventDestroy(fork);
+    }
+  } sides;
+  SC_HIP(hipEventCreateWithFlags(&sides.fork, hipEventDisableTiming));
+  for (size_t i = 0; i < klass.size(); ++i) {
+    hipStream_t x = nullptr;
+    hipEvent_t e = nullptr;
+    SC_HIP(hipStreamCreateWithFlags(&x, hipStreamNonBlocking));
+    sides.st.push_back(x);
+    SC_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    sides.done.push_back(e);
+  }
   auto iteration = [&](int mult) -> int {
+    SC_HIP(hipEventRecord(sides.fork, s));
+    size_t i = 0;
     for (auto& kv : klass) {
       Klass& K = kv.second;
       const int64_t ni = static_cast<int64_t>(K.items.size());
       const unsigned grid = static_cast<unsigned>(std::min<int64_t>(ni, static_cast<int64_t>(mult) * cus));
+      SC_HIP(hipStreamWaitEvent(sides.st[i], sides.fork, 0));
       if (int rc = launch_step(K.kbm, K.ksm, dP, K.buf.as<uint2>(), ni, grid, K.exact, K.kmax,
-                               K.ldmax, s))
+                               K.ldmax, sides.st[i]))
         return rc;
+      SC_HIP(hipEventRecord(sides.done[i], sides.st[i]));
+      ++i;
     }
+    for (hipEvent_t e : sides.done) SC_HIP(hipStreamWaitEvent(s, e, 0));
     return kmeans_reduce_launch(dP, red, true, true, s);
   };
 
