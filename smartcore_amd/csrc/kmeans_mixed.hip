@@ -966,9 +966,16 @@ extern "C" int sc_kmeans_batch_run(const sc_kmeans_problem* problems, size_t cou
   if (mult == 0) {
     const int cand[3] = {1, 2, 4};
     float best = 1e30f;
-    hipEvent_t e0, e1;
-    SC_HIP(hipEventCreate(&e0));
-    SC_HIP(hipEventCreate(&e1));
+    struct TimerPair {
+      hipEvent_t a = nullptr, b = nullptr;
+      ~TimerPair() {
+        if (a) (void)hipEventDestroy(a);
+        if (b) (void)hipEventDestroy(b);
+      }
+    } tp;
+    SC_HIP(hipEventCreate(&tp.a));
+    SC_HIP(hipEventCreate(&tp.b));
+    hipEvent_t e0 = tp.a, e1 = tp.b;
     for (int c = 0; c < 3 && it_done < max_iter; ++c) {
       SC_HIP(hipEventRecord(e0, s));
       if (int rc = iteration(cand[c])) return rc;
@@ -982,8 +989,6 @@ extern "C" int sc_kmeans_batch_run(const sc_kmeans_problem* problems, size_t cou
         mult = cand[c];
       }
     }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
     std::lock_guard<std::mutex> lk(g_tune_mu);
     g_tune.push_back(TuneEntry{key, mult});
   }
