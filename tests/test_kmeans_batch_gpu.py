@@ -55,3 +55,20 @@ def test_batch_errors():
         S.kmeans_batch_run([(x, np.ones((1, 4)))], 5)
     with pytest.raises(S.Failed):
         S.kmeans_batch_run([(x, np.ones((2, 4)))], 0)
+
+
+def test_batch_with_a_problem_outside_the_shape_classes():
+    # d = k = 100 needs more LDS than a workgroup has (three 128 x 136 BF16
+    # tables and a 128-row stage): the batch runs its problems one by one,
+    # each equal to its single run
+    probs = synth.kmeans_batch_problems(seed=53, count=3, n_lo=300, n_hi=2_000)
+    probs += synth.kmeans_batch_problems(seed=54, count=1, n_lo=1_500, n_hi=1_500, dims=(100,),
+                                         k_lo=100, k_hi=100)
+    got = S.kmeans_batch_run(probs, 4, S.SC_MIXED)
+    for (x, init), (c, lab, size, dist, it) in zip(probs, got):
+        c1, l1, s1, d1, i1 = single(x, init, 4, S.SC_MIXED)
+        np.testing.assert_array_equal(lab, l1)
+        np.testing.assert_array_equal(c.view(np.uint64), c1.view(np.uint64))
+        assert it == i1
+        c2, l2, s2, d2, i2 = single(x, init, 4, S.SC_PLAIN_FP64)
+        np.testing.assert_array_equal(lab, l2)
