@@ -4,7 +4,7 @@
  * search and the k-means Lloyd step.
  *
  * This is the boundary a smartcore build would bind through `extern "C"`
- * declarations (the Rust stub is written out in INTEGRATION.md).  Every
+ * declarations in a Rust stub.  Every
  * entry point names the reference interface it replaces (paths relative to
  * the smartcore v0.4.0 crate root).  Pointers are plain host pointers unless
  * the entry point's name ends in `_device`, in which case they are device

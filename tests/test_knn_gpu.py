@@ -86,7 +86,9 @@ def test_tie_heavy_integer_data_keeps_heap_order():
         assert_exact(*run_both(ref, qs, k, "manhattan"))
 
 
-def test_tie_counterexample_from_survey():
+def test_tie_counterexample():
+    # ten 1-d rows with repeated values: the k = 5 nearest to 0 under
+    # Manhattan include a tie at distance 3, which the heap order decides
     ref = np.array([3, 0, 3, 0, 2, 4, 4, 4, 3, 1], np.float64)[:, None]
     ix = S.Index(ref)
     gi, gd, gc = ix.knn(np.zeros((1, 1)), 5, S.SC_MANHATTAN)
