@@ -639,7 +639,7 @@ __global__ __launch_bounds__(kNT, 2) void fixed_scan_kernel(const ScanArgs a) {
   const int lcap = a.lay.cap(lseg);
 
   bf16x8 bq[2][KS + 1];
-  f32x2 thr, x2v, kfix;
+  f32x2 thr;
   i32x2 qvalid;
 #pragma unroll
   for (int cb = 0; cb < 2; ++cb) {
@@ -649,11 +649,10 @@ __global__ __launch_bounds__(kNT, 2) void fixed_scan_kernel(const ScanArgs a) {
 #pragma unroll
     for (int ks = 0; ks < KS; ++ks)
       bq[cb][ks] = load_frag(a.q_b + qr * S::LD + 16 * ks + 8 * half);
-    x2v[cb] = qvalid[cb] ? a.x2m[qr] : INFINITY;
     // the seed's k-th value T; the epilogue threshold is T - x2m rounded up
-    kfix[cb] = qvalid[cb] ? a.seed[qr * k + k - 1] : INFINITY;
+    const float x2q = a.x2m[qr], tq = a.seed[qr * k + k - 1];
     thr[cb] = qvalid[cb]
-                  ? __double2float_ru(static_cast<double>(kfix[cb]) - static_cast<double>(x2v[cb]))
+                  ? __double2float_ru(static_cast<double>(tq) - static_cast<double>(x2q))
                   : INFINITY;
     const uint16_t b1 = bf16_ru_pos(a.qa1[qr]), b2 = bf16_ru_pos(a.qa2[qr]);
     const uint32_t one = 0x3f80u, n1 = 0x8000u | b1, n2 = 0x8000u | b2;
@@ -738,11 +737,12 @@ __global__ __launch_bounds__(kNT, 2) void fixed_scan_kernel(const ScanArgs a) {
 #pragma unroll
       for (int v = 0; v < 16; ++v) {
         const float val = acc[cb][v];
+        // val < thr (T - x2m rounded up) keeps every row with L < T; a row
+        // that only the rounding lets in is a harmless extra candidate
         if (qvalid[cb] != 0 && val < thr[cb]) {
           const int r = 8 * (v >> 2) + 4 * half + (v & 3);
           const int64_t row = t0 + r;
-          const double L = static_cast<double>(x2v[cb]) + static_cast<double>(val);
-          if (row < r_end && L < static_cast<double>(kfix[cb])) {
+          if (row < r_end) {
             if (cntf[cb] < static_cast<uint32_t>(lcap)) {
               const size_t e = static_cast<size_t>(qi) * perq + loff + cntf[cb];
               a.cand_idx[e] = static_cast<uint32_t>(row);
