@@ -84,13 +84,16 @@ def test_mixed_rejects_other_metrics():
         ix.knn(np.ones((1, 3), np.float32), 2, S.SC_MANHATTAN, 1, S.SC_MIXED)
 
 
-@pytest.mark.parametrize("d,k,kind", [(16, 10, "lognormal"), (48, 32, "ties"), (16, 1, "lognormal")])
+@pytest.mark.parametrize("d,k,kind", [(16, 10, "lognormal"), (48, 32, "ties"), (16, 1, "lognormal"),
+                                      (128, 10, "lognormal"), (128, 1, "lognormal"), (128, 10, "ties")])
 def test_mixed_seeded_large_index(d, k, kind):
     """Indexes of 2,097,152 rows or more scan a 65,536-row prefix first, then
     row ranges growing 8 times, each tested against the k-th exact distance
     of the earlier candidates.  The result must still equal the FP64 form bit
     for bit (and the oracle on a few queries), also with heavy exact ties and
-    with a single neighbour."""
+    with a single neighbour.  The d = 128 cases run the scan the 4,000,000 x 128
+    workload times (eight K steps plus the folded bound step) on both the
+    middle and the last level."""
     n, nq = 2_200_000, 3_000
     if kind == "lognormal":
         ref = synth.lognormal_rows(90 + d, n, d)
