@@ -76,6 +76,17 @@ def lib():
         L.or_lloyd_step_brute.restype = dbl
         L.or_lloyd_step_brute.argtypes = [vp, i32, sz, sz, vp, sz, vp, vp, vp]
         L.or_kmeans_predict.argtypes = [vp, i32, sz, sz, vp, sz, vp]
+        u64 = C.c_uint64
+        L.or_pcg32_output.restype = C.c_uint32
+        L.or_pcg32_output.argtypes = [u64]
+        L.or_smallrng_seed.argtypes = [u64, vp]
+        L.or_smallrng_next.restype = u64
+        L.or_smallrng_next.argtypes = [vp]
+        L.or_smallrng_gen_range.restype = u64
+        L.or_smallrng_gen_range.argtypes = [vp, u64]
+        L.or_smallrng_gen_f64.restype = dbl
+        L.or_smallrng_gen_f64.argtypes = [vp]
+        L.or_kmeans_plus_plus.argtypes = [vp, i32, sz, sz, sz, i32, u64, vp]
         L.or_dbscan_fit.restype = i32
         L.or_dbscan_fit.argtypes = [vp, i32, sz, sz, dbl, sz, i32, u32, vp,
                                     C.POINTER(C.c_size_t)]
@@ -287,6 +298,42 @@ def kmeans_fit(data: np.ndarray, init_centroids: np.ndarray, max_iter: int, use_
     if rc:
         raise OracleError(rc)
     return cent, labels, size, dist.value, iters.value
+
+
+class SmallRng:
+    """rand 0.8.5's SmallRng (xoshiro256++) as SmallRng::seed_from_u64 seeds
+    it (rand_core 0.6's PCG32 seed fill); `state` sets the four words
+    directly, as xoshiro256++'s from_seed does."""
+
+    def __init__(self, seed: int | None = None, state=None):
+        self.s = np.zeros(4, np.uint64)
+        if state is not None:
+            self.s[:] = np.asarray(state, dtype=np.uint64)
+        else:
+            lib().or_smallrng_seed(int(seed or 0), _p(self.s))
+
+    def next_u64(self) -> int:
+        return int(lib().or_smallrng_next(_p(self.s)))
+
+    def gen_range(self, n: int) -> int:
+        return int(lib().or_smallrng_gen_range(_p(self.s), n))
+
+    def gen_f64(self) -> float:
+        return float(lib().or_smallrng_gen_f64(_p(self.s)))
+
+
+def pcg32_output(state: int) -> int:
+    return int(lib().or_pcg32_output(state))
+
+
+def kmeans_plus_plus(data: np.ndarray, k: int, seed: int | None = None) -> np.ndarray:
+    """KMeans::kmeans_plus_plus (kmeans.rs:354-413): initial labels."""
+    data = np.ascontiguousarray(data)
+    n, d = data.shape
+    y = np.zeros(max(n, 1), np.uint32)
+    lib().or_kmeans_plus_plus(_p(data), _dt(data), n, d, k, int(seed is not None), int(seed or 0),
+                              _p(y))
+    return y[:n]
 
 
 def kmeans_predict(data: np.ndarray, centroids: np.ndarray) -> np.ndarray:

@@ -13,6 +13,7 @@
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
+#include <float.h>
 
 /* ------------------------------------------------------------------ */
 /* distances                                                            */
@@ -593,6 +594,152 @@ static uint32_t assign_row(const void* data, int dtype, size_t d, size_t i,
   }
   *best_dist = min_dist;
   return best;
+}
+
+/* ------------------------------------------------------------------ */
+/* random stream of KMeans::kmeans_plus_plus                            */
+/* ------------------------------------------------------------------ */
+/* get_rng_impl (rand_custom.rs:8-33, default features) returns rand 0.8.5's
+ * SmallRng::seed_from_u64(seed), seed 0 for None.  rand 0.8.5's SmallRng
+ * (64-bit targets: xoshiro256++) implements only from_seed / from_rng, so
+ * seed_from_u64 is rand_core 0.6's trait default: eight PCG32 (XSH-RR)
+ * outputs, each taken after advancing the LCG state, fill the 32-byte seed
+ * little-endian; from_seed reads four little-endian u64 words, and an
+ * all-zero seed is replaced by xoshiro's own seed_from_u64(0) (SplitMix64).
+ * The rand crates are not vendored in the reference tree; this follows their
+ * published source. */
+
+/* PCG XSH-RR output of a 64-bit LCG state. */
+uint32_t or_pcg32_output(uint64_t state) {
+  uint32_t xorshifted = (uint32_t)(((state >> 18) ^ state) >> 27);
+  uint32_t rot = (uint32_t)(state >> 59);
+  return (xorshifted >> rot) | (xorshifted << ((32u - rot) & 31u));
+}
+
+static uint64_t rotl64(uint64_t x, int k) { return (x << k) | (x >> (64 - k)); }
+
+static void xoshiro_from_splitmix(uint64_t state, uint64_t s[4]) {
+  for (int i = 0; i < 4; ++i) {
+    state += 0x9e3779b97f4a7c15ULL;
+    uint64_t z = state;
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+    s[i] = z ^ (z >> 31);
+  }
+}
+
+void or_smallrng_seed(uint64_t seed, uint64_t s[4]) {
+  const uint64_t MUL = 6364136223846793005ULL;
+  const uint64_t INC = 11634580027462260723ULL;
+  uint8_t bytes[32];
+  uint64_t st = seed;
+  for (int c = 0; c < 8; ++c) {
+    st = st * MUL + INC;
+    uint32_t x = or_pcg32_output(st);
+    for (int b = 0; b < 4; ++b) bytes[4 * c + b] = (uint8_t)(x >> (8 * b));
+  }
+  int zero = 1;
+  for (int i = 0; i < 32; ++i) zero &= bytes[i] == 0;
+  if (zero) {
+    xoshiro_from_splitmix(0, s);
+    return;
+  }
+  for (int w = 0; w < 4; ++w) {
+    uint64_t v = 0;
+    for (int b = 7; b >= 0; --b) v = (v << 8) | bytes[8 * w + b];
+    s[w] = v;
+  }
+}
+
+/* xoshiro256++ next_u64 */
+uint64_t or_smallrng_next(uint64_t s[4]) {
+  const uint64_t result = rotl64(s[0] + s[3], 23) + s[0];
+  const uint64_t t = s[1] << 17;
+  s[2] ^= s[0];
+  s[3] ^= s[1];
+  s[1] ^= s[2];
+  s[0] ^= s[3];
+  s[2] ^= t;
+  s[3] = rotl64(s[3], 45);
+  return result;
+}
+
+/* Rng::gen_range(0..n) for usize (rand 0.8.5 UniformInt::sample_single ->
+ * sample_single_inclusive(0, n-1)): range = n; zone = (range << lz(range))
+ * - 1; draw v until the low word of v * range is <= zone, return the high
+ * word. */
+uint64_t or_smallrng_gen_range(uint64_t s[4], uint64_t n) {
+  const uint64_t range = n;
+  if (range == 0) return or_smallrng_next(s);
+  const uint64_t zone = (range << __builtin_clzll(range)) - 1;
+  for (;;) {
+    const uint64_t v = or_smallrng_next(s);
+    const unsigned __int128 m = (unsigned __int128)v * range;
+    if ((uint64_t)m <= zone) return (uint64_t)(m >> 64);
+  }
+}
+
+/* Standard for f64: (next_u64 >> 11) * 2^-53 */
+double or_smallrng_gen_f64(uint64_t s[4]) {
+  const double scale = 1.0 / (double)((uint64_t)1 << 53);
+  return scale * (double)(or_smallrng_next(s) >> 11);
+}
+
+/* KMeans::kmeans_plus_plus, kmeans.rs:354-413: labels of the rows. */
+void or_kmeans_plus_plus(const void* data, int dtype, size_t n, size_t d, size_t k,
+                         int has_seed, uint64_t seed, uint32_t* y) {
+  uint64_t s[4];
+  or_smallrng_seed(has_seed ? seed : 0, s);
+  const size_t es = dtype == OR_F64 ? 8 : 4;
+  const char* base = (const char*)data;
+  double* dm = (double*)malloc((n ? n : 1) * sizeof(double));
+  for (size_t i = 0; i < n; ++i) {
+    y[i] = 0;
+    dm[i] = DBL_MAX;
+  }
+  /* :358-362 */
+  size_t c = (size_t)or_smallrng_gen_range(s, n);
+  for (size_t j = 1; j < k; ++j) {
+    /* :367-379: squared distance in the rows' own type */
+    for (size_t i = 0; i < n; ++i) {
+      double dist = dtype == OR_F64
+                        ? or_squared_distance_f64((const double*)(base + i * d * es),
+                                                  (const double*)(base + c * d * es), d)
+                        : or_squared_distance_f32((const float*)(base + i * d * es),
+                                                  (const float*)(base + c * d * es), d);
+      if (dist < dm[i]) {
+        dm[i] = dist;
+        y[i] = (uint32_t)(j - 1);
+      }
+    }
+    /* :381-394 */
+    double sum = 0.0;
+    for (size_t i = 0; i < n; ++i) sum += dm[i];
+    const double cutoff = or_smallrng_gen_f64(s) * sum;
+    double cost = 0.0;
+    size_t index = 0;
+    while (index < n) {
+      cost += dm[index];
+      if (cost >= cutoff) break;
+      index += 1;
+    }
+    /* the reference indexes row `index` (a panic when index == n, reachable
+     * only with a NaN sum); stay on the last row */
+    c = index < n ? index : n - 1;
+  }
+  /* :399-410 */
+  for (size_t i = 0; i < n; ++i) {
+    double dist = dtype == OR_F64
+                      ? or_squared_distance_f64((const double*)(base + i * d * es),
+                                                (const double*)(base + c * d * es), d)
+                      : or_squared_distance_f32((const float*)(base + i * d * es),
+                                                (const float*)(base + c * d * es), d);
+    if (dist < dm[i]) {
+      dm[i] = dist;
+      y[i] = (uint32_t)(k - 1);
+    }
+  }
+  free(dm);
 }
 
 void or_kmeans_predict(const void* data, int dtype, size_t n, size_t d,

@@ -129,6 +129,15 @@ int or_kmeans_fit(const void* data, int dtype, size_t n, size_t d, size_t k,
 double or_lloyd_step_brute(const void* data, int dtype, size_t n, size_t d,
                            const double* centroids, size_t k, double* sums,
                            uint64_t* counts, uint32_t* membership);
+/* rand 0.8.5 SmallRng as get_rng_impl seeds it (rand_custom.rs:8-33) and
+ * KMeans::kmeans_plus_plus (kmeans.rs:354-413). */
+uint32_t or_pcg32_output(uint64_t state);
+void or_smallrng_seed(uint64_t seed, uint64_t s[4]);
+uint64_t or_smallrng_next(uint64_t s[4]);
+uint64_t or_smallrng_gen_range(uint64_t s[4], uint64_t n);
+double or_smallrng_gen_f64(uint64_t s[4]);
+void or_kmeans_plus_plus(const void* data, int dtype, size_t n, size_t d, size_t k,
+                         int has_seed, uint64_t seed, uint32_t* y);
 void or_kmeans_predict(const void* data, int dtype, size_t n, size_t d,
                        const double* centroids, size_t k, uint32_t* labels);
 

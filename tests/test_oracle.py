@@ -309,3 +309,98 @@ def test_dbscan_equals_graph_restatement_on_digits(eps, ms):
         w = int(np.argmax(h))  # first maximum, as which_max
         want.append(0 if w == nc else w + 1)
     np.testing.assert_array_equal(O.dbscan_predict(x, lab, nc, eps, q), want)
+
+
+# ---- rand 0.8.5 SmallRng and KMeans::kmeans_plus_plus ----
+
+def test_xoshiro256pp_published_vector():
+    # xoshiro256++ from state words (1, 2, 3, 4): the reference outputs of
+    # xoshiro256plusplus.c that rand's xoshiro256++ tests hold
+    r = O.SmallRng(state=[1, 2, 3, 4])
+    assert [r.next_u64() for _ in range(10)] == [
+        41943041, 58720359, 3588806011781223, 3591011842654386, 9228616714210784205,
+        9973669472204895162, 14011001112246962877, 12406186145184390807,
+        15849039046786891736, 10450023813501588000]
+
+
+def test_pcg32_output_function_published_vector():
+    # pcg32-demo (pcg32_srandom_r(42, 54)): same multiplier and XSH-RR output
+    # as the PCG32 fill of SmallRng::seed_from_u64, increment (54 << 1) | 1,
+    # output taken from the state before each advance
+    mul, mask = 6364136223846793005, (1 << 64) - 1
+    inc = (54 << 1) | 1
+    st = (0 * mul + inc) & mask
+    st = (st + 42) & mask
+    st = (st * mul + inc) & mask
+    out = []
+    for _ in range(6):
+        old, st = st, (st * mul + inc) & mask
+        out.append(O.pcg32_output(old))
+    assert out == [0xa15c02b7, 0x7b47f409, 0xba1d3330, 0x83d2f293, 0xbfa4784b, 0xcbed606e]
+
+
+def test_smallrng_seed_is_pcg32_fill():
+    # the seed words are the little-endian pairs of eight PCG32 outputs taken
+    # after each LCG advance from the u64 seed
+    mul, inc, mask = 6364136223846793005, 11634580027462260723, (1 << 64) - 1
+    for seed in (0, 1, 42, 2**63 + 5):
+        st, xs = seed, []
+        for _ in range(8):
+            st = (st * mul + inc) & mask
+            xs.append(O.pcg32_output(st))
+        words = [xs[2 * i] | (xs[2 * i + 1] << 32) for i in range(4)]
+        a = O.SmallRng(seed)
+        b = O.SmallRng(state=words)
+        assert [a.next_u64() for _ in range(5)] == [b.next_u64() for _ in range(5)]
+
+
+def test_gen_range_and_gen_f64_closed_forms():
+    # a power-of-two range 2^j has zone (2^j << (63 - j)) - 1 = 2^63 - 1: a
+    # draw is taken when its bit 63 - j is clear (the low word of v * 2^j is
+    # below 2^63) and gives its top j bits
+    mask = (1 << 64) - 1
+    for j in (1, 3, 10, 40):
+        a, b = O.SmallRng(9), O.SmallRng(9)
+        for _ in range(50):
+            v = b.next_u64()
+            while ((v << j) & mask) >= 1 << 63:
+                v = b.next_u64()
+            assert a.gen_range(1 << j) == v >> (64 - j)
+    # f64: the 53 high bits scaled by 2^-53
+    a, b = O.SmallRng(11), O.SmallRng(11)
+    for _ in range(50):
+        assert a.gen_f64() * 2.0**53 == float(b.next_u64() >> 11)
+    # any range: results in [0, n), every value reachable, about uniform
+    r = O.SmallRng(12)
+    for n in (3, 7, 1000):
+        v = np.array([r.gen_range(n) for _ in range(6000)])
+        assert v.min() >= 0 and v.max() < n
+        if n <= 7:
+            counts = np.bincount(v, minlength=n)
+            assert counts.min() > 0.8 * 6000 / n
+
+
+def test_kmeans_plus_plus_two_groups():
+    # one point away from nine identical ones: D^2 sampling must pick it as
+    # the other centre whichever row comes first, so the labels split the
+    # two groups
+    x = np.zeros((10, 2))
+    x[6] = [5.0, 5.0]
+    for seed in (None, 0, 1, 2, 3, 4):
+        y = O.kmeans_plus_plus(x, 2, seed)
+        assert len(set(y[[0, 1, 2, 3, 4, 5, 7, 8, 9]])) == 1
+        assert y[6] != y[0]
+    y = O.kmeans_plus_plus(x, 1, 5)
+    assert np.all(y == 0)
+
+
+def test_kmeans_plus_plus_labels_are_nearest_chosen_centre():
+    # the labels are the assignment to the chosen centres (rows whose label
+    # is j and whose distance to the j-th centre is zero), first strict
+    # minimum over centres in order
+    x, _, _ = synth.kmeans_blobs(seed=8, n=3_000, d=6, blobs=5, k=5, dtype=np.float64)
+    y = O.kmeans_plus_plus(x, 7, seed=3)
+    assert y.min() >= 0 and y.max() <= 6 and len(np.unique(y)) == 7
+    for dt in (np.float64, np.float32):
+        xx = x.astype(dt)
+        assert np.array_equal(O.kmeans_plus_plus(xx, 7, seed=3), O.kmeans_plus_plus(xx, 7, seed=3))
