@@ -13,6 +13,7 @@
 #include <cmath>
 #include <cstdint>
 #include <memory>
+#include <mutex>
 #include <optional>
 #include <stdexcept>
 #include <string>
@@ -96,6 +97,90 @@ constexpr sc_dtype dtype_of() {
   static_assert(std::is_same<T, double>::value || std::is_same<T, float>::value,
                 "device path supports f64 and f32 rows");
   return std::is_same<T, double>::value ? SC_F64 : SC_F32;
+}
+
+// Little-endian byte stream of the estimators' save / load (the analogue of
+// the reference's serde derives, knn_classifier.rs:72-73, kmeans.rs:71-72).
+// Layout: "SCAMD" | version u8 | kind u8 | the estimator's fields in
+// declaration order; vectors are a u64 length and their elements.
+class ByteWriter {
+ public:
+  void u8(uint8_t v) { b_.push_back(v); }
+  void u16(uint16_t v) { raw(&v, 2); }
+  void u32(uint32_t v) { raw(&v, 4); }
+  void u64(uint64_t v) { raw(&v, 8); }
+  void f64(double v) { raw(&v, 8); }
+  void i64(int64_t v) { raw(&v, 8); }
+  template <typename T>
+  void num(T v) {
+    raw(&v, sizeof(T));
+  }
+  template <typename T>
+  void vec(const std::vector<T>& v) {
+    u64(v.size());
+    for (const T& x : v) num<T>(x);
+  }
+  void header(uint8_t kind) {
+    for (char c : std::string("SCAMD")) u8(static_cast<uint8_t>(c));
+    u8(1);
+    u8(kind);
+  }
+  std::vector<uint8_t> take() { return std::move(b_); }
+
+ private:
+  // the hosts this library runs on are little-endian
+  void raw(const void* p, size_t n) {
+    const uint8_t* c = static_cast<const uint8_t*>(p);
+    b_.insert(b_.end(), c, c + n);
+  }
+  std::vector<uint8_t> b_;
+};
+
+class ByteReader {
+ public:
+  explicit ByteReader(const std::vector<uint8_t>& b) : b_(b) {}
+  uint8_t u8() { uint8_t v; raw(&v, 1); return v; }
+  uint16_t u16() { uint16_t v; raw(&v, 2); return v; }
+  uint32_t u32() { uint32_t v; raw(&v, 4); return v; }
+  uint64_t u64() { uint64_t v; raw(&v, 8); return v; }
+  double f64() { double v; raw(&v, 8); return v; }
+  template <typename T>
+  T num() {
+    T v;
+    raw(&v, sizeof(T));
+    return v;
+  }
+  template <typename T>
+  std::vector<T> vec() {
+    const uint64_t n = u64();
+    if (n > (b_.size() - at_) / std::max<size_t>(sizeof(T), 1)) throw_short();
+    std::vector<T> v(static_cast<size_t>(n));
+    for (auto& x : v) x = num<T>();
+    return v;
+  }
+  void header(uint8_t kind);
+  bool done() const { return at_ == b_.size(); }
+
+ private:
+  [[noreturn]] static void throw_short();
+  void raw(void* p, size_t n) {
+    if (n > b_.size() - at_) throw_short();
+    std::copy(b_.begin() + at_, b_.begin() + at_ + n, static_cast<uint8_t*>(p));
+    at_ += n;
+  }
+  const std::vector<uint8_t>& b_;
+  size_t at_ = 0;
+};
+}  // namespace detail
+
+namespace detail {
+inline void ByteReader::throw_short() { throw Failed::input("serialized model is truncated"); }
+inline void ByteReader::header(uint8_t kind) {
+  std::string magic;
+  for (int i = 0; i < 5; ++i) magic.push_back(static_cast<char>(u8()));
+  if (magic != "SCAMD") throw Failed::input("not a serialized smartcore_amd model");
+  if (u8() != 1) throw Failed::input("unsupported serialized model version");
+  if (u8() != kind) throw Failed::input("serialized model is of another estimator");
 }
 }  // namespace detail
 
@@ -204,6 +289,22 @@ struct Minkowski {
   std::optional<DeviceMetric> device_metric() const { return DeviceMetric{SC_MINKOWSKI, p}; }
 };
 
+// Rebuilds a distance from what device_metric() reports (model load).
+template <typename D>
+struct DistanceFrom;
+template <typename T>
+struct DistanceFrom<Euclidian<T>> {
+  static Euclidian<T> make(uint16_t) { return {}; }
+};
+template <typename T>
+struct DistanceFrom<Manhattan<T>> {
+  static Manhattan<T> make(uint16_t) { return {}; }
+};
+template <typename T>
+struct DistanceFrom<Minkowski<T>> {
+  static Minkowski<T> make(uint16_t p) { return Minkowski<T>(p); }
+};
+
 // metrics/distance/mod.rs:46-61
 struct Distances {
   template <typename T>
@@ -230,22 +331,17 @@ enum class SearchMode { PlainFp64 = SC_PLAIN_FP64, Mixed = SC_MIXED };
 template <typename T, typename D>
 class LinearKNNSearch {
  public:
-  // linear_search.rs:45-47 (row-major rows, one vector per row)
+  // linear_search.rs:45-47 (row-major rows, one vector per row).  The device
+  // copy of the rows is made on the first search, so a model that was fitted
+  // or loaded but never searched holds no device memory.
   static LinearKNNSearch create(std::vector<std::vector<T>> data, D distance) {
     return LinearKNNSearch(std::move(data), std::move(distance));
   }
   LinearKNNSearch(std::vector<std::vector<T>> data, D distance)
-      : data_(std::move(data)), distance_(std::move(distance)) {
-    const size_t d = data_.empty() ? 0 : data_[0].size();
-    std::vector<T> packed(data_.size() * d);
-    for (size_t i = 0; i < data_.size(); ++i) {
-      if (data_[i].size() != d) throw Panic("Input vector sizes are different.");
-      std::copy(data_[i].begin(), data_[i].end(), packed.begin() + i * d);
-    }
-    sc_index* ix = nullptr;
-    detail::check(sc_index_create(packed.data(), detail::dtype_of<T>(), data_.size(), d, &ix));
-    index_.reset(ix, [](sc_index* p) { sc_index_destroy(p); });
-    dim_ = d;
+      : data_(std::move(data)), distance_(std::move(distance)), dev_(std::make_shared<Device>()) {
+    dim_ = data_.empty() ? 0 : data_[0].size();
+    for (const auto& r : data_)
+      if (r.size() != dim_) throw Panic("Input vector sizes are different.");
   }
 
   using Neighbour = std::tuple<size_t, double, const std::vector<T>*>;
@@ -268,7 +364,7 @@ class LinearKNNSearch {
     std::vector<int64_t> idx(from.size() * std::max<size_t>(k, 1));
     std::vector<double> dist(idx.size());
     std::vector<uint32_t> cnt(std::max<size_t>(from.size(), 1));
-    detail::check(sc_knn_search(index_.get(), q.data(), from.size(), k, dm->metric, dm->p,
+    detail::check(sc_knn_search(index(), q.data(), from.size(), k, dm->metric, dm->p,
                                 static_cast<sc_mode>(mode), idx.data(), dist.data(), cnt.data()));
     std::vector<std::vector<Neighbour>> out(from.size());
     for (size_t i = 0; i < from.size(); ++i) {
@@ -287,7 +383,7 @@ class LinearKNNSearch {
     if (!dm) throw Failed::because(FailedError::FindFailed, "metric has no device kernel");
     if (from.size() != dim_) throw Panic("Input vector sizes are different.");
     sc_radius_result* r = nullptr;
-    detail::check(sc_radius_search(index_.get(), from.data(), 1, radius, dm->metric, dm->p, &r));
+    detail::check(sc_radius_search(index(), from.data(), 1, radius, dm->metric, dm->p, &r));
     const size_t total = sc_radius_result_total(r);
     std::vector<uint64_t> off(2);
     std::vector<int64_t> idx(std::max<size_t>(total, 1));
@@ -301,12 +397,67 @@ class LinearKNNSearch {
   }
 
   size_t len() const { return data_.size(); }
-  const sc_index* index() const { return index_.get(); }
+  // The device-resident rows (uploaded on first use).
+  const sc_index* index() const {
+    std::lock_guard<std::mutex> lk(dev_->mu);
+    if (!dev_->ix) {
+      std::vector<T> packed(data_.size() * dim_);
+      for (size_t i = 0; i < data_.size(); ++i)
+        std::copy(data_[i].begin(), data_[i].end(), packed.begin() + i * dim_);
+      sc_index* ix = nullptr;
+      detail::check(sc_index_create(packed.data(), detail::dtype_of<T>(), data_.size(), dim_, &ix));
+      dev_->ix = ix;
+    }
+    return dev_->ix;
+  }
+  bool on_device() const {
+    std::lock_guard<std::mutex> lk(dev_->mu);
+    return dev_->ix != nullptr;
+  }
+
+  // The rows and the distance (serde's view of LinearKNNSearch,
+  // linear_search.rs:34-39); the device copy is not part of it.
+  void save(detail::ByteWriter& w) const {
+    const auto dm = distance_.device_metric();
+    w.u8(static_cast<uint8_t>(detail::dtype_of<T>()));
+    w.u8(static_cast<uint8_t>(dm ? dm->metric : SC_EUCLIDIAN));
+    w.u16(dm ? dm->p : 2);
+    w.u64(data_.size());
+    w.u64(dim_);
+    for (const auto& r : data_)
+      for (const T& v : r) w.num<T>(v);
+  }
+  static LinearKNNSearch load(detail::ByteReader& r) {
+    if (r.u8() != static_cast<uint8_t>(detail::dtype_of<T>()))
+      throw Failed::input("serialized rows are of another type");
+    const auto metric = static_cast<sc_metric>(r.u8());
+    const uint16_t p = r.u16();
+    D dist = DistanceFrom<D>::make(p);
+    const auto dm = dist.device_metric();
+    if (!dm || dm->metric != metric) throw Failed::input("serialized model uses another distance");
+    const uint64_t n = r.u64(), d = r.u64();
+    std::vector<std::vector<T>> rows(static_cast<size_t>(n), std::vector<T>(static_cast<size_t>(d)));
+    for (auto& row : rows)
+      for (T& v : row) v = r.num<T>();
+    return LinearKNNSearch(std::move(rows), std::move(dist));
+  }
+  bool operator==(const LinearKNNSearch& o) const {
+    const auto a = distance_.device_metric(), b = o.distance_.device_metric();
+    return data_ == o.data_ && a.has_value() == b.has_value() &&
+           (!a || (a->metric == b->metric && a->p == b->p));
+  }
 
  private:
+  struct Device {
+    std::mutex mu;
+    sc_index* ix = nullptr;
+    ~Device() {
+      if (ix) sc_index_destroy(ix);
+    }
+  };
   std::vector<std::vector<T>> data_;
   D distance_;
-  std::shared_ptr<sc_index> index_;
+  std::shared_ptr<Device> dev_;
   size_t dim_ = 0;
 };
 
@@ -399,6 +550,41 @@ class KNNClassifier {
   }
 
   const std::vector<TY>& classes() const { return classes_; }
+  // Whether the rows have been uploaded (by a predict since fit or load).
+  bool on_device() const { return search_->on_device(); }
+
+  // Save / load (serde round trip, knn_classifier.rs:352-363): the classes,
+  // the label map, k, the weight, the mode, the rows and the distance.  A
+  // loaded model uploads its rows on its first predict.
+  std::vector<uint8_t> serialize() const {
+    detail::ByteWriter w;
+    w.header(1);
+    w.vec(classes_);
+    w.vec(std::vector<uint64_t>(y_.begin(), y_.end()));
+    w.u64(k_);
+    w.u8(static_cast<uint8_t>(weight_));
+    w.u8(static_cast<uint8_t>(mode_));
+    search_->save(w);
+    return w.take();
+  }
+  static KNNClassifier deserialize(const std::vector<uint8_t>& bytes) {
+    detail::ByteReader r(bytes);
+    r.header(1);
+    KNNClassifier m;
+    m.classes_ = r.vec<TY>();
+    const auto y64 = r.vec<uint64_t>();
+    m.y_.assign(y64.begin(), y64.end());
+    m.k_ = static_cast<size_t>(r.u64());
+    m.weight_ = static_cast<KNNWeightFunction>(r.u8());
+    m.mode_ = static_cast<SearchMode>(r.u8());
+    m.search_ = std::make_shared<LinearKNNSearch<TX, D>>(LinearKNNSearch<TX, D>::load(r));
+    if (!r.done()) throw Failed::input("trailing bytes after the serialized model");
+    return m;
+  }
+  bool operator==(const KNNClassifier& o) const {
+    return classes_ == o.classes_ && y_ == o.y_ && k_ == o.k_ && weight_ == o.weight_ &&
+           mode_ == o.mode_ && *search_ == *o.search_;
+  }
 
  private:
   // knn_classifier.rs:279-299
@@ -475,6 +661,35 @@ class KNNRegressor {
       result[i] = acc;
     }
     return result;
+  }
+
+  bool on_device() const { return search_->on_device(); }
+  // Save / load, as KNNClassifier's.
+  std::vector<uint8_t> serialize() const {
+    detail::ByteWriter w;
+    w.header(2);
+    w.vec(y_);
+    w.u64(k_);
+    w.u8(static_cast<uint8_t>(weight_));
+    w.u8(static_cast<uint8_t>(mode_));
+    search_->save(w);
+    return w.take();
+  }
+  static KNNRegressor deserialize(const std::vector<uint8_t>& bytes) {
+    detail::ByteReader r(bytes);
+    r.header(2);
+    KNNRegressor m;
+    m.y_ = r.vec<TY>();
+    m.k_ = static_cast<size_t>(r.u64());
+    m.weight_ = static_cast<KNNWeightFunction>(r.u8());
+    m.mode_ = static_cast<SearchMode>(r.u8());
+    m.search_ = std::make_shared<LinearKNNSearch<TX, D>>(LinearKNNSearch<TX, D>::load(r));
+    if (!r.done()) throw Failed::input("trailing bytes after the serialized model");
+    return m;
+  }
+  bool operator==(const KNNRegressor& o) const {
+    return y_ == o.y_ && k_ == o.k_ && weight_ == o.weight_ && mode_ == o.mode_ &&
+           *search_ == *o.search_;
   }
 
  private:
@@ -594,6 +809,44 @@ class KMeans {
   const std::vector<double>& centroids() const { return centroids_; }
   double distortion() const { return distortion_; }
   uint32_t iterations() const { return iters_; }
+
+  // Save / load (serde round trip, kmeans.rs:512-545): k, the labels, the
+  // sizes, the distortion, the iteration count and the centroids.  KMeans
+  // keeps nothing on the device between calls (predict uploads its rows).
+  std::vector<uint8_t> serialize() const {
+    detail::ByteWriter w;
+    w.header(3);
+    w.u8(static_cast<uint8_t>(detail::dtype_of<TX>()));
+    w.u64(k_);
+    w.vec(y_);
+    w.vec(size_);
+    w.f64(distortion_);
+    w.u32(iters_);
+    w.vec(centroids_);
+    return w.take();
+  }
+  static KMeans deserialize(const std::vector<uint8_t>& bytes) {
+    detail::ByteReader r(bytes);
+    r.header(3);
+    if (r.u8() != static_cast<uint8_t>(detail::dtype_of<TX>()))
+      throw Failed::input("serialized rows are of another type");
+    KMeans m;
+    m.k_ = static_cast<size_t>(r.u64());
+    m.y_ = r.vec<uint32_t>();
+    m.size_ = r.vec<uint64_t>();
+    m.distortion_ = r.f64();
+    m.iters_ = r.u32();
+    m.centroids_ = r.vec<double>();
+    if (!r.done()) throw Failed::input("trailing bytes after the serialized model");
+    if (m.size_.size() != m.k_ || (m.k_ && m.centroids_.size() % m.k_ != 0))
+      throw Failed::input("serialized k-means model is inconsistent");
+    return m;
+  }
+  bool operator==(const KMeans& o) const {
+    return k_ == o.k_ && y_ == o.y_ && size_ == o.size_ && centroids_ == o.centroids_ &&
+           (distortion_ == o.distortion_ || (std::isnan(distortion_) && std::isnan(o.distortion_))) &&
+           iters_ == o.iters_;
+  }
 
  private:
   static KMeans fit_from(sc_index* ix, size_t d, const KMeansParameters& p, std::vector<double> c) {

@@ -11,6 +11,7 @@
 #include <variant>
 
 #include "../../include/smartcore_amd.hpp"
+#include "../../include/smartcore_amd_io.hpp"
 
 namespace py = pybind11;
 using namespace smartcore;
@@ -62,6 +63,22 @@ class PyKNN {
     return PyKNN(fit_t<double>(x, yv, p), false);
   }
 
+  py::bytes serialize() const {
+    const auto b = std::visit([](const auto& m) { return m.serialize(); }, m_);
+    return py::bytes(reinterpret_cast<const char*>(b.data()), b.size());
+  }
+  // The element type and distance are read from the bytes: every
+  // alternative is tried and the one whose layout matches is kept.
+  static PyKNN deserialize(py::bytes data) {
+    const std::string s = data;
+    const std::vector<uint8_t> b(s.begin(), s.end());
+    return try_load<0>(b);
+  }
+  bool equals(const PyKNN& o) const { return m_ == o.m_; }
+  bool on_device() const {
+    return std::visit([](const auto& m) { return m.on_device(); }, m_);
+  }
+
   py::array_t<TY> predict(py::array x) const {
     std::vector<TY> out = std::visit(
         [&](const auto& m) -> std::vector<TY> {
@@ -77,6 +94,20 @@ class PyKNN {
 
  private:
   PyKNN(V m, bool f32) : m_(std::move(m)), f32_(f32) {}
+
+  template <size_t I>
+  static PyKNN try_load(const std::vector<uint8_t>& b) {
+    using M = std::variant_alternative_t<I, V>;
+    try {
+      return PyKNN(V(std::in_place_index<I>, M::deserialize(b)),
+                   std::is_same<typename M::value_type, float>::value);
+    } catch (const Failed&) {
+      if constexpr (I + 1 < std::variant_size_v<V>)
+        return try_load<I + 1>(b);
+      else
+        throw;
+    }
+  }
 
   template <typename T>
   static V fit_t(py::array x, const std::vector<TY>& y, const PyKNNParams& p) {
@@ -216,6 +247,20 @@ class PyKMeans {
   double distortion() const {
     return std::visit([](const auto& m) { return m.distortion(); }, m_);
   }
+  py::bytes serialize() const {
+    const auto b = std::visit([](const auto& m) { return m.serialize(); }, m_);
+    return py::bytes(reinterpret_cast<const char*>(b.data()), b.size());
+  }
+  static PyKMeans deserialize(py::bytes data) {
+    const std::string s = data;
+    const std::vector<uint8_t> b(s.begin(), s.end());
+    try {
+      return PyKMeans(KMeans<double, uint32_t>::deserialize(b));
+    } catch (const Failed&) {
+      return PyKMeans(KMeans<float, uint32_t>::deserialize(b));
+    }
+  }
+  bool equals(const PyKMeans& o) const { return m_ == o.m_; }
   uint32_t iterations() const {
     return std::visit([](const auto& m) { return m.iterations(); }, m_);
   }
@@ -344,12 +389,20 @@ PYBIND11_MODULE(smartcore, m) {
   py::class_<Cls>(m, "KNNClassifier")
       .def_static("fit", &Cls::fit, py::arg("x"), py::arg("y"),
                   py::arg("parameters") = PyKNNParams{})
-      .def("predict", &Cls::predict);
+      .def("predict", &Cls::predict)
+      .def("serialize", &Cls::serialize)
+      .def_static("deserialize", &Cls::deserialize)
+      .def("on_device", &Cls::on_device)
+      .def("__eq__", &Cls::equals);
   using Reg = PyKNN<KNNRegressor, double>;
   py::class_<Reg>(m, "KNNRegressor")
       .def_static("fit", &Reg::fit, py::arg("x"), py::arg("y"),
                   py::arg("parameters") = PyKNNParams{})
-      .def("predict", &Reg::predict);
+      .def("predict", &Reg::predict)
+      .def("serialize", &Reg::serialize)
+      .def_static("deserialize", &Reg::deserialize)
+      .def("on_device", &Reg::on_device)
+      .def("__eq__", &Reg::equals);
 
   py::class_<KMeansParameters>(m, "KMeansParameters")
       .def(py::init<>())
@@ -376,7 +429,64 @@ PYBIND11_MODULE(smartcore, m) {
       .def_property_readonly("size", &PyKMeans::size)
       .def_property_readonly("centroids", &PyKMeans::centroids)
       .def_property_readonly("distortion", &PyKMeans::distortion)
-      .def_property_readonly("iterations", &PyKMeans::iterations);
+      .def_property_readonly("iterations", &PyKMeans::iterations)
+      .def("serialize", &PyKMeans::serialize)
+      .def_static("deserialize", &PyKMeans::deserialize)
+      .def("__eq__", &PyKMeans::equals);
+
+  // readers (src/readers/csv.rs) and the .xy datasets (src/dataset/mod.rs)
+  static py::exception<readers::ReadingError> reading_exc(m, "ReadingError");
+  py::register_exception_translator([](std::exception_ptr p) {
+    try {
+      if (p) std::rethrow_exception(p);
+    } catch (const readers::ReadingError& e) {
+      static const char* kinds[] = {"CouldNotReadFileSystem", "NoRowsProvided", "InvalidField",
+                                    "InvalidRow", "ShapesDoNotMatch"};
+      py::object err = reading_exc;
+      py::object inst = err(e.what());
+      inst.attr("kind") = kinds[static_cast<int>(e.kind())];
+      inst.attr("msg") = e.message() ? py::cast(*e.message()) : py::none();
+      PyErr_SetObject(reading_exc.ptr(), inst.ptr());
+    }
+  });
+  m.def(
+      "matrix_from_csv_source",
+      [](const std::string& text, size_t n_rows_header, const std::string& sep, bool f32) -> py::array {
+        readers::CSVDefinition def{n_rows_header, sep};
+        auto out = [](auto mx) {
+          using T = typename std::decay_t<decltype(mx.row_major())>::value_type;
+          const auto sh = mx.shape();
+          py::array_t<T> a({static_cast<py::ssize_t>(sh.first), static_cast<py::ssize_t>(sh.second)});
+          const auto v = mx.row_major();
+          std::copy(v.begin(), v.end(), a.mutable_data());
+          return py::array(a);
+        };
+        if (f32) return out(readers::matrix_from_csv_source<float>(text, def));
+        return out(readers::matrix_from_csv_source<double>(text, def));
+      },
+      py::arg("text"), py::arg("n_rows_header") = 1, py::arg("field_seperator") = ",",
+      py::arg("f32") = false);
+  m.def("deserialize_data", [](py::bytes data) {
+    const std::string s = data;
+    const auto ds = dataset::deserialize_data(std::vector<uint8_t>(s.begin(), s.end()));
+    py::array_t<float> x({static_cast<py::ssize_t>(ds.num_samples), static_cast<py::ssize_t>(ds.num_features)});
+    std::copy(ds.data.begin(), ds.data.end(), x.mutable_data());
+    py::array_t<float> y(ds.target.size());
+    std::copy(ds.target.begin(), ds.target.end(), y.mutable_data());
+    return py::make_tuple(x, y);
+  });
+  m.def("serialize_data", [](py::array_t<float, py::array::c_style | py::array::forcecast> x,
+                             py::array_t<float, py::array::c_style | py::array::forcecast> y) {
+    if (x.ndim() != 2 || y.ndim() != 1 || y.shape(0) != x.shape(0))
+      throw Failed::input("x must be n x f and y of length n");
+    dataset::Dataset ds;
+    ds.num_samples = x.shape(0);
+    ds.num_features = x.shape(1);
+    ds.data.assign(x.data(), x.data() + x.size());
+    ds.target.assign(y.data(), y.data() + y.size());
+    const auto b = dataset::serialize_data(ds);
+    return py::bytes(reinterpret_cast<const char*>(b.data()), b.size());
+  });
 
   py::class_<PyDBSCANParams>(m, "DBSCANParameters")
       .def(py::init<>())
