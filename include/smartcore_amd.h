@@ -61,11 +61,12 @@ typedef enum { SC_F64 = 0, SC_F32 = 1 } sc_dtype;
 typedef enum { SC_EUCLIDIAN = 0, SC_MANHATTAN = 1, SC_MINKOWSKI = 2 } sc_metric;
 
 /* SC_PLAIN_FP64: distances in the reference's own arithmetic order (bit-exact
- * distances, heap-replayed output).  SC_MIXED: Euclidian only; BF16 MFMA
- * cross terms with FP32 accumulation select a certified shortlist that is
+ * distances, heap-replayed output).  SC_MIXED: for Euclidian, BF16 MFMA
+ * cross terms with FP32 accumulation select a certified shortlist; for
+ * Manhattan, FP32 sums with a rigorous error bound do.  The shortlist is
  * rescored in the SC_PLAIN_FP64 arithmetic, so the output is identical to
  * SC_PLAIN_FP64's (queries the certificate cannot cover are recomputed in
- * SC_PLAIN_FP64). */
+ * SC_PLAIN_FP64).  Minkowski has no SC_MIXED form. */
 typedef enum { SC_PLAIN_FP64 = 0, SC_MIXED = 1 } sc_mode;
 
 /* Per-call counters of the calling thread's last search (sc_last_stats). */
@@ -115,6 +116,25 @@ int sc_knn_search_device(const sc_index* index, const void* d_queries,
                          size_t q, size_t k, sc_metric metric, uint16_t p,
                          sc_mode mode, int64_t* d_idx, double* d_dist,
                          uint32_t* d_count, void* stream);
+
+/* Many uneven query blocks against one index in one call (no reference
+ * counterpart: smartcore calls LinearKNNSearch::find, linear_search.rs:52-84,
+ * once per row; this is the batched form of that loop for many independent
+ * callers).  Each block has its own host queries (q*d, index dtype), k and
+ * outputs in the sc_knn_search layout.  Blocks with the same k run as one
+ * device search over their concatenated rows; every block gets exactly what
+ * sc_knn_search gives it alone.  Errors as sc_knn_search, for the first
+ * block that fails validation. */
+typedef struct {
+  const void* queries;
+  size_t q;
+  size_t k;
+  int64_t* idx;
+  double* dist;
+  uint32_t* count;
+} sc_knn_block;
+int sc_knn_search_batch(const sc_index* index, const sc_knn_block* blocks, size_t count,
+                        sc_metric metric, uint16_t p, sc_mode mode);
 
 /* LinearKNNSearch::find_radius (linear_search.rs:89-108) for q queries.
  * Result rows are in index order.  Errors: SC_FIND_FAILED "radius should be

@@ -163,6 +163,11 @@ def dtype_code(dt) -> int:
     raise TypeError(f"rows must be float64 or float32, got {dt}")
 
 
+class sc_knn_block(C.Structure):
+    _fields_ = [("queries", C.c_void_p), ("q", C.c_size_t), ("k", C.c_size_t),
+                ("idx", C.c_void_p), ("dist", C.c_void_p), ("count", C.c_void_p)]
+
+
 class Index:
     """Device-resident reference rows (an sc_index handle)."""
 
@@ -214,6 +219,28 @@ class Index:
         check(lib().sc_knn_search(self._h, _ptr(queries), q, kk, metric, p, mode, _ptr(idx),
                                   _ptr(dist), _ptr(cnt)))
         return idx[:, :kk], dist[:, :kk], cnt[:q]
+
+    def knn_batch(self, blocks, metric: int = SC_EUCLIDIAN, p: int = 2,
+                  mode: int = SC_PLAIN_FP64):
+        """sc_knn_search_batch over [(queries, k), ...]: a list of
+        (idx, dist, count) per block."""
+        arr = (sc_knn_block * max(len(blocks), 1))()
+        keep, outs = [], []
+        for i, (qs, k) in enumerate(blocks):
+            qs = np.ascontiguousarray(qs, dtype=self.dtype)
+            q = qs.shape[0]
+            idx = np.empty((q, max(k, 1)), np.int64)
+            dist = np.empty((q, max(k, 1)), np.float64)
+            cnt = np.empty(max(q, 1), np.uint32)
+            keep += [qs, idx, dist, cnt]
+            outs.append((idx, dist, cnt, q, k))
+            arr[i] = sc_knn_block(qs.ctypes.data, q, k, idx.ctypes.data, dist.ctypes.data,
+                                  cnt.ctypes.data)
+        L = lib()
+        L.sc_knn_search_batch.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int,
+                                          C.c_uint16, C.c_int]
+        check(L.sc_knn_search_batch(self._h, C.cast(arr, C.c_void_p), len(blocks), metric, p, mode))
+        return [(i[:, :k], d[:, :k], c[:q]) for i, d, c, q, k in outs]
 
     def knn_device(self, q_ptr: int, q: int, k: int, idx_ptr: int, dist_ptr: int, cnt_ptr: int,
                    metric: int = SC_EUCLIDIAN, p: int = 2, mode: int = SC_PLAIN_FP64,
@@ -271,6 +298,15 @@ class Index:
         check(lib().sc_kmeans_step(self._h, _ptr(centroids), k, _ptr(sums), _ptr(counts),
                                    _ptr(memb), C.byref(dist)))
         return dist.value, sums, counts, memb[: self.n]
+
+    def kmeans_plus_plus(self, k: int, seed: int | None = None) -> np.ndarray:
+        """sc_kmeans_plus_plus: initial labels (KMeans::kmeans_plus_plus)."""
+        L = lib()
+        L.sc_kmeans_plus_plus.argtypes = [C.c_void_p, C.c_size_t, C.c_int, C.c_uint64,
+                                          C.c_void_p]
+        y = np.zeros(max(self.n, 1), np.uint32)
+        check(L.sc_kmeans_plus_plus(self._h, k, int(seed is not None), int(seed or 0), _ptr(y)))
+        return y[: self.n]
 
     def kmeans_run(self, init_centroids: np.ndarray, max_iter: int, mode: int = SC_PLAIN_FP64):
         """sc_kmeans_run: (centroids, labels, size, distortion, iters)."""

@@ -2,10 +2,12 @@
 // points, error and stats reporting.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <atomic>
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "common.hpp"
@@ -50,28 +52,168 @@ void clear_error() { t_last_error.clear(); }
 sc_stats& last_stats() { return t_stats; }
 
 int hip_fail(hipError_t e, const char* what) {
+  // HIP keeps the last failure of any call for hipGetLastError; clear it so
+  // that the launch checks of the next call do not report this one again
+  (void)hipGetLastError();
   const int code = (e == hipErrorOutOfMemory) ? SC_ERR_OUT_OF_MEMORY : SC_ERR_DEVICE;
   return fail(code, std::string("HIP error ") + hipGetErrorString(e) + " in " + what);
+}
+
+namespace {
+
+constexpr size_t kStageBytes = size_t{16} << 20;  // per pinned buffer
+constexpr int kCopyThreads = 8;
+
+// The buffers are kept for the life of the process (like the per-thread
+// streams): freeing them from a thread-exit destructor could run after the
+// HIP runtime has shut down.
+struct Stager {
+  void* buf[2] = {nullptr, nullptr};
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  bool used[2] = {false, false};
+  hipError_t init() {
+    for (int i = 0; i < 2; ++i) {
+      if (!buf[i]) {
+        hipError_t e = hipHostMalloc(&buf[i], kStageBytes, hipHostMallocDefault);
+        if (e != hipSuccess) {
+          buf[i] = nullptr;
+          return e;
+        }
+      }
+      if (!ev[i]) {
+        hipError_t e = hipEventCreateWithFlags(&ev[i], hipEventDisableTiming);
+        if (e != hipSuccess) {
+          ev[i] = nullptr;
+          return e;
+        }
+      }
+    }
+    return hipSuccess;
+  }
+};
+
+// One stager per host thread (its buffers are reused by the thread's calls).
+Stager& thread_stager() {
+  thread_local Stager st;
+  return st;
+}
+
+// memcpy split over a few threads for large spans.
+void parallel_copy(void* dst, const void* src, size_t bytes) {
+  if (bytes < (size_t{4} << 20)) {
+    std::memcpy(dst, src, bytes);
+    return;
+  }
+  const size_t part = (bytes / kCopyThreads + 4095) / 4096 * 4096;
+  std::thread th[kCopyThreads - 1];
+  for (int t = 1; t < kCopyThreads; ++t) {
+    const size_t off = part * t;
+    if (off >= bytes) break;
+    const size_t len = std::min(part, bytes - off);
+    th[t - 1] = std::thread([=] {
+      std::memcpy(static_cast<char*>(dst) + off, static_cast<const char*>(src) + off, len);
+    });
+  }
+  std::memcpy(dst, src, std::min(part, bytes));
+  for (auto& t : th)
+    if (t.joinable()) t.join();
+}
+
+}  // namespace
+
+int upload_staged(void* dev, const void* host, size_t bytes, hipStream_t stream) {
+  if (bytes == 0) return SC_OK;
+  Stager& st = thread_stager();
+  SC_HIP(st.init());
+  int b = 0;
+  for (size_t off = 0; off < bytes; off += kStageBytes, b ^= 1) {
+    const size_t len = std::min(kStageBytes, bytes - off);
+    if (st.used[b]) SC_HIP(hipEventSynchronize(st.ev[b]));
+    parallel_copy(st.buf[b], static_cast<const char*>(host) + off, len);
+    SC_HIP(hipMemcpyAsync(static_cast<char*>(dev) + off, st.buf[b], len, hipMemcpyHostToDevice,
+                          stream));
+    SC_HIP(hipEventRecord(st.ev[b], stream));
+    st.used[b] = true;
+  }
+  return SC_OK;
+}
+
+int download_staged(void* host, const void* dev, size_t bytes, hipStream_t stream) {
+  if (bytes == 0) return SC_OK;
+  Stager& st = thread_stager();
+  SC_HIP(st.init());
+  // chunk i is copied out of its buffer while chunk i + 1 is in flight
+  const size_t nch = (bytes + kStageBytes - 1) / kStageBytes;
+  auto issue = [&](size_t c) -> hipError_t {
+    const size_t off = c * kStageBytes;
+    const size_t len = std::min(kStageBytes, bytes - off);
+    const int b = static_cast<int>(c & 1);
+    hipError_t e = hipMemcpyAsync(st.buf[b], static_cast<const char*>(dev) + off, len,
+                                  hipMemcpyDeviceToHost, stream);
+    if (e != hipSuccess) return e;
+    st.used[b] = true;
+    return hipEventRecord(st.ev[b], stream);
+  };
+  SC_HIP(issue(0));
+  for (size_t c = 0; c < nch; ++c) {
+    if (c + 1 < nch) SC_HIP(issue(c + 1));
+    const int b = static_cast<int>(c & 1);
+    SC_HIP(hipEventSynchronize(st.ev[b]));
+    const size_t off = c * kStageBytes;
+    parallel_copy(static_cast<char*>(host) + off, st.buf[b], std::min(kStageBytes, bytes - off));
+  }
+  return SC_OK;
+}
+
+int upload_staged_rows(void* dev, const void* host, size_t n, size_t row_bytes, size_t ld_bytes,
+                       hipStream_t stream) {
+  if (n == 0) return SC_OK;
+  if (row_bytes == ld_bytes) return upload_staged(dev, host, n * row_bytes, stream);
+  Stager& st = thread_stager();
+  SC_HIP(st.init());
+  // rows are widened to the padded stride while they are copied into the
+  // pinned buffer, so the DMA moves the final layout
+  const size_t rows_per_chunk = std::max<size_t>(kStageBytes / ld_bytes, 1);
+  int b = 0;
+  for (size_t r0 = 0; r0 < n; r0 += rows_per_chunk, b ^= 1) {
+    const size_t nr = std::min(rows_per_chunk, n - r0);
+    if (st.used[b]) SC_HIP(hipEventSynchronize(st.ev[b]));
+    char* dst = static_cast<char*>(st.buf[b]);
+    const char* src = static_cast<const char*>(host) + r0 * row_bytes;
+    auto pad_rows = [=](size_t a, size_t e) {
+      for (size_t r = a; r < e; ++r) {
+        std::memcpy(dst + r * ld_bytes, src + r * row_bytes, row_bytes);
+        std::memset(dst + r * ld_bytes + row_bytes, 0, ld_bytes - row_bytes);
+      }
+    };
+    if (nr * ld_bytes < (size_t{4} << 20)) {
+      pad_rows(0, nr);
+    } else {
+      std::thread th[kCopyThreads - 1];
+      const size_t part = (nr + kCopyThreads - 1) / kCopyThreads;
+      for (int t = 1; t < kCopyThreads; ++t)
+        th[t - 1] = std::thread(pad_rows, std::min(nr, part * t), std::min(nr, part * (t + 1)));
+      pad_rows(0, std::min(nr, part));
+      for (auto& t : th)
+        if (t.joinable()) t.join();
+    }
+    SC_HIP(hipMemcpyAsync(static_cast<char*>(dev) + r0 * ld_bytes, st.buf[b], nr * ld_bytes,
+                          hipMemcpyHostToDevice, stream));
+    SC_HIP(hipEventRecord(st.ev[b], stream));
+    st.used[b] = true;
+  }
+  return SC_OK;
 }
 
 int upload_padded(const void* host, sc_dtype dtype, size_t n, size_t d, size_t ld, void* dev,
                   hipStream_t stream) {
   const size_t es = elem_size(dtype);
   if (n == 0) return SC_OK;
-  if (ld == d) {
-    SC_HIP(hipMemcpyAsync(dev, host, n * d * es, hipMemcpyHostToDevice, stream));
-    return SC_OK;
-  }
   if (d == 0) {
     SC_HIP(hipMemsetAsync(dev, 0, n * ld * es, stream));
     return SC_OK;
   }
-  // one contiguous transfer, then the padding on the device (a strided
-  // copy from pageable memory moves row by row)
-  DevBuf tmp;
-  SC_HIP(tmp.alloc(n * d * es, stream));
-  SC_HIP(hipMemcpyAsync(tmp.p, host, n * d * es, hipMemcpyHostToDevice, stream));
-  return copy_padded_device(tmp.p, dtype, n, d, ld, dev, stream);
+  return upload_staged_rows(dev, host, n, d * es, ld * es, stream);
 }
 
 int copy_padded_device(const void* src, sc_dtype dtype, size_t n, size_t d, size_t ld, void* dev,
@@ -103,7 +245,8 @@ std::vector<ProfRecord> g_prof;
 
 ProfScope::ProfScope(const char* name, hipStream_t stream) : name_(name), stream_(stream) {
   if (!g_prof_on.load(std::memory_order_relaxed)) return;
-  // launches recorded into a graph are timed by the scope around the graph
+  // Launches recorded into a graph are timed by the scope around the graph
+  // launch: events recorded during a capture cannot be timed.
   hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
   if (hipStreamIsCapturing(stream_, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) return;
   if (hipEventCreate(&start_) != hipSuccess) {
@@ -124,6 +267,8 @@ ProfScope::~ProfScope() {
   std::lock_guard<std::mutex> lk(g_prof_mu);
   g_prof.push_back(ProfRecord{name_, start_, stop});
 }
+
+bool profiling_enabled() { return g_prof_on.load(std::memory_order_relaxed); }
 
 void prof_clear_locked() {
   for (auto& r : g_prof) {
@@ -198,8 +343,8 @@ int validate_search(const sc_index* index, size_t k, sc_metric metric, uint16_t 
   // minkowski.rs:57-59
   if (metric == SC_MINKOWSKI && p < 1) return fail(SC_ERR_PANIC, "p must be at least 1");
   if (mode != SC_PLAIN_FP64 && mode != SC_MIXED) return fail(SC_ERR_UNSUPPORTED, "unknown mode");
-  if (mode == SC_MIXED && metric != SC_EUCLIDIAN)
-    return fail(SC_ERR_UNSUPPORTED, "SC_MIXED supports the Euclidian metric only");
+  if (mode == SC_MIXED && metric == SC_MINKOWSKI)
+    return fail(SC_ERR_UNSUPPORTED, "SC_MIXED supports the Euclidian and Manhattan metrics only");
   return SC_OK;
 }
 
@@ -209,8 +354,10 @@ int search_device_impl(const sc_index* index, const void* d_padded_queries, size
   sc_stats& st = last_stats();
   st = sc_stats{};
   sc_stats* sp = with_stats ? &st : nullptr;
-  if (mode == SC_MIXED && index->dtype == SC_F32)
+  if (mode == SC_MIXED && index->dtype == SC_F32 && metric == SC_EUCLIDIAN)
     return knn_mixed_device(index, d_padded_queries, q, k, d_idx, d_dist, d_count, s, sp);
+  if (mode == SC_MIXED && metric == SC_MANHATTAN)
+    return knn_l1_mixed_device(index, d_padded_queries, q, k, d_idx, d_dist, d_count, s, sp);
   return knn_exact_device(index, d_padded_queries, q, k, metric, p, d_idx, d_dist, d_count, s, sp);
 }
 
@@ -269,9 +416,9 @@ int sc_knn_search(const sc_index* index, const void* queries, size_t q, size_t k
   if (int rc = search_device_impl(index, dq.p, q, k, metric, p, mode, di.as<int64_t>(),
                                   dd.as<double>(), dc.as<uint32_t>(), s, true))
     return rc;
-  SC_HIP(hipMemcpyAsync(idx, di.p, q * k * sizeof(int64_t), hipMemcpyDeviceToHost, s));
-  SC_HIP(hipMemcpyAsync(dist, dd.p, q * k * sizeof(double), hipMemcpyDeviceToHost, s));
-  SC_HIP(hipMemcpyAsync(count, dc.p, q * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  if (int rc = download_staged(idx, di.p, q * k * sizeof(int64_t), s)) return rc;
+  if (int rc = download_staged(dist, dd.p, q * k * sizeof(double), s)) return rc;
+  if (int rc = download_staged(count, dc.p, q * sizeof(uint32_t), s)) return rc;
   SC_HIP(hipStreamSynchronize(s));
   return SC_OK;
 }
@@ -297,6 +444,77 @@ int sc_knn_search_device(const sc_index* index, const void* d_queries, size_t q,
   if (int rc = copy_padded_device(d_queries, index->dtype, q, index->d, index->ld, dq.p, s))
     return rc;
   return search_device_impl(index, dq.p, q, k, metric, p, mode, d_idx, d_dist, d_count, s, false);
+}
+
+int sc_knn_search_batch(const sc_index* index, const sc_knn_block* blocks, size_t count,
+                        sc_metric metric, uint16_t p, sc_mode mode) {
+  clear_error();
+  if (count == 0) return SC_OK;
+  if (!index || !blocks) return fail(SC_PARAMETERS_ERROR, "null argument");
+  for (size_t b = 0; b < count; ++b) {
+    if (int rc = validate_search(index, blocks[b].k, metric, p, mode)) return rc;
+    if (blocks[b].q > 0 && (!blocks[b].queries || !blocks[b].idx || !blocks[b].dist ||
+                            !blocks[b].count))
+      return fail(SC_PARAMETERS_ERROR, "block " + std::to_string(b) + " has a null buffer");
+  }
+  SC_HIP(hipSetDevice(index->device));
+  hipStream_t s = stream_for(index);
+  const size_t es = elem_size(index->dtype);
+  // blocks with the same k are searched together: one device search over
+  // their concatenated rows (each query is searched independently of the
+  // rows beside it, so every block gets what sc_knn_search gives it)
+  std::vector<size_t> order(count);
+  for (size_t b = 0; b < count; ++b) order[b] = b;
+  std::stable_sort(order.begin(), order.end(),
+                   [&](size_t a, size_t b) { return blocks[a].k < blocks[b].k; });
+  sc_stats total{};
+  for (size_t g0 = 0; g0 < count;) {
+    size_t g1 = g0;
+    size_t rows = 0;
+    const size_t k = blocks[order[g0]].k;
+    while (g1 < count && blocks[order[g1]].k == k) rows += blocks[order[g1++]].q;
+    if (rows > 0) {
+      DevBuf dq, di, dd, dc;
+      SC_HIP(dq.alloc(rows * index->ld * es, s));
+      SC_HIP(di.alloc(rows * k * sizeof(int64_t), s));
+      SC_HIP(dd.alloc(rows * k * sizeof(double), s));
+      SC_HIP(dc.alloc(rows * sizeof(uint32_t), s));
+      size_t at = 0;
+      for (size_t g = g0; g < g1; ++g) {
+        const sc_knn_block& bl = blocks[order[g]];
+        if (int rc = upload_padded(bl.queries, index->dtype, bl.q, index->d, index->ld,
+                                   dq.as<char>() + at * index->ld * es, s))
+          return rc;
+        at += bl.q;
+      }
+      if (int rc = search_device_impl(index, dq.p, rows, k, metric, p, mode, di.as<int64_t>(),
+                                      dd.as<double>(), dc.as<uint32_t>(), s, true))
+        return rc;
+      const sc_stats& st = last_stats();
+      total.queries += st.queries;
+      total.candidates += st.candidates;
+      total.overflow_queries += st.overflow_queries;
+      total.shortlist_rescored += st.shortlist_rescored;
+      total.uncertified += st.uncertified;
+      at = 0;
+      for (size_t g = g0; g < g1; ++g) {
+        const sc_knn_block& bl = blocks[order[g]];
+        if (bl.q > 0) {
+          if (int rc = download_staged(bl.idx, di.as<int64_t>() + at * k, bl.q * k * sizeof(int64_t), s))
+            return rc;
+          if (int rc = download_staged(bl.dist, dd.as<double>() + at * k, bl.q * k * sizeof(double), s))
+            return rc;
+          if (int rc = download_staged(bl.count, dc.as<uint32_t>() + at, bl.q * sizeof(uint32_t), s))
+            return rc;
+        }
+        at += bl.q;
+      }
+    }
+    g0 = g1;
+  }
+  SC_HIP(hipStreamSynchronize(s));
+  last_stats() = total;
+  return SC_OK;
 }
 
 size_t sc_last_error(char* buf, size_t len) {

@@ -104,6 +104,19 @@ struct DevBuf {
   }
 };
 
+// Host <-> device copies of pageable caller memory through two pinned
+// staging buffers of the calling thread: the host copies chunk i into one
+// buffer (split over a few threads) while the DMA engine moves chunk i - 1
+// from the other.  upload_staged returns once the caller's memory has been
+// read (the last DMA may still be in flight, from pinned memory);
+// download_staged returns once `host` holds the data.
+int upload_staged(void* dev, const void* host, size_t bytes, hipStream_t stream);
+// n rows of row_bytes to the device at a stride of ld_bytes (the padding is
+// zeroed on the host side of the copy).
+int upload_staged_rows(void* dev, const void* host, size_t n, size_t row_bytes, size_t ld_bytes,
+                       hipStream_t stream);
+int download_staged(void* host, const void* dev, size_t bytes, hipStream_t stream);
+
 // Copies `n` host rows of `d` elements into a zero-padded device block with
 // stride ld (async on `stream`).
 int upload_padded(const void* host, sc_dtype dtype, size_t n, size_t d, size_t ld, void* dev,
@@ -128,6 +141,9 @@ class ProfScope {
   hipEvent_t start_ = nullptr;
 };
 
+// Whether the tracing hooks are on (sc_profile_enable).
+bool profiling_enabled();
+
 // The calling host thread's stream for the index's device.
 hipStream_t stream_for(const sc_index* index);
 hipStream_t stream_for_device(int device);
@@ -139,6 +155,12 @@ hipStream_t stream_for_device(int device);
 int knn_exact_device(const sc_index* index, const void* d_queries, size_t q, size_t k,
                      sc_metric metric, int p, int64_t* d_idx, double* d_dist, uint32_t* d_count,
                      hipStream_t stream, sc_stats* stats);
+
+// SC_MIXED Manhattan: bounded f32 scan, candidates rescored in the
+// reference arithmetic (f32 rows; f64 rows take the exact form).
+int knn_l1_mixed_device(const sc_index* index, const void* d_queries, size_t q, size_t k,
+                        int64_t* d_idx, double* d_dist, uint32_t* d_count, hipStream_t stream,
+                        sc_stats* stats);
 
 int knn_mixed_device(const sc_index* index, const void* d_queries, size_t q, size_t k,
                      int64_t* d_idx, double* d_dist, uint32_t* d_count, hipStream_t stream,

@@ -147,8 +147,178 @@ __global__ __launch_bounds__(NT) void knn_scan_kernel(
   if (tid < QT && my_q < q) cand_count[static_cast<size_t>(my_q) * nseg + seg] = emitted;
 }
 
+// Manhattan on f32 rows with f32 sums (SC_MIXED).  Each term
+// |a - b| is the reference's own f32 difference (manhattan.rs:56-59, the
+// widening to f64 is exact), so only the summation differs: a sequential
+// f32 sum A of the ld terms differs from their exact sum, and so from the
+// reference's f64 fold S, by at most about (ld + 1) 2^-24 A.  With
+// e = (ld + 4) 2^-24 (the extra terms cover the roundings of the bound
+// arithmetic itself) and an absolute 2^-126 for underflow,
+// L = A (1 - e) - 2^-126 <= S <= U = A (1 + e) + 2^-126.  A segment keeps
+// the k smallest U it has seen (starting from `seed`, when given, the k-th
+// smallest U of earlier rows, else +inf) and stores, in row order, every
+// row whose L is below the current k-th of them: the k-th U is at or above
+// the k-th S, so every row the reference heap admits is stored.  Their S is
+// recomputed in the reference arithmetic (knn_rescore_kernel) and the
+// usual replay runs over them.  A non-finite A (inf / NaN rows or an f32
+// overflow) is always stored and never lowers the threshold.  With
+// `out_thr` the segment's final k-th U is written per query (the seed of
+// later segments).
+__global__ __launch_bounds__(NT) void knn_scan_l1f32_kernel(
+    const float* __restrict__ refs, int64_t r0, int64_t n, int ld,
+    const float* __restrict__ queries, int64_t q, int k, int64_t seg_len, int nseg,
+    const float* __restrict__ seed, float* __restrict__ out_thr,
+    uint32_t* __restrict__ cand_idx, uint32_t* __restrict__ cand_count, int cap) {
+  __shared__ float Qs[kTileD][QT + 1];
+  __shared__ float Rs[kTileD][RT + 1];
+  __shared__ uint32_t mask[QT][RT / 32];
+  __shared__ float sL[QT][RT];
+  __shared__ float sU[QT][RT];
+  __shared__ float topk[QT][KMAX_TILED];
+  __shared__ float thr[QT];
+
+  const int tid = threadIdx.x;
+  const int tq = tid >> 5;
+  const int tr = tid & 31;
+  const int64_t q0 = static_cast<int64_t>(blockIdx.x) * QT;
+  const int seg = blockIdx.y;
+  const int64_t r_begin = r0 + static_cast<int64_t>(seg) * seg_len;
+  const int64_t r_end = min(n, r_begin + seg_len);
+  const float e = static_cast<float>(ld + 4) * 5.9604644775390625e-08f;
+  const float tiny = 1.1754943508222875e-38f;
+
+  for (int i = tid; i < QT * KMAX_TILED; i += NT) (&topk[0][0])[i] = INFINITY;
+  for (int i = tid; i < QT * (RT / 32); i += NT) (&mask[0][0])[i] = 0u;
+  if (tid < QT) thr[tid] = (seed && q0 + tid < q) ? seed[q0 + tid] : INFINITY;
+  __syncthreads();
+  // a seed is an upper bound on the k-th S: it caps every slot
+  if (seed && tid < QT)
+    for (int j = 0; j < k; ++j) topk[tid][j] = thr[tid];
+
+  uint32_t emitted = 0;
+  const int64_t my_q = q0 + tid;
+  const size_t cand_base = (tid < QT && my_q < q)
+                               ? (static_cast<size_t>(my_q) * nseg + seg) * static_cast<size_t>(cap)
+                               : 0;
+
+  for (int64_t t0 = r_begin; t0 < r_end; t0 += RT) {
+    float acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) acc[i][j] = 0.0f;
+    for (int c0 = 0; c0 < ld; c0 += kTileD) {
+      __syncthreads();
+      for (int x = tid; x < RT * kTileD; x += NT) {
+        const int r = x / kTileD, c = x % kTileD;
+        const int64_t row = t0 + r;
+        Rs[c][r] = row < r_end ? refs[row * ld + c0 + c] : 0.0f;
+      }
+      for (int x = tid; x < QT * kTileD; x += NT) {
+        const int r = x / kTileD, c = x % kTileD;
+        const int64_t row = q0 + r;
+        Qs[c][r] = row < q ? queries[row * ld + c0 + c] : 0.0f;
+      }
+      __syncthreads();
+#pragma unroll 4
+      for (int kk = 0; kk < kTileD; ++kk) {
+        float a[4], b[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) a[i] = Qs[kk][tq * 4 + i];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) b[j] = Rs[kk][tr + 32 * j];
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+          for (int j = 0; j < 4; ++j) acc[i][j] = acc[i][j] + fabsf(a[i] - b[j]);
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int qi = tq * 4 + i;
+      const float th = thr[qi];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int rl = tr + 32 * j;
+        const float A = acc[i][j];
+        const bool fin = A < INFINITY;
+        const float L = fin ? (A - A * e) - tiny : -INFINITY;
+        if (q0 + qi < q && t0 + rl < r_end && L < th) {
+          sL[qi][rl] = L;
+          sU[qi][rl] = fin ? (A + A * e) + tiny : INFINITY;
+          atomicOr(&mask[qi][rl >> 5], 1u << (rl & 31));
+        }
+      }
+    }
+    __syncthreads();
+    if (tid < QT && my_q < q) {
+      float* tk = topk[tid];
+      for (int w = 0; w < RT / 32; ++w) {
+        uint32_t bits = mask[tid][w];
+        mask[tid][w] = 0u;
+        while (bits) {
+          const int b = __builtin_ctz(bits);
+          bits &= bits - 1;
+          const int rl = w * 32 + b;
+          // the running threshold in row order (it fell since the tile began)
+          if (!(sL[tid][rl] < tk[k - 1])) continue;
+          if (emitted < static_cast<uint32_t>(cap))
+            cand_idx[cand_base + emitted] = static_cast<uint32_t>(t0 + rl);
+          ++emitted;
+          const float u = sU[tid][rl];
+          if (u < tk[k - 1]) {
+            int pos = k - 1;
+            while (pos > 0 && tk[pos - 1] > u) {
+              tk[pos] = tk[pos - 1];
+              --pos;
+            }
+            tk[pos] = u;
+          }
+        }
+      }
+      thr[tid] = tk[k - 1];
+    }
+  }
+  if (tid < QT && my_q < q) {
+    cand_count[static_cast<size_t>(my_q) * nseg + seg] = emitted;
+    if (out_thr) out_thr[my_q] = topk[tid][k - 1];
+  }
+}
+
+// The reference sum of every stored candidate (one workgroup per (query,
+// segment) list, one thread per candidate).
+template <typename T, int M>
+__global__ void knn_rescore_kernel(const T* __restrict__ refs, int ld, const T* __restrict__ queries,
+                                   int64_t q, double p, int nseg, int cap,
+                                   const uint32_t* __restrict__ cand_idx,
+                                   const uint32_t* __restrict__ cand_count,
+                                   double* __restrict__ cand_s) {
+  const int64_t list = blockIdx.x;
+  const int64_t qi = list / nseg;
+  if (qi >= q) return;
+  const uint32_t cnt = min(cand_count[list], static_cast<uint32_t>(cap));
+  const T* qrow = queries + qi * ld;
+  const size_t base = static_cast<size_t>(list) * static_cast<size_t>(cap);
+  for (uint32_t c = threadIdx.x; c < cnt; c += blockDim.x) {
+    const T* rrow = refs + static_cast<int64_t>(cand_idx[base + c]) * ld;
+    double s = 0.0;
+    for (int j = 0; j < ld; ++j) s = s + Term<T, M>::f(qrow[j], rrow[j], p);
+    cand_s[base + c] = s;
+  }
+}
+
+// A candidate list set: per query `nseg` lists of up to `cap` (row, s)
+// entries, each list in row order and the lists in row order.
+struct CandSet {
+  const uint32_t* idx;
+  const double* s;
+  const uint32_t* count;
+  int nseg;
+  int cap;
+};
+
 template <int M>
-__global__ void knn_replay_kernel(int64_t q, int k, double p, int nseg, int cap,
+__global__ void knn_replay_kernel(int64_t q, int k, double p, CandSet pre, int nseg, int cap,
                                   const uint32_t* __restrict__ cand_idx,
                                   const double* __restrict__ cand_s,
                                   const uint32_t* __restrict__ cand_count, int64_t* __restrict__ out_idx,
@@ -165,6 +335,22 @@ __global__ void knn_replay_kernel(int64_t q, int k, double p, int nseg, int cap,
     hi[j] = -1;
   }
   unsigned long long seen = 0;
+  // the earlier rows' lists (an optional prefix set), then the main set
+  for (int s = 0; s < pre.nseg; ++s) {
+    const uint32_t cnt = pre.count[static_cast<size_t>(qi) * pre.nseg + s];
+    if (cnt > static_cast<uint32_t>(pre.cap)) {
+      const uint32_t slot = atomicAdd(overflow_n, 1u);
+      overflow_list[slot] = static_cast<uint32_t>(qi);
+      out_count[qi] = 0;
+      return;
+    }
+    const size_t base = (static_cast<size_t>(qi) * pre.nseg + s) * static_cast<size_t>(pre.cap);
+    for (uint32_t c = 0; c < cnt; ++c) {
+      const double d = finalize<M>(pre.s[base + c], p);
+      if (d < hd[0]) heap_push_root(hd, hi, k, d, static_cast<int64_t>(pre.idx[base + c]));
+    }
+    seen += cnt;
+  }
   for (int s = 0; s < nseg; ++s) {
     const uint32_t cnt = cand_count[static_cast<size_t>(qi) * nseg + s];
     if (cnt > static_cast<uint32_t>(cap)) {
@@ -339,7 +525,8 @@ int run_exact(const sc_index* index, const T* d_queries, int64_t q, int k, doubl
       {
         ProfScope ps("knn_replay", stream);
         hipLaunchKernelGGL((knn_replay_kernel<M>), dim3(static_cast<unsigned>((qn + 127) / 128)),
-                           dim3(128), 0, stream, qn, k, p, plan.nseg, plan.cap,
+                           dim3(128), 0, stream, qn, k, p, CandSet{nullptr, nullptr, nullptr, 0, 1},
+                           plan.nseg, plan.cap,
                            cidx.as<uint32_t>(), cs.as<double>(), ccnt.as<uint32_t>(),
                            d_idx + b0 * k, d_dist + b0 * k, d_count + b0, olist.as<uint32_t>(),
                            overflow_n, total_cand);
@@ -380,6 +567,123 @@ int run_exact(const sc_index* index, const T* d_queries, int64_t q, int k, doubl
   return SC_OK;
 }
 
+// SC_MIXED Manhattan on f32 rows: bounded f32 scan, rescoring in the
+// reference arithmetic, replay; overflowing queries take the serial kernel.
+// Indexes of at least 4 x 65,536 rows scan their first 65,536 rows as one
+// segment first; the k-th U found there seeds the threshold of every later
+// segment (it is at or above the heap's threshold from that row on), so the
+// later segments store only the rows that can still enter.
+constexpr int64_t kL1Prefix = 65536;
+
+int run_l1_mixed(const sc_index* index, const float* d_queries, int64_t q, int k, int64_t* d_idx,
+                 double* d_dist, uint32_t* d_count, hipStream_t stream, sc_stats* stats) {
+  if (k > KMAX_TILED)
+    return run_exact<float, kManhattan>(index, d_queries, q, k, 1.0, d_idx, d_dist, d_count,
+                                        stream, stats);
+  const int64_t n = static_cast<int64_t>(index->n);
+  const int ld = static_cast<int>(index->ld);
+  const float* refs = static_cast<const float*>(index->rows);
+  DevBuf flags;
+  SC_HIP(flags.alloc(64, stream));
+  SC_HIP(hipMemsetAsync(flags.p, 0, 64, stream));
+  uint32_t* overflow_n = flags.as<uint32_t>();
+  unsigned long long* total_cand = reinterpret_cast<unsigned long long*>(flags.as<char>() + 8);
+  DevBuf olist;
+  SC_HIP(olist.alloc(sizeof(uint32_t) * std::max<int64_t>(q, 1), stream));
+  const int64_t prefix = n >= 4 * kL1Prefix ? kL1Prefix : 0;
+  // the prefix list: the running-threshold bound for one segment of
+  // `prefix` rows
+  const int cap_p = prefix ? static_cast<int>(std::min<int64_t>(
+                                 prefix, static_cast<int64_t>(3.0 * k * (1.0 + std::log(
+                                             static_cast<double>(prefix) / k))) + 64))
+                           : 1;
+  Plan plan = make_plan(n - prefix, q, k);
+  const size_t per_query = static_cast<size_t>(plan.nseg) * plan.cap * 12 + plan.nseg * 4 +
+                           (prefix ? static_cast<size_t>(cap_p) * 12 + 8 : 0);
+  int64_t qb = std::max<int64_t>(QT, static_cast<int64_t>((size_t(1) << 30) / per_query) / QT * QT);
+  qb = std::min(qb, q);
+  DevBuf cidx, cs, ccnt, pidx, ps, pcnt, seed;
+  SC_HIP(cidx.alloc(sizeof(uint32_t) * qb * plan.nseg * plan.cap, stream));
+  SC_HIP(cs.alloc(sizeof(double) * qb * plan.nseg * plan.cap, stream));
+  SC_HIP(ccnt.alloc(sizeof(uint32_t) * qb * plan.nseg, stream));
+  if (prefix) {
+    SC_HIP(pidx.alloc(sizeof(uint32_t) * qb * cap_p, stream));
+    SC_HIP(ps.alloc(sizeof(double) * qb * cap_p, stream));
+    SC_HIP(pcnt.alloc(sizeof(uint32_t) * qb, stream));
+    SC_HIP(seed.alloc(sizeof(float) * qb, stream));
+  }
+  for (int64_t b0 = 0; b0 < q; b0 += qb) {
+    const int64_t qn = std::min(qb, q - b0);
+    const float* qs = d_queries + b0 * ld;
+    const unsigned qtiles = static_cast<unsigned>((qn + QT - 1) / QT);
+    {
+      ProfScope pscope("knn_scan", stream);
+      if (prefix) {
+        hipLaunchKernelGGL(knn_scan_l1f32_kernel, dim3(qtiles, 1), dim3(NT), 0, stream, refs,
+                           int64_t{0}, prefix, ld, qs, qn, k, prefix, 1,
+                           static_cast<const float*>(nullptr), seed.as<float>(),
+                           pidx.as<uint32_t>(), pcnt.as<uint32_t>(), cap_p);
+        SC_HIP(hipGetLastError());
+      }
+      hipLaunchKernelGGL(knn_scan_l1f32_kernel, dim3(qtiles, static_cast<unsigned>(plan.nseg)),
+                         dim3(NT), 0, stream, refs, prefix, n, ld, qs, qn, k, plan.seg_len,
+                         plan.nseg, prefix ? static_cast<const float*>(seed.as<float>()) : nullptr,
+                         static_cast<float*>(nullptr), cidx.as<uint32_t>(), ccnt.as<uint32_t>(),
+                         plan.cap);
+      SC_HIP(hipGetLastError());
+    }
+    {
+      ProfScope pscope("knn_mixed_rescore", stream);
+      if (prefix) {
+        hipLaunchKernelGGL((knn_rescore_kernel<float, kManhattan>), dim3(static_cast<unsigned>(qn)),
+                           dim3(64), 0, stream, refs, ld, qs, qn, 1.0, 1, cap_p,
+                           pidx.as<uint32_t>(), pcnt.as<uint32_t>(), ps.as<double>());
+        SC_HIP(hipGetLastError());
+      }
+      hipLaunchKernelGGL((knn_rescore_kernel<float, kManhattan>),
+                         dim3(static_cast<unsigned>(qn * plan.nseg)), dim3(64), 0, stream, refs, ld,
+                         qs, qn, 1.0, plan.nseg, plan.cap, cidx.as<uint32_t>(),
+                         ccnt.as<uint32_t>(), cs.as<double>());
+      SC_HIP(hipGetLastError());
+    }
+    {
+      ProfScope pscope("knn_replay", stream);
+      const CandSet pre = prefix ? CandSet{pidx.as<uint32_t>(), ps.as<double>(), pcnt.as<uint32_t>(),
+                                           1, cap_p}
+                                 : CandSet{nullptr, nullptr, nullptr, 0, 1};
+      hipLaunchKernelGGL((knn_replay_kernel<kManhattan>), dim3(static_cast<unsigned>((qn + 127) / 128)),
+                         dim3(128), 0, stream, qn, k, 1.0, pre, plan.nseg, plan.cap,
+                         cidx.as<uint32_t>(), cs.as<double>(), ccnt.as<uint32_t>(),
+                         d_idx + b0 * k, d_dist + b0 * k, d_count + b0, olist.as<uint32_t>(),
+                         overflow_n, total_cand);
+      SC_HIP(hipGetLastError());
+    }
+    {
+      ProfScope pscope("knn_serial", stream);
+      hipLaunchKernelGGL((knn_serial_kernel<float, kManhattan>), dim3(512), dim3(NT), 0, stream,
+                         refs, n, ld, qs, k, 1.0, olist.as<uint32_t>(), overflow_n, d_idx + b0 * k,
+                         d_dist + b0 * k, d_count + b0);
+      SC_HIP(hipGetLastError());
+    }
+    if (stats) {
+      uint32_t ov = 0;
+      SC_HIP(hipMemcpyAsync(&ov, overflow_n, 4, hipMemcpyDeviceToHost, stream));
+      SC_HIP(hipStreamSynchronize(stream));
+      stats->overflow_queries += ov;
+    }
+    SC_HIP(hipMemsetAsync(overflow_n, 0, 4, stream));
+  }
+  if (stats) {
+    unsigned long long tc = 0;
+    SC_HIP(hipMemcpyAsync(&tc, total_cand, 8, hipMemcpyDeviceToHost, stream));
+    SC_HIP(hipStreamSynchronize(stream));
+    stats->candidates += tc;
+    stats->shortlist_rescored += tc;
+    stats->queries += static_cast<uint64_t>(q);
+  }
+  return SC_OK;
+}
+
 template <typename T>
 int dispatch_metric(const sc_index* index, const void* d_queries, size_t q, size_t k,
                     sc_metric metric, int p, int64_t* d_idx, double* d_dist, uint32_t* d_count,
@@ -411,6 +715,17 @@ int launch_serial_exact_f32(const sc_index* index, const float* d_queries, int k
                      d_count);
   SC_HIP(hipGetLastError());
   return SC_OK;
+}
+
+int knn_l1_mixed_device(const sc_index* index, const void* d_queries, size_t q, size_t k,
+                        int64_t* d_idx, double* d_dist, uint32_t* d_count, hipStream_t stream,
+                        sc_stats* stats) {
+  if (q == 0) return SC_OK;
+  if (index->dtype != SC_F32)
+    return dispatch_metric<double>(index, d_queries, q, k, SC_MANHATTAN, 1, d_idx, d_dist, d_count,
+                                   stream, stats);
+  return run_l1_mixed(index, static_cast<const float*>(d_queries), static_cast<int64_t>(q),
+                      static_cast<int>(k), d_idx, d_dist, d_count, stream, stats);
 }
 
 int knn_exact_device(const sc_index* index, const void* d_queries, size_t q, size_t k,

@@ -522,9 +522,11 @@ __global__ __launch_bounds__(kNT, 2) void mixed_scan_kernel(const ScanArgs a) {
         cand_idx[sl] = static_cast<uint32_t>(row);
       }
       ++cnt;
-      // T - x2m rounded up (the f64 difference of two f32 values is exact)
+      // T - x2m rounded up: the f64 difference of two f32 values rounds
+      // when their exponents are far apart, so it is taken rounded toward
+      // +inf before the narrowing (also toward +inf)
       kcur = kth_of();
-      tcur = __double2float_ru(static_cast<double>(kcur) - xo);
+      tcur = __double2float_ru(__ocml_sub_rtp_f64(static_cast<double>(kcur), xo));
     }
     set_thr(tcur);
   };
@@ -650,9 +652,12 @@ __global__ __launch_bounds__(kNT, 2) void fixed_scan_kernel(const ScanArgs a) {
     for (int ks = 0; ks < KS; ++ks)
       bq[cb][ks] = load_frag(a.q_b + qr * S::LD + 16 * ks + 8 * half);
     // the seed's k-th value T; the epilogue threshold is T - x2m rounded up
+    // (the f64 difference rounded toward +inf, then narrowed toward +inf:
+    // at or above the exact T - x2m even when the f64 difference rounds)
     const float x2q = a.x2m[qr], tq = a.seed[qr * k + k - 1];
     thr[cb] = qvalid[cb]
-                  ? __double2float_ru(static_cast<double>(tq) - static_cast<double>(x2q))
+                  ? __double2float_ru(__ocml_sub_rtp_f64(static_cast<double>(tq),
+                                                         static_cast<double>(x2q)))
                   : INFINITY;
     const uint16_t b1 = bf16_ru_pos(a.qa1[qr]), b2 = bf16_ru_pos(a.qa2[qr]);
     const uint32_t one = 0x3f80u, n1 = 0x8000u | b1, n2 = 0x8000u | b2;

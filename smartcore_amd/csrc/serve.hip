@@ -306,6 +306,8 @@ int search_batch_device(const sc_index* index, const void* d_padded_queries, siz
                         double* d_dist, uint32_t* d_count, hipStream_t s) {
   if (mode == SC_MIXED && index->dtype == SC_F32 && metric == SC_EUCLIDIAN)
     return knn_mixed_device(index, d_padded_queries, q, k, d_idx, d_dist, d_count, s, nullptr);
+  if (mode == SC_MIXED && metric == SC_MANHATTAN)
+    return knn_l1_mixed_device(index, d_padded_queries, q, k, d_idx, d_dist, d_count, s, nullptr);
   return knn_exact_device(index, d_padded_queries, q, k, metric, p, d_idx, d_dist, d_count, s,
                           nullptr);
 }
@@ -353,8 +355,8 @@ int sc_server_submit(sc_server* sv, const void* queries, size_t q, size_t k, sc_
   // the checks LinearKNNSearch::find makes (linear_search.rs:53-58)
   if (k < 1 || k > ix->n) return fail(SC_FIND_FAILED, "k should be >= 1 and <= length(data)");
   if (metric == SC_MINKOWSKI && p < 1) return fail(SC_ERR_PANIC, "p must be at least 1");
-  if (mode == SC_MIXED && metric != SC_EUCLIDIAN)
-    return fail(SC_ERR_UNSUPPORTED, "SC_MIXED supports the Euclidian metric only");
+  if (mode == SC_MIXED && metric == SC_MINKOWSKI)
+    return fail(SC_ERR_UNSUPPORTED, "SC_MIXED supports the Euclidian and Manhattan metrics only");
   if (q == 0 || q > sv->cfg.max_batch_queries)
     return fail(SC_PARAMETERS_ERROR, "request must carry 1..max_batch_queries rows");
   auto r = std::make_shared<Request>();

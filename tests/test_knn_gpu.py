@@ -161,3 +161,50 @@ def test_classifier_vote_on_gpu_neighbours():
             a = O.classifier_predict_row(gi[r, :gc[r]], gd[r, :gc[r]], ycls, len(classes), w)
             b = O.classifier_predict_row(oi[r, :oc[r]], od[r, :oc[r]], ycls, len(classes), w)
             assert a == b
+
+
+@pytest.mark.parametrize("mode,metric", [(S.SC_PLAIN_FP64, S.SC_EUCLIDIAN), (S.SC_MIXED, S.SC_EUCLIDIAN),
+                                         (S.SC_MIXED, S.SC_MANHATTAN)])
+def test_batch_of_uneven_query_blocks_equals_single_searches(mode, metric):
+    """sc_knn_search_batch: blocks of 1..400 rows with k in {1, 5, 10, 20}
+    searched in one call give exactly what each gets alone."""
+    rng = np.random.default_rng(140)
+    ref = (rng.normal(size=(30_000, 40)) * rng.lognormal(0, 0.5, size=(30_000, 1))).astype(np.float32)
+    ix = S.Index(ref)
+    blocks = []
+    for _ in range(25):
+        q = int(round(np.exp(rng.uniform(0, np.log(400)))))
+        blocks.append((rng.normal(size=(q, 40)).astype(np.float32), int(rng.choice([1, 5, 10, 20]))))
+    p = 2 if metric == S.SC_EUCLIDIAN else 1
+    got = ix.knn_batch(blocks, metric, p, mode)
+    for (qs, k), (gi, gd, gc) in zip(blocks, got):
+        ai, ad, ac = ix.knn(qs, k, metric, p, mode)
+        np.testing.assert_array_equal(gi, ai)
+        np.testing.assert_array_equal(gd.view(np.uint64), ad.view(np.uint64))
+        np.testing.assert_array_equal(gc, ac)
+    with pytest.raises(S.Failed):
+        ix.knn_batch([(blocks[0][0], 5), (blocks[1][0], 0)], metric, p, mode)
+
+
+def test_device_allocation_failure_is_a_clean_error_and_the_library_recovers():
+    """A host-forced device fault: a call whose query buffer (2^40 rows)
+    cannot be reserved fails with SC_ERR_OUT_OF_MEMORY before it reads any
+    query, and the next calls on the same index and thread work."""
+    import ctypes as C
+
+    rng = np.random.default_rng(150)
+    ref = rng.normal(size=(2_000, 8))
+    qs = rng.normal(size=(3, 8))
+    ix = S.Index(ref)
+    idx = np.zeros(8, np.int64)
+    dist = np.zeros(8)
+    cnt = np.zeros(1, np.uint32)
+    rc = S.lib().sc_knn_search(ix.handle, qs.ctypes.data_as(C.c_void_p), 1 << 40, 5, S.SC_EUCLIDIAN,
+                               2, S.SC_PLAIN_FP64, idx.ctypes.data_as(C.c_void_p),
+                               dist.ctypes.data_as(C.c_void_p), cnt.ctypes.data_as(C.c_void_p))
+    assert rc == S.SC_ERR_OUT_OF_MEMORY
+    assert "HIP error" in S.last_error()
+    for mode in (S.SC_PLAIN_FP64, S.SC_MIXED):
+        gi, gd, gc = ix.knn(qs, 5, S.SC_EUCLIDIAN, 2, mode)
+        oi, od, oc = O.linear_find_batch(ref, qs, 5)
+        np.testing.assert_array_equal(gi, oi)

@@ -78,10 +78,57 @@ def test_mixed_large_k_and_f64_index_use_exact_form():
     check(ref.astype(np.float64), qs.astype(np.float64), 7)
 
 
-def test_mixed_rejects_other_metrics():
+def test_mixed_rejects_minkowski():
     ix = S.Index(np.ones((4, 3), np.float32))
     with pytest.raises(S.Failed):
-        ix.knn(np.ones((1, 3), np.float32), 2, S.SC_MANHATTAN, 1, S.SC_MIXED)
+        ix.knn(np.ones((1, 3), np.float32), 2, S.SC_MINKOWSKI, 3, S.SC_MIXED)
+
+
+def check_l1(ref, qs, k):
+    """SC_MIXED Manhattan (f32 sums with a rigorous bound, candidates rescored
+    in the reference arithmetic) against the oracle and the FP64 form."""
+    ix = S.Index(ref)
+    gi, gd, gc = ix.knn(qs, k, S.SC_MANHATTAN, 1, S.SC_MIXED)
+    st = S.last_stats()
+    fi, fd, fc = ix.knn(qs, k, S.SC_MANHATTAN, 1, S.SC_PLAIN_FP64)
+    np.testing.assert_array_equal(gc, fc)
+    np.testing.assert_array_equal(gi, fi)
+    np.testing.assert_array_equal(gd.view(np.uint64), fd.view(np.uint64))
+    oi, od, oc = O.linear_find_batch(ref, qs[:5], k, O.MANHATTAN)
+    for r in range(min(5, qs.shape[0])):
+        c = int(oc[r])
+        np.testing.assert_array_equal(gi[r, :c], oi[r, :c])
+        np.testing.assert_array_equal(gd[r, :c].view(np.uint64), od[r, :c].view(np.uint64))
+    return st
+
+
+@pytest.mark.parametrize("n,d,k", [(40_000, 96, 20), (12_000, 16, 1), (9_000, 33, 64),
+                                   (300_000, 96, 20)])
+def test_mixed_manhattan_equals_fp64_form(n, d, k):
+    ref = synth.lognormal_rows(120 + d, n, d)
+    qs = synth.lognormal_rows(121 + d, 150, d)
+    st = check_l1(ref, qs, k)
+    assert st["shortlist_rescored"] >= 150 * k
+
+
+def test_mixed_manhattan_ties_non_finite_and_fallbacks():
+    rng = np.random.default_rng(125)
+    # integer rows: many exact ties, heap order decides
+    ref = rng.integers(0, 3, size=(20_000, 12)).astype(np.float32)
+    qs = rng.integers(0, 3, size=(200, 12)).astype(np.float32)
+    for k in (1, 7, 33):
+        check_l1(ref, qs, k)
+    # NaN / inf rows and a huge-valued query
+    ref = rng.normal(size=(5_000, 20)).astype(np.float32)
+    ref[10, 3] = np.nan
+    ref[11, 0] = np.inf
+    ref[12, :] = 3e38
+    qs = rng.normal(size=(30, 20)).astype(np.float32)
+    qs[4, :] = -3e38
+    check_l1(ref, qs, 9)
+    # k above the tiled limit and f64 rows take the exact form
+    check_l1(ref, qs, 70)
+    check_l1(ref.astype(np.float64), qs.astype(np.float64), 9)
 
 
 @pytest.mark.parametrize("d,k,kind", [(16, 10, "lognormal"), (48, 32, "ties"), (16, 1, "lognormal"),

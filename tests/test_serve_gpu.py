@@ -26,7 +26,9 @@ def test_results_same_alone_and_in_the_mix():
     for i in range(40):
         q = int(round(np.exp(rng.uniform(0, np.log(600)))))
         metric = S.SC_MANHATTAN if i % 5 == 0 else S.SC_EUCLIDIAN
-        mode = S.SC_PLAIN_FP64 if metric == S.SC_MANHATTAN else S.SC_MIXED
+        # Manhattan requests alternate between the bounded f32 form and the
+        # FP64 form
+        mode = S.SC_PLAIN_FP64 if metric == S.SC_MANHATTAN and i % 10 == 0 else S.SC_MIXED
         reqs.append((rng.normal(size=(q, 96)).astype(np.float32), metric, mode))
     results = {}
 
@@ -52,9 +54,14 @@ def test_results_same_alone_and_in_the_mix():
         np.testing.assert_array_equal(gd.view(np.uint64), ad.view(np.uint64))
         np.testing.assert_array_equal(gc, ac)
         assert lat > 0.0
-    # and the oracle on a few rows
+    # and the oracle on a few rows of a Euclidean and a mixed Manhattan
+    # request from the mix
     oi, od, oc = O.linear_find_batch(ref, reqs[1][0][:3], 20)
     np.testing.assert_array_equal(results[1][0][:3], oi)
+    assert reqs[5][1] == S.SC_MANHATTAN and reqs[5][2] == S.SC_MIXED
+    oi, od, oc = O.linear_find_batch(ref, reqs[5][0][:3], 20, O.MANHATTAN)
+    np.testing.assert_array_equal(results[5][0][:3], oi)
+    np.testing.assert_array_equal(results[5][1][:3].view(np.uint64), od.view(np.uint64))
 
 
 def test_admission_queues_under_a_small_budget():
