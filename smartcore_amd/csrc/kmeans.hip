@@ -19,9 +19,13 @@
 //             centroid_i = sums_i / size_i when size_i > 0
 //             (kmeans.rs:297-303); the last workgroup of a problem adds the
 //             terms into the distortion
-//               D = sum_i ||x_i||^2 - 2 sum_c c.S_c + sum_c n_c ||c||^2
+//               D = sum_i ||x_i - mu||^2 - 2 sum_c (c - mu).(S_c - n_c mu)
+//                   + sum_c n_c ||c - mu||^2
 //             (the cost BBDTree::clustering returns, bbd_tree.rs:162 and
-//             :297-305, is the same quantity) and runs the stop rule
+//             :297-305, is the same quantity, sum_i ||x_i - c_i||^2; the
+//             shift mu, the first row, keeps the three sums near the data's
+//             own spread, so rows far from the origin do not turn D into a
+//             difference of large numbers) and runs the stop rule
 //             `if distortion <= dist break` (kmeans.rs:305-309) on the
 //             device; later kernels return at once, so a fit needs one host
 //             synchronisation.
@@ -138,10 +142,21 @@ __global__ __launch_bounds__(NT) void kmeans_assign_kernel(const T* __restrict__
   }
 }
 
-// Per block: sum of ||x||^2 over its rows (each row a sequential f64 fold,
-// 8 consecutive rows per thread, then a fixed tree).
+// The shift mu of the distortion sums: the first row, widened to f64, with
+// non-finite values replaced by 0.
+template <typename T>
+__global__ void mu_kernel(const T* __restrict__ rows, int ld, double* __restrict__ mu) {
+  for (int j = threadIdx.x; j < ld; j += blockDim.x) {
+    const double v = static_cast<double>(rows[j]);
+    mu[j] = isfinite(v) ? v : 0.0;
+  }
+}
+
+// Per block: sum of ||x - mu||^2 over its rows (each row a sequential f64
+// fold, 8 consecutive rows per thread, then a fixed tree).
 template <typename T>
 __global__ __launch_bounds__(NT) void x2_block_kernel(const T* __restrict__ rows, int64_t n, int ld,
+                                                      const double* __restrict__ mu,
                                                       double* __restrict__ x2block) {
   __shared__ double red[NT];
   const int tid = threadIdx.x;
@@ -152,7 +167,7 @@ __global__ __launch_bounds__(NT) void x2_block_kernel(const T* __restrict__ rows
     if (r < n) {
       double t = 0.0;
       for (int c = 0; c < ld; ++c) {
-        const double v = static_cast<double>(rows[r * ld + c]);
+        const double v = static_cast<double>(rows[r * ld + c]) - mu[c];
         t = t + v * v;
       }
       s = s + t;
@@ -392,8 +407,11 @@ __global__ __launch_bounds__(128) void kmeans_final_kernel(const KmProb* __restr
       to_global(p.sums)[static_cast<int64_t>(c) * d + j] = S;
       const double cj = to_global(p.cent)[static_cast<int64_t>(c) * p.ld + j];
       const double nc = upd ? S / static_cast<double>(n) : cj;
-      s_dot[tid] = cj * S;
-      s_c2[tid] = cj * cj;
+      const double mj = to_global(p.mu)[j];
+      const double dc = cj - mj;
+      const double dS = S - static_cast<double>(n) * mj;
+      s_dot[tid] = dc * dS;
+      s_c2[tid] = dc * dc;
       s_n2[tid] = nc * nc;
       if (upd) to_global(p.cent)[static_cast<int64_t>(c) * p.ld + j] = nc;
       if (tables) {
@@ -563,15 +581,26 @@ int kmeans_assign_launch(const sc_index* ix, const void* rows, int64_t n, const 
 
 int kmeans_x2_launch(const sc_index* ix, KmWork& w, hipStream_t s) {
   const int64_t n = static_cast<int64_t>(ix->n);
+  const int ld = static_cast<int>(ix->ld);
+  if (n > 0) {
+    if (ix->dtype == SC_F64)
+      hipLaunchKernelGGL(mu_kernel<double>, dim3(1), dim3(NT), 0, s,
+                         static_cast<const double*>(ix->rows), ld, w.p.mu);
+    else
+      hipLaunchKernelGGL(mu_kernel<float>, dim3(1), dim3(NT), 0, s,
+                         static_cast<const float*>(ix->rows), ld, w.p.mu);
+  } else {
+    hipLaunchKernelGGL(fill_kernel, dim3(static_cast<unsigned>((ld + NT - 1) / NT)), dim3(NT), 0,
+                       s, w.p.mu, static_cast<int64_t>(ld), 0.0);
+  }
+  SC_HIP(hipGetLastError());
   if (w.p.nb > 0) {
     if (ix->dtype == SC_F64)
       hipLaunchKernelGGL(x2_block_kernel<double>, dim3(static_cast<unsigned>(w.p.nb)), dim3(NT),
-                         0, s, static_cast<const double*>(ix->rows), n, static_cast<int>(ix->ld),
-                         w.p.x2block);
+                         0, s, static_cast<const double*>(ix->rows), n, ld, w.p.mu, w.p.x2block);
     else
       hipLaunchKernelGGL(x2_block_kernel<float>, dim3(static_cast<unsigned>(w.p.nb)), dim3(NT),
-                         0, s, static_cast<const float*>(ix->rows), n, static_cast<int>(ix->ld),
-                         w.p.x2block);
+                         0, s, static_cast<const float*>(ix->rows), n, ld, w.p.mu, w.p.x2block);
     SC_HIP(hipGetLastError());
   }
   hipLaunchKernelGGL(x2_total_kernel, dim3(1), dim3(1), 0, s, w.p.x2block, w.p.nb, w.p.st);
@@ -681,8 +710,8 @@ int KmWork::alloc(const sc_index* ix, int k, const double* d_cent, bool mixed, h
   SC_HIP(b_gcount.alloc(sizeof(uint64_t) * ng1 * k, s));
   SC_HIP(b_sums.alloc(sizeof(double) * k * d1, s));
   SC_HIP(b_counts.alloc(sizeof(uint64_t) * k, s));
-  // x2 per block | k distortion terms | k centroid norms (MFMA form)
-  SC_HIP(b_x2.alloc(sizeof(double) * (nb1 + 2 * static_cast<int64_t>(k)), s));
+  // x2 per block | k distortion terms | k centroid norms (MFMA form) | mu
+  SC_HIP(b_x2.alloc(sizeof(double) * (nb1 + 2 * static_cast<int64_t>(k) + p.ld), s));
   // the state, then the arrival counter
   SC_HIP(b_state.alloc(sizeof(KmState) + 16, s));
   SC_HIP(b_prob.alloc(sizeof(KmProb), s));
@@ -697,6 +726,7 @@ int KmWork::alloc(const sc_index* ix, int k, const double* d_cent, bool mixed, h
   p.counts = b_counts.as<uint64_t>();
   p.x2block = b_x2.as<double>();
   p.term = p.x2block + nb1;
+  p.mu = p.term + 2 * static_cast<int64_t>(k);
   p.st = b_state.as<KmState>();
   p.arrive = reinterpret_cast<uint32_t*>(b_state.as<char>() + sizeof(KmState));
   SC_HIP(hipMemsetAsync(p.arrive, 0, 4, s));
@@ -729,11 +759,12 @@ using namespace sc;
 namespace {
 
 int upload_centroids(const double* host, int k, size_t d, size_t ld, double* dev, hipStream_t s) {
-  SC_HIP(hipMemsetAsync(dev, 0, sizeof(double) * k * ld, s));
-  if (d > 0)
-    SC_HIP(hipMemcpy2DAsync(dev, ld * sizeof(double), host, d * sizeof(double), d * sizeof(double),
-                            k, hipMemcpyHostToDevice, s));
-  return SC_OK;
+  if (d == 0) {
+    SC_HIP(hipMemsetAsync(dev, 0, sizeof(double) * k * ld, s));
+    return SC_OK;
+  }
+  return upload_staged_rows(dev, host, static_cast<size_t>(k), d * sizeof(double),
+                            ld * sizeof(double), s);
 }
 
 // One Lloyd step of the chosen form into w (labels, sums, counts, st->dist;
@@ -763,14 +794,42 @@ int kmeans_run_impl(const sc_index* index, int k, size_t max_iter, bool mixed, d
   if (int rc = kmeans_x2_launch(index, w, s)) return rc;
   if (mixed)
     if (int rc = kmeans_tables_launch(w.dprob, 1, s)) return rc;
-  for (size_t it = 0; it < max_iter; ++it)
-    if (int rc = lloyd_step(index, w, mixed, true, true, s)) return rc;
+  // The max_iter steps (each a step launch and two reduce launches, all
+  // returning at once after the stop rule fires) are captured in one hipGraph
+  // and launched once: the graph removes the per-launch gaps between the
+  // dependent kernels.  With the tracing hooks on, the steps are launched
+  // one by one instead, so that every launch is timed.
+  if (profiling_enabled()) {
+    for (size_t it = 0; it < max_iter; ++it)
+      if (int rc = lloyd_step(index, w, mixed, true, true, s)) return rc;
+  } else {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    SC_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    int crc = SC_OK;
+    for (size_t it = 0; it < max_iter && crc == SC_OK; ++it)
+      crc = lloyd_step(index, w, mixed, true, true, s);
+    hipError_t ce = hipStreamEndCapture(s, &graph);
+    if (crc) {
+      if (graph) (void)hipGraphDestroy(graph);
+      return crc;
+    }
+    if (ce != hipSuccess) return hip_fail(ce, "hipStreamEndCapture");
+    ce = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (ce != hipSuccess) return hip_fail(ce, "hipGraphInstantiate");
+    ce = hipGraphLaunch(exec, s);
+    const hipError_t se = hipStreamSynchronize(s);
+    (void)hipGraphExecDestroy(exec);
+    if (ce != hipSuccess) return hip_fail(ce, "hipGraphLaunch");
+    if (se != hipSuccess) return hip_fail(se, "hipStreamSynchronize");
+  }
   KmState hs{};
   if (index->d > 0)
     SC_HIP(hipMemcpy2DAsync(centroids, index->d * sizeof(double), cent.p,
                             index->ld * sizeof(double), index->d * sizeof(double), k,
                             hipMemcpyDeviceToHost, s));
-  SC_HIP(hipMemcpyAsync(labels, w.p.labels, sizeof(uint32_t) * index->n, hipMemcpyDeviceToHost, s));
+  if (int rc = download_staged(labels, w.p.labels, sizeof(uint32_t) * index->n, s)) return rc;
   SC_HIP(hipMemcpyAsync(size, w.p.counts, sizeof(uint64_t) * k, hipMemcpyDeviceToHost, s));
   SC_HIP(hipMemcpyAsync(&hs, w.p.st, sizeof(KmState), hipMemcpyDeviceToHost, s));
   SC_HIP(hipStreamSynchronize(s));
@@ -812,7 +871,7 @@ int sc_kmeans_step(const sc_index* index, const double* centroids, size_t k, dou
   if (index->d > 0)
     SC_HIP(hipMemcpyAsync(sums, w.p.sums, sizeof(double) * k * index->d, hipMemcpyDeviceToHost, s));
   SC_HIP(hipMemcpyAsync(counts, w.p.counts, sizeof(uint64_t) * k, hipMemcpyDeviceToHost, s));
-  SC_HIP(hipMemcpyAsync(membership, w.p.labels, sizeof(uint32_t) * index->n, hipMemcpyDeviceToHost, s));
+  if (int rc = download_staged(membership, w.p.labels, sizeof(uint32_t) * index->n, s)) return rc;
   SC_HIP(hipMemcpyAsync(&hs, w.p.st, sizeof(KmState), hipMemcpyDeviceToHost, s));
   SC_HIP(hipStreamSynchronize(s));
   *distortion = hs.dist;
@@ -870,7 +929,7 @@ int sc_kmeans_predict(const void* x, sc_dtype dtype, size_t n, size_t d, const d
   if (int rc = kmeans_assign_launch(&tmp, rows.p, static_cast<int64_t>(n), cent.as<double>(),
                                     static_cast<int>(k), lab.as<uint32_t>(), nullptr, s))
     return rc;
-  SC_HIP(hipMemcpyAsync(labels, lab.p, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, s));
+  if (int rc = download_staged(labels, lab.p, sizeof(uint32_t) * n, s)) return rc;
   SC_HIP(hipStreamSynchronize(s));
   return SC_OK;
 }

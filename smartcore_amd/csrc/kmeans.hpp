@@ -28,7 +28,7 @@ constexpr int kReduceChunk = 256;
 struct KmState {
   double distortion;  // best distortion so far (f64::MAX at start)
   double dist;        // distortion of the last step
-  double x2total;     // sum over rows of ||x||^2 (fixed order)
+  double x2total;     // sum over rows of ||x - mu||^2 (fixed order)
   uint32_t iters;     // steps executed
   uint32_t done;      // stop rule fired
   unsigned long long fallback;  // rows reassigned in FP64 by the certified MFMA form
@@ -51,6 +51,8 @@ struct KmProb {
   uint64_t* counts;     // k
   double* term;         // k distortion terms of the last step
   double* x2block;      // nb
+  double* mu;           // ld: the shift of the distortion sums (the first row,
+                        // non-finite values replaced by 0)
   KmState* st;
   uint32_t* arrive;     // arrival counter of the final reduce kernel
   // MFMA form only (null otherwise): the terms of the current centroids
@@ -85,14 +87,17 @@ struct KmWork {
 };
 
 int kmeans_state_init(KmState* st, hipStream_t s);
-// Sum over the index's rows of ||x||^2 in f64 into st->x2total.
+// The shift point mu (the first row, widened) and the sum over the index's
+// rows of ||x - mu||^2 in f64 into st->x2total.
 int kmeans_x2_launch(const sc_index* ix, KmWork& w, hipStream_t s);
 int kmeans_assign_launch(const sc_index* ix, const void* rows, int64_t n, const double* d_cent,
                          int k, uint32_t* d_labels, const KmState* st, hipStream_t s);
 // Block partial sums of the rows by label (FP64 form).
 int kmeans_partial_launch(const sc_index* ix, KmWork& w, int k, const KmState* st, hipStream_t s);
-// Group and final sums, counts, distortion from the aggregates
-//   D = sum_i ||x_i||^2 - 2 sum_c c.S_c + sum_c n_c ||c||^2,
+// Group and final sums, counts, distortion from the aggregates, shifted by
+// mu so that data far from the origin lose no precision to cancellation:
+//   D = sum_i ||x_i - mu||^2 - 2 sum_c (c - mu).(S_c - n_c mu)
+//       + sum_c n_c ||c - mu||^2,
 // and (update) centroid_i = sums_i / size_i for size_i > 0, (control) the
 // stop rule, and the MFMA terms of the new centroids when the problem has
 // them.  `probs` is a device array of `count` problems.

@@ -53,6 +53,7 @@
 #include <algorithm>
 #include <cfloat>
 #include <cmath>
+#include <cstring>
 #include <map>
 #include <mutex>
 #include <string>
@@ -78,12 +79,29 @@ constexpr double kKappa = 9.5367431640625e-07;   // 2^-20
 constexpr size_t kLdsBudget = 150 * 1024;         // dynamic LDS one workgroup may take
 
 // Shape class of the step kernel: k <= 32 KBM, padded dimension <= 16 KSM.
+// Fixed LDS of a shape class at a given number of waves: stage | lab | cnt |
+// c2 | A tables (hi | mid | lo).
+constexpr size_t class_fixed_lds(int kbm, int ksm, int waves) {
+  return sizeof(float) * (32 * waves) * (16 * ksm + 4) + sizeof(uint32_t) * (32 * waves) +
+         sizeof(uint32_t) * (32 * kbm) + sizeof(float) * (32 * kbm) +
+         3 * sizeof(uint16_t) * (32 * kbm) * (16 * ksm + 8);
+}
+// Two four-wave workgroups per CU when both fit its LDS (the sums of one
+// overlap the MFMA phases of the other); otherwise one eight-wave workgroup
+// (four waves for the widest rows, whose stage and tables take most of it).
+constexpr size_t kTwoGroupLds = 78 * 1024;
+constexpr int class_waves(int kbm, int ksm) {
+  return ksm == 8 ? 4 : (class_fixed_lds(kbm, ksm, 4) <= kTwoGroupLds ? 4 : 8);
+}
+
 template <int KBM, int KSM>
 struct Cfg {
   static constexpr int LD = 16 * KSM;                    // largest padded dimension
   static constexpr int LDP = LD + 4;                     // stage row stride (floats)
   static constexpr int KP = 32 * KBM;                    // largest padded k
-  static constexpr int WAVES = KSM == 8 ? 4 : 8;         // waves per workgroup
+  static constexpr int WAVES = class_waves(KBM, KSM);    // waves per workgroup
+  static constexpr bool TWO_GROUPS = KSM != 8 && WAVES == 4;  // two workgroups per CU
+  static constexpr int MIN_WAVES_PER_EU = TWO_GROUPS ? 2 : WAVES / 4;
   static constexpr int NTH = 64 * WAVES;                 // threads per workgroup
   static constexpr int CR = 32 * WAVES;                  // rows per chunk
   static constexpr int SLOTS = (KP + WAVES - 1) / WAVES; // clusters one wave owns
@@ -95,6 +113,7 @@ struct Cfg {
   static constexpr size_t ATAB = sizeof(uint16_t) * KP * LDA;
   static constexpr size_t FIXED = STAGE + sizeof(uint32_t) * CR + sizeof(uint32_t) * KP +
                                   sizeof(float) * KP + 3 * ATAB;
+  static_assert(FIXED == class_fixed_lds(KBM, KSM, WAVES), "LDS layout");
   static_assert(kBlockRows % CR == 0, "a block is a whole number of chunks");
   static_assert(CR % 64 == 0, "chunks are whole ballots");
 };
@@ -546,7 +565,7 @@ __device__ __forceinline__ void km_block(const KmProb& p, int64_t b, unsigned ch
 // Persistent step kernel: items are (problem, block) pairs of one shape
 // class; with items == nullptr the launch covers every block of problem 0.
 template <int KBM, int KSM, bool EXACT>
-__global__ __launch_bounds__((Cfg<KBM, KSM>::NTH), (Cfg<KBM, KSM>::WAVES / 4)) void km_step_kernel(
+__global__ __launch_bounds__((Cfg<KBM, KSM>::NTH), (Cfg<KBM, KSM>::MIN_WAVES_PER_EU)) void km_step_kernel(
     const KmProb* __restrict__ P, const uint2* __restrict__ items, int64_t nitems, int ctab_lds) {
   extern __shared__ __align__(16) unsigned char smem[];
   unsigned long long fb = 0;
@@ -575,7 +594,9 @@ template <int KBM, int KSM>
 size_t lds_bytes(int k, int ld, bool* ctab_lds) {
   const size_t fixed = Cfg<KBM, KSM>::FIXED;
   const size_t ctab = sizeof(double) * static_cast<size_t>(k) * ld;
-  *ctab_lds = fixed + ctab <= kLdsBudget;
+  // the f64 centroid copy goes to LDS only when the workgroups per CU still
+  // fit
+  *ctab_lds = fixed + ctab <= (Cfg<KBM, KSM>::TWO_GROUPS ? kTwoGroupLds : kLdsBudget);
   return fixed + (*ctab_lds ? ctab : 0);
 }
 
@@ -668,7 +689,19 @@ __global__ void kmb_init_kernel(const KmProb* __restrict__ P, int count) {
   *P[i].arrive = 0u;
 }
 
-// ||x||^2 per (problem, block): same order as the single-problem x2 kernel.
+// The shift mu of every problem: its first row, non-finite values as 0 (as
+// the single-problem mu kernel).
+__global__ void kmb_mu_kernel(const KmProb* __restrict__ P, int count) {
+  const KmProb p = P[blockIdx.x];
+  const float* rows = static_cast<const float*>(p.rows);
+  for (int j = threadIdx.x; j < p.ld; j += blockDim.x) {
+    const double v = p.n > 0 ? static_cast<double>(rows[j]) : 0.0;
+    p.mu[j] = isfinite(v) ? v : 0.0;
+  }
+}
+
+// ||x - mu||^2 per (problem, block): same order as the single-problem x2
+// kernel.
 __global__ __launch_bounds__(NT) void kmb_x2_kernel(const KmProb* __restrict__ P,
                                                     const uint2* __restrict__ items, int64_t nitems) {
   __shared__ double red[NT];
@@ -684,7 +717,7 @@ __global__ __launch_bounds__(NT) void kmb_x2_kernel(const KmProb* __restrict__ P
       if (r < p.n) {
         double t = 0.0;
         for (int c = 0; c < p.ld; ++c) {
-          const double v = static_cast<double>(rows[r * p.ld + c]);
+          const double v = static_cast<double>(rows[r * p.ld + c]) - p.mu[c];
           t = t + v * v;
         }
         s = s + t;
@@ -805,7 +838,7 @@ extern "C" int sc_kmeans_batch_run(const sc_kmeans_problem* problems, size_t cou
     o.gc = tot.gc;      tot.gc += p.ng * p.k;
     o.sum = tot.sum;    tot.sum += kd;
     o.cnt = tot.cnt;    tot.cnt += p.k;
-    o.x2 = tot.x2;      tot.x2 += p.nb + 2 * static_cast<int64_t>(p.k);
+    o.x2 = tot.x2;      tot.x2 += p.nb + 2 * static_cast<int64_t>(p.k) + p.ld;
     o.tab = tot.tab;    tot.tab += static_cast<int64_t>(p.kpad) * p.ld;
     o.c2f = tot.c2f;    tot.c2f += p.kpad + 4;
   }
@@ -843,6 +876,7 @@ extern "C" int sc_kmeans_batch_run(const sc_kmeans_problem* problems, size_t cou
     p.x2block = bX2.as<double>() + o.x2;
     p.term = p.x2block + p.nb;
     p.c2d = p.term + p.k;
+    p.mu = p.c2d + p.k;
     p.st = reinterpret_cast<KmState*>(bSt.as<char>() + st_stride * i);
     p.arrive = reinterpret_cast<uint32_t*>(bSt.as<char>() + st_stride * i + sizeof(KmState));
     p.bhi = bHi.as<uint16_t>() + o.tab;
@@ -851,15 +885,20 @@ extern "C" int sc_kmeans_batch_run(const sc_kmeans_problem* problems, size_t cou
     p.c2f = bC2f.as<float>() + o.c2f;
     p.cmax = p.c2f + p.kpad;
   }
-  SC_HIP(hipMemcpyAsync(bP.p, P.data(), sizeof(KmProb) * count, hipMemcpyHostToDevice, s));
-  SC_HIP(hipMemsetAsync(bCent.p, 0, sizeof(double) * tot.cent, s));
+  if (int rc = upload_staged(bP.p, P.data(), sizeof(KmProb) * count, s)) return rc;
+  // the initial centroids of every problem, padded on the host, in one copy
+  std::vector<double> hcent0(static_cast<size_t>(tot.cent), 0.0);
+  for (size_t i = 0; i < count; ++i) {
+    const sc_kmeans_problem& pr = problems[i];
+    for (size_t c = 0; c < pr.k; ++c)
+      std::copy(pr.init_centroids + c * pr.d, pr.init_centroids + (c + 1) * pr.d,
+                hcent0.data() + off[i].cent + c * P[i].ld);
+  }
+  if (int rc = upload_staged(bCent.p, hcent0.data(), sizeof(double) * tot.cent, s)) return rc;
   for (size_t i = 0; i < count; ++i) {
     const sc_kmeans_problem& pr = problems[i];
     if (int rc = upload_padded(pr.rows, SC_F32, pr.n, pr.d, P[i].ld, const_cast<void*>(P[i].rows), s))
       return rc;
-    SC_HIP(hipMemcpy2DAsync(P[i].cent, P[i].ld * sizeof(double), pr.init_centroids,
-                            pr.d * sizeof(double), pr.d * sizeof(double), pr.k,
-                            hipMemcpyHostToDevice, s));
   }
   const KmProb* dP = bP.as<KmProb>();
 
@@ -899,6 +938,8 @@ extern "C" int sc_kmeans_batch_run(const sc_kmeans_problem* problems, size_t cou
 
   const unsigned cgrid = static_cast<unsigned>((count + 255) / 256);
   hipLaunchKernelGGL(kmb_init_kernel, dim3(cgrid), dim3(256), 0, s, dP, static_cast<int>(count));
+  hipLaunchKernelGGL(kmb_mu_kernel, dim3(static_cast<unsigned>(count)), dim3(128), 0, s, dP,
+                     static_cast<int>(count));
   hipLaunchKernelGGL(kmb_x2_kernel, dim3(1024), dim3(NT), 0, s, dP, bAll.as<uint2>(),
                      static_cast<int64_t>(all.size()));
   hipLaunchKernelGGL(kmb_x2_total_kernel, dim3(cgrid), dim3(256), 0, s, dP, static_cast<int>(count));
@@ -1019,24 +1060,27 @@ extern "C" int sc_kmeans_batch_run(const sc_kmeans_problem* problems, size_t cou
     if (se != hipSuccess) return hip_fail(se, "hipStreamSynchronize");
   }
   // ---- results ----
+  // the states sit st_stride apart in one arena: one copy for all
+  std::vector<char> hst(st_stride * count);
+  if (int rc = download_staged(hst.data(), bSt.p, st_stride * count, s)) return rc;
   std::vector<KmState> hs(count);
-  for (size_t i = 0; i < count; ++i)
-    SC_HIP(hipMemcpyAsync(&hs[i], P[i].st, sizeof(KmState), hipMemcpyDeviceToHost, s));
-  size_t co = 0, lo = 0, so = 0;
+  for (size_t i = 0; i < count; ++i) std::memcpy(&hs[i], hst.data() + st_stride * i, sizeof(KmState));
+  // the label and size arenas hold the problems in order, like the outputs;
+  // the centroid arena is copied whole and its row padding dropped here
+  if (int rc = download_staged(labels_out, bLab.p, sizeof(uint32_t) * tot.lab, s)) return rc;
+  if (int rc = download_staged(sizes_out, bCnt.p, sizeof(uint64_t) * tot.cnt, s)) return rc;
+  std::vector<double> hcent(static_cast<size_t>(tot.cent));
+  if (int rc = download_staged(hcent.data(), bCent.p, sizeof(double) * tot.cent, s)) return rc;
+  SC_HIP(hipStreamSynchronize(s));
+  size_t co = 0;
   for (size_t i = 0; i < count; ++i) {
     const KmProb& p = P[i];
-    SC_HIP(hipMemcpy2DAsync(centroids_out + co, p.d * sizeof(double), p.cent,
-                            p.ld * sizeof(double), p.d * sizeof(double), p.k,
-                            hipMemcpyDeviceToHost, s));
-    SC_HIP(hipMemcpyAsync(labels_out + lo, p.labels, sizeof(uint32_t) * p.n,
-                          hipMemcpyDeviceToHost, s));
-    SC_HIP(hipMemcpyAsync(sizes_out + so, p.counts, sizeof(uint64_t) * p.k,
-                          hipMemcpyDeviceToHost, s));
+    const double* src = hcent.data() + off[i].cent;
+    for (int c = 0; c < p.k; ++c)
+      std::copy(src + static_cast<size_t>(c) * p.ld, src + static_cast<size_t>(c) * p.ld + p.d,
+                centroids_out + co + static_cast<size_t>(c) * p.d);
     co += static_cast<size_t>(p.k) * p.d;
-    lo += p.n;
-    so += p.k;
   }
-  SC_HIP(hipStreamSynchronize(s));
   uint64_t fb = 0;
   for (size_t i = 0; i < count; ++i) {
     distortion_out[i] = hs[i].distortion;

@@ -80,6 +80,22 @@ def test_kmeans_fit_predict_self_consistent():
 
 
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_kmeans_plus_plus_equals_oracle(dtype):
+    """sc_kmeans_plus_plus draws the same stream as the oracle's restatement of
+    rand 0.8.5's SmallRng (seed_from_u64 = rand_core's PCG32 fill) and takes
+    the same D^2-sampling walk, for a given seed and for None (seed 0)."""
+    from smartcore_amd import _capi as S
+
+    x, _, _ = synth.kmeans_blobs(seed=14, n=20_000, d=12, blobs=9, k=9, dtype=dtype)
+    ix = S.Index(x)
+    try:
+        for seed, k in ((None, 9), (7, 9), (123456789, 16), (2, 2)):
+            np.testing.assert_array_equal(ix.kmeans_plus_plus(k, seed), O.kmeans_plus_plus(x, k, seed))
+    finally:
+        ix.close()
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
 def test_kmeans_fit_with_centroids_vs_oracle(dtype):
     x, _, init = synth.kmeans_blobs(seed=13, n=30_000, d=16, blobs=12, k=12, dtype=dtype)
     p = sc.KMeansParameters().with_k(12).with_max_iter(25)

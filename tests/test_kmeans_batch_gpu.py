@@ -5,6 +5,7 @@ the FP64 form's."""
 import numpy as np
 import pytest
 
+import oracle as O
 from smartcore_amd import _capi as S
 from smartcore_amd import synth
 
@@ -72,3 +73,25 @@ def test_batch_with_a_problem_outside_the_shape_classes():
         assert it == i1
         c2, l2, s2, d2, i2 = single(x, init, 4, S.SC_PLAIN_FP64)
         np.testing.assert_array_equal(lab, l2)
+
+
+def test_batch_config4_problems_vs_bbd_oracle():
+    """Sixteen problems drawn like the 512-problem workload (n log-uniform in
+    [1,000, 200,000], d in {8, 16, 32, 64}, k in 4..128) plus three with n
+    near 200,000, 20 iterations in one batched call: every problem's labels,
+    sizes and iteration count equal the BBD-tree oracle's fit, centroids to
+    1e-12 relative."""
+    probs = synth.kmeans_batch_problems(seed=60, count=16)
+    probs += synth.kmeans_batch_problems(seed=61, count=3, n_lo=180_000, n_hi=200_000)
+    assert {x.shape[1] for x, _ in probs} == {8, 16, 32, 64}
+    got = S.kmeans_batch_run(probs, 20, S.SC_MIXED)
+    worst = 0.0
+    for (x, init), (c, lab, size, dist, it) in zip(probs, got):
+        ocent, olabels, osize, odist, oiters = O.kmeans_fit(x, init, 20, use_bbd=True)
+        np.testing.assert_array_equal(lab, olabels)
+        np.testing.assert_array_equal(size, osize)
+        assert it == oiters
+        rel = np.max(np.abs(c - ocent)) / np.max(np.abs(ocent))
+        worst = max(worst, rel)
+        assert rel <= 1e-12
+    print(f"worst centroid relative error vs oracle over {len(probs)} problems: {worst:.3e}")

@@ -48,6 +48,48 @@ def test_mixed_labels_match_bbd_oracle():
     assert np.max(np.abs(cent - ocent)) <= 1e-12 * np.max(np.abs(ocent))
 
 
+@pytest.mark.parametrize("mode", [S.SC_MIXED, S.SC_PLAIN_FP64])
+def test_config2_shape_vs_bbd_oracle(mode):
+    """The 1,000,000 x 64, k = 64 workload's shape class (d = 64, k = 64,
+    64 blobs of U(0.5, 2) spread) on 60,000 rows, 20 Lloyd iterations from the
+    same initial centroids: labels, sizes and iteration count equal the
+    BBD-tree oracle's; the centroids agree to 1e-12 relative (the oracle adds
+    the sums in tree order, the device in fixed row-block order)."""
+    x, _, init = synth.kmeans_blobs(seed=2, n=60_000, d=64, blobs=64, k=64)
+    ix = S.Index(x)
+    cent, labels, size, dist, iters = ix.kmeans_run(init, 20, mode)
+    ocent, olabels, osize, odist, oiters = O.kmeans_fit(x, init, 20, use_bbd=True)
+    np.testing.assert_array_equal(labels, olabels)
+    np.testing.assert_array_equal(size, osize)
+    assert iters == oiters
+    rel = np.max(np.abs(cent - ocent)) / np.max(np.abs(ocent))
+    print(f"centroid relative error vs oracle: {rel:.3e}; distortion {dist!r} vs {odist!r}")
+    assert rel <= 1e-12
+    assert abs(dist - odist) <= 1e-9 * abs(odist)
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_stop_rule_on_data_far_from_the_origin(dtype):
+    """Blobs shifted by 1e5 in every dimension, up to 200 iterations: the
+    distortion compared by the stop rule (kmeans.rs:305-309) must not lose
+    the step-to-step decrease to cancellation, so the fit stops at the
+    oracle's iteration with the oracle's labels (160 for f64 rows, 132 for
+    f32 rows, both before max_iter)."""
+    x, _, init = synth.kmeans_blobs(seed=21, n=30_000, d=16, blobs=10, k=10, dtype=np.float64)
+    x = (x + 1e5).astype(dtype)
+    init = init + 1e5
+    ocent, olabels, osize, odist, oiters = O.kmeans_fit(x, init, 200, use_bbd=True)
+    assert oiters < 200
+    ix = S.Index(x)
+    modes = [S.SC_PLAIN_FP64] + ([S.SC_MIXED] if dtype == np.float32 else [])
+    for mode in modes:
+        cent, labels, size, dist, iters = ix.kmeans_run(init, 200, mode)
+        assert iters == oiters
+        np.testing.assert_array_equal(labels, olabels)
+        np.testing.assert_array_equal(size, osize)
+        assert abs(dist - odist) <= 1e-9 * abs(odist)
+
+
 def test_mixed_near_ties_use_fp64_fallback():
     """Centroids placed almost on top of each other cannot be separated by
     the BF16 bound, so those rows go through the FP64 path."""
