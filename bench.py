@@ -52,15 +52,16 @@ def cpu_model() -> str:
 
 
 def measured_traffic(kernel: str):
-    """HBM bytes per launch of `kernel` from the last counter runs
-    (profiles/traffic.json: FETCH_SIZE and WRITE_SIZE scaled by a plain copy),
-    or None when no measurement is on file."""
+    """(HBM bytes per launch of `kernel`, the summary they come from): two
+    rocprofv3 --pmc runs of the same workload (FETCH_SIZE, then WRITE_SIZE,
+    scaled by what a plain copy of known size reads), filed in
+    profiles/traffic.json; (None, None) when no measurement is on file."""
     try:
         with open(os.path.join(ROOT, "profiles", "traffic.json")) as f:
             t = json.load(f).get(kernel)
     except (OSError, ValueError):
-        return None
-    return None if not t else t["bytes"]
+        return None, None
+    return (None, None) if not t else (t["bytes"], t.get("source"))
 
 
 def profile_read(lib, name: str):
@@ -124,10 +125,12 @@ def bench_knn(args):
             while start * 8 < N:
                 start *= 8
         flops_per_query = 2.0 * (N - start) * D
+        e2e_flops_per_query = 2.0 * N * D
         peak, bound, unit_peak = PEAK_BF16, "mfma", "TFLOP/s"
     else:
         kname = "knn_scan"
         flops_per_query = 3.0 * N * D
+        e2e_flops_per_query = flops_per_query
         peak, bound, unit_peak = PEAK_FP64, "fp64-valu", "TFLOP/s"
     kms, launches = profile_read(lib, kname)
     per_launch_q = Q * args.steps / max(launches, 1)
@@ -179,9 +182,17 @@ def bench_knn(args):
                    "mode": args.mode},
         "roofline": {"bound": bound, "achieved": achieved / 1e12, "peak": peak / 1e12,
                      "unit": unit_peak, "frac": achieved / peak,
-                     "traffic": measured_traffic(kname),
+                     "traffic": measured_traffic(kname)[0],
+                     "traffic_source": measured_traffic(kname)[1],
                      "kernel": kname, "avg_ms": avg_s * 1e3,
                      "work_per_launch": f"{flops_per_query:.4g} FLOP/query x {per_launch_q:.0f} queries"},
+        # every kernel of a step (query preparation, all levels, rescoring,
+        # replay) against the same peak: the cross-term FLOPs of the whole row
+        # set per query times the step's query rate
+        "roofline_end_to_end": {"bound": bound, "unit": unit_peak,
+                                "achieved": e2e_flops_per_query * qps / 1e12, "peak": peak / 1e12,
+                                "frac": e2e_flops_per_query * qps / peak,
+                                "work": f"{e2e_flops_per_query:.4g} FLOP/query x {qps:.0f} queries/s"},
         "kernels": others,
     }
     if plain:
@@ -206,13 +217,61 @@ def bench_knn(args):
     return out
 
 
+def bench_classifier(args):
+    """KNNClassifier, k = 5, Euclidian, LinearSearch, 10,000 x 32 f64 rows from
+    a 10-class Gaussian mixture, 2,000 queries from the same mixture: fit +
+    predict through the C++ estimator mirror (one batched device search per
+    predict).  The CPU baseline is the oracle's LinearKNNSearch::find per
+    query plus the vote, on one core, over all 2,000 queries; the labels of
+    both are compared."""
+    import oracle as O
+    from smartcore_amd import smartcore as sc
+    from smartcore_amd import synth
+
+    xtr, ytr, xq, _ = synth.knn_mixture(seed=1)
+    p = sc.KNNClassifierParameters().with_k(5).with_algorithm(sc.KNNAlgorithmName.LinearSearch)
+    for _ in range(max(args.warmup, 1)):
+        sc.KNNClassifier.fit(xtr, ytr, p).predict(xq)
+    reps = max(args.steps, 1)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        model = sc.KNNClassifier.fit(xtr, ytr, p)
+        t1 = time.perf_counter()
+        got = model.predict(xq)
+        t2 = time.perf_counter()
+    total_s = (time.perf_counter() - t0) / reps
+    predict_s = t2 - t1
+    out = {"workload": "KNNClassifier k=5, Euclidian, LinearSearch, 10,000x32 f64 train, "
+                       "2,000 queries (10-class Gaussian mixture)",
+           "value": xq.shape[0] / predict_s, "unit": "queries/s (predict)",
+           "ms_fit_predict": total_s * 1e3, "dtype": "f64"}
+    if not args.no_cpu:
+        classes = np.unique(ytr)
+        ycls = np.searchsorted(classes, ytr)
+        O.lib()
+        t0 = time.perf_counter()
+        oi, od, oc = O.linear_find_batch(xtr, xq, 5)
+        want = np.array([classes[O.classifier_predict_row(oi[r, :oc[r]], od[r, :oc[r]], ycls,
+                                                          len(classes), 0)]
+                         for r in range(xq.shape[0])])
+        cpu_s = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": xq.shape[0] / cpu_s, "unit": "queries/s", "cores": 1,
+                               "kind": "port",
+                               "sample": f"all {xq.shape[0]:,} queries, fit + predict "
+                                         f"({cpu_s:.2f} s); host {cpu_model()}, nproc {os.cpu_count()}"}
+        out["label_mismatches_vs_oracle"] = int(np.sum(np.asarray(got) != want))
+    return out
+
+
 def bench_serve(args):
     """Continuous-batching server over a resident 2,000,000 x 96 f32 index
     (rows ~ N(0, I)), k = 20.  Requests arrive by a Poisson process at fixed
-    rates, each with round(exp(U[0, ln 4096])) query rows; 80% Euclidean on
-    the mixed form, 20% Manhattan on the FP64 form.  Reports per rate the
-    query throughput and the p50 / p99 submit-to-result latency, and checks
-    sampled requests against the same search run alone."""
+    rates, each with round(exp(U[0, ln 4096])) query rows; 80% Euclidean
+    (BF16 MFMA shortlist, FP64 rescoring), 20% Manhattan (bounded f32 sums,
+    FP64 rescoring), both the SC_MIXED form.  Reports per rate the query
+    throughput and the p50 / p99 submit-to-result latency, overall and per
+    metric, and checks sampled requests against the same search run alone
+    and a few rows of each metric against the oracle."""
     import threading
 
     import torch
@@ -235,11 +294,10 @@ def bench_serve(args):
         s0 = int(rng.integers(0, pool.shape[0] - q + 1))
         euclid = rng.random() < 0.8
         metric = S.SC_EUCLIDIAN if euclid else S.SC_MANHATTAN
-        mode = S.SC_MIXED if euclid else S.SC_PLAIN_FP64
-        return pool[s0:s0 + q], metric, mode
+        return pool[s0:s0 + q], metric, S.SC_MIXED
 
     # warm-up: both forms compiled and the BF16 copy built
-    for metric, mode in ((S.SC_EUCLIDIAN, S.SC_MIXED), (S.SC_MANHATTAN, S.SC_PLAIN_FP64)):
+    for metric, mode in ((S.SC_EUCLIDIAN, S.SC_MIXED), (S.SC_MANHATTAN, S.SC_MIXED)):
         srv.wait(srv.submit(pool[:64], K, metric, 2, mode))
     levels = []
     for rate in args.serve_rates:
@@ -279,9 +337,16 @@ def bench_serve(args):
             mism += int(np.sum(outs[j][0] != ai))
         lat_ms = np.array(lat)
         nq = sum(r[0].shape[0] for r in reqs)
+        is_l1 = np.array([r[1] == S.SC_MANHATTAN for r in reqs])
+        per_metric = {}
+        for name, sel in (("euclidian", ~is_l1), ("manhattan", is_l1)):
+            if sel.any():
+                per_metric[name] = {"requests": int(sel.sum()),
+                                    "p50_ms": float(np.percentile(lat_ms[sel], 50)),
+                                    "p99_ms": float(np.percentile(lat_ms[sel], 99))}
         levels.append({"offered_requests_per_s": rate, "requests": n_req,
                        "queries_per_s": nq / wall, "p50_ms": float(np.percentile(lat_ms, 50)),
-                       "p99_ms": float(np.percentile(lat_ms, 99)),
+                       "p99_ms": float(np.percentile(lat_ms, 99)), "per_metric": per_metric,
                        "checked_requests": len(sample), "index_mismatches_vs_alone": mism})
         log(f"[bench] serve {rate} req/s: {nq / wall:.0f} q/s p50 {np.percentile(lat_ms, 50):.1f} ms "
             f"p99 {np.percentile(lat_ms, 99):.1f} ms")
@@ -289,22 +354,32 @@ def bench_serve(args):
     srv.close()
     # the oracle on a few rows of one Euclidean and one Manhattan request
     oracle_mism = None
+    cpu = None
     if not args.no_cpu:
         import oracle as O
 
         href = refs.cpu().numpy()
         qs = pool[:2]
         ai, _, _ = ix.knn(qs, K, S.SC_EUCLIDIAN, 2, S.SC_MIXED)
-        bi, _, _ = ix.knn(qs, K, S.SC_MANHATTAN, 1, S.SC_PLAIN_FP64)
+        bi, _, _ = ix.knn(qs, K, S.SC_MANHATTAN, 1, S.SC_MIXED)
         oi, _, _ = O.linear_find_batch(href, qs, K, O.EUCLIDIAN)
         mi, _, _ = O.linear_find_batch(href, qs, K, O.MANHATTAN)
         oracle_mism = int(np.sum(ai != oi) + np.sum(bi != mi))
+        t0 = time.perf_counter()
+        O.linear_find_batch(href, pool[2:4], K, O.EUCLIDIAN)
+        t1 = time.perf_counter()
+        O.linear_find_batch(href, pool[4:6], K, O.MANHATTAN)
+        t2 = time.perf_counter()
+        cpu = {"value": (t1 - t0) / 2 * 1e3, "unit": "ms/query (Euclidian)",
+               "manhattan_ms_per_query": (t2 - t1) / 2 * 1e3, "cores": 1, "kind": "port",
+               "sample": f"2 Euclidian + 2 Manhattan queries against all {N:,} rows; host "
+                         f"{cpu_model()}, nproc {os.cpu_count()}"}
     ix.close()
     return {"workload": f"kNN server, {N:,}x{D} f32 resident rows, k={K}, requests of 1-4096 "
-                        "rows (log-uniform), Poisson arrivals, 80% Euclidean mixed / 20% "
-                        "Manhattan FP64",
+                        "rows (log-uniform), Poisson arrivals, 80% Euclidean / 20% Manhattan, "
+                        "both SC_MIXED (FP64-rescored)",
             "levels": levels, "batches": st["batches"], "hbm_peak_bytes": st["hbm_peak"],
-            "oracle_index_mismatches_4_queries": oracle_mism}
+            "oracle_index_mismatches_4_queries": oracle_mism, "cpu_baseline": cpu}
 
 
 def bench_copy(args):
@@ -340,7 +415,6 @@ def bench_kmeans(args):
     for w in range(args.warmup):
         ix.kmeans_run(init, iters, mode)
         log(f"[bench] warmup {w + 1}/{args.warmup} done")
-    lib.sc_profile_enable(1)
     torch.cuda.synchronize()
     total_iters = 0
     t0 = time.perf_counter()
@@ -349,8 +423,15 @@ def bench_kmeans(args):
         total_iters += it
         log(f"[bench] step {s + 1}/{args.steps}: {it} iterations")
     elapsed = time.perf_counter() - t0
-    lib.sc_profile_enable(0)
     ips = total_iters / elapsed
+    # The timed fits replay their Lloyd steps from one hipGraph, whose
+    # launches HIP events cannot bracket one by one; one more fit with the
+    # tracing hooks on launches the same kernels one by one on the same
+    # stream, and its per-launch event timings give the kernel durations.
+    lib.sc_profile_enable(1)
+    pc_, pl_, ps_, pd_, pit_ = ix.kmeans_run(init, iters, mode)
+    lib.sc_profile_enable(0)
+    assert np.array_equal(pl_, labels) and pit_ == it
     vs_fp64 = None
     if mode == S.SC_MIXED:
         t1 = time.perf_counter()
@@ -368,13 +449,16 @@ def bench_kmeans(args):
         bytes_per_launch = n * d * 4 + 4 * n + 2 * k * d * 8
         achieved = bytes_per_launch / avg_s
         roof = {"bound": "hbm", "achieved": achieved / 1e9, "peak": PEAK_HBM / 1e9, "unit": "GB/s",
-                "frac": achieved / PEAK_HBM, "traffic": measured_traffic(kname)}
+                "frac": achieved / PEAK_HBM, "traffic": measured_traffic(kname)[0],
+                "traffic_source": measured_traffic(kname)[1]}
     else:
         flops = 3.0 * n * k * d
         achieved = flops / avg_s
         roof = {"bound": "fp64-valu", "achieved": achieved / 1e12, "peak": PEAK_FP64 / 1e12,
                 "unit": "TFLOP/s", "frac": achieved / PEAK_FP64, "traffic": None}
-    roof.update({"kernel": kname, "avg_ms": avg_s * 1e3})
+    roof.update({"kernel": kname, "avg_ms": avg_s * 1e3,
+                 "timing": "HIP events per launch on a profiled fit of the same workload run "
+                           "right after the timed fits (those replay a hipGraph)"})
     kern = {}
     for name in ("kmeans_assign", "kmeans_partial", "kmeans_reduce", "kmeans_mixed_assign"):
         ms, c = profile_read(lib, name)
@@ -418,6 +502,14 @@ def bench_kmeans(args):
                                "sample": f"{nit} BBD-tree Lloyd steps on the full {n:,}x{d} set "
                                          f"(tree build {build_s:.1f} s excluded); host {cpu_model()}, "
                                          f"nproc {os.cpu_count()}"}
+        # the same steps on the device from the same initial centroids: labels
+        # of the last step and the centroids after it against the oracle's
+        gc, gl, gsz, gd, git = ix.kmeans_run(init, nit, mode)
+        out["vs_oracle_bbd"] = {
+            "steps": nit, "device_steps": int(git),
+            "label_mismatches": int(np.sum(gl != memb)),
+            "size_mismatches": int(np.sum(gsz != counts)),
+            "centroid_rel_error": float(np.max(np.abs(gc - c)) / np.max(np.abs(c)))}
     ix.close()
     return out
 
@@ -462,6 +554,28 @@ def bench_batch(args):
     it_bad = sum(1 for a, b in zip(got, ref) if a[4] != b[4])
     log(f"[bench] batch graph {graph_s * 1e3:.1f} ms, call {wall * 1e3:.1f} ms; "
         f"FP64 form {fp64_s * 1e3:.0f} ms; label-mismatched problems {len(bad)}")
+    cpu = None
+    if not args.no_cpu:
+        # the oracle's KMeans fit loop (BBD tree) over the first problems one
+        # after another, on one core; its labels checked against the batch's
+        import oracle as O
+
+        t0 = time.perf_counter()
+        done_iters, sample, bad_cpu = 0, 0, 0
+        for (x, init), g in zip(probs, got):
+            oc, ol, osz, odist, oit = O.kmeans_fit(x, init, iters, use_bbd=True)
+            done_iters += oit
+            sample += 1
+            bad_cpu += int(not np.array_equal(ol, g[1]) or oit != g[4])
+            if time.perf_counter() - t0 > args.cpu_batch_seconds:
+                break
+        cpu_s = time.perf_counter() - t0
+        cpu = {"value": done_iters / cpu_s, "unit": "problem-iterations/s", "cores": 1,
+               "kind": "port",
+               "sample": f"first {sample} of {len(probs)} problems fitted one after another "
+                         f"({cpu_s:.1f} s, BBD tree build included); host {cpu_model()}, "
+                         f"nproc {os.cpu_count()}",
+               "problems_differing_from_batch": bad_cpu}
     return {
         "value": inst_iters / graph_s,
         "unit": "problem-iterations/s",
@@ -477,6 +591,7 @@ def bench_batch(args):
                          "speedup_vs_fp64_form": fp64_s / wall,
                          "problems_with_label_mismatch": len(bad),
                          "problems_with_iteration_mismatch": it_bad},
+        "cpu_baseline": cpu,
     }
 
 
@@ -485,7 +600,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", choices=["all", "knn", "kmeans", "batch", "serve", "copy"],
+    ap.add_argument("--workload", choices=["all", "knn", "classifier", "kmeans", "batch", "serve",
+                                           "copy"],
                     default="all",
                     help="all = the kNN line plus the k-means and serving figures")
     ap.add_argument("--serve-n", type=int, default=2_000_000)
@@ -500,6 +616,7 @@ def main():
     ap.add_argument("--batch-count", type=int, default=512)
     ap.add_argument("--cpu-sample", type=int, default=64)
     ap.add_argument("--cpu-kmeans-iters", type=int, default=2)
+    ap.add_argument("--cpu-batch-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -514,6 +631,8 @@ def main():
         out = bench_kmeans(args)
     elif args.workload == "serve":
         out = bench_serve(args)
+    elif args.workload == "classifier":
+        out = bench_classifier(args)
     elif args.workload == "batch":
         out = bench_batch(args)
         out.update({"metric": METRIC, "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -528,7 +647,8 @@ def main():
             km = bench_kmeans(km_args)
             out["kmeans"] = {k: km[k] for k in ("value", "unit", "ms_per_step", "dtype", "config",
                                                 "roofline", "kernels", "cpu_baseline",
-                                                "vs_fp64_form") if k in km}
+                                                "vs_fp64_form", "vs_oracle_bbd") if k in km}
+            out["knn_classifier"] = bench_classifier(args)
             b_args = argparse.Namespace(**vars(args))
             b_args.steps, b_args.warmup = 1, 1
             out["kmeans_batch"] = bench_batch(b_args)
