@@ -158,6 +158,20 @@ void sc_radius_result_free(sc_radius_result* r);
 int sc_kmeans_step(const sc_index* index, const double* centroids, size_t k,
                    double* sums, uint64_t* counts, uint32_t* membership,
                    double* distortion);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): sc_kmeans_step with device pointers (d_centroids k*d
+ * f64; d_sums k*d, d_counts k, d_membership n, d_distortion 1), ordered on
+ * `stream`, returning after enqueueing.  It allocates nothing: its scratch
+ * is one fixed 64 MiB device array per GPU, held until the process ends.
+ * Shapes whose scratch exceeds it (about 8*(nb+ng)*k*d + 12*nb*k + 8*k*ld
+ * bytes, nb = ceil(n/2048), ng = ceil(nb/16)) fail with SC_ERR_UNSUPPORTED.
+ * All calls on one GPU share that array: calls that overlap on one GPU
+ * race, so the caller serialises them.  Outputs equal sc_kmeans_step's bit
+ * for bit. */
+int sc_kmeans_step_device_noalloc(const sc_index* index, const double* d_centroids, size_t k,
+                                  double* d_sums, uint64_t* d_counts, uint32_t* d_membership,
+                                  double* d_distortion, void* stream);
 /* The Lloyd loop of KMeans::fit (kmeans.rs:294-310) from caller-supplied
  * initial centroids (k*d f64, updated in place): at most max_iter steps,
  * centroid_i = sums_i / size_i when size_i > 0, stop when the previous
