@@ -31,6 +31,28 @@ hipStream_t thread_stream(int device) {
   return streams[device];
 }
 
+// Scratch of a call is stream-ordered (hipMallocAsync / hipFreeAsync from
+// the device's default pool).  The pool keeps up to kPoolKeepBytes of freed
+// scratch across synchronisations instead of handing it back at each one,
+// so repeated searches do not map their candidate buffers afresh.
+constexpr uint64_t kPoolKeepBytes = uint64_t(16) << 30;
+
+void configure_pool(int device) {
+  static constexpr int kMaxDev = 64;
+  static std::once_flag once[kMaxDev];
+  if (device < 0 || device >= kMaxDev) return;
+  std::call_once(once[device], [device] {
+    hipMemPool_t pool = nullptr;
+    if (hipDeviceGetDefaultMemPool(&pool, device) != hipSuccess || !pool) {
+      (void)hipGetLastError();
+      return;
+    }
+    uint64_t keep = kPoolKeepBytes;
+    if (hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep) != hipSuccess)
+      (void)hipGetLastError();
+  });
+}
+
 __global__ void pad_rows_kernel(const char* __restrict__ src, char* __restrict__ dst, size_t n,
                                 size_t row_bytes, size_t ld_bytes) {
   const size_t total = n * ld_bytes;
@@ -309,6 +331,7 @@ int index_create_impl(const void* rows, bool on_device, sc_dtype dtype, size_t n
   ix->ld = padded_dim(d);
   ix->device = dev;
   ix->mu = new std::mutex();
+  configure_pool(dev);
   hipStream_t s = thread_stream(dev);
   const size_t bytes = std::max<size_t>(n * ix->ld * elem_size(dtype), 16);
   hipError_t e = hipMalloc(&ix->rows, bytes);

@@ -61,6 +61,7 @@
 #include <cfloat>
 #include <cmath>
 #include <mutex>
+#include <type_traits>
 #include <vector>
 
 #include "common.hpp"
@@ -94,6 +95,7 @@ constexpr int kTargetBlocks = 256;        // one workgroup per CU on 256 CUs
 constexpr int64_t kPrefixRows = 65536;    // rows of the seeding launch
 constexpr int64_t kSeedMinRows = 32 * kPrefixRows;  // indexes this large are seeded
 constexpr int64_t kLevelRatio = 8;        // row-range growth of the fixed-threshold levels
+constexpr int64_t kScanPadRows = 1024;    // padding rows past the index read by fixed launches
 constexpr int kMaxLists = 64 + 2 * 64 * 8;  // running lists + fixed pairs, any plan
 constexpr double kC1 = 0.0078125 + 1.52587890625e-05 + 128.0 * 2.384185791015625e-07;
 constexpr double kLoMargin = 3.814697265625e-06;   // 2^-18
@@ -270,6 +272,7 @@ struct ScanArgs {
   uint32_t* cand_idx;  // q x lay.per_query()
   uint32_t* cand_count;  // q x lay.nlists()
   const float* seed;   // q x k, or null
+  const uint32_t* fold;  // (npad + kScanPadRows) x 4 folded-column dwords
 };
 
 // Running launch.  KT: registers of the per-query top-k (16 for k <= 16,
@@ -560,6 +563,12 @@ __global__ __launch_bounds__(kNT, 2) void mixed_scan_kernel(const ScanArgs a) {
   if (qo < q) a.cand_count[static_cast<size_t>(qo) * nl + lseg] = cnt;
 }
 
+__device__ __forceinline__ uint16_t bf16_ru_pos(float f) {
+  // f >= 0 finite: the smallest BF16 >= f
+  const uint32_t u = __float_as_uint(f);
+  return static_cast<uint16_t>((u >> 16) + ((u & 0xffffu) != 0u ? 1u : 0u));
+}
+
 // Fixed-threshold launch (see the file comment).  The bound's terms are
 // folded into one more MFMA K step: a tile row carries, after its ld
 // values, the BF16 columns [y2m hi, mid, lo, RU(nry), RU(nyh), 0 ...] and
@@ -567,44 +576,67 @@ __global__ __launch_bounds__(kNT, 2) void mixed_scan_kernel(const ScanArgs a) {
 // L - x2m = y2m - 2C - e (e = RU(qa1) RU(nry) + RU(qa2) RU(nyh), exact BF16
 // products, >= the tight bound) and the epilogue is a minimum.  The FP32
 // roundings of those adds and of the y2m split stay inside the 2^-18
-// margin.  Each lane writes its own rows (its half of the tile) of both
-// column blocks' queries to the queries' lists of that half.  Sixteen tile
-// slots (152 KiB of LDS; the registers allow one workgroup per CU): step w
-// (w = 0 mod 4) stores tiles w + 8 .. w + 11 into the slots of tiles
-// w - 8 .. w - 5 and loads tiles w + 12 .. w + 15 into registers.  The
-// MFMAs of tile t + 1 are issued before the epilogue of tile t, so the
-// last LDS reads of the slot of tile w - 5 are in step w - 6.  A barrier
-// at the start of every step 8 m + 7 falls between those reads and the
-// stores of step w, and between the stores of step w and the reads of tile
-// w + 8 in step w + 7: one barrier per eight tiles.  The candidate path
-// reads the row terms from global memory, so the slots hold rows only.
+// margin.  The folded columns of every row are built once with the index
+// (fold_terms_kernel).  Each lane writes its own rows (its half of the
+// tile) of both column blocks' queries to the queries' lists of that half.
+//
+// Schedule.  Sixteen tile slots (152 KiB of LDS at d = 128; the registers
+// allow one workgroup per CU, two waves per SIMD).  Step t issues the MFMAs
+// of tile t + 1 with the minimum of tile t's results between them, so a
+// wave keeps the matrix pipe fed during its own epilogue; the candidate
+// path (rare) is a branch after the MFMAs.  Step t also stores tile t + 9
+// from registers into its slot and loads tile t + 13 into the same
+// registers (four tiles in flight per thread).  The slot of tile t + 9 last
+// held tile t - 7, whose fragments are read in step t - 8, and the new tile
+// is read in step t + 8.  A barrier at the start of every step 8 m falls in
+// (t - 8, t] and in (t, t + 8]: one barrier per eight tiles.  Steps run in
+// groups of eight (ntiles rounded up).  Tiles past the segment read real or
+// padding rows (the BF16 rows and the folded columns carry kScanPadRows
+// rows of padding past the index) and their rows fail the row < r_end test.
 template <int KS>
 struct FixTile {
   static constexpr int LD = 16 * KS;
-  static constexpr int RS = LD + 16 + 8;  // row | folded K step | 16 B padding (bf16)
-  static constexpr int VEC = kRows * LD / 8;
-  static constexpr int NP = (VEC + kNT - 1) / kNT;
-  static constexpr int SLOT = kRows * RS * 2;  // rows with their folded columns (bytes)
+  static constexpr int RS = LD + 16 + 8;      // row | folded K step | 16 B padding (bf16)
+  static constexpr int VEC = kRows * LD / 8;  // 16-byte pieces of a tile's rows
+  static constexpr int SLOT = kRows * RS * 2;  // bytes
+  static_assert(VEC <= kNT, "one row piece per thread");
 };
 constexpr int kFixSlots = 16;
-constexpr int kFixAhead = kFixSlots / 2;  // tiles between a store and the first read of its slot
+constexpr int kFixAhead = 9;  // step t stores tile t + kFixAhead
+constexpr int kFixRing = 4;   // steps between a tile's load and its store
+constexpr int kFixGroup = 8;  // steps per barrier
+// the store of step t into the slot read last in step t + kFixAhead - kFixSlots - 1
+// and next in step t + kFixAhead - 1: a barrier step in each interval
+static_assert(kFixAhead - 1 >= kFixGroup && kFixSlots + 1 - kFixAhead >= kFixGroup,
+              "a barrier between each slot's last read and its next store, and after it");
 template <int KS>
 size_t fixed_lds_bytes() {
   return kFixSlots * static_cast<size_t>(FixTile<KS>::SLOT);
 }
 
-__device__ __forceinline__ uint16_t bf16_ru_pos(float f) {
-  // f >= 0 finite: the smallest BF16 >= f
-  const uint32_t u = __float_as_uint(f);
-  return static_cast<uint16_t>((u >> 16) + ((u & 0xffffu) != 0u ? 1u : 0u));
+// Folded columns of every row (four dwords): [hi | mid << 16, lo | RU(nry)
+// << 16, RU(nyh), 0], hi + mid + lo = y2m up to 2^-24 y2m.  Rows from npad on
+// are padding (y2m = +inf: a NaN sum that never passes a test).
+__global__ void fold_terms_kernel(const float* __restrict__ y2m, const float* __restrict__ nry,
+                                  const float* __restrict__ nyh, u32x4* __restrict__ fold,
+                                  int64_t npad, int64_t nfold) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= nfold) return;
+  const float y = i < npad ? y2m[i] : INFINITY;
+  const uint32_t h = bf16_rne(y);
+  const float e1 = y - __uint_as_float(h << 16);
+  const uint32_t m = bf16_rne(e1);
+  const uint32_t l = bf16_rne(e1 - __uint_as_float(m << 16));
+  const uint32_t c1 = i < npad ? bf16_ru_pos(nry[i]) : 0u;
+  const uint32_t c2 = i < npad ? bf16_ru_pos(nyh[i]) : 0u;
+  fold[i] = u32x4{h | (m << 16), l | (c1 << 16), c2, 0u};
 }
 
 template <int KS>
 __global__ __launch_bounds__(kNT, 2) void fixed_scan_kernel(const ScanArgs a) {
   using S = FixTile<KS>;
   const uint16_t* __restrict__ rows_b = a.rows_b;
-  const float* __restrict__ y2m = a.y2m;
-  const int64_t n = a.n;
+  const uint32_t* __restrict__ fold = a.fold;
   const int64_t q = a.q;
   const int k = a.k;
   const int nseg = a.nseg;
@@ -641,123 +673,30 @@ __global__ __launch_bounds__(kNT, 2) void fixed_scan_kernel(const ScanArgs a) {
   const int lcap = a.lay.cap(lseg);
 
   bf16x8 bq[2][KS + 1];
-  f32x2 thr;
-  i32x2 qvalid;
+  float thr[2];
 #pragma unroll
   for (int cb = 0; cb < 2; ++cb) {
     const int64_t qi = qw0 + 32 * cb + col;
-    qvalid[cb] = qi < q && a.x2m[qi < q ? qi : 0] < INFINITY;
+    const bool valid = qi < q && a.x2m[qi < q ? qi : 0] < INFINITY;
     const int64_t qr = qi < q ? qi : 0;
 #pragma unroll
     for (int ks = 0; ks < KS; ++ks)
       bq[cb][ks] = load_frag(a.q_b + qr * S::LD + 16 * ks + 8 * half);
     // the seed's k-th value T; the epilogue threshold is T - x2m rounded up
     // (the f64 difference rounded toward +inf, then narrowed toward +inf:
-    // at or above the exact T - x2m even when the f64 difference rounds)
+    // at or above the exact T - x2m even when the f64 difference rounds).
+    // Queries that take no part get -inf, which no value is below.
     const float x2q = a.x2m[qr], tq = a.seed[qr * k + k - 1];
-    thr[cb] = qvalid[cb]
-                  ? __double2float_ru(__ocml_sub_rtp_f64(static_cast<double>(tq),
-                                                         static_cast<double>(x2q)))
-                  : INFINITY;
+    thr[cb] = valid ? __double2float_ru(__ocml_sub_rtp_f64(static_cast<double>(tq),
+                                                           static_cast<double>(x2q)))
+                    : -INFINITY;
     const uint16_t b1 = bf16_ru_pos(a.qa1[qr]), b2 = bf16_ru_pos(a.qa2[qr]);
     const uint32_t one = 0x3f80u, n1 = 0x8000u | b1, n2 = 0x8000u | b2;
-    const u32x4 fold = half == 0 ? u32x4{one | (one << 16), one | (n1 << 16), n2, 0u}
-                                 : u32x4{0u, 0u, 0u, 0u};
-    bq[cb][KS] = __builtin_bit_cast(bf16x8, fold);
+    const u32x4 fq = half == 0 ? u32x4{one | (one << 16), one | (n1 << 16), n2, 0u}
+                               : u32x4{0u, 0u, 0u, 0u};
+    bq[cb][KS] = __builtin_bit_cast(bf16x8, fq);
   }
   uint32_t cntf[2] = {0u, 0u};
-
-  // tile loads: thread tid carries pieces tid and tid + kNT of the rows.
-  // The terms and folded columns of tile t (t = 4 m + j) are carried by
-  // lanes 0..15 of waves 2 j and 2 j + 1 (rows 16 (w & 1) + lane), so the
-  // work is spread over all eight waves of a group of four tiles.
-  struct TileRegs {
-    u32x4 v[S::NP];
-    f32x4 tm;
-  };
-  auto gload = [&](int64_t t, TileRegs& r) {
-    const int64_t t0 = r_begin + t * kRows;
-#pragma unroll
-    for (int pc = 0; pc < S::NP; ++pc) {
-      const int e = tid + kNT * pc;
-      if (e < S::VEC) {
-        const int64_t row = min<int64_t>(t0 + e / (S::LD / 8), n - 1);
-        r.v[pc] = *reinterpret_cast<const u32x4*>(rows_b + row * S::LD + 8 * (e % (S::LD / 8)));
-      }
-    }
-    // y2m, nyb, nry and nyh are consecutive arrays of a.npad floats
-    if ((wave >> 1) == static_cast<int>(t & 3) && lane < 16) {
-      const float* p = y2m + min<int64_t>(t0 + 16 * (wave & 1) + lane, a.npad - 1);
-      r.tm = f32x4{p[0], 0.0f, p[2 * a.npad], p[3 * a.npad]};
-    }
-  };
-  auto swrite = [&](int sl, const TileRegs& r) {
-    unsigned char* base = smem + sl * S::SLOT;
-#pragma unroll
-    for (int pc = 0; pc < S::NP; ++pc) {
-      const int e = tid + kNT * pc;
-      if (e < S::VEC)
-        *reinterpret_cast<u32x4*>(base + 2 * ((e / (S::LD / 8)) * S::RS + 8 * (e % (S::LD / 8)))) =
-            r.v[pc];
-    }
-    // slot sl holds a tile t with t = sl (mod 4)
-    if ((wave >> 1) == (sl & 3) && lane < 16) {
-      // folded columns: y2m split in three (exact up to 2^-24 y2m), the
-      // row factors rounded up
-      const int fr = 16 * (wave & 1) + lane;
-      const float y = r.tm[0];
-      const uint32_t h = bf16_rne(y);
-      const float e1 = y - __uint_as_float(h << 16);
-      const uint32_t m = bf16_rne(e1);
-      const uint32_t l = bf16_rne(e1 - __uint_as_float(m << 16));
-      const uint32_t c1 = bf16_ru_pos(r.tm[2]), c2 = bf16_ru_pos(r.tm[3]);
-      u32x4* fc = reinterpret_cast<u32x4*>(base + 2 * (fr * S::RS + S::LD));
-      fc[0] = u32x4{h | (m << 16), l | (c1 << 16), c2, 0u};
-      fc[1] = u32x4{0u, 0u, 0u, 0u};
-    }
-  };
-  // all A fragments of the tile are read before its MFMAs, so the LDS
-  // latency is paid once per tile, not once per K step
-  auto mfma_tile = [&](int sl, f32x16 (&acc)[2]) {
-    const uint16_t* A = reinterpret_cast<const uint16_t*>(smem + sl * S::SLOT);
-    bf16x8 fa[KS + 1];
-#pragma unroll
-    for (int ks = 0; ks <= KS; ++ks) fa[ks] = load_frag(A + col * S::RS + 16 * ks + 8 * half);
-    __builtin_amdgcn_sched_barrier(0);
-    acc[0] = f32x16{};
-    acc[1] = f32x16{};
-#pragma unroll
-    for (int ks = 0; ks <= KS; ++ks) {
-      acc[0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[ks], bq[0][ks], acc[0], 0, 0, 0);
-      acc[1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[ks], bq[1][ks], acc[1], 0, 0, 0);
-    }
-  };
-  // hits of tile t: rows of this lane's half, ascending
-  auto hits = [&](int64_t t, const f32x16 (&acc)[2], const bool (&hit)[2]) {
-    const int64_t t0 = r_begin + t * kRows;
-#pragma unroll
-    for (int cb = 0; cb < 2; ++cb) {
-      if (!hit[cb]) continue;
-      const int64_t qi = qw0 + 32 * cb + col;
-#pragma unroll
-      for (int v = 0; v < 16; ++v) {
-        const float val = acc[cb][v];
-        // val < thr (T - x2m rounded up) keeps every row with L < T; a row
-        // that only the rounding lets in is a harmless extra candidate
-        if (qvalid[cb] != 0 && val < thr[cb]) {
-          const int r = 8 * (v >> 2) + 4 * half + (v & 3);
-          const int64_t row = t0 + r;
-          if (row < r_end) {
-            if (cntf[cb] < static_cast<uint32_t>(lcap)) {
-              const size_t e = static_cast<size_t>(qi) * perq + loff + cntf[cb];
-              a.cand_idx[e] = static_cast<uint32_t>(row);
-            }
-            ++cntf[cb];
-          }
-        }
-      }
-    }
-  };
 
   if (ntiles == 0) {  // the same for the whole workgroup
 #pragma unroll
@@ -767,52 +706,136 @@ __global__ __launch_bounds__(kNT, 2) void fixed_scan_kernel(const ScanArgs a) {
     }
     return;
   }
-  // Loads and stores of the ring are unconditional (tiles past the
-  // segment read clamped rows into slots nobody reads): a skipped store
-  // would leave its registers' load outstanding, and the compiler then
-  // waits for every load before each later load into them.
-  TileRegs tr[4];
-  f32x16 accA[2], accB[2];
-  // tiles 0 .. kFixAhead - 1 into their slots, the next four into registers
+
+  // The second half of each row's folded K step is zero on both sides;
+  // the LDS there must hold finite values, and no tile store touches it.
+  static_assert(kFixSlots * kRows == kNT, "one zero piece per thread");
+  *reinterpret_cast<u32x4*>(smem + (tid / kRows) * S::SLOT +
+                            2 * ((tid % kRows) * S::RS + S::LD + 8)) = u32x4{0u, 0u, 0u, 0u};
+
+  // Tile loads: thread tid carries row piece tid (when the tile has that
+  // many) and dword tid % 128 of the tile's folded columns.  Addresses are
+  // a per-tile base (the same for the whole wave) plus a fixed lane offset.
+  const int prow = (tid < S::VEC ? tid : 0) / (S::LD / 8);
+  const int pcol = 8 * ((tid < S::VEC ? tid : 0) % (S::LD / 8));
+  const int foff = tid & 127;  // dword of the tile's 32 x 4 folded dwords
+  const uint32_t lds_piece = 2 * (prow * S::RS + pcol);
+  const uint32_t lds_fold = 2 * ((foff >> 2) * S::RS + S::LD) + 4 * (foff & 3);
+  struct TileRegs {
+    u32x4 v;
+    uint32_t f;
+  };
+  auto gload = [&](int64_t t, TileRegs& r) {
+    const int64_t t0 = r_begin + t * kRows;
+    if (S::VEC == kNT || tid < S::VEC)
+      r.v = *reinterpret_cast<const u32x4*>(rows_b + t0 * S::LD + (prow * S::LD + pcol));
+    r.f = fold[t0 * 4 + foff];
+  };
+  auto swrite = [&](int sl, const TileRegs& r) {
+    unsigned char* base = smem + sl * S::SLOT;
+    if (S::VEC == kNT || tid < S::VEC) *reinterpret_cast<u32x4*>(base + lds_piece) = r.v;
+    *reinterpret_cast<uint32_t*>(base + lds_fold) = r.f;
+  };
+  auto frags = [&](int sl, bf16x8 (&fa)[KS + 1]) {
+    const uint16_t* A = reinterpret_cast<const uint16_t*>(smem + sl * S::SLOT);
 #pragma unroll
-  for (int g = 0; g < kFixAhead; g += 4) {
+    for (int ks = 0; ks <= KS; ++ks) fa[ks] = load_frag(A + col * S::RS + 16 * ks + 8 * half);
+  };
+  auto mfmas = [&](const bf16x8 (&fa)[KS + 1], f32x16 (&acc)[2]) {
+    acc[0] = f32x16{};
+    acc[1] = f32x16{};
 #pragma unroll
-    for (int j = 0; j < 4; ++j) gload(g + j, tr[j]);
-#pragma unroll
-    for (int j = 0; j < 4; ++j) swrite(g + j, tr[j]);
-  }
-#pragma unroll
-  for (int j = 0; j < 4; ++j) gload(kFixAhead + j, tr[j]);
-  __syncthreads();
-  auto step = [&](int64_t t, f32x16 (&cur)[2], f32x16 (&nxt)[2]) {
-    if ((t & (kFixAhead - 1)) == kFixAhead - 1) __syncthreads();
-    if (t + 1 < ntiles) mfma_tile(static_cast<int>((t + 1) & (kFixSlots - 1)), nxt);
-    bool hit[2];
-#pragma unroll
-    for (int cb = 0; cb < 2; ++cb) {
-      float m = cur[cb][0];
-#pragma unroll
-      for (int v = 1; v < 16; ++v) m = fminf(m, cur[cb][v]);
-      hit[cb] = __ballot(qvalid[cb] != 0 && m < thr[cb]) != 0;
-    }
-    if (hit[0] || hit[1]) hits(t, cur, hit);
-    if ((t & 3) == 0) {
-      // vmcnt(0): the loads of four steps ago are done.  Said once here,
-      // on every path, it spares the compiler's own waits before each load.
-      __builtin_amdgcn_s_waitcnt(0x0F70);
-      // the slots of tiles t + kFixAhead + j are those of t - kFixAhead + j
-#pragma unroll
-      for (int j = 0; j < 4; ++j)
-        swrite(static_cast<int>((t + kFixAhead + j) & (kFixSlots - 1)), tr[j]);
-#pragma unroll
-      for (int j = 0; j < 4; ++j) gload(t + kFixAhead + 4 + j, tr[j]);
+    for (int ks = 0; ks <= KS; ++ks) {
+      acc[0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[ks], bq[0][ks], acc[0], 0, 0, 0);
+      acc[1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[ks], bq[1][ks], acc[1], 0, 0, 0);
     }
   };
-  mfma_tile(0, accA);
-  for (int64_t t = 0; t < ntiles; t += 2) {
-    step(t, accA, accB);
-    if (t + 1 >= ntiles) break;
-    step(t + 1, accB, accA);
+  // hits of tile t: rows of this lane's half, ascending
+  auto hits = [&](int64_t t, const f32x16 (&acc)[2], bool h0, bool h1) {
+    const int64_t t0 = r_begin + t * kRows;
+#pragma unroll
+    for (int cb = 0; cb < 2; ++cb) {
+      if (!(cb == 0 ? h0 : h1)) continue;
+      const int64_t qi = qw0 + 32 * cb + col;
+#pragma unroll
+      for (int v = 0; v < 16; ++v) {
+        // val < thr (T - x2m rounded up) keeps every row with L < T; a row
+        // that only the rounding lets in is a harmless extra candidate
+        if (acc[cb][v] < thr[cb]) {
+          const int64_t row = t0 + 8 * (v >> 2) + 4 * half + (v & 3);
+          if (row < r_end) {
+            if (cntf[cb] < static_cast<uint32_t>(lcap))
+              a.cand_idx[static_cast<size_t>(qi) * perq + loff + cntf[cb]] =
+                  static_cast<uint32_t>(row);
+            ++cntf[cb];
+          }
+        }
+      }
+    }
+  };
+
+  TileRegs tr[kFixRing];
+  f32x16 accA[2], accB[2];
+  // tiles 0 .. kFixAhead - 1 into their slots, the next kFixRing into registers
+  static_assert(kFixAhead <= 3 * kFixRing, "prologue in three batches");
+#pragma unroll
+  for (int g = 0; g < kFixAhead; g += kFixRing) {
+#pragma unroll
+    for (int j = 0; j < kFixRing; ++j)
+      if (g + j < kFixAhead) gload(g + j, tr[j]);
+#pragma unroll
+    for (int j = 0; j < kFixRing; ++j)
+      if (g + j < kFixAhead) swrite(g + j, tr[j]);
+  }
+#pragma unroll
+  for (int j = 0; j < kFixRing; ++j) gload(kFixAhead + j, tr[j]);
+  __syncthreads();
+  {
+    bf16x8 fa[KS + 1];
+    frags(0, fa);
+    mfmas(fa, accA);
+  }
+  // step t: epilogue of tile t (cur), MFMAs of tile t + 1 (nxt), ring
+  // register set t % kFixRing
+  auto step = [&](auto jc, int64_t t, f32x16 (&cur)[2], f32x16 (&nxt)[2]) {
+    constexpr int R = decltype(jc)::value % kFixRing;
+    bf16x8 fa[KS + 1];
+    frags(static_cast<int>((t + 1) & (kFixSlots - 1)), fa);
+    swrite(static_cast<int>((t + kFixAhead) & (kFixSlots - 1)), tr[R]);
+    gload(t + kFixAhead + kFixRing, tr[R]);
+    float m0 = cur[0][0], m1 = cur[1][0];
+#pragma unroll
+    for (int v = 1; v < 16; ++v) {
+      m0 = fminf(m0, cur[0][v]);
+      m1 = fminf(m1, cur[1][v]);
+    }
+    mfmas(fa, nxt);
+    const bool h0 = __ballot(m0 < thr[0]) != 0;
+    const bool h1 = __ballot(m1 < thr[1]) != 0;
+    // LDS reads first, then the ring's stores and loads, then each MFMA
+    // followed by two VALU instructions (the minimum, addresses, tests)
+    __builtin_amdgcn_sched_group_barrier(0x100, KS + 1, 0);
+    __builtin_amdgcn_sched_group_barrier(0x200, 2, 0);
+    __builtin_amdgcn_sched_group_barrier(0x020, 2, 0);
+    __builtin_amdgcn_sched_group_barrier(0x002, 4, 0);
+#pragma unroll
+    for (int i = 0; i < 2 * (KS + 1); ++i) {
+      __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+      __builtin_amdgcn_sched_group_barrier(0x002, 2, 0);
+    }
+    if (h0 || h1) hits(t, cur, h0, h1);
+  };
+  const int64_t nsteps = (ntiles + kFixGroup - 1) / kFixGroup * kFixGroup;
+  for (int64_t t = 0; t < nsteps; t += kFixGroup) {
+    __syncthreads();
+    step(std::integral_constant<int, 0>{}, t + 0, accA, accB);
+    step(std::integral_constant<int, 1>{}, t + 1, accB, accA);
+    step(std::integral_constant<int, 2>{}, t + 2, accA, accB);
+    step(std::integral_constant<int, 3>{}, t + 3, accB, accA);
+    step(std::integral_constant<int, 4>{}, t + 4, accA, accB);
+    step(std::integral_constant<int, 5>{}, t + 5, accB, accA);
+    step(std::integral_constant<int, 6>{}, t + 6, accA, accB);
+    step(std::integral_constant<int, 7>{}, t + 7, accB, accA);
   }
 #pragma unroll
   for (int cb = 0; cb < 2; ++cb) {
@@ -1043,8 +1066,13 @@ int ensure_bf16(sc_index* ix, hipStream_t s, bool* bad_out, int64_t* npad_out) {
   const int ldb = static_cast<int>(ix->ld);
   void* b = nullptr;
   float* terms = nullptr;
-  SC_HIP(hipMalloc(&b, std::max<size_t>(1, n) * ldb * sizeof(uint16_t)));
-  SC_HIP(hipMalloc(reinterpret_cast<void**>(&terms), sizeof(float) * 4 * npad));
+  // BF16 rows, then kScanPadRows zero rows; terms: y2m | nyb | nry | nyh
+  // (npad floats each) | folded columns (npad + kScanPadRows rows of 4 dwords)
+  const int64_t nfold = npad + kScanPadRows;
+  SC_HIP(hipMalloc(&b, static_cast<size_t>(n + kScanPadRows) * ldb * sizeof(uint16_t)));
+  SC_HIP(hipMalloc(reinterpret_cast<void**>(&terms), sizeof(float) * (4 * npad + 4 * nfold)));
+  SC_HIP(hipMemsetAsync(static_cast<uint16_t*>(b) + static_cast<size_t>(n) * ldb, 0,
+                        static_cast<size_t>(kScanPadRows) * ldb * sizeof(uint16_t), s));
   DevBuf flag;
   SC_HIP(flag.alloc(4, s));
   SC_HIP(hipMemsetAsync(flag.p, 0, 4, s));
@@ -1055,12 +1083,15 @@ int ensure_bf16(sc_index* ix, hipStream_t s, bool* bad_out, int64_t* npad_out) {
                        s, static_cast<const float*>(ix->rows), n, static_cast<int>(ix->ld),
                        static_cast<uint16_t*>(b), terms, terms + npad, terms + 2 * npad,
                        terms + 3 * npad, flag.as<uint32_t>());
+  hipLaunchKernelGGL(fold_terms_kernel, dim3(static_cast<unsigned>((nfold + 255) / 256)), dim3(256), 0,
+                     s, terms, terms + 2 * npad, terms + 3 * npad,
+                     reinterpret_cast<u32x4*>(terms + 4 * npad), npad, nfold);
   SC_HIP(hipGetLastError());
   uint32_t bad = 0;
   SC_HIP(hipMemcpyAsync(&bad, flag.p, 4, hipMemcpyDeviceToHost, s));
   SC_HIP(hipStreamSynchronize(s));
   ix->rows_bf16 = b;
-  ix->norms2 = terms;  // y2m | nyb | nry | nyh, npad floats each
+  ix->norms2 = terms;  // y2m | nyb | nry | nyh (npad floats each) | folded columns
   ix->ldb = bad ? 0 : static_cast<size_t>(ldb);
   *bad_out = bad != 0;
   return SC_OK;
@@ -1227,6 +1258,7 @@ int knn_mixed_device(const sc_index* cix, const void* d_queries, size_t q_, size
   sa.nyb = nyb;
   sa.nry = ix->norms2 + 2 * npad;
   sa.npad = npad;
+  sa.fold = reinterpret_cast<const uint32_t*>(ix->norms2 + 4 * npad);
   sa.n = n;
   sa.q_b = qb.as<uint16_t>();
   sa.x2m = x2.as<float>();

@@ -131,20 +131,31 @@ def test_mixed_manhattan_ties_non_finite_and_fallbacks():
     check_l1(ref.astype(np.float64), qs.astype(np.float64), 9)
 
 
-@pytest.mark.parametrize("d,k,kind", [(16, 10, "lognormal"), (48, 32, "ties"), (16, 1, "lognormal"),
-                                      (128, 10, "lognormal"), (128, 1, "lognormal"), (128, 10, "ties")])
-def test_mixed_seeded_large_index(d, k, kind):
+@pytest.mark.parametrize("d,k,kind,n", [(16, 10, "lognormal", 2_200_000), (48, 32, "ties", 2_200_000),
+                                        (16, 1, "lognormal", 2_200_000),
+                                        (128, 10, "lognormal", 2_200_000),
+                                        (128, 1, "lognormal", 2_200_000),
+                                        (128, 10, "ties", 2_200_000),
+                                        (96, 20, "gaussian", 2_000_000),
+                                        (96, 20, "lognormal", 600_000)])
+def test_mixed_seeded_large_index(d, k, kind, n):
     """Indexes of 2,097,152 rows or more scan a 65,536-row prefix first, then
     row ranges growing 8 times, each tested against the k-th exact distance
     of the earlier candidates.  The result must still equal the FP64 form bit
     for bit (and the oracle on a few queries), also with heavy exact ties and
     with a single neighbour.  The d = 128 cases run the scan the 4,000,000 x 128
     workload times (eight K steps plus the folded bound step) on both the
-    middle and the last level."""
-    n, nq = 2_200_000, 3_000
+    middle and the last level.  The 2,000,000 x 96 Gaussian case is the
+    served index's shape and 600,000 rows a mid-size one; both stay below the
+    level threshold and run one running launch over many segments."""
+    nq = 3_000
     if kind == "lognormal":
         ref = synth.lognormal_rows(90 + d, n, d)
         qs = synth.lognormal_rows(91 + d, nq, d)
+    elif kind == "gaussian":
+        rng = np.random.default_rng(93)
+        ref = rng.normal(size=(n, d)).astype(np.float32)
+        qs = rng.normal(size=(nq, d)).astype(np.float32)
     else:
         rng = np.random.default_rng(92)
         ref = rng.integers(0, 4, size=(n, d)).astype(np.float32)
