@@ -408,6 +408,8 @@ int sc_index_destroy(sc_index* index) {
   if (index->rows) (void)hipFree(index->rows);
   if (index->rows_bf16) (void)hipFree(index->rows_bf16);
   if (index->norms2) (void)hipFree(index->norms2);
+  if (index->rows_q16) (void)hipFree(index->rows_q16);
+  if (index->err_q16) (void)hipFree(index->err_q16);
   (void)hipSetDevice(cur);
   delete static_cast<std::mutex*>(index->mu);
   delete index;

@@ -79,6 +79,14 @@ struct sc_index {
   void* rows_bf16;  // n x ldb bf16 (hi part)
   float* norms2;    // n squared norms (FP32 of the exact FP64 value)
   size_t ldb;
+  // Lazily built 16-bit quantised copy for the mixed Manhattan form
+  // (guarded by mu): rows_q16 holds n x ld u16 codes (two per dword),
+  // err_q16 per row ceil(sum_i |y_i - (q16_lo + code_i * q16_step)| / q16_step).
+  // q16_state: 0 not built, 1 built, 2 outside the form's domain.
+  void* rows_q16;
+  uint32_t* err_q16;
+  double q16_lo, q16_step;
+  int q16_state;
   void* mu;         // std::mutex*
 };
 

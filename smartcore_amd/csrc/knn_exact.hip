@@ -26,6 +26,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 #include <vector>
 
 #include "common.hpp"
@@ -283,6 +284,259 @@ __global__ __launch_bounds__(NT) void knn_scan_l1f32_kernel(
     cand_count[static_cast<size_t>(my_q) * nseg + seg] = emitted;
     if (out_thr) out_thr[my_q] = topk[tid][k - 1];
   }
+}
+
+// ---------------------------------------------------------------------------
+// Mixed Manhattan on 16-bit codes.  The index's values are coded as
+// c = clamp(rint((v - lo) / h), 0, 65535) with h a power of two and lo a
+// multiple of h, so deq(c) = lo + c h is exact in f64.  Per row the code
+// error er >= sum_i |y_i - deq(c_i)| / h (an integer, rounded up); per query
+// eq >= sum_i |x_i - deq(c_i)| / h (f64, rounded up; clamping included).
+// With D = sum_i |x_i - y_i| and sad = sum_i |cx_i - cy_i| (v_sad_u16, two
+// dimensions per instruction, exact in u32), the triangle inequality gives
+//   h (sad - eq - er) <= D <= h (sad + eq + er).
+// The reference sum S = f64 fold of |fl32(x_i - y_i)| (manhattan.rs:56-59)
+// lies in [(1 - eps) D, (1 + eps) D], eps = 2^-24 + (d + 1) 2^-52 (one f32
+// rounding per term, at most d f64 roundings of the fold), so
+//   L = (1 - eps) h (sad - eq - er) <= S <= U = (1 + eps) h (sad + eq + er).
+// The scan keeps per query the k smallest V = sad + er (the k smallest U)
+// and stores every row with L below the running k-th U, i.e. with
+//   W = sad - er < rho (V_k + eq) + eq,  rho >= (1 + eps) / (1 - eps),
+// tested in integers against the right side rounded up.  Padding dimensions
+// code to 0 on both sides and add nothing.
+constexpr int kQ16Chunk = 8;  // dwords (16 dimensions) staged per step
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+constexpr double kQ16Limit = 1e30;
+
+// thr = ceil(rho (vk + eq) + eq) with every operation rounded up; "no
+// threshold" (fewer than k rows seen, or a query outside the domain) is
+// INT_MAX, which every W is below.
+__device__ __forceinline__ int q16_threshold(uint32_t vk, double eq, double rho) {
+  if (vk == 0xffffffffu || !(eq < INFINITY)) return 0x7fffffff;
+  const double r = __ocml_add_rtp_f64(
+      __ocml_mul_rtp_f64(__ocml_add_rtp_f64(static_cast<double>(vk), eq), rho), eq);
+  const double c = ceil(r);
+  return c >= 2147483647.0 ? 0x7fffffff : static_cast<int>(c);
+}
+
+__global__ __launch_bounds__(NT) void knn_scan_l1q_kernel(
+    const uint32_t* __restrict__ rows_q, const uint32_t* __restrict__ er, int64_t r0, int64_t n,
+    int ldq, const uint32_t* __restrict__ qcodes, const double* __restrict__ eqv, int64_t q,
+    int k, int64_t seg_len, int nseg, double rho, const uint32_t* __restrict__ seed,
+    uint32_t* __restrict__ out_vk, uint32_t* __restrict__ cand_idx,
+    uint32_t* __restrict__ cand_count, int cap) {
+  extern __shared__ __align__(16) unsigned char q16_smem[];
+  // queries: all their dwords, staged once; rows: kQ16Chunk dwords at a time
+  // (query rows padded to 16 bytes: a lane reads its four queries' dword
+  // with one 16-byte load)
+  uint32_t (*Qs)[QT + 4] = reinterpret_cast<uint32_t (*)[QT + 4]>(q16_smem);
+  uint32_t (*Rs)[RT + 1] = reinterpret_cast<uint32_t (*)[RT + 1]>(
+      q16_smem + sizeof(uint32_t) * ldq * (QT + 4));
+  __shared__ uint32_t mask[QT][RT / 32];
+  __shared__ int sW[QT][RT];
+  __shared__ uint32_t topk[QT][KMAX_TILED];
+  __shared__ int thr[QT];
+
+  const int tid = threadIdx.x;
+  const int tq = tid >> 5;
+  const int tr = tid & 31;
+  const int64_t q0 = static_cast<int64_t>(blockIdx.x) * QT;
+  const int seg = blockIdx.y;
+  const int64_t r_begin = r0 + static_cast<int64_t>(seg) * seg_len;
+  const int64_t r_end = min(n, r_begin + seg_len);
+
+  for (int i = tid; i < QT * KMAX_TILED; i += NT) (&topk[0][0])[i] = 0xffffffffu;
+  for (int i = tid; i < QT * (RT / 32); i += NT) (&mask[0][0])[i] = 0u;
+  for (int x = tid; x < QT * ldq; x += NT) {
+    const int r = x / ldq, c = x % ldq;
+    const int64_t row = q0 + r;
+    Qs[c][r] = row < q ? qcodes[row * ldq + c] : 0u;
+  }
+  __syncthreads();
+  const int64_t my_q = q0 + tid;
+  const double my_eq = (tid < QT && my_q < q) ? eqv[my_q] : INFINITY;
+  if (tid < QT) {
+    // a seed is the k-th V of earlier rows: it caps every slot
+    const uint32_t s0 = (seed && my_q < q) ? seed[my_q] : 0xffffffffu;
+    for (int j = 0; j < k; ++j) topk[tid][j] = s0;
+    thr[tid] = q16_threshold(s0, my_eq, rho);
+  }
+  uint32_t emitted = 0;
+  const size_t cand_base = (tid < QT && my_q < q)
+                               ? (static_cast<size_t>(my_q) * nseg + seg) * static_cast<size_t>(cap)
+                               : 0;
+
+  for (int64_t t0 = r_begin; t0 < r_end; t0 += RT) {
+    uint32_t acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) acc[i][j] = 0u;
+    for (int c0 = 0; c0 < ldq; c0 += kQ16Chunk) {
+      __syncthreads();
+      for (int x = tid; x < RT * kQ16Chunk; x += NT) {
+        const int r = x / kQ16Chunk, c = x % kQ16Chunk;
+        const int64_t row = t0 + r;
+        Rs[c][r] = row < r_end ? rows_q[row * ldq + c0 + c] : 0u;
+      }
+      __syncthreads();
+#pragma unroll
+      for (int kk = 0; kk < kQ16Chunk; ++kk) {
+        uint32_t b[4];
+        const u32x4 a = *reinterpret_cast<const u32x4*>(&Qs[c0 + kk][tq * 4]);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) b[j] = Rs[kk][tr + 32 * j];
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+          for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_sad_u16(a[i], b[j], acc[i][j]);
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int rl = tr + 32 * j;
+      const int64_t row = t0 + rl;
+      if (row >= r_end) continue;
+      const uint32_t e = er[row];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int qi = tq * 4 + i;
+        const int W = static_cast<int>(acc[i][j]) - static_cast<int>(e);
+        if (q0 + qi < q && W < thr[qi]) {
+          sW[qi][rl] = W;
+          atomicOr(&mask[qi][rl >> 5], 1u << (rl & 31));
+        }
+      }
+    }
+    __syncthreads();
+    if (tid < QT && my_q < q) {
+      uint32_t* tk = topk[tid];
+      int th = thr[tid];
+      for (int w = 0; w < RT / 32; ++w) {
+        uint32_t bits = mask[tid][w];
+        mask[tid][w] = 0u;
+        while (bits) {
+          const int b = __builtin_ctz(bits);
+          bits &= bits - 1;
+          const int rl = w * 32 + b;
+          // the running threshold in row order (it fell since the tile began)
+          if (!(sW[tid][rl] < th)) continue;
+          if (emitted < static_cast<uint32_t>(cap))
+            cand_idx[cand_base + emitted] = static_cast<uint32_t>(t0 + rl);
+          ++emitted;
+          const uint32_t v = static_cast<uint32_t>(sW[tid][rl] + 2 * static_cast<int>(er[t0 + rl]));
+          if (v < tk[k - 1]) {
+            int pos = k - 1;
+            while (pos > 0 && tk[pos - 1] > v) {
+              tk[pos] = tk[pos - 1];
+              --pos;
+            }
+            tk[pos] = v;
+            th = q16_threshold(tk[k - 1], my_eq, rho);
+          }
+        }
+      }
+      thr[tid] = th;
+    }
+  }
+  if (tid < QT && my_q < q) {
+    cand_count[static_cast<size_t>(my_q) * nseg + seg] = emitted;
+    if (out_vk) out_vk[my_q] = topk[tid][k - 1];
+  }
+}
+
+// Range of the index's values (first d of each row): per block min, max,
+// sum |v| and a flag for values outside the form's domain.
+__global__ void q16_range_kernel(const float* __restrict__ rows, int64_t n, int d, int ld,
+                                 double* __restrict__ part) {
+  __shared__ double smin[256], smax[256], ssum[256], sbad[256];
+  double mn = INFINITY, mx = -INFINITY, sm = 0.0, bad = 0.0;
+  const int64_t total = n * d;
+  for (int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; e < total;
+       e += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    const double v = rows[(e / d) * ld + e % d];
+    if (!(fabs(v) <= kQ16Limit)) {
+      bad = 1.0;
+      continue;
+    }
+    mn = fmin(mn, v);
+    mx = fmax(mx, v);
+    sm += fabs(v);
+  }
+  const int t = threadIdx.x;
+  smin[t] = mn;
+  smax[t] = mx;
+  ssum[t] = sm;
+  sbad[t] = bad;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (t < s) {
+      smin[t] = fmin(smin[t], smin[t + s]);
+      smax[t] = fmax(smax[t], smax[t + s]);
+      ssum[t] += ssum[t + s];
+      sbad[t] = fmax(sbad[t], sbad[t + s]);
+    }
+    __syncthreads();
+  }
+  if (t == 0) {
+    part[4 * blockIdx.x + 0] = smin[0];
+    part[4 * blockIdx.x + 1] = smax[0];
+    part[4 * blockIdx.x + 2] = ssum[0];
+    part[4 * blockIdx.x + 3] = sbad[0];
+  }
+}
+
+// Codes of one vector's value (f64 arithmetic; deq(c) = lo + c h exact) and
+// the absolute code error.
+__device__ __forceinline__ uint32_t q16_code(float v, double lo, double h, double inv_h,
+                                             double* err) {
+  double c = rint((static_cast<double>(v) - lo) * inv_h);
+  c = fmin(fmax(c, 0.0), 65535.0);
+  *err += fabs(static_cast<double>(v) - (lo + c * h));
+  return static_cast<uint32_t>(c);
+}
+
+// Rows: codes (dimension pairs in dwords, padding dimensions 0) and er.
+__global__ void q16_rows_kernel(const float* __restrict__ rows, int64_t n, int d, int ld,
+                                double lo, double h, uint32_t* __restrict__ codes,
+                                uint32_t* __restrict__ er) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float* r = rows + i * ld;
+  uint32_t* o = codes + i * (ld / 2);
+  const double inv_h = 1.0 / h;
+  double err = 0.0;
+  for (int j = 0; j < ld; j += 2) {
+    const uint32_t c0 = j < d ? q16_code(r[j], lo, h, inv_h, &err) : 0u;
+    const uint32_t c1 = j + 1 < d ? q16_code(r[j + 1], lo, h, inv_h, &err) : 0u;
+    o[j / 2] = c0 | (c1 << 16);
+  }
+  // err carries at most two roundings per term (the difference, the add):
+  // scale it up by more than that, then round up to an integer
+  er[i] = static_cast<uint32_t>(ceil(err * inv_h * (1.0 + (2.0 * d + 4.0) * 2.220446049250313e-16)));
+}
+
+// Queries: codes (clamped) and eq; a query with a value outside the domain
+// gets eq = +inf (no threshold: its lists overflow and the serial kernel
+// answers it).
+__global__ void q16_queries_kernel(const float* __restrict__ qs, int64_t q, int d, int ld,
+                                   double lo, double h, uint32_t* __restrict__ codes,
+                                   double* __restrict__ eq) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= q) return;
+  const float* r = qs + i * ld;
+  uint32_t* o = codes + i * (ld / 2);
+  const double inv_h = 1.0 / h;
+  double err = 0.0;
+  bool ok = true;
+  for (int j = 0; j < d; ++j) ok = ok && fabsf(r[j]) <= static_cast<float>(kQ16Limit);
+  for (int j = 0; j < ld; j += 2) {
+    const uint32_t c0 = j < d ? q16_code(r[j], lo, h, inv_h, &err) : 0u;
+    const uint32_t c1 = j + 1 < d ? q16_code(r[j + 1], lo, h, inv_h, &err) : 0u;
+    o[j / 2] = c0 | (c1 << 16);
+  }
+  eq[i] = ok ? __ocml_mul_rtp_f64(err * inv_h, 1.0 + (2.0 * d + 4.0) * 2.220446049250313e-16)
+             : INFINITY;
 }
 
 // The reference sum of every stored candidate (one workgroup per (query,
@@ -575,6 +829,85 @@ int run_exact(const sc_index* index, const T* d_queries, int64_t q, int k, doubl
 // later segments store only the rows that can still enter.
 constexpr int64_t kL1Prefix = 65536;
 
+// Builds the 16-bit coded copy of an f32 index once (under its lock).  The
+// form is taken when every value is finite with |v| <= 1e30 and the code step
+// h (the smallest power of two with (max - min) / h <= 65534) is at most
+// 1/64 of the mean |v|, so the bound's band (about d h) stays a few percent
+// of a typical distance; otherwise, and for rows so wide that the staged
+// queries would not fit the LDS, the index keeps the f32 scan.
+constexpr int kQ16MaxLdq = 416;  // dwords of a staged query row (LDS: 36 x 416 x 4 B)
+
+int ensure_q16(sc_index* ix, hipStream_t s) {
+  std::lock_guard<std::mutex> lk(*static_cast<std::mutex*>(ix->mu));
+  if (ix->q16_state != 0) return SC_OK;
+  const int64_t n = static_cast<int64_t>(ix->n);
+  const int d = static_cast<int>(ix->d), ld = static_cast<int>(ix->ld);
+  if (n == 0 || d == 0 || ld / 2 > kQ16MaxLdq) {
+    ix->q16_state = 2;
+    return SC_OK;
+  }
+  const float* rows = static_cast<const float*>(ix->rows);
+  const int blocks = static_cast<int>(std::min<int64_t>(1024, (n * d + 255) / 256));
+  DevBuf part;
+  SC_HIP(part.alloc(sizeof(double) * 4 * blocks, s));
+  hipLaunchKernelGGL(q16_range_kernel, dim3(blocks), dim3(256), 0, s, rows, n, d, ld,
+                     part.as<double>());
+  SC_HIP(hipGetLastError());
+  std::vector<double> h(4 * static_cast<size_t>(blocks));
+  SC_HIP(hipMemcpyAsync(h.data(), part.p, sizeof(double) * h.size(), hipMemcpyDeviceToHost, s));
+  SC_HIP(hipStreamSynchronize(s));
+  double mn = INFINITY, mx = -INFINITY, sum = 0.0, bad = 0.0;
+  for (int b = 0; b < blocks; ++b) {
+    mn = std::min(mn, h[4 * b]);
+    mx = std::max(mx, h[4 * b + 1]);
+    sum += h[4 * b + 2];
+    bad = std::max(bad, h[4 * b + 3]);
+  }
+  if (bad != 0.0 || !(mn <= mx)) {
+    ix->q16_state = 2;
+    return SC_OK;
+  }
+  const double range = mx - mn;
+  double step = std::ldexp(1.0, -100);
+  if (range > 0.0) {
+    int e = 0;
+    (void)std::frexp(range / 65534.0, &e);  // range / 65534 < 2^e
+    step = std::max(step, std::ldexp(1.0, e));
+  }
+  double lo = std::floor(mn / step) * step;
+  while ((mx - lo) / step > 65535.0) {  // not reached for the step above
+    step *= 2.0;
+    lo = std::floor(mn / step) * step;
+  }
+  const double mean_abs = sum / (static_cast<double>(n) * d);
+  if (range > 0.0 && step > mean_abs / 64.0) {
+    ix->q16_state = 2;
+    return SC_OK;
+  }
+  void* codes = nullptr;
+  uint32_t* er = nullptr;
+  SC_HIP(hipMalloc(&codes, sizeof(uint32_t) * static_cast<size_t>(n) * (ld / 2)));
+  if (hipMalloc(reinterpret_cast<void**>(&er), sizeof(uint32_t) * n) != hipSuccess) {
+    (void)hipFree(codes);
+    return hip_fail(hipErrorOutOfMemory, "hipMalloc(row code errors)");
+  }
+  hipLaunchKernelGGL(q16_rows_kernel, dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0, s,
+                     rows, n, d, ld, lo, step, static_cast<uint32_t*>(codes), er);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e != hipSuccess) {
+    (void)hipFree(codes);
+    (void)hipFree(er);
+    return hip_fail(e, "q16_rows_kernel");
+  }
+  ix->rows_q16 = codes;
+  ix->err_q16 = er;
+  ix->q16_lo = lo;
+  ix->q16_step = step;
+  ix->q16_state = 1;
+  return SC_OK;
+}
+
 int run_l1_mixed(const sc_index* index, const float* d_queries, int64_t q, int k, int64_t* d_idx,
                  double* d_dist, uint32_t* d_count, hipStream_t stream, sc_stats* stats) {
   if (k > KMAX_TILED)
@@ -583,6 +916,15 @@ int run_l1_mixed(const sc_index* index, const float* d_queries, int64_t q, int k
   const int64_t n = static_cast<int64_t>(index->n);
   const int ld = static_cast<int>(index->ld);
   const float* refs = static_cast<const float*>(index->rows);
+  if (int rc = ensure_q16(const_cast<sc_index*>(index), stream)) return rc;
+  const bool coded = index->q16_state == 1;
+  const int ldq = ld / 2;
+  // rho >= (1 + eps) / (1 - eps), eps = 2^-24 + (d + 1) 2^-52
+  const double eps = 5.9604644775390625e-08 + (static_cast<double>(index->d) + 1.0) *
+                                                  2.220446049250313e-16;
+  const double rho = 1.0 + 3.0 * eps;
+  const size_t q16_lds = sizeof(uint32_t) * (static_cast<size_t>(ldq) * (QT + 4) +
+                                             static_cast<size_t>(kQ16Chunk) * (RT + 1));
   DevBuf flags;
   SC_HIP(flags.alloc(64, stream));
   SC_HIP(hipMemsetAsync(flags.p, 0, 64, stream));
@@ -602,7 +944,11 @@ int run_l1_mixed(const sc_index* index, const float* d_queries, int64_t q, int k
                            (prefix ? static_cast<size_t>(cap_p) * 12 + 8 : 0);
   int64_t qb = std::max<int64_t>(QT, static_cast<int64_t>((size_t(1) << 30) / per_query) / QT * QT);
   qb = std::min(qb, q);
-  DevBuf cidx, cs, ccnt, pidx, ps, pcnt, seed;
+  DevBuf cidx, cs, ccnt, pidx, ps, pcnt, seed, qcodes, qerr;
+  if (coded) {
+    SC_HIP(qcodes.alloc(sizeof(uint32_t) * qb * ldq, stream));
+    SC_HIP(qerr.alloc(sizeof(double) * qb, stream));
+  }
   SC_HIP(cidx.alloc(sizeof(uint32_t) * qb * plan.nseg * plan.cap, stream));
   SC_HIP(cs.alloc(sizeof(double) * qb * plan.nseg * plan.cap, stream));
   SC_HIP(ccnt.alloc(sizeof(uint32_t) * qb * plan.nseg, stream));
@@ -616,7 +962,29 @@ int run_l1_mixed(const sc_index* index, const float* d_queries, int64_t q, int k
     const int64_t qn = std::min(qb, q - b0);
     const float* qs = d_queries + b0 * ld;
     const unsigned qtiles = static_cast<unsigned>((qn + QT - 1) / QT);
-    {
+    if (coded) {
+      hipLaunchKernelGGL(q16_queries_kernel, dim3(static_cast<unsigned>((qn + 255) / 256)),
+                         dim3(256), 0, stream, qs, qn, static_cast<int>(index->d), ld,
+                         index->q16_lo, index->q16_step, qcodes.as<uint32_t>(), qerr.as<double>());
+      SC_HIP(hipGetLastError());
+      ProfScope pscope("knn_scan_q16", stream);
+      const uint32_t* rq = static_cast<const uint32_t*>(index->rows_q16);
+      if (prefix) {
+        hipLaunchKernelGGL(knn_scan_l1q_kernel, dim3(qtiles, 1), dim3(NT), q16_lds, stream, rq,
+                           index->err_q16, int64_t{0}, prefix, ldq, qcodes.as<uint32_t>(),
+                           qerr.as<double>(), qn, k, prefix, 1, rho,
+                           static_cast<const uint32_t*>(nullptr), seed.as<uint32_t>(),
+                           pidx.as<uint32_t>(), pcnt.as<uint32_t>(), cap_p);
+        SC_HIP(hipGetLastError());
+      }
+      hipLaunchKernelGGL(knn_scan_l1q_kernel, dim3(qtiles, static_cast<unsigned>(plan.nseg)),
+                         dim3(NT), q16_lds, stream, rq, index->err_q16, prefix, n, ldq,
+                         qcodes.as<uint32_t>(), qerr.as<double>(), qn, k, plan.seg_len, plan.nseg,
+                         rho, prefix ? static_cast<const uint32_t*>(seed.as<uint32_t>()) : nullptr,
+                         static_cast<uint32_t*>(nullptr), cidx.as<uint32_t>(), ccnt.as<uint32_t>(),
+                         plan.cap);
+      SC_HIP(hipGetLastError());
+    } else {
       ProfScope pscope("knn_scan", stream);
       if (prefix) {
         hipLaunchKernelGGL(knn_scan_l1f32_kernel, dim3(qtiles, 1), dim3(NT), 0, stream, refs,

@@ -65,10 +65,13 @@ def test_manhattan(dtype):
     assert_exact(*run_both(ref, qs, 8, "manhattan"))
 
 
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
 @pytest.mark.parametrize("p", [1, 2, 3])
-def test_minkowski(p):
-    ref = synth.lognormal_rows(31, 4_000, 24).astype(np.float64)
-    qs = synth.lognormal_rows(32, 60, 24).astype(np.float64)
+def test_minkowski(p, dtype):
+    """f32 rows take (a - b) in f32, widened, |.|^p summed in f64
+    (minkowski.rs:50-67), like the f64 rows."""
+    ref = synth.lognormal_rows(31, 4_000, 24).astype(dtype)
+    qs = synth.lognormal_rows(32, 60, 24).astype(dtype)
     (gi, gd, gc), (oi, od, oc) = run_both(ref, qs, 6, "minkowski", p)
     np.testing.assert_array_equal(gc, oc)
     np.testing.assert_array_equal(gi, oi)
@@ -208,3 +211,49 @@ def test_device_allocation_failure_is_a_clean_error_and_the_library_recovers():
         gi, gd, gc = ix.knn(qs, 5, S.SC_EUCLIDIAN, 2, mode)
         oi, od, oc = O.linear_find_batch(ref, qs, 5)
         np.testing.assert_array_equal(gi, oi)
+
+
+def test_concurrent_searches_on_one_index_from_several_threads():
+    """Calls on one index from several host threads run on per-thread streams
+    with per-call workspaces: every answer equals the same search run alone.
+    The index is fresh, so the first mixed calls also race to build its BF16
+    copy (built once, under the index's lock)."""
+    import threading
+
+    ref = synth.lognormal_rows(61, 60_000, 48)
+    qs = synth.lognormal_rows(62, 2_000, 48)
+    jobs = [(S.SC_EUCLIDIAN, S.SC_MIXED, 10), (S.SC_EUCLIDIAN, S.SC_PLAIN_FP64, 7),
+            (S.SC_MANHATTAN, S.SC_MIXED, 5), (S.SC_MANHATTAN, S.SC_PLAIN_FP64, 3)]
+    ix0 = S.Index(ref)
+    want = {}
+    for j, (m, mode, k) in enumerate(jobs):
+        want[j] = ix0.knn(qs[200 * j:200 * j + 600], k, m, 2, mode)
+    ix0.close()
+
+    ix = S.Index(ref)
+    got = {}
+    errors = []
+
+    def worker(t):
+        try:
+            for rep in range(3):
+                j = (t + rep) % len(jobs)
+                m, mode, k = jobs[j]
+                got[(t, rep)] = (j, ix.knn(qs[200 * j:200 * j + 600], k, m, 2, mode))
+        except Exception as e:  # reported below, with the thread
+            errors.append((t, repr(e)))
+
+    threads = [threading.Thread(target=worker, args=(t,)) for t in range(6)]
+    for th in threads:
+        th.start()
+    for th in threads:
+        th.join()
+    ix.close()
+    assert not errors, errors
+    assert len(got) == 18
+    for (t, rep), (j, (gi, gd, gc)) in got.items():
+        wi, wd, wc = want[j]
+        np.testing.assert_array_equal(gc, wc, err_msg=f"thread {t} call {rep}")
+        np.testing.assert_array_equal(gi, wi, err_msg=f"thread {t} call {rep}")
+        np.testing.assert_array_equal(gd.view(np.uint64), wd.view(np.uint64),
+                                      err_msg=f"thread {t} call {rep}")

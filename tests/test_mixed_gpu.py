@@ -1,6 +1,8 @@
 """SC_MIXED kNN: BF16 MFMA candidates + FP64 replay must equal the oracle bit
 for bit (indices in heap-array order and distance bits), i.e. zero index
 mismatches against the FP64 brute-force form on every input."""
+import ctypes as C
+
 import numpy as np
 import pytest
 
@@ -129,6 +131,83 @@ def test_mixed_manhattan_ties_non_finite_and_fallbacks():
     # k above the tiled limit and f64 rows take the exact form
     check_l1(ref, qs, 70)
     check_l1(ref.astype(np.float64), qs.astype(np.float64), 9)
+
+
+def _coded_scan_ran(fn):
+    """Runs fn with the launch timers on; True when the 16-bit coded
+    Manhattan scan was launched."""
+    lib = S.lib()
+    lib.sc_profile_enable.argtypes = [C.c_int]
+    lib.sc_profile_read.argtypes = [C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_uint64)]
+    lib.sc_profile_enable(1)
+    try:
+        out = fn()
+        ms, cnt = C.c_double(), C.c_uint64()
+        lib.sc_profile_read(b"knn_scan_q16", C.byref(ms), C.byref(cnt))
+    finally:
+        lib.sc_profile_enable(0)
+    return out, cnt.value > 0
+
+
+@pytest.mark.parametrize("kind,n,d,k", [("gaussian", 30_000, 96, 20), ("gaussian", 300_000, 40, 10),
+                                        ("uniform01", 20_000, 128, 5), ("lognormal", 25_000, 33, 64),
+                                        ("gaussian", 15_000, 7, 1)])
+def test_mixed_manhattan_coded_scan(kind, n, d, k):
+    """Indexes whose values span a range small against their mean |v| take
+    the 16-bit coded scan (v_sad_u16 over code pairs, bound from the code
+    errors); the answers must still equal the FP64 form bit for bit and the
+    oracle.  Covers odd widths (padding dimensions), the prefix-seeded plan
+    (300,000 rows), k = 1 and k = 64."""
+    rng = np.random.default_rng(7 + d)
+    if kind == "gaussian":
+        ref = rng.normal(size=(n, d)).astype(np.float32)
+        qs = rng.normal(size=(120, d)).astype(np.float32)
+    elif kind == "uniform01":
+        ref = rng.random(size=(n, d)).astype(np.float32)
+        qs = rng.random(size=(120, d)).astype(np.float32)
+    else:
+        ref = synth.lognormal_rows(130 + d, n, d)
+        qs = synth.lognormal_rows(131 + d, 120, d)
+    _, coded = _coded_scan_ran(lambda: check_l1(ref, qs, k))
+    assert coded
+
+
+def test_mixed_manhattan_coded_scan_query_edges():
+    """Queries outside the index's code range are clamped (their code error
+    widens the bound), a query with a NaN or a huge value has no threshold
+    (its lists overflow and the serial kernel answers it), and exact integer
+    data codes without error; every answer equals the FP64 form's."""
+    rng = np.random.default_rng(140)
+    ref = rng.uniform(-1.0, 1.0, size=(40_000, 24)).astype(np.float32)
+    qs = rng.uniform(-3.0, 3.0, size=(100, 24)).astype(np.float32)
+    qs[5, 2] = np.nan
+    qs[6, :] = 1e35
+    qs[7, :] = 0.0
+    _, coded = _coded_scan_ran(lambda: check_l1(ref, qs, 8))
+    assert coded
+    ref = rng.integers(-5, 6, size=(30_000, 16)).astype(np.float32)
+    qs = rng.integers(-5, 6, size=(80, 16)).astype(np.float32)
+    _, coded = _coded_scan_ran(lambda: check_l1(ref, qs, 12))
+    assert coded
+
+
+def test_mixed_manhattan_coded_scan_declined():
+    """Non-finite rows, a value range far wider than the typical value, and
+    rows too wide to stage keep the f32 scan; answers are unchanged."""
+    rng = np.random.default_rng(150)
+    ref = rng.normal(size=(8_000, 20)).astype(np.float32)
+    ref[3, 3] = np.inf
+    qs = rng.normal(size=(40, 20)).astype(np.float32)
+    _, coded = _coded_scan_ran(lambda: check_l1(ref, qs, 5))
+    assert not coded
+    ref = rng.normal(size=(8_000, 20)).astype(np.float32)
+    ref[100, 0] = 1e9
+    _, coded = _coded_scan_ran(lambda: check_l1(ref, qs, 5))
+    assert not coded
+    ref = rng.normal(size=(3_000, 900)).astype(np.float32)
+    qs = rng.normal(size=(20, 900)).astype(np.float32)
+    _, coded = _coded_scan_ran(lambda: check_l1(ref, qs, 5))
+    assert not coded
 
 
 @pytest.mark.parametrize("d,k,kind,n", [(16, 10, "lognormal", 2_200_000), (48, 32, "ties", 2_200_000),
