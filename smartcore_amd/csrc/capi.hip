@@ -5,7 +5,9 @@
 #include <algorithm>
 #include <atomic>
 #include <cstring>
+#include <map>
 #include <mutex>
+#include <unordered_map>
 #include <string>
 #include <thread>
 #include <vector>
@@ -19,7 +21,7 @@ thread_local std::string t_last_error;
 thread_local sc_stats t_stats{};
 
 // One non-blocking stream per host thread and device, at the default
-// priority.
+// priority; its scratch is cached.
 hipStream_t thread_stream(int device) {
   static constexpr int kMaxDev = 64;
   thread_local hipStream_t streams[kMaxDev] = {};
@@ -27,9 +29,26 @@ hipStream_t thread_stream(int device) {
   if (!streams[device]) {
     if (hipStreamCreateWithFlags(&streams[device], hipStreamNonBlocking) != hipSuccess)
       streams[device] = nullptr;
+    else
+      scratch_register(streams[device]);
   }
   return streams[device];
 }
+
+struct ScratchCache {
+  struct PerStream {
+    std::multimap<size_t, void*> blocks;  // by size
+    size_t bytes = 0;
+  };
+  std::mutex mu;
+  std::unordered_map<hipStream_t, PerStream> streams;
+};
+// never destroyed: threads may return blocks during process exit
+ScratchCache& scratch_cache() {
+  static ScratchCache* c = new ScratchCache();
+  return *c;
+}
+constexpr size_t kScratchKeepPerStream = size_t(8) << 30;
 
 // Scratch of a call is stream-ordered (hipMallocAsync / hipFreeAsync from
 // the device's default pool).  The pool keeps up to kPoolKeepBytes of freed
@@ -291,6 +310,78 @@ ProfScope::~ProfScope() {
 }
 
 bool profiling_enabled() { return g_prof_on.load(std::memory_order_relaxed); }
+
+void scratch_register(hipStream_t s) {
+  ScratchCache& c = scratch_cache();
+  std::lock_guard<std::mutex> lk(c.mu);
+  c.streams[s];
+}
+
+void scratch_release(hipStream_t s) {
+  ScratchCache& c = scratch_cache();
+  std::lock_guard<std::mutex> lk(c.mu);
+  auto it = c.streams.find(s);
+  if (it == c.streams.end()) return;
+  for (auto& b : it->second.blocks) (void)hipFreeAsync(b.second, s);
+  c.streams.erase(it);
+}
+
+void scratch_trim(hipStream_t s) {
+  ScratchCache& c = scratch_cache();
+  std::lock_guard<std::mutex> lk(c.mu);
+  auto it = c.streams.find(s);
+  if (it == c.streams.end()) return;
+  for (auto& b : it->second.blocks) (void)hipFreeAsync(b.second, s);
+  it->second.blocks.clear();
+  it->second.bytes = 0;
+}
+
+size_t scratch_round(size_t want) {
+  // powers of two up to 64 MiB, then whole 64 MiB
+  constexpr size_t kBig = size_t(64) << 20;
+  if (want >= kBig) return (want + kBig - 1) / kBig * kBig;
+  size_t r = 256;
+  while (r < want) r <<= 1;
+  return r;
+}
+
+namespace {
+bool capturing(hipStream_t s) {
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  return hipStreamIsCapturing(s, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone;
+}
+}  // namespace
+
+void* scratch_take(hipStream_t s, size_t want, size_t* cap) {
+  ScratchCache& c = scratch_cache();
+  {
+    std::lock_guard<std::mutex> lk(c.mu);
+    auto it = c.streams.find(s);
+    if (it == c.streams.end()) return nullptr;
+    auto& ps = it->second;
+    // the smallest block that fits and is not far larger than needed
+    auto b = ps.blocks.lower_bound(want);
+    if (b == ps.blocks.end() || b->first > 2 * scratch_round(want)) return nullptr;
+    if (capturing(s)) return nullptr;
+    void* p = b->second;
+    *cap = b->first;
+    ps.bytes -= b->first;
+    ps.blocks.erase(b);
+    return p;
+  }
+}
+
+bool scratch_give(hipStream_t s, void* p, size_t cap) {
+  ScratchCache& c = scratch_cache();
+  std::lock_guard<std::mutex> lk(c.mu);
+  auto it = c.streams.find(s);
+  if (it == c.streams.end() || capturing(s)) return false;
+  auto& ps = it->second;
+  if (ps.bytes + cap > kScratchKeepPerStream) return false;
+  ps.blocks.emplace(cap, p);
+  ps.bytes += cap;
+  return true;
+}
 
 void prof_clear_locked() {
   for (auto& r : g_prof) {

@@ -92,19 +92,52 @@ struct sc_index {
 
 namespace sc {
 
+// Scratch blocks kept per library-owned stream (the calling thread's stream
+// and the server's slot streams).  A block freed on such a stream goes back
+// to that stream's cache and serves a later call on the same stream: reuse
+// is ordered by the stream itself, with no allocator call.  Other streams
+// (a caller's, a graph being captured) allocate and free stream-ordered.
+// A stream keeps at most a few GiB; beyond that blocks are freed.
+void scratch_register(hipStream_t s);
+// Frees the stream's cached blocks (stream-ordered) and forgets the stream.
+void scratch_release(hipStream_t s);
+// A cached block of at least `want` bytes for stream s (its size in *cap),
+// or null.
+void* scratch_take(hipStream_t s, size_t want, size_t* cap);
+// Returns the block to s's cache; false when s keeps no cache.
+bool scratch_give(hipStream_t s, void* p, size_t cap);
+// Allocation size for a request (rounded so blocks serve similar requests).
+size_t scratch_round(size_t want);
+// Frees every block cached for s (after an allocation failure).
+void scratch_trim(hipStream_t s);
+
 // Stream-ordered scratch allocation that is released when the object dies.
 struct DevBuf {
   void* p = nullptr;
   hipStream_t s = nullptr;
+  size_t cap = 0;
   DevBuf() = default;
   DevBuf(const DevBuf&) = delete;
   DevBuf& operator=(const DevBuf&) = delete;
-  ~DevBuf() {
-    if (p) (void)hipFreeAsync(p, s);
+  ~DevBuf() { release(); }
+  void release() {
+    if (p && !scratch_give(s, p, cap)) (void)hipFreeAsync(p, s);
+    p = nullptr;
   }
   hipError_t alloc(size_t bytes, hipStream_t stream) {
+    release();
     s = stream;
-    return hipMallocAsync(&p, bytes ? bytes : 16, stream);
+    const size_t want = bytes ? bytes : 16;
+    if ((p = scratch_take(stream, want, &cap)) != nullptr) return hipSuccess;
+    cap = scratch_round(want);
+    hipError_t e = hipMallocAsync(&p, cap, stream);
+    if (e == hipErrorOutOfMemory) {
+      (void)hipGetLastError();
+      scratch_trim(stream);
+      e = hipMallocAsync(&p, cap, stream);
+    }
+    if (e != hipSuccess) p = nullptr;
+    return e;
   }
   template <typename T>
   T* as() const {

@@ -282,7 +282,8 @@ __global__ __launch_bounds__(NT) void knn_scan_l1f32_kernel(
   }
   if (tid < QT && my_q < q) {
     cand_count[static_cast<size_t>(my_q) * nseg + seg] = emitted;
-    if (out_thr) out_thr[my_q] = topk[tid][k - 1];
+    if (out_thr)
+      atomicMin(reinterpret_cast<unsigned int*>(out_thr) + my_q, __float_as_uint(topk[tid][k - 1]));
   }
 }
 
@@ -441,7 +442,7 @@ __global__ __launch_bounds__(NT) void knn_scan_l1q_kernel(
   }
   if (tid < QT && my_q < q) {
     cand_count[static_cast<size_t>(my_q) * nseg + seg] = emitted;
-    if (out_vk) out_vk[my_q] = topk[tid][k - 1];
+    if (out_vk) atomicMin(out_vk + my_q, topk[tid][k - 1]);
   }
 }
 
@@ -571,70 +572,106 @@ struct CandSet {
   int cap;
 };
 
+// HeapSelection's replay over the stored candidates, one wavefront per
+// query: the earlier rows' lists (an optional prefix set), then the main
+// set, each in row order.  Entries are taken 64 at a time (lane = entry);
+// those whose distance is below the heap's largest (the reference's strict
+// <) enter one after another, in row order, by lane 0's sift-down on the
+// wave's LDS heap, and the rest are tested again against the new top.
+constexpr int kReplayWaves = 4;
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+__device__ __forceinline__ double lane_value(double v, int l) {
+  const int lo = __builtin_amdgcn_readlane(static_cast<int>(__double_as_longlong(v)), l);
+  const int hi = __builtin_amdgcn_readlane(static_cast<int>(__double_as_longlong(v) >> 32), l);
+  return __longlong_as_double((static_cast<long long>(hi) << 32) |
+                              static_cast<unsigned long long>(static_cast<uint32_t>(lo)));
+}
+
 template <int M>
-__global__ void knn_replay_kernel(int64_t q, int k, double p, CandSet pre, int nseg, int cap,
-                                  const uint32_t* __restrict__ cand_idx,
-                                  const double* __restrict__ cand_s,
-                                  const uint32_t* __restrict__ cand_count, int64_t* __restrict__ out_idx,
-                                  double* __restrict__ out_dist, uint32_t* __restrict__ out_count,
-                                  uint32_t* __restrict__ overflow_list,
-                                  uint32_t* __restrict__ overflow_n,
-                                  unsigned long long* __restrict__ total_cand) {
-  const int64_t qi = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (qi >= q) return;
-  double* hd = out_dist + qi * k;
-  int64_t* hi = out_idx + qi * k;
-  for (int j = 0; j < k; ++j) {
-    hd[j] = INFINITY;
-    hi[j] = -1;
+__global__ __launch_bounds__(64 * kReplayWaves) void knn_replay_kernel(
+    int64_t q, int k, double p, CandSet pre, int nseg, int cap,
+    const uint32_t* __restrict__ cand_idx, const double* __restrict__ cand_s,
+    const uint32_t* __restrict__ cand_count, int64_t* __restrict__ out_idx,
+    double* __restrict__ out_dist, uint32_t* __restrict__ out_count,
+    uint32_t* __restrict__ overflow_list, uint32_t* __restrict__ overflow_n,
+    unsigned long long* __restrict__ total_cand) {
+  __shared__ double s_hd[kReplayWaves][KMAX_TILED];
+  __shared__ int64_t s_hi[kReplayWaves][KMAX_TILED];
+  const int lane = threadIdx.x & 63;
+  const int w = threadIdx.x >> 6;
+  const int64_t qi = static_cast<int64_t>(blockIdx.x) * kReplayWaves + w;
+  if (qi >= q) return;  // the whole wave
+  bool over = false;
+  for (int s2 = lane; s2 < pre.nseg; s2 += 64)
+    over = over || pre.count[static_cast<size_t>(qi) * pre.nseg + s2] > static_cast<uint32_t>(pre.cap);
+  for (int s2 = lane; s2 < nseg; s2 += 64)
+    over = over || cand_count[static_cast<size_t>(qi) * nseg + s2] > static_cast<uint32_t>(cap);
+  if (__ballot(over) != 0) {
+    if (lane == 0) {
+      const uint32_t slot = atomicAdd(overflow_n, 1u);
+      overflow_list[slot] = static_cast<uint32_t>(qi);
+      out_count[qi] = 0;
+    }
+    return;
   }
+  double* hd = s_hd[w];
+  int64_t* hi = s_hi[w];
+  if (lane < k) {
+    hd[lane] = INFINITY;
+    hi[lane] = -1;
+  }
+  wave_sync();
+  double top = INFINITY;
   unsigned long long seen = 0;
-  // the earlier rows' lists (an optional prefix set), then the main set
-  for (int s = 0; s < pre.nseg; ++s) {
-    const uint32_t cnt = pre.count[static_cast<size_t>(qi) * pre.nseg + s];
-    if (cnt > static_cast<uint32_t>(pre.cap)) {
-      const uint32_t slot = atomicAdd(overflow_n, 1u);
-      overflow_list[slot] = static_cast<uint32_t>(qi);
-      out_count[qi] = 0;
-      return;
-    }
-    const size_t base = (static_cast<size_t>(qi) * pre.nseg + s) * static_cast<size_t>(pre.cap);
-    for (uint32_t c = 0; c < cnt; ++c) {
-      const double d = finalize<M>(pre.s[base + c], p);
-      if (d < hd[0]) heap_push_root(hd, hi, k, d, static_cast<int64_t>(pre.idx[base + c]));
-    }
+  auto walk = [&](const uint32_t* idx, const double* sv, size_t base, uint32_t cnt) {
     seen += cnt;
-  }
-  for (int s = 0; s < nseg; ++s) {
-    const uint32_t cnt = cand_count[static_cast<size_t>(qi) * nseg + s];
-    if (cnt > static_cast<uint32_t>(cap)) {
-      const uint32_t slot = atomicAdd(overflow_n, 1u);
-      overflow_list[slot] = static_cast<uint32_t>(qi);
-      out_count[qi] = 0;
-      return;
+    for (uint32_t c0 = 0; c0 < cnt; c0 += 64) {
+      const bool valid = c0 + lane < cnt;
+      const double d = valid ? finalize<M>(sv[base + c0 + lane], p) : 0.0;
+      const uint32_t row = valid ? idx[base + c0 + lane] : 0u;
+      uint64_t m = __ballot(valid && d < top);
+      while (m) {
+        const int src = __builtin_ctzll(m);
+        const double dl = lane_value(d, src);
+        const uint32_t rl = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(row), src));
+        if (lane == 0) heap_push_root(hd, hi, k, dl, static_cast<int64_t>(rl));
+        wave_sync();
+        top = hd[0];
+        m &= m - 1;
+        m &= __ballot(valid && d < top);
+      }
     }
-    const size_t base = (static_cast<size_t>(qi) * nseg + s) * static_cast<size_t>(cap);
-    for (uint32_t c = 0; c < cnt; ++c) {
-      const double d = finalize<M>(cand_s[base + c], p);
-      if (d < hd[0]) heap_push_root(hd, hi, k, d, static_cast<int64_t>(cand_idx[base + c]));
-    }
-    seen += cnt;
+  };
+  for (int s2 = 0; s2 < pre.nseg; ++s2) {
+    const size_t base = (static_cast<size_t>(qi) * pre.nseg + s2) * static_cast<size_t>(pre.cap);
+    walk(pre.idx, pre.s, base, pre.count[static_cast<size_t>(qi) * pre.nseg + s2]);
   }
-  atomicAdd(total_cand, seen);
+  for (int s2 = 0; s2 < nseg; ++s2) {
+    const size_t base = (static_cast<size_t>(qi) * nseg + s2) * static_cast<size_t>(cap);
+    walk(cand_idx, cand_s, base, cand_count[static_cast<size_t>(qi) * nseg + s2]);
+  }
+  if (lane == 0) atomicAdd(total_cand, seen);
   // linear_search.rs:79-83: keep heap-array order, drop None entries.
-  int c = 0;
-  for (int j = 0; j < k; ++j) {
-    if (hi[j] >= 0) {
-      hi[c] = hi[j];
-      hd[c] = hd[j];
-      ++c;
-    }
+  const bool full = lane < k && hi[lane] >= 0;
+  const uint64_t fm = __ballot(full);
+  const int c = __popcll(fm);
+  const int64_t vi = lane < k ? hi[lane] : -1;
+  const double vd = lane < k ? hd[lane] : 0.0;
+  wave_sync();
+  if (full) {
+    const int pos = __popcll(fm & ((uint64_t(1) << lane) - 1));
+    out_idx[qi * k + pos] = vi;
+    out_dist[qi * k + pos] = vd;
   }
-  for (int j = c; j < k; ++j) {
-    hi[j] = -1;
-    hd[j] = NAN;
+  if (lane >= c && lane < k) {
+    out_idx[qi * k + lane] = -1;
+    out_dist[qi * k + lane] = NAN;
   }
-  out_count[qi] = static_cast<uint32_t>(c);
+  if (lane == 0) out_count[qi] = static_cast<uint32_t>(c);
 }
 
 // One workgroup per listed query; persistent over the list so the grid can
@@ -778,8 +815,9 @@ int run_exact(const sc_index* index, const T* d_queries, int64_t q, int k, doubl
       }
       {
         ProfScope ps("knn_replay", stream);
-        hipLaunchKernelGGL((knn_replay_kernel<M>), dim3(static_cast<unsigned>((qn + 127) / 128)),
-                           dim3(128), 0, stream, qn, k, p, CandSet{nullptr, nullptr, nullptr, 0, 1},
+        hipLaunchKernelGGL((knn_replay_kernel<M>),
+                           dim3(static_cast<unsigned>((qn + kReplayWaves - 1) / kReplayWaves)),
+                           dim3(64 * kReplayWaves), 0, stream, qn, k, p, CandSet{nullptr, nullptr, nullptr, 0, 1},
                            plan.nseg, plan.cap,
                            cidx.as<uint32_t>(), cs.as<double>(), ccnt.as<uint32_t>(),
                            d_idx + b0 * k, d_dist + b0 * k, d_count + b0, olist.as<uint32_t>(),
@@ -823,10 +861,12 @@ int run_exact(const sc_index* index, const T* d_queries, int64_t q, int k, doubl
 
 // SC_MIXED Manhattan on f32 rows: bounded f32 scan, rescoring in the
 // reference arithmetic, replay; overflowing queries take the serial kernel.
-// Indexes of at least 4 x 65,536 rows scan their first 65,536 rows as one
-// segment first; the k-th U found there seeds the threshold of every later
-// segment (it is at or above the heap's threshold from that row on), so the
-// later segments store only the rows that can still enter.
+// Indexes of at least 4 x 65,536 rows scan their first 65,536 rows first, in
+// one segment or (for few queries) up to 64; the smallest of the segments'
+// k-th U seeds the threshold of every later segment (each is at or above
+// the k-th S of the whole prefix, hence at or above the heap's threshold
+// from that row on), so the later segments store only the rows that can
+// still enter.
 constexpr int64_t kL1Prefix = 65536;
 
 // Builds the 16-bit coded copy of an f32 index once (under its lock).  The
@@ -933,15 +973,20 @@ int run_l1_mixed(const sc_index* index, const float* d_queries, int64_t q, int k
   DevBuf olist;
   SC_HIP(olist.alloc(sizeof(uint32_t) * std::max<int64_t>(q, 1), stream));
   const int64_t prefix = n >= 4 * kL1Prefix ? kL1Prefix : 0;
-  // the prefix list: the running-threshold bound for one segment of
-  // `prefix` rows
+  // The prefix runs in nseg_p segments (more when few query tiles would
+  // leave the chip idle); each segment's k-th bound caps the k-th sum of all
+  // the prefix rows, so the smallest of them seeds every later segment.
+  const int64_t qtiles_all = (q + QT - 1) / QT;
+  int nseg_p = 1;  // a power of two, so the segments tile the prefix exactly
+  while (prefix && nseg_p < 64 && qtiles_all * nseg_p * 2 <= 256) nseg_p *= 2;
+  const int64_t seg_p = prefix / nseg_p;
   const int cap_p = prefix ? static_cast<int>(std::min<int64_t>(
-                                 prefix, static_cast<int64_t>(3.0 * k * (1.0 + std::log(
-                                             static_cast<double>(prefix) / k))) + 64))
+                                 seg_p, static_cast<int64_t>(3.0 * k * (1.0 + std::log(std::max(
+                                            1.0, static_cast<double>(seg_p) / k)))) + 64))
                            : 1;
   Plan plan = make_plan(n - prefix, q, k);
   const size_t per_query = static_cast<size_t>(plan.nseg) * plan.cap * 12 + plan.nseg * 4 +
-                           (prefix ? static_cast<size_t>(cap_p) * 12 + 8 : 0);
+                           (prefix ? static_cast<size_t>(nseg_p) * (cap_p * 12 + 4) + 4 : 0);
   int64_t qb = std::max<int64_t>(QT, static_cast<int64_t>((size_t(1) << 30) / per_query) / QT * QT);
   qb = std::min(qb, q);
   DevBuf cidx, cs, ccnt, pidx, ps, pcnt, seed, qcodes, qerr;
@@ -953,9 +998,9 @@ int run_l1_mixed(const sc_index* index, const float* d_queries, int64_t q, int k
   SC_HIP(cs.alloc(sizeof(double) * qb * plan.nseg * plan.cap, stream));
   SC_HIP(ccnt.alloc(sizeof(uint32_t) * qb * plan.nseg, stream));
   if (prefix) {
-    SC_HIP(pidx.alloc(sizeof(uint32_t) * qb * cap_p, stream));
-    SC_HIP(ps.alloc(sizeof(double) * qb * cap_p, stream));
-    SC_HIP(pcnt.alloc(sizeof(uint32_t) * qb, stream));
+    SC_HIP(pidx.alloc(sizeof(uint32_t) * qb * nseg_p * cap_p, stream));
+    SC_HIP(ps.alloc(sizeof(double) * qb * nseg_p * cap_p, stream));
+    SC_HIP(pcnt.alloc(sizeof(uint32_t) * qb * nseg_p, stream));
     SC_HIP(seed.alloc(sizeof(float) * qb, stream));
   }
   for (int64_t b0 = 0; b0 < q; b0 += qb) {
@@ -970,9 +1015,11 @@ int run_l1_mixed(const sc_index* index, const float* d_queries, int64_t q, int k
       ProfScope pscope("knn_scan_q16", stream);
       const uint32_t* rq = static_cast<const uint32_t*>(index->rows_q16);
       if (prefix) {
-        hipLaunchKernelGGL(knn_scan_l1q_kernel, dim3(qtiles, 1), dim3(NT), q16_lds, stream, rq,
-                           index->err_q16, int64_t{0}, prefix, ldq, qcodes.as<uint32_t>(),
-                           qerr.as<double>(), qn, k, prefix, 1, rho,
+        // the segments' k-th V meet by atomicMin: start at "none"
+        SC_HIP(hipMemsetAsync(seed.p, 0xff, sizeof(uint32_t) * qn, stream));
+        hipLaunchKernelGGL(knn_scan_l1q_kernel, dim3(qtiles, static_cast<unsigned>(nseg_p)),
+                           dim3(NT), q16_lds, stream, rq, index->err_q16, int64_t{0}, prefix, ldq,
+                           qcodes.as<uint32_t>(), qerr.as<double>(), qn, k, seg_p, nseg_p, rho,
                            static_cast<const uint32_t*>(nullptr), seed.as<uint32_t>(),
                            pidx.as<uint32_t>(), pcnt.as<uint32_t>(), cap_p);
         SC_HIP(hipGetLastError());
@@ -987,9 +1034,11 @@ int run_l1_mixed(const sc_index* index, const float* d_queries, int64_t q, int k
     } else {
       ProfScope pscope("knn_scan", stream);
       if (prefix) {
-        hipLaunchKernelGGL(knn_scan_l1f32_kernel, dim3(qtiles, 1), dim3(NT), 0, stream, refs,
-                           int64_t{0}, prefix, ld, qs, qn, k, prefix, 1,
-                           static_cast<const float*>(nullptr), seed.as<float>(),
+        // the segments' k-th U (>= 0) meet by atomicMin on their bits
+        SC_HIP(hipMemsetAsync(seed.p, 0xff, sizeof(float) * qn, stream));
+        hipLaunchKernelGGL(knn_scan_l1f32_kernel, dim3(qtiles, static_cast<unsigned>(nseg_p)),
+                           dim3(NT), 0, stream, refs, int64_t{0}, prefix, ld, qs, qn, k, seg_p,
+                           nseg_p, static_cast<const float*>(nullptr), seed.as<float>(),
                            pidx.as<uint32_t>(), pcnt.as<uint32_t>(), cap_p);
         SC_HIP(hipGetLastError());
       }
@@ -1003,9 +1052,10 @@ int run_l1_mixed(const sc_index* index, const float* d_queries, int64_t q, int k
     {
       ProfScope pscope("knn_mixed_rescore", stream);
       if (prefix) {
-        hipLaunchKernelGGL((knn_rescore_kernel<float, kManhattan>), dim3(static_cast<unsigned>(qn)),
-                           dim3(64), 0, stream, refs, ld, qs, qn, 1.0, 1, cap_p,
-                           pidx.as<uint32_t>(), pcnt.as<uint32_t>(), ps.as<double>());
+        hipLaunchKernelGGL((knn_rescore_kernel<float, kManhattan>),
+                           dim3(static_cast<unsigned>(qn * nseg_p)), dim3(64), 0, stream, refs, ld,
+                           qs, qn, 1.0, nseg_p, cap_p, pidx.as<uint32_t>(), pcnt.as<uint32_t>(),
+                           ps.as<double>());
         SC_HIP(hipGetLastError());
       }
       hipLaunchKernelGGL((knn_rescore_kernel<float, kManhattan>),
@@ -1017,10 +1067,11 @@ int run_l1_mixed(const sc_index* index, const float* d_queries, int64_t q, int k
     {
       ProfScope pscope("knn_replay", stream);
       const CandSet pre = prefix ? CandSet{pidx.as<uint32_t>(), ps.as<double>(), pcnt.as<uint32_t>(),
-                                           1, cap_p}
+                                           nseg_p, cap_p}
                                  : CandSet{nullptr, nullptr, nullptr, 0, 1};
-      hipLaunchKernelGGL((knn_replay_kernel<kManhattan>), dim3(static_cast<unsigned>((qn + 127) / 128)),
-                         dim3(128), 0, stream, qn, k, 1.0, pre, plan.nseg, plan.cap,
+      hipLaunchKernelGGL((knn_replay_kernel<kManhattan>),
+                         dim3(static_cast<unsigned>((qn + kReplayWaves - 1) / kReplayWaves)),
+                         dim3(64 * kReplayWaves), 0, stream, qn, k, 1.0, pre, plan.nseg, plan.cap,
                          cidx.as<uint32_t>(), cs.as<double>(), ccnt.as<uint32_t>(),
                          d_idx + b0 * k, d_dist + b0 * k, d_count + b0, olist.as<uint32_t>(),
                          overflow_n, total_cand);

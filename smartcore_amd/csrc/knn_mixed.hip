@@ -50,8 +50,9 @@
 // Candidates are kept only if the per-row bound, which uses the stored
 // norms of y - bf16(y) and x - bf16(x), also has L below the threshold.
 // Pass 2 (replay, FP64).  Every candidate's exact s is computed in
-// parallel (a workgroup per query); then one thread per query walks its
-// lists in index order and applies the heap rule.  Rows that would not
+// parallel (a workgroup per query); then one wavefront per query walks its
+// lists in index order, 64 entries at a time, and applies the heap rule to
+// the entries below the heap's top, one after another.  Rows that would not
 // enter the reference heap are no-ops, so the final heap array equals the
 // reference's.  Queries whose lists overflow, or whose rows are not finite
 // / too large for the bound, are recomputed by the sequential exact kernel.
@@ -93,7 +94,7 @@ constexpr int kMaxK = 32;                 // largest k served by this form
 constexpr int kXcds = 8;                  // MI355X: 8 XCDs, round-robin dispatch
 constexpr int kTargetBlocks = 256;        // one workgroup per CU on 256 CUs
 constexpr int64_t kPrefixRows = 65536;    // rows of the seeding launch
-constexpr int64_t kSeedMinRows = 32 * kPrefixRows;  // indexes this large are seeded
+constexpr int64_t kSeedMinRows = 8 * kPrefixRows;  // indexes this large are seeded
 constexpr int64_t kLevelRatio = 8;        // row-range growth of the fixed-threshold levels
 constexpr int64_t kScanPadRows = 1024;    // padding rows past the index read by fixed launches
 constexpr int kMaxLists = 64 + 2 * 64 * 8;  // running lists + fixed pairs, any plan
@@ -844,50 +845,76 @@ __global__ __launch_bounds__(kNT, 2) void fixed_scan_kernel(const ScanArgs a) {
   }
 }
 
+// One wavefront per query for the sequential passes below; four per
+// workgroup.  LDS traffic of a wave is ordered by the hardware; the fence
+// keeps the compiler from moving it across the exchange points.
+constexpr int kWavesPerBlock = 4;
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+__device__ __forceinline__ double lane_value(double v, int l) {
+  const int lo = __builtin_amdgcn_readlane(static_cast<int>(__double_as_longlong(v)), l);
+  const int hi = __builtin_amdgcn_readlane(static_cast<int>(__double_as_longlong(v) >> 32), l);
+  return __longlong_as_double((static_cast<long long>(hi) << 32) |
+                              static_cast<unsigned long long>(static_cast<uint32_t>(lo)));
+}
+__device__ __forceinline__ float lane_value(float v, int l) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+
 // The k smallest of a seed (null: +inf) and the exact distances, rounded
 // up to f32, of the entries in lists [l0, l0 + nl) (candidates of rows after
 // the seed's rows, distinct): the next seed, ascending.  Entries past a
-// list's cap were not stored; leaving them out keeps a valid seed.
-__global__ void select_seed_kernel(const float* __restrict__ seed_in, const double* __restrict__ cand_S,
-                                   const uint32_t* __restrict__ cand_count, ListLayout lay, int l0,
-                                   int nl, int64_t q, int k, float* __restrict__ seed_out) {
-  const int64_t qi = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (qi >= q) return;
-  float best[kMaxK];
-  for (int j = 0; j < k; ++j) best[j] = seed_in ? seed_in[qi * k + j] : INFINITY;
+// list's cap were not stored; leaving them out keeps a valid seed.  Lanes
+// 0 .. k-1 hold the current k smallest in ascending order; a value enters
+// by one shift across the lanes.
+__global__ __launch_bounds__(64 * kWavesPerBlock) void select_seed_kernel(
+    const float* __restrict__ seed_in, const double* __restrict__ cand_S,
+    const uint32_t* __restrict__ cand_count, ListLayout lay, int l0, int nl, int64_t q, int k,
+    float* __restrict__ seed_out) {
+  const int lane = threadIdx.x & 63;
+  const int64_t qi = static_cast<int64_t>(blockIdx.x) * kWavesPerBlock + (threadIdx.x >> 6);
+  if (qi >= q) return;  // the whole wave
+  float b = (lane < k && seed_in) ? seed_in[qi * k + lane] : INFINITY;
+  if (lane >= k) b = INFINITY;
+  float kth = lane_value(b, k - 1);
   const int64_t perq = lay.per_query();
   for (int l = l0; l < l0 + nl; ++l) {
     const uint32_t c = min(cand_count[static_cast<size_t>(qi) * lay.nlists() + l],
                            static_cast<uint32_t>(lay.cap(l)));
     const double* sv = cand_S + static_cast<size_t>(qi) * perq + lay.offset(l);
-    for (uint32_t e = 0; e < c; ++e) {
-      const float v = __double2float_ru(sv[e]);
-      if (!(v < best[k - 1])) continue;
-      int pos = k - 1;
-      while (pos > 0 && best[pos - 1] > v) {
-        best[pos] = best[pos - 1];
-        --pos;
+    for (uint32_t c0 = 0; c0 < c; c0 += 64) {
+      const float v = c0 + lane < c ? __double2float_ru(sv[c0 + lane]) : INFINITY;
+      uint64_t m = __ballot(v < kth);
+      while (m) {
+        const int src = __builtin_ctzll(m);
+        const float x = lane_value(v, src);
+        float prev = __shfl_up(b, 1);
+        if (lane == 0) prev = -INFINITY;
+        if (lane < k && x < b) b = x < prev ? prev : x;
+        kth = lane_value(b, k - 1);
+        m &= m - 1;
+        m &= __ballot(v < kth);
       }
-      best[pos] = v;
     }
   }
-  for (int j = 0; j < k; ++j) seed_out[qi * k + j] = best[j];
+  if (lane < k) seed_out[qi * k + lane] = b;
 }
 
-__device__ inline void sift_down_s(double* hd, int64_t* hi, double* hs, int kk, int n) {
+// The reference heap's sift-down (heap_select.rs), on a wave's LDS heap.
+__device__ inline void sift_down_lds(double* hd, uint32_t* hi, int kk, int n) {
   while (2 * kk <= n) {
     int j = 2 * kk;
     if (j < n && hd[j] < hd[j + 1]) j += 1;
     if (hd[kk] == hd[j] || hd[kk] > hd[j]) break;
-    double td = hd[kk];
+    const double td = hd[kk];
     hd[kk] = hd[j];
     hd[j] = td;
-    int64_t ti = hi[kk];
+    const uint32_t ti = hi[kk];
     hi[kk] = hi[j];
     hi[j] = ti;
-    double ts = hs[kk];
-    hs[kk] = hs[j];
-    hs[j] = ts;
     kk = j;
   }
 }
@@ -941,114 +968,150 @@ __global__ __launch_bounds__(kRescoreThreads) void mixed_rescore_kernel(
   }
 }
 
-// Pass 2b: replay of the heap rule over the candidates in row order
-// (see the file comment): running lists one by one, fixed-threshold list
-// pairs merged by row index.  A candidate that would not enter the
-// reference heap is a no-op, so the final heap array equals the
-// reference's.
-__global__ void mixed_replay_kernel(int64_t q, int k, ListLayout lay,
-                                    const uint32_t* __restrict__ cand_idx,
-                                    const double* __restrict__ cand_S,
-                                    const uint32_t* __restrict__ cand_count,
-                                    const float* __restrict__ x2m, double* __restrict__ hs_all,
-                                    int64_t* __restrict__ out_idx, double* __restrict__ out_dist,
-                                    uint32_t* __restrict__ out_count, uint32_t* __restrict__ list,
-                                    uint32_t* __restrict__ list_n,
-                                    unsigned long long* __restrict__ counters) {
-  const int64_t qi = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (qi >= q) return;
-  if (!(x2m[qi] < INFINITY)) return;  // listed for the exact kernel already
-  double* hd = out_dist + qi * k;
-  int64_t* hi = out_idx + qi * k;
-  double* hs = hs_all + qi * k;
-  for (int j = 0; j < k; ++j) {
-    hd[j] = INFINITY;
-    hi[j] = -1;
-    hs[j] = INFINITY;
-  }
+// Pass 2b: replay of the heap rule over the candidates in row order, one
+// wavefront per query.  Entries are taken 64 at a time (lane = entry); the
+// lanes whose distance is below the heap's largest (the reference's strict
+// <) enter one after another, in row order, by lane 0's sift-down on the
+// wave's LDS heap, and the rest are tested again against the new top.  A
+// fixed launch's list pair is merged by row 64 entries at a time: an entry's
+// place is its lane plus the count of smaller rows in the other list's
+// window, and the places below 64 are the next 64 rows of the merge.
+// Queries whose lists overflowed go to the serial exact kernel's list.
+__global__ __launch_bounds__(64 * kWavesPerBlock) void mixed_replay_kernel(
+    int64_t q, int k, ListLayout lay, const uint32_t* __restrict__ cand_idx,
+    const double* __restrict__ cand_S, const uint32_t* __restrict__ cand_count,
+    const float* __restrict__ x2m, int64_t* __restrict__ out_idx, double* __restrict__ out_dist,
+    uint32_t* __restrict__ out_count, uint32_t* __restrict__ list, uint32_t* __restrict__ list_n,
+    unsigned long long* __restrict__ counters) {
+  __shared__ double s_hd[kWavesPerBlock][kMaxK];
+  __shared__ uint32_t s_hi[kWavesPerBlock][kMaxK];
+  __shared__ uint32_t s_win[kWavesPerBlock][2][64];
+  __shared__ uint32_t s_mrow[kWavesPerBlock][64];
+  __shared__ double s_mS[kWavesPerBlock][64];
+  const int lane = threadIdx.x & 63;
+  const int w = threadIdx.x >> 6;
+  const int64_t qi = static_cast<int64_t>(blockIdx.x) * kWavesPerBlock + w;
+  if (qi >= q) return;                     // the whole wave
+  if (!(x2m[qi] < INFINITY)) return;       // listed for the exact kernel already
   const int nl = lay.nlists();
   const uint32_t* cc = cand_count + static_cast<size_t>(qi) * nl;
-  for (int l = 0; l < nl; ++l) {
-    if (cc[l] > static_cast<uint32_t>(lay.cap(l))) {
+  bool over = false;
+  for (int l = lane; l < nl; l += 64) over = over || cc[l] > static_cast<uint32_t>(lay.cap(l));
+  if (__ballot(over) != 0) {
+    if (lane == 0) {
       const uint32_t slot = atomicAdd(list_n, 1u);
       list[slot] = static_cast<uint32_t>(qi);
-      return;
     }
+    return;
   }
-  const size_t qbase = static_cast<size_t>(qi) * lay.per_query();
-  unsigned long long seen = 0;
-  // hd[0] (the heap's largest distance) is kept in a register; candidates
-  // below it enter the heap (the reference's strict <)
+  double* hd = s_hd[w];
+  uint32_t* hi = s_hi[w];
+  if (lane < k) {
+    hd[lane] = INFINITY;
+    hi[lane] = 0xffffffffu;
+  }
+  wave_sync();
   double top = INFINITY;
-  auto visit = [&](double sum, uint32_t row) {
-    const double d = __builtin_sqrt(sum);
-    if (d < top) {
-      hd[0] = d;
-      hi[0] = row;
-      hs[0] = sum;
-      if (k > 1) sift_down_s(hd, hi, hs, 0, k - 1);
+  // entries of one chunk, in row order over the lanes (valid lanes first)
+  auto consume = [&](double S, uint32_t row, bool valid) {
+    const double d = __builtin_sqrt(S);
+    uint64_t m = __ballot(valid && d < top);
+    while (m) {
+      const int src = __builtin_ctzll(m);
+      const double dl = lane_value(d, src);
+      const uint32_t rl = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(row), src));
+      if (lane == 0) {
+        hd[0] = dl;
+        hi[0] = rl;
+        if (k > 1) sift_down_lds(hd, hi, 0, k - 1);
+      }
+      wave_sync();
       top = hd[0];
+      m &= m - 1;
+      m &= __ballot(valid && d < top);
     }
   };
+  const size_t qbase = static_cast<size_t>(qi) * lay.per_query();
+  unsigned long long seen = 0;
   for (int l = 0; l < lay.nrun; ++l) {
     const size_t base = qbase + lay.offset(l);
     const uint32_t n = cc[l];
     seen += n;
-    // eight entries loaded together, visited in order
-    for (uint32_t c0 = 0; c0 < n; c0 += 8) {
-      double sv[8];
-      uint32_t rv[8];
-#pragma unroll
-      for (int j = 0; j < 8; ++j) {
-        const size_t e = base + min(c0 + j, n - 1);
-        sv[j] = cand_S[e];
-        rv[j] = cand_idx[e];
-      }
-#pragma unroll
-      for (int j = 0; j < 8; ++j)
-        if (c0 + j < n) visit(sv[j], rv[j]);
+    for (uint32_t c0 = 0; c0 < n; c0 += 64) {
+      const bool valid = c0 + lane < n;
+      const double S = valid ? cand_S[base + c0 + lane] : 0.0;
+      const uint32_t row = valid ? cand_idx[base + c0 + lane] : 0u;
+      consume(S, row, valid);
     }
   }
+  uint32_t* winA = s_win[w][0];
+  uint32_t* winB = s_win[w][1];
+  uint32_t* mrow = s_mrow[w];
+  double* mS = s_mS[w];
+  // count of entries below v in a window of 64 ascending rows (absent
+  // entries are 0xffffffff)
+  auto rank_in = [&](const uint32_t* win, uint32_t v) {
+    int lo = 0;
+#pragma unroll
+    for (int step = 64; step > 0; step >>= 1)
+      if (lo + step <= 64 && win[lo + step - 1] < v) lo += step;
+    return lo;
+  };
   for (int l = lay.nrun; l < nl; l += 2) {
-    const size_t b0 = qbase + lay.offset(l), b1 = qbase + lay.offset(l + 1);
-    const uint32_t n0 = cc[l], n1 = cc[l + 1];
-    seen += n0 + n1;
-    // merge of the pair by row index; each stream's next row and distance
-    // are held in registers
-    uint32_t i0 = 0, i1 = 0;
-    uint32_t r0 = n0 > 0 ? cand_idx[b0] : 0xffffffffu, r1 = n1 > 0 ? cand_idx[b1] : 0xffffffffu;
-    double s0 = n0 > 0 ? cand_S[b0] : 0.0, s1 = n1 > 0 ? cand_S[b1] : 0.0;
-    while (i0 < n0 || i1 < n1) {
-      if (i1 >= n1 || (i0 < n0 && r0 < r1)) {
-        visit(s0, r0);
-        if (++i0 < n0) {
-          r0 = cand_idx[b0 + i0];
-          s0 = cand_S[b0 + i0];
-        }
-      } else {
-        visit(s1, r1);
-        if (++i1 < n1) {
-          r1 = cand_idx[b1 + i1];
-          s1 = cand_S[b1 + i1];
-        }
+    const size_t bA = qbase + lay.offset(l), bB = qbase + lay.offset(l + 1);
+    const uint32_t nA = cc[l], nB = cc[l + 1];
+    seen += nA + nB;
+    uint32_t ia = 0, ib = 0;
+    while (ia < nA || ib < nB) {
+      const bool okA = ia + lane < nA, okB = ib + lane < nB;
+      const uint32_t ra = okA ? cand_idx[bA + ia + lane] : 0xffffffffu;
+      const uint32_t rb = okB ? cand_idx[bB + ib + lane] : 0xffffffffu;
+      const double Sa = okA ? cand_S[bA + ia + lane] : 0.0;
+      const double Sb = okB ? cand_S[bB + ib + lane] : 0.0;
+      winA[lane] = ra;
+      winB[lane] = rb;
+      wave_sync();
+      const int pa = lane + rank_in(winB, ra);
+      const int pb = lane + rank_in(winA, rb);
+      const bool takeA = okA && pa < 64, takeB = okB && pb < 64;
+      if (takeA) {
+        mrow[pa] = ra;
+        mS[pa] = Sa;
       }
+      if (takeB) {
+        mrow[pb] = rb;
+        mS[pb] = Sb;
+      }
+      const uint32_t cA = static_cast<uint32_t>(__popcll(__ballot(takeA)));
+      const uint32_t cB = static_cast<uint32_t>(__popcll(__ballot(takeB)));
+      wave_sync();
+      const bool valid = static_cast<uint32_t>(lane) < cA + cB;
+      const double S = valid ? mS[lane] : 0.0;
+      const uint32_t row = valid ? mrow[lane] : 0u;
+      wave_sync();  // the window and merge slots are rewritten next round
+      consume(S, row, valid);
+      ia += cA;
+      ib += cB;
     }
   }
-  atomicAdd(&counters[0], seen);
-  atomicAdd(&counters[1], seen);
-  int c = 0;
-  for (int j = 0; j < k; ++j) {
-    if (hi[j] >= 0) {
-      hi[c] = hi[j];
-      hd[c] = hd[j];
-      ++c;
-    }
+  if (lane == 0) {
+    atomicAdd(&counters[0], seen);
+    atomicAdd(&counters[1], seen);
   }
-  for (int j = c; j < k; ++j) {
-    hi[j] = -1;
-    hd[j] = NAN;
+  // the heap array without its empty slots, in heap order
+  const bool full = lane < k && hi[lane] != 0xffffffffu;
+  const uint64_t fm = __ballot(full);
+  const int c = __popcll(fm);
+  if (full) {
+    const int pos = __popcll(fm & ((uint64_t(1) << lane) - 1));
+    out_idx[qi * k + pos] = static_cast<int64_t>(hi[lane]);
+    out_dist[qi * k + pos] = hd[lane];
   }
-  out_count[qi] = static_cast<uint32_t>(c);
+  if (lane >= c && lane < k) {
+    out_idx[qi * k + lane] = -1;
+    out_dist[qi * k + lane] = NAN;
+  }
+  if (lane == 0) out_count[qi] = static_cast<uint32_t>(c);
 }
 
 // Builds the BF16 copy and row terms of an f32 index once.  ldb (the BF16
@@ -1229,7 +1292,7 @@ int knn_mixed_device(const sc_index* cix, const void* d_queries, size_t q_, size
   }
   const int64_t perq = lay.per_query();
 
-  DevBuf qb, x2, cx, cidx, cS, ccnt, hs, lst, flags, seedA, seedB;
+  DevBuf qb, x2, cx, cidx, cS, ccnt, lst, flags, seedA, seedB;
   SC_HIP(qb.alloc(sizeof(uint16_t) * q * ld, stream));
   SC_HIP(x2.alloc(sizeof(float) * q, stream));
   SC_HIP(cx.alloc(sizeof(float) * 3 * q, stream));  // cxb | qa1 | qa2
@@ -1240,7 +1303,6 @@ int knn_mixed_device(const sc_index* cix, const void* d_queries, size_t q_, size
     SC_HIP(seedA.alloc(sizeof(float) * q * k, stream));
     SC_HIP(seedB.alloc(sizeof(float) * q * k, stream));
   }
-  SC_HIP(hs.alloc(sizeof(double) * q * k, stream));
   SC_HIP(lst.alloc(sizeof(uint32_t) * q, stream));
   SC_HIP(flags.alloc(64, stream));
   SC_HIP(hipMemsetAsync(flags.p, 0, 64, stream));
@@ -1284,8 +1346,9 @@ int knn_mixed_device(const sc_index* cix, const void* d_queries, size_t q_, size
   };
   // the next seed from the current one (null: none yet) and lists [l0, l0 + nl)
   auto select = [&](const float* in, int l0, int nl) -> int {
-    hipLaunchKernelGGL(select_seed_kernel, dim3(static_cast<unsigned>((q + 255) / 256)), dim3(256),
-                       0, stream, in, cS.as<double>(), ccnt.as<uint32_t>(), lay, l0, nl, q, k,
+    hipLaunchKernelGGL(select_seed_kernel,
+                       dim3(static_cast<unsigned>((q + kWavesPerBlock - 1) / kWavesPerBlock)),
+                       dim3(64 * kWavesPerBlock), 0, stream, in, cS.as<double>(), ccnt.as<uint32_t>(), lay, l0, nl, q, k,
                        seed_next);
     SC_HIP(hipGetLastError());
     std::swap(seed_cur, seed_next);
@@ -1332,10 +1395,11 @@ int knn_mixed_device(const sc_index* cix, const void* d_queries, size_t q_, size
   {
     ProfScope ps("knn_mixed_rescore", stream);
     if (int rc = rescore(l_last, lay.nlists())) return rc;
-    hipLaunchKernelGGL(mixed_replay_kernel, dim3(static_cast<unsigned>((q + 63) / 64)), dim3(64), 0,
-                       stream, q, k, lay, cidx.as<uint32_t>(), cS.as<double>(), ccnt.as<uint32_t>(),
-                       x2.as<float>(), hs.as<double>(), d_idx, d_dist, d_count, lst.as<uint32_t>(),
-                       list_n, counters);
+    hipLaunchKernelGGL(mixed_replay_kernel,
+                       dim3(static_cast<unsigned>((q + kWavesPerBlock - 1) / kWavesPerBlock)),
+                       dim3(64 * kWavesPerBlock), 0, stream, q, k, lay, cidx.as<uint32_t>(),
+                       cS.as<double>(), ccnt.as<uint32_t>(), x2.as<float>(), d_idx, d_dist, d_count,
+                       lst.as<uint32_t>(), list_n, counters);
     SC_HIP(hipGetLastError());
   }
   if (int rc = launch_serial_exact_f32(ix, qf, k, lst.as<uint32_t>(), list_n, d_idx, d_dist,

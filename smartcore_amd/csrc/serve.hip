@@ -337,8 +337,21 @@ int sc_server_create(const sc_index* index, const sc_server_config* cfg, sc_serv
   sv->budget = c.hbm_budget_bytes;
   sv->slots.resize(c.max_inflight);
   for (auto& s : sv->slots) {
-    SC_HIP(hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking));
-    SC_HIP(hipEventCreateWithFlags(&s.done, hipEventDisableTiming));
+    hipError_t e = hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking);
+    if (e == hipSuccess) {
+      scratch_register(s.stream);
+      e = hipEventCreateWithFlags(&s.done, hipEventDisableTiming);
+    }
+    if (e != hipSuccess) {
+      for (auto& t : sv->slots) {
+        if (t.done) (void)hipEventDestroy(t.done);
+        if (t.stream) {
+          scratch_release(t.stream);
+          (void)hipStreamDestroy(t.stream);
+        }
+      }
+      return hip_fail(e, "server stream");
+    }
   }
   sc_server* raw = sv.release();
   raw->sched = std::thread(scheduler_loop, raw);
@@ -452,7 +465,10 @@ int sc_server_destroy(sc_server* sv) {
   if (sv->compl_.joinable()) sv->compl_.join();
   (void)hipSetDevice(sv->index->device);
   for (auto& s : sv->slots) {
-    if (s.stream) (void)hipStreamSynchronize(s.stream);
+    if (s.stream) {
+      scratch_release(s.stream);
+      (void)hipStreamSynchronize(s.stream);
+    }
     if (s.h_q) {
       (void)hipHostFree(s.h_q);
       (void)hipHostFree(s.h_idx);

@@ -218,15 +218,14 @@ def test_mixed_manhattan_coded_scan_declined():
                                         (96, 20, "gaussian", 2_000_000),
                                         (96, 20, "lognormal", 600_000)])
 def test_mixed_seeded_large_index(d, k, kind, n):
-    """Indexes of 2,097,152 rows or more scan a 65,536-row prefix first, then
+    """Indexes of 524,288 rows or more scan a 65,536-row prefix first, then
     row ranges growing 8 times, each tested against the k-th exact distance
     of the earlier candidates.  The result must still equal the FP64 form bit
     for bit (and the oracle on a few queries), also with heavy exact ties and
     with a single neighbour.  The d = 128 cases run the scan the 4,000,000 x 128
     workload times (eight K steps plus the folded bound step) on both the
     middle and the last level.  The 2,000,000 x 96 Gaussian case is the
-    served index's shape and 600,000 rows a mid-size one; both stay below the
-    level threshold and run one running launch over many segments."""
+    served index's shape, and 600,000 rows end in a short last level."""
     nq = 3_000
     if kind == "lognormal":
         ref = synth.lognormal_rows(90 + d, n, d)
