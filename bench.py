@@ -519,8 +519,9 @@ def bench_batch(args):
     [1,000, 200,000], d in {8, 16, 32, 64}, k ~ U{4..128}, f32 blobs), 20
     Lloyd iterations each, in one sc_kmeans_batch_run call: one persistent
     kernel per iteration over every problem's row blocks, the iterations
-    replayed as one hipGraph.  The warmup call picks the grid size by timing;
-    the timed calls reuse it, so their graph covers all iterations."""
+    replayed as one hipGraph.  The warmup call picks the launch shape (waves
+    per workgroup, workgroups per CU) by timing; the timed calls reuse it, so
+    their graph covers all iterations."""
     from smartcore_amd import _capi as S
     from smartcore_amd import synth
 
@@ -541,6 +542,11 @@ def bench_batch(args):
         got = S.kmeans_batch_run(probs, iters, S.SC_MIXED)
     wall = (time.perf_counter() - t0) / args.steps
     lib.sc_profile_enable(0)
+    st = S.last_stats()
+    shape = {"waves_per_workgroup": st["launch_waves"], "rows_per_chunk": 32 * st["launch_waves"],
+             "workgroups_per_cu": st["launch_groups_per_cu"],
+             "how": "timed on the warmup call's first six iterations (4 or 8 waves x 1, 2, 4 "
+                    "workgroups per CU), reused by the timed calls"}
     gms, launches = profile_read(lib, "kmeans_batch")
     graph_s = gms / 1e3 / max(launches, 1)
     inst_iters = sum(g[4] for g in got)
@@ -581,6 +587,7 @@ def bench_batch(args):
         "unit": "problem-iterations/s",
         "ms_per_call_device": graph_s * 1e3,
         "ms_per_call_with_transfers": wall * 1e3,
+        "launch_shape": shape,
         "config": {"workload": f"{len(probs)} k-means problems, n log-uniform[1e3, 2e5] "
                                f"({rows:,} rows), d in {{8,16,32,64}}, k in 4..128, "
                                f"max_iter={iters}, one persistent launch per iteration in a hipGraph"},

@@ -76,6 +76,9 @@ typedef struct {
   uint64_t overflow_queries;   /* queries recomputed by the sequential kernel */
   uint64_t shortlist_rescored; /* SC_MIXED: shortlist rows rescored in FP64 */
   uint64_t uncertified;        /* SC_MIXED: queries whose shortlist was not certified */
+  uint64_t launch_waves;       /* batched k-means: waves per workgroup the launch-shape
+                                  tuner chose (0 for other calls) */
+  uint64_t launch_groups_per_cu; /* batched k-means: workgroups per CU it chose */
 } sc_stats;
 
 /* Opaque device-resident reference set (packed row-major copy in HBM).

@@ -72,6 +72,8 @@ class sc_stats(C.Structure):
         ("overflow_queries", C.c_uint64),
         ("shortlist_rescored", C.c_uint64),
         ("uncertified", C.c_uint64),
+        ("launch_waves", C.c_uint64),
+        ("launch_groups_per_cu", C.c_uint64),
     ]
 
 

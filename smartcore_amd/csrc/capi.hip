@@ -206,6 +206,93 @@ int download_staged(void* host, const void* dev, size_t bytes, hipStream_t strea
   return SC_OK;
 }
 
+namespace {
+// n rows of rb bytes to a stride of lb bytes, the padding zeroed (word
+// loops: rows here are a few dozen bytes, too short for memcpy calls)
+void copy_rows(char* __restrict__ dst, const char* __restrict__ src, size_t n, size_t rb, size_t lb) {
+  if (rb == lb) {
+    std::memcpy(dst, src, n * rb);
+    return;
+  }
+  if (rb % 4 == 0 && lb % 4 == 0) {
+    const size_t rw = rb / 4, lw = lb / 4;
+    for (size_t r = 0; r < n; ++r) {
+      uint32_t* __restrict__ d = reinterpret_cast<uint32_t*>(dst + r * lb);
+      const char* sr = src + r * rb;
+      for (size_t w = 0; w < rw; ++w) {
+        uint32_t v;
+        std::memcpy(&v, sr + 4 * w, 4);
+        d[w] = v;
+      }
+      for (size_t w = rw; w < lw; ++w) d[w] = 0u;
+    }
+    return;
+  }
+  for (size_t r = 0; r < n; ++r) {
+    std::memcpy(dst + r * lb, src + r * rb, rb);
+    std::memset(dst + r * lb + rb, 0, lb - rb);
+  }
+}
+}  // namespace
+
+int upload_staged_segments(void* dev, const StageSeg* segs, size_t count, hipStream_t stream) {
+  Stager& st = thread_stager();
+  SC_HIP(st.init());
+  size_t si = 0, r = 0;  // next row to stage
+  size_t dev_off = 0;
+  int b = 0;
+  struct Piece {
+    const char* src;
+    size_t n, rb, lb, at;  // rows, row bytes, padded bytes, offset in the chunk
+  };
+  std::vector<Piece> pieces;
+  while (si < count) {
+    // whole rows, in order across segments, up to one staging buffer
+    pieces.clear();
+    size_t fill = 0;
+    while (si < count) {
+      const StageSeg& g = segs[si];
+      if (r >= g.n) {
+        ++si;
+        r = 0;
+        continue;
+      }
+      const size_t room = (kStageBytes - fill) / g.ld_bytes;
+      if (room == 0) break;
+      const size_t take = std::min(room, g.n - r);
+      pieces.push_back(Piece{static_cast<const char*>(g.host) + r * g.row_bytes, take, g.row_bytes,
+                             g.ld_bytes, fill});
+      fill += take * g.ld_bytes;
+      r += take;
+    }
+    if (fill == 0) break;
+    if (st.used[b]) SC_HIP(hipEventSynchronize(st.ev[b]));
+    char* dst = static_cast<char*>(st.buf[b]);
+    auto work = [&](size_t t, size_t nt) {
+      // piece rows split evenly over nt workers
+      for (const Piece& pc : pieces) {
+        const size_t a = pc.n * t / nt, e = pc.n * (t + 1) / nt;
+        if (e > a) copy_rows(dst + pc.at + a * pc.lb, pc.src + a * pc.rb, e - a, pc.rb, pc.lb);
+      }
+    };
+    if (fill < (size_t{4} << 20)) {
+      work(0, 1);
+    } else {
+      std::thread th[kCopyThreads - 1];
+      for (int t = 1; t < kCopyThreads; ++t) th[t - 1] = std::thread(work, t, kCopyThreads);
+      work(0, kCopyThreads);
+      for (auto& t : th) t.join();
+    }
+    SC_HIP(hipMemcpyAsync(static_cast<char*>(dev) + dev_off, st.buf[b], fill, hipMemcpyHostToDevice,
+                          stream));
+    SC_HIP(hipEventRecord(st.ev[b], stream));
+    st.used[b] = true;
+    dev_off += fill;
+    b ^= 1;
+  }
+  return SC_OK;
+}
+
 int upload_staged_rows(void* dev, const void* host, size_t n, size_t row_bytes, size_t ld_bytes,
                        hipStream_t stream) {
   if (n == 0) return SC_OK;
@@ -222,10 +309,7 @@ int upload_staged_rows(void* dev, const void* host, size_t n, size_t row_bytes, 
     char* dst = static_cast<char*>(st.buf[b]);
     const char* src = static_cast<const char*>(host) + r0 * row_bytes;
     auto pad_rows = [=](size_t a, size_t e) {
-      for (size_t r = a; r < e; ++r) {
-        std::memcpy(dst + r * ld_bytes, src + r * row_bytes, row_bytes);
-        std::memset(dst + r * ld_bytes + row_bytes, 0, ld_bytes - row_bytes);
-      }
+      copy_rows(dst + a * ld_bytes, src + a * row_bytes, e - a, row_bytes, ld_bytes);
     };
     if (nr * ld_bytes < (size_t{4} << 20)) {
       pad_rows(0, nr);

@@ -157,6 +157,14 @@ int upload_staged(void* dev, const void* host, size_t bytes, hipStream_t stream)
 int upload_staged_rows(void* dev, const void* host, size_t n, size_t row_bytes, size_t ld_bytes,
                        hipStream_t stream);
 int download_staged(void* host, const void* dev, size_t bytes, hipStream_t stream);
+// Several host row blocks, each widened to its own stride, placed back to
+// back on the device from `dev`: rows of consecutive blocks share staging
+// buffers, so many small blocks move as a few large copies.
+struct StageSeg {
+  const void* host;
+  size_t n, row_bytes, ld_bytes;
+};
+int upload_staged_segments(void* dev, const StageSeg* segs, size_t count, hipStream_t stream);
 
 // Copies `n` host rows of `d` elements into a zero-padded device block with
 // stride ld (async on `stream`).

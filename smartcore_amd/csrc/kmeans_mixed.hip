@@ -94,13 +94,21 @@ constexpr int class_waves(int kbm, int ksm) {
   return ksm == 8 ? 4 : (class_fixed_lds(kbm, ksm, 4) <= kTwoGroupLds ? 4 : 8);
 }
 
-template <int KBM, int KSM>
+// Waves per workgroup a class may run with (the tuner of the batched form
+// tries both): four, or eight where the stage and tables fit.
+constexpr bool class_allows(int kbm, int ksm, int waves) {
+  return (waves == 4 || (waves == 8 && ksm != 8)) &&
+         class_fixed_lds(kbm, ksm, waves) <= kLdsBudget;
+}
+
+template <int KBM, int KSM, int W = class_waves(KBM, KSM)>
 struct Cfg {
   static constexpr int LD = 16 * KSM;                    // largest padded dimension
   static constexpr int LDP = LD + 4;                     // stage row stride (floats)
   static constexpr int KP = 32 * KBM;                    // largest padded k
-  static constexpr int WAVES = class_waves(KBM, KSM);    // waves per workgroup
-  static constexpr bool TWO_GROUPS = KSM != 8 && WAVES == 4;  // two workgroups per CU
+  static constexpr int WAVES = W;                        // waves per workgroup
+  static constexpr bool TWO_GROUPS =                     // two workgroups per CU
+      KSM != 8 && W == 4 && class_fixed_lds(KBM, KSM, 4) <= kTwoGroupLds;
   static constexpr int MIN_WAVES_PER_EU = TWO_GROUPS ? 2 : WAVES / 4;
   static constexpr int NTH = 64 * WAVES;                 // threads per workgroup
   static constexpr int CR = 32 * WAVES;                  // rows per chunk
@@ -153,10 +161,10 @@ __device__ __forceinline__ uint32_t slot_centroid(int sl, int half) {
 
 // One block b of problem p: labels of its rows and its partial sums.
 // EXACT: the problem has exactly KBM centroid blocks and KSM K steps.
-template <int KBM, int KSM, bool EXACT>
+template <int KBM, int KSM, bool EXACT, int W>
 __device__ __forceinline__ void km_block(const KmProb& p, int64_t b, unsigned char* smem,
                                          bool ctab_lds, unsigned long long& fallback) {
-  using C = Cfg<KBM, KSM>;
+  using C = Cfg<KBM, KSM, W>;
   constexpr int NTH = C::NTH;
   constexpr int CR = C::CR;
   const int tid = threadIdx.x;
@@ -564,8 +572,8 @@ __device__ __forceinline__ void km_block(const KmProb& p, int64_t b, unsigned ch
 
 // Persistent step kernel: items are (problem, block) pairs of one shape
 // class; with items == nullptr the launch covers every block of problem 0.
-template <int KBM, int KSM, bool EXACT>
-__global__ __launch_bounds__((Cfg<KBM, KSM>::NTH), (Cfg<KBM, KSM>::MIN_WAVES_PER_EU)) void km_step_kernel(
+template <int KBM, int KSM, bool EXACT, int W>
+__global__ __launch_bounds__((Cfg<KBM, KSM, W>::NTH), (Cfg<KBM, KSM, W>::MIN_WAVES_PER_EU)) void km_step_kernel(
     const KmProb* __restrict__ P, const uint2* __restrict__ items, int64_t nitems, int ctab_lds) {
   extern __shared__ __align__(16) unsigned char smem[];
   unsigned long long fb = 0;
@@ -584,19 +592,19 @@ __global__ __launch_bounds__((Cfg<KBM, KSM>::NTH), (Cfg<KBM, KSM>::MIN_WAVES_PER
       cur = i;
       st = p.st;
     }
-    km_block<KBM, KSM, EXACT>(p, b, smem, ctab_lds != 0, fb);
+    km_block<KBM, KSM, EXACT, W>(p, b, smem, ctab_lds != 0, fb);
   }
   // each wave adds its own count
   if (st && fb && (threadIdx.x & 63) == 0) atomicAdd(&st->fallback, fb);
 }
 
-template <int KBM, int KSM>
+template <int KBM, int KSM, int W>
 size_t lds_bytes(int k, int ld, bool* ctab_lds) {
-  const size_t fixed = Cfg<KBM, KSM>::FIXED;
+  const size_t fixed = Cfg<KBM, KSM, W>::FIXED;
   const size_t ctab = sizeof(double) * static_cast<size_t>(k) * ld;
   // the f64 centroid copy goes to LDS only when the workgroups per CU still
   // fit
-  *ctab_lds = fixed + ctab <= (Cfg<KBM, KSM>::TWO_GROUPS ? kTwoGroupLds : kLdsBudget);
+  *ctab_lds = fixed + ctab <= (Cfg<KBM, KSM, W>::TWO_GROUPS ? kTwoGroupLds : kLdsBudget);
   return fixed + (*ctab_lds ? ctab : 0);
 }
 
@@ -619,37 +627,43 @@ bool class_fits(int k, int ld) {
   return false;
 }
 
-template <int KBM, int KSM>
+template <int KBM, int KSM, int W>
 int launch_class(const KmProb* dP, const uint2* items, int64_t nitems, unsigned grid, bool exact,
                  int kmax, int ldmax, hipStream_t s) {
   bool ctl = false;
-  const size_t lds = lds_bytes<KBM, KSM>(kmax, ldmax, &ctl);
-  const dim3 block(Cfg<KBM, KSM>::NTH);
+  const size_t lds = lds_bytes<KBM, KSM, W>(kmax, ldmax, &ctl);
+  const dim3 block(Cfg<KBM, KSM, W>::NTH);
   // allow the largest dynamic LDS this class can ask for (once per kernel)
   static const bool attr = [] {
     const int mx = static_cast<int>(kLdsBudget);
-    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&km_step_kernel<KBM, KSM, true>),
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&km_step_kernel<KBM, KSM, true, W>),
                               hipFuncAttributeMaxDynamicSharedMemorySize, mx);
-    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&km_step_kernel<KBM, KSM, false>),
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&km_step_kernel<KBM, KSM, false, W>),
                               hipFuncAttributeMaxDynamicSharedMemorySize, mx);
     return true;
   }();
   (void)attr;
   if (exact)
-    hipLaunchKernelGGL((km_step_kernel<KBM, KSM, true>), dim3(grid), block, lds, s, dP, items,
+    hipLaunchKernelGGL((km_step_kernel<KBM, KSM, true, W>), dim3(grid), block, lds, s, dP, items,
                        nitems, ctl ? 1 : 0);
   else
-    hipLaunchKernelGGL((km_step_kernel<KBM, KSM, false>), dim3(grid), block, lds, s, dP, items,
+    hipLaunchKernelGGL((km_step_kernel<KBM, KSM, false, W>), dim3(grid), block, lds, s, dP, items,
                        nitems, ctl ? 1 : 0);
   SC_HIP(hipGetLastError());
   return SC_OK;
 }
 
-int launch_step(int kbm, int ksm, const KmProb* dP, const uint2* items, int64_t nitems,
+// waves: the preferred waves per workgroup (0: the class's default); a
+// class that does not allow it runs its default.
+int launch_step(int kbm, int ksm, int waves, const KmProb* dP, const uint2* items, int64_t nitems,
                 unsigned grid, bool exact, int kmax, int ldmax, hipStream_t s) {
+  const int w = waves != 0 && class_allows(kbm, ksm, waves) ? waves : class_waves(kbm, ksm);
 #define SC_KM_CASE(A, B)                                                                     \
-  if (kbm == A && ksm == B)                                                                  \
-    return launch_class<A, B>(dP, items, nitems, grid, exact, kmax, ldmax, s);
+  if (kbm == A && ksm == B) {                                                                \
+    if constexpr (class_allows(A, B, 8))                                                     \
+      if (w == 8) return launch_class<A, B, 8>(dP, items, nitems, grid, exact, kmax, ldmax, s); \
+    return launch_class<A, B, 4>(dP, items, nitems, grid, exact, kmax, ldmax, s);           \
+  }
   SC_KM_CASE(1, 1) SC_KM_CASE(1, 2) SC_KM_CASE(1, 4) SC_KM_CASE(1, 8)
   SC_KM_CASE(2, 1) SC_KM_CASE(2, 2) SC_KM_CASE(2, 4) SC_KM_CASE(2, 8)
   SC_KM_CASE(4, 1) SC_KM_CASE(4, 2) SC_KM_CASE(4, 4) SC_KM_CASE(4, 8)
@@ -669,8 +683,8 @@ int kmeans_mixed_step_launch(const sc_index* ix, KmWork& w, hipStream_t s) {
   const int kbm = kbm_of(k), ksm = ksm_of(ld);
   const bool exact = (k + 31) / 32 == kbm && ld / 16 == ksm;
   ProfScope ps("kmeans_mixed_assign", s);
-  return launch_step(kbm, ksm, w.dprob, nullptr, w.p.nb, static_cast<unsigned>(w.p.nb), exact, k,
-                     ld, s);
+  return launch_step(kbm, ksm, 0, w.dprob, nullptr, w.p.nb, static_cast<unsigned>(w.p.nb), exact,
+                     k, ld, s);
 }
 
 // ------------------------------------------------------------------ batch
@@ -751,9 +765,11 @@ struct Klass {
   DevBuf buf;
 };
 
+// The batched form's launch shape for a set of problems: waves per
+// workgroup (and so rows per chunk, 32 per wave) and workgroups per CU.
 struct TuneEntry {
   std::vector<int64_t> key;
-  int mult;
+  int waves, mult;
 };
 std::mutex g_tune_mu;
 std::vector<TuneEntry> g_tune;
@@ -895,10 +911,14 @@ extern "C" int sc_kmeans_batch_run(const sc_kmeans_problem* problems, size_t cou
                 hcent0.data() + off[i].cent + c * P[i].ld);
   }
   if (int rc = upload_staged(bCent.p, hcent0.data(), sizeof(double) * tot.cent, s)) return rc;
-  for (size_t i = 0; i < count; ++i) {
-    const sc_kmeans_problem& pr = problems[i];
-    if (int rc = upload_padded(pr.rows, SC_F32, pr.n, pr.d, P[i].ld, const_cast<void*>(P[i].rows), s))
-      return rc;
+  {
+    // the problems' rows are consecutive in the row arena: one staged
+    // stream of rows, padded on the host
+    std::vector<StageSeg> segs(count);
+    for (size_t i = 0; i < count; ++i)
+      segs[i] = StageSeg{problems[i].rows, problems[i].n, problems[i].d * sizeof(float),
+                         static_cast<size_t>(P[i].ld) * sizeof(float)};
+    if (int rc = upload_staged_segments(bRows.p, segs.data(), count, s)) return rc;
   }
   const KmProb* dP = bP.as<KmProb>();
 
@@ -973,7 +993,7 @@ extern "C" int sc_kmeans_batch_run(const sc_kmeans_problem* problems, size_t cou
     SC_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     sides.done.push_back(e);
   }
-  auto iteration = [&](int mult) -> int {
+  auto iteration = [&](int waves, int mult) -> int {
     SC_HIP(hipEventRecord(sides.fork, s));
     size_t i = 0;
     for (auto& kv : klass) {
@@ -981,8 +1001,8 @@ extern "C" int sc_kmeans_batch_run(const sc_kmeans_problem* problems, size_t cou
       const int64_t ni = static_cast<int64_t>(K.items.size());
       const unsigned grid = static_cast<unsigned>(std::min<int64_t>(ni, static_cast<int64_t>(mult) * cus));
       SC_HIP(hipStreamWaitEvent(sides.st[i], sides.fork, 0));
-      if (int rc = launch_step(K.kbm, K.ksm, dP, K.buf.as<uint2>(), ni, grid, K.exact, K.kmax,
-                               K.ldmax, sides.st[i]))
+      if (int rc = launch_step(K.kbm, K.ksm, waves, dP, K.buf.as<uint2>(), ni, grid, K.exact,
+                               K.kmax, K.ldmax, sides.st[i]))
         return rc;
       SC_HIP(hipEventRecord(sides.done[i], sides.st[i]));
       ++i;
@@ -991,21 +1011,28 @@ extern "C" int sc_kmeans_batch_run(const sc_kmeans_problem* problems, size_t cou
     return kmeans_reduce_launch(dP, red, true, true, s);
   };
 
-  // ---- grid size: cached, or timed on the first iterations ----
+  // ---- launch shape: cached, or timed on the first iterations ----
+  // Candidates: waves per workgroup (4 or 8, where a class allows it; rows
+  // per chunk follow) x workgroups per CU (1, 2, 4).  Every shape gives the
+  // same labels and sums (the sums' order is fixed per block), so the
+  // timed iterations are real ones.
   std::vector<int64_t> key;
   for (auto& kv : klass) {
     key.push_back(kv.first);
     key.push_back(static_cast<int64_t>(kv.second.items.size()));
   }
-  int mult = 0;
+  int waves = 0, mult = 0;
   {
     std::lock_guard<std::mutex> lk(g_tune_mu);
     for (auto& e : g_tune)
-      if (e.key == key) mult = e.mult;
+      if (e.key == key) {
+        waves = e.waves;
+        mult = e.mult;
+      }
   }
   size_t it_done = 0;
   if (mult == 0) {
-    const int cand[3] = {1, 2, 4};
+    const int cand[6][2] = {{4, 1}, {4, 2}, {4, 4}, {8, 1}, {8, 2}, {8, 4}};
     float best = 1e30f;
     struct TimerPair {
       hipEvent_t a = nullptr, b = nullptr;
@@ -1017,9 +1044,9 @@ extern "C" int sc_kmeans_batch_run(const sc_kmeans_problem* problems, size_t cou
     SC_HIP(hipEventCreate(&tp.a));
     SC_HIP(hipEventCreate(&tp.b));
     hipEvent_t e0 = tp.a, e1 = tp.b;
-    for (int c = 0; c < 3 && it_done < max_iter; ++c) {
+    for (int c = 0; c < 6 && it_done < max_iter; ++c) {
       SC_HIP(hipEventRecord(e0, s));
-      if (int rc = iteration(cand[c])) return rc;
+      if (int rc = iteration(cand[c][0], cand[c][1])) return rc;
       SC_HIP(hipEventRecord(e1, s));
       SC_HIP(hipEventSynchronize(e1));
       float ms = 0.0f;
@@ -1027,11 +1054,14 @@ extern "C" int sc_kmeans_batch_run(const sc_kmeans_problem* problems, size_t cou
       ++it_done;
       if (ms < best) {
         best = ms;
-        mult = cand[c];
+        waves = cand[c][0];
+        mult = cand[c][1];
       }
     }
-    std::lock_guard<std::mutex> lk(g_tune_mu);
-    g_tune.push_back(TuneEntry{key, mult});
+    if (it_done == 6) {  // a full comparison: kept for later calls
+      std::lock_guard<std::mutex> lk(g_tune_mu);
+      g_tune.push_back(TuneEntry{key, waves, mult});
+    }
   }
   // ---- remaining iterations as one graph ----
   if (it_done < max_iter) {
@@ -1040,7 +1070,7 @@ extern "C" int sc_kmeans_batch_run(const sc_kmeans_problem* problems, size_t cou
     hipGraphExec_t exec = nullptr;
     SC_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     int crc = SC_OK;
-    for (size_t it = it_done; it < max_iter && crc == SC_OK; ++it) crc = iteration(mult);
+    for (size_t it = it_done; it < max_iter && crc == SC_OK; ++it) crc = iteration(waves, mult);
     hipError_t ce = hipStreamEndCapture(s, &graph);
     if (crc) {
       if (graph) (void)hipGraphDestroy(graph);
@@ -1091,5 +1121,7 @@ extern "C" int sc_kmeans_batch_run(const sc_kmeans_problem* problems, size_t cou
   st = sc_stats{};
   st.queries = count;
   st.uncertified = fb;
+  st.launch_waves = static_cast<uint64_t>(waves);
+  st.launch_groups_per_cu = static_cast<uint64_t>(mult);
   return SC_OK;
 }

@@ -41,6 +41,40 @@ def test_batch_second_call_uses_tuned_grid_and_agrees():
         assert ia == ib
 
 
+def test_batch_launch_shape_tuner_is_invariant_and_reported():
+    """The first call on a new set of shape classes times six launch shapes
+    (4 or 8 waves per workgroup, i.e. 128- or 256-row chunks, times 1, 2 or
+    4 workgroups per CU) on its first six iterations, keeps the fastest and
+    reports it; every problem must still equal its single-problem fit (which
+    runs the class's default shape), and a second call (the kept shape for
+    all iterations) must repeat the first bit for bit.  The problems cover
+    classes that allow eight waves and the widest rows (four only)."""
+    rng = np.random.default_rng(53)
+    probs = []
+    for n, d, k in [(5_000, 8, 3), (12_000, 24, 40), (9_000, 64, 64), (7_000, 40, 100),
+                    (6_000, 100, 20), (3_000, 64, 128), (20_000, 16, 12)]:
+        x = rng.normal(size=(n, d)).astype(np.float32)
+        x += rng.integers(0, 5, size=(n, 1)).astype(np.float32) * 3.0
+        init = x[rng.choice(n, size=k, replace=False)].astype(np.float64)
+        probs.append((x, init))
+    a = S.kmeans_batch_run(probs, 9, S.SC_MIXED)
+    st = S.last_stats()
+    assert st["launch_waves"] in (4, 8)
+    assert st["launch_groups_per_cu"] in (1, 2, 4)
+    b = S.kmeans_batch_run(probs, 9, S.SC_MIXED)
+    st2 = S.last_stats()
+    assert (st2["launch_waves"], st2["launch_groups_per_cu"]) == (st["launch_waves"],
+                                                                 st["launch_groups_per_cu"])
+    for (x, init), ra, rb in zip(probs, a, b):
+        c1, l1, s1, d1, i1 = single(x, init, 9, S.SC_MIXED)
+        for c, lab, size, dist, it in (ra, rb):
+            np.testing.assert_array_equal(lab, l1)
+            np.testing.assert_array_equal(size, s1)
+            assert it == i1
+            np.testing.assert_array_equal(c.view(np.uint64), c1.view(np.uint64))
+            assert np.float64(dist).view(np.uint64) == np.float64(d1).view(np.uint64)
+
+
 def test_batch_fp64_mode_runs_problems_one_by_one():
     probs = synth.kmeans_batch_problems(seed=52, count=4, n_lo=300, n_hi=3_000)
     got = S.kmeans_batch_run(probs, 5, S.SC_PLAIN_FP64)
