@@ -115,13 +115,13 @@ def bench_knn(args):
 
     if mode == S.SC_MIXED:
         # the timed main scan is the last row range: on indexes of at least
-        # 524,288 rows the mixed form scans the first 65,536 rows, then
+        # 524,288 rows the mixed form scans the first 16,384 rows, then
         # ranges growing 8 times; the last one starts at the first
-        # 65,536 * 8^j whose 8-fold reaches n (524,288 at 4M rows)
+        # 16,384 * 8^j whose 8-fold reaches n (1,048,576 at 4M rows)
         kname = "knn_mixed_gemm"
         start = 0
         if N >= 8 * 65536:
-            start = 65536
+            start = 16384
             while start * 8 < N:
                 start *= 8
         flops_per_query = 2.0 * (N - start) * D

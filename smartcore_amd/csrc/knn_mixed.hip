@@ -93,8 +93,8 @@ constexpr int kMaxKS = 8;                 // K steps of 16 -> dimension <= 128
 constexpr int kMaxK = 32;                 // largest k served by this form
 constexpr int kXcds = 8;                  // MI355X: 8 XCDs, round-robin dispatch
 constexpr int kTargetBlocks = 256;        // one workgroup per CU on 256 CUs
-constexpr int64_t kPrefixRows = 65536;    // rows of the seeding launch
-constexpr int64_t kSeedMinRows = 8 * kPrefixRows;  // indexes this large are seeded
+constexpr int64_t kPrefixRows = 16384;    // rows of the seeding launch
+constexpr int64_t kSeedMinRows = 524288;  // indexes this large are seeded
 constexpr int64_t kLevelRatio = 8;        // row-range growth of the fixed-threshold levels
 constexpr int64_t kScanPadRows = 1024;    // padding rows past the index read by fixed launches
 constexpr int kMaxLists = 64 + 2 * 64 * 8;  // running lists + fixed pairs, any plan

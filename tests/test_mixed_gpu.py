@@ -218,7 +218,7 @@ def test_mixed_manhattan_coded_scan_declined():
                                         (96, 20, "gaussian", 2_000_000),
                                         (96, 20, "lognormal", 600_000)])
 def test_mixed_seeded_large_index(d, k, kind, n):
-    """Indexes of 524,288 rows or more scan a 65,536-row prefix first, then
+    """Indexes of 524,288 rows or more scan a 16,384-row prefix first, then
     row ranges growing 8 times, each tested against the k-th exact distance
     of the earlier candidates.  The result must still equal the FP64 form bit
     for bit (and the oracle on a few queries), also with heavy exact ties and
