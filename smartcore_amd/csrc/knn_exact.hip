@@ -576,19 +576,50 @@ struct CandSet {
 // query: the earlier rows' lists (an optional prefix set), then the main
 // set, each in row order.  Entries are taken 64 at a time (lane = entry);
 // those whose distance is below the heap's largest (the reference's strict
-// <) enter one after another, in row order, by lane 0's sift-down on the
-// wave's LDS heap, and the rest are tested again against the new top.
+// <) enter one after another, in row order, by a sift-down of the heap
+// held across the lanes, and the rest are tested again against the new top.
 constexpr int kReplayWaves = 4;
-__device__ __forceinline__ void wave_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
 __device__ __forceinline__ double lane_value(double v, int l) {
   const int lo = __builtin_amdgcn_readlane(static_cast<int>(__double_as_longlong(v)), l);
   const int hi = __builtin_amdgcn_readlane(static_cast<int>(__double_as_longlong(v) >> 32), l);
   return __longlong_as_double((static_cast<long long>(hi) << 32) |
                               static_cast<unsigned long long>(static_cast<uint32_t>(lo)));
+}
+
+// The reference heap held across a wave's lanes: lane j < k holds entry j
+// (distance hd, row hi); root replacement and HeapSelection::sift_down
+// (heap_select.rs:71-86) with readlane reads and select swaps.
+__device__ __forceinline__ void lane_heap_push_root(double& hd, uint32_t& hi, int lane, int k,
+                                                    double d, uint32_t row) {
+  if (lane == 0) {
+    hd = d;
+    hi = row;
+  }
+  const int n = k - 1;
+  int kk = 0;
+  const double vk = d;
+  while (2 * kk <= n) {
+    int j = 2 * kk;
+    double vj = lane_value(hd, j);
+    if (j < n) {
+      const double vj1 = lane_value(hd, j + 1);
+      if (vj < vj1) {
+        j += 1;
+        vj = vj1;
+      }
+    }
+    if (vk == vj || vk > vj) break;
+    const uint32_t ij = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(hi), j));
+    const uint32_t ik = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(hi), kk));
+    if (lane == kk) {
+      hd = vj;
+      hi = ij;
+    } else if (lane == j) {
+      hd = vk;
+      hi = ik;
+    }
+    kk = j;
+  }
 }
 
 template <int M>
@@ -599,11 +630,8 @@ __global__ __launch_bounds__(64 * kReplayWaves) void knn_replay_kernel(
     double* __restrict__ out_dist, uint32_t* __restrict__ out_count,
     uint32_t* __restrict__ overflow_list, uint32_t* __restrict__ overflow_n,
     unsigned long long* __restrict__ total_cand) {
-  __shared__ double s_hd[kReplayWaves][KMAX_TILED];
-  __shared__ int64_t s_hi[kReplayWaves][KMAX_TILED];
   const int lane = threadIdx.x & 63;
-  const int w = threadIdx.x >> 6;
-  const int64_t qi = static_cast<int64_t>(blockIdx.x) * kReplayWaves + w;
+  const int64_t qi = static_cast<int64_t>(blockIdx.x) * kReplayWaves + (threadIdx.x >> 6);
   if (qi >= q) return;  // the whole wave
   bool over = false;
   for (int s2 = lane; s2 < pre.nseg; s2 += 64)
@@ -618,13 +646,9 @@ __global__ __launch_bounds__(64 * kReplayWaves) void knn_replay_kernel(
     }
     return;
   }
-  double* hd = s_hd[w];
-  int64_t* hi = s_hi[w];
-  if (lane < k) {
-    hd[lane] = INFINITY;
-    hi[lane] = -1;
-  }
-  wave_sync();
+  // the heap in lanes 0 .. k-1 (seeded with k x {inf, None})
+  double hd = INFINITY;
+  uint32_t hi = 0xffffffffu;
   double top = INFINITY;
   unsigned long long seen = 0;
   auto walk = [&](const uint32_t* idx, const double* sv, size_t base, uint32_t cnt) {
@@ -638,9 +662,8 @@ __global__ __launch_bounds__(64 * kReplayWaves) void knn_replay_kernel(
         const int src = __builtin_ctzll(m);
         const double dl = lane_value(d, src);
         const uint32_t rl = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(row), src));
-        if (lane == 0) heap_push_root(hd, hi, k, dl, static_cast<int64_t>(rl));
-        wave_sync();
-        top = hd[0];
+        lane_heap_push_root(hd, hi, lane, k, dl, rl);
+        top = lane_value(hd, 0);
         m &= m - 1;
         m &= __ballot(valid && d < top);
       }
@@ -656,16 +679,13 @@ __global__ __launch_bounds__(64 * kReplayWaves) void knn_replay_kernel(
   }
   if (lane == 0) atomicAdd(total_cand, seen);
   // linear_search.rs:79-83: keep heap-array order, drop None entries.
-  const bool full = lane < k && hi[lane] >= 0;
+  const bool full = lane < k && hi != 0xffffffffu;
   const uint64_t fm = __ballot(full);
   const int c = __popcll(fm);
-  const int64_t vi = lane < k ? hi[lane] : -1;
-  const double vd = lane < k ? hd[lane] : 0.0;
-  wave_sync();
   if (full) {
     const int pos = __popcll(fm & ((uint64_t(1) << lane) - 1));
-    out_idx[qi * k + pos] = vi;
-    out_dist[qi * k + pos] = vd;
+    out_idx[qi * k + pos] = static_cast<int64_t>(hi);
+    out_dist[qi * k + pos] = hd;
   }
   if (lane >= c && lane < k) {
     out_idx[qi * k + lane] = -1;

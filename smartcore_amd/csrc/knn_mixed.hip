@@ -903,18 +903,41 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void select_seed_kernel(
   if (lane < k) seed_out[qi * k + lane] = b;
 }
 
-// The reference heap's sift-down (heap_select.rs), on a wave's LDS heap.
-__device__ inline void sift_down_lds(double* hd, uint32_t* hi, int kk, int n) {
+// The reference heap (heap_select.rs) held across a wave's lanes: lane j < k
+// holds entry j (distance hd, row hi).  Replacing the root and sifting it
+// down follows HeapSelection::sift_down (heap_select.rs:71-86: the children
+// of kk are 2kk and 2kk + 1; stop on Equal or Greater) with the node values
+// read by readlane and the swap done by a select in every lane, so an
+// insertion costs a few instructions per level and no memory traffic.
+__device__ __forceinline__ void lane_heap_push_root(double& hd, uint32_t& hi, int lane, int k,
+                                                    double d, uint32_t row) {
+  if (lane == 0) {
+    hd = d;
+    hi = row;
+  }
+  const int n = k - 1;
+  int kk = 0;
+  double vk = d;
   while (2 * kk <= n) {
     int j = 2 * kk;
-    if (j < n && hd[j] < hd[j + 1]) j += 1;
-    if (hd[kk] == hd[j] || hd[kk] > hd[j]) break;
-    const double td = hd[kk];
-    hd[kk] = hd[j];
-    hd[j] = td;
-    const uint32_t ti = hi[kk];
-    hi[kk] = hi[j];
-    hi[j] = ti;
+    double vj = lane_value(hd, j);
+    if (j < n) {
+      const double vj1 = lane_value(hd, j + 1);
+      if (vj < vj1) {
+        j += 1;
+        vj = vj1;
+      }
+    }
+    if (vk == vj || vk > vj) break;
+    const uint32_t ij = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(hi), j));
+    const uint32_t ik = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(hi), kk));
+    if (lane == kk) {
+      hd = vj;
+      hi = ij;
+    } else if (lane == j) {
+      hd = vk;
+      hi = ik;
+    }
     kk = j;
   }
 }
@@ -971,8 +994,8 @@ __global__ __launch_bounds__(kRescoreThreads) void mixed_rescore_kernel(
 // Pass 2b: replay of the heap rule over the candidates in row order, one
 // wavefront per query.  Entries are taken 64 at a time (lane = entry); the
 // lanes whose distance is below the heap's largest (the reference's strict
-// <) enter one after another, in row order, by lane 0's sift-down on the
-// wave's LDS heap, and the rest are tested again against the new top.  A
+// <) enter one after another, in row order, by a sift-down of the heap
+// held across the lanes, and the rest are tested again against the new top.  A
 // fixed launch's list pair is merged by row 64 entries at a time: an entry's
 // place is its lane plus the count of smaller rows in the other list's
 // window, and the places below 64 are the next 64 rows of the merge.
@@ -983,8 +1006,6 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void mixed_replay_kernel(
     const float* __restrict__ x2m, int64_t* __restrict__ out_idx, double* __restrict__ out_dist,
     uint32_t* __restrict__ out_count, uint32_t* __restrict__ list, uint32_t* __restrict__ list_n,
     unsigned long long* __restrict__ counters) {
-  __shared__ double s_hd[kWavesPerBlock][kMaxK];
-  __shared__ uint32_t s_hi[kWavesPerBlock][kMaxK];
   __shared__ uint32_t s_win[kWavesPerBlock][2][64];
   __shared__ uint32_t s_mrow[kWavesPerBlock][64];
   __shared__ double s_mS[kWavesPerBlock][64];
@@ -1004,13 +1025,9 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void mixed_replay_kernel(
     }
     return;
   }
-  double* hd = s_hd[w];
-  uint32_t* hi = s_hi[w];
-  if (lane < k) {
-    hd[lane] = INFINITY;
-    hi[lane] = 0xffffffffu;
-  }
-  wave_sync();
+  // the heap in lanes 0 .. k-1 (the reference seeds it with k x {inf, None})
+  double hd = INFINITY;
+  uint32_t hi = 0xffffffffu;
   double top = INFINITY;
   // entries of one chunk, in row order over the lanes (valid lanes first)
   auto consume = [&](double S, uint32_t row, bool valid) {
@@ -1020,13 +1037,8 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void mixed_replay_kernel(
       const int src = __builtin_ctzll(m);
       const double dl = lane_value(d, src);
       const uint32_t rl = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(row), src));
-      if (lane == 0) {
-        hd[0] = dl;
-        hi[0] = rl;
-        if (k > 1) sift_down_lds(hd, hi, 0, k - 1);
-      }
-      wave_sync();
-      top = hd[0];
+      lane_heap_push_root(hd, hi, lane, k, dl, rl);
+      top = lane_value(hd, 0);
       m &= m - 1;
       m &= __ballot(valid && d < top);
     }
@@ -1099,13 +1111,13 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void mixed_replay_kernel(
     atomicAdd(&counters[1], seen);
   }
   // the heap array without its empty slots, in heap order
-  const bool full = lane < k && hi[lane] != 0xffffffffu;
+  const bool full = lane < k && hi != 0xffffffffu;
   const uint64_t fm = __ballot(full);
   const int c = __popcll(fm);
   if (full) {
     const int pos = __popcll(fm & ((uint64_t(1) << lane) - 1));
-    out_idx[qi * k + pos] = static_cast<int64_t>(hi[lane]);
-    out_dist[qi * k + pos] = hd[lane];
+    out_idx[qi * k + pos] = static_cast<int64_t>(hi);
+    out_dist[qi * k + pos] = hd;
   }
   if (lane >= c && lane < k) {
     out_idx[qi * k + lane] = -1;
