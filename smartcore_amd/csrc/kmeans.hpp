@@ -16,8 +16,8 @@ namespace sc {
 // and group sums in group order.  Nothing depends on the launch shape.
 constexpr int kBlockRows = 2048;
 constexpr int kGroupBlocks = 16;
-// Rows a workgroup stages through LDS at a time in the partial and fused
-// kernels (4 waves x 32 rows).
+// Rows a workgroup stages through LDS at a time in the FP64 form's partial
+// kernel (4 waves x 32 rows).
 constexpr int kChunkRows = 128;
 // Largest k * d whose f64 accumulators fit the LDS partial kernels.
 constexpr int kMaxLdsAcc = 8192;
@@ -106,9 +106,9 @@ int kmeans_reduce_launch(const KmProb* d_probs, const KmItems& items, bool updat
 // MFMA terms of the current centroids (before the first step).
 int kmeans_tables_launch(const KmProb* d_probs, size_t count, hipStream_t s);
 
-// One certified MFMA assignment fused with the partial sums, for every
-// block of a single problem (same labels and partials as assign + partial of
-// the FP64 form).
+// One certified MFMA assignment of every row of a single problem, then the
+// block partial sums from the labels (same labels and partials as assign +
+// partial of the FP64 form).
 int kmeans_mixed_step_launch(const sc_index* ix, KmWork& w, hipStream_t s);
 bool kmeans_mixed_supported(const sc_index* ix, int k);
 

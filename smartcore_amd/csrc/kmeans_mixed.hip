@@ -1,10 +1,11 @@
 // Certified BF16-MFMA Lloyd step for f32 rows (sc_kmeans_run and
-// sc_kmeans_batch_run, SC_MIXED), fused with the centroid partial sums.
+// sc_kmeans_batch_run, SC_MIXED): a label kernel, then a block-sums kernel.
 //
-// A workgroup owns one block of kBlockRows rows and streams it through LDS
-// 256 rows at a time (128 for the widest rows); the next chunk is loaded
-// into registers while the current one is processed, so each row is read
-// from HBM once per step.
+// The label kernel runs four-wave workgroups over 128-row chunks, each wave
+// 32 rows; a lane loads the values of its row that it feeds the MFMA
+// straight from memory (no LDS stage), and the rows of the workgroup's next
+// chunk are requested while the current one is labelled.  The centroid
+// tables sit in LDS, loaded once per problem.
 //
 // Labels.  Each wave takes 32 rows of the chunk and splits them into three
 // BF16 terms (x = xh + xm + xl + r, |r| <= 2^-24 |x|); the centroid tables
@@ -35,19 +36,19 @@
 // norm is at least 1e30 or below 1e-30 (but not zero), compute all k
 // distances.  So the labels equal the FP64 form's on every input.
 //
-// Sums.  After the labels, wave w adds the chunk's rows of its clusters
-// (c % waves == w) in row order into f64 registers, lane = dimension, one
-// ballot per (cluster, 64-row group).  That is the order of the FP64 form's
-// partial kernel, so both forms write the same block partials bit for bit,
-// and the shared reduce kernels give the same sums, centroids and
-// distortions.
+// Sums.  A second kernel reads each 2048-row block's labels, sorts the
+// block's rows by label in LDS (stable: row order inside each cluster) and
+// adds each cluster's run of rows in that order into f64 registers, lane =
+// dimension.  That is the order of the FP64 form's partial kernel, so both
+// forms write the same block partials bit for bit, and the shared reduce
+// kernels give the same sums, centroids and distortions.
 //
 // Batched form.  Many independent problems (uneven n, d, k) run per Lloyd
-// iteration as one persistent launch per shape class over (problem, block)
-// items, then the shared reduce kernels over every problem; the iterations
-// are captured in one hipGraph.  The grid size is picked by timing the first
-// iterations.  Every problem gets bit-identical results to sc_kmeans_run on
-// it alone.
+// iteration as one persistent label launch per shape class over (problem,
+// chunk) items, one sums launch over every (problem, block), then the shared
+// reduce kernels over every problem; the iterations are captured in one
+// hipGraph.  The label grid size is picked by timing the first iterations.
+// Every problem gets bit-identical results to sc_kmeans_run on it alone.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -78,52 +79,28 @@ constexpr uint32_t kUncert = 0xffffffffu;
 constexpr double kKappa = 9.5367431640625e-07;   // 2^-20
 constexpr size_t kLdsBudget = 150 * 1024;         // dynamic LDS one workgroup may take
 
-// Shape class of the step kernel: k <= 32 KBM, padded dimension <= 16 KSM.
-// Fixed LDS of a shape class at a given number of waves: stage | lab | cnt |
-// c2 | A tables (hi | mid | lo).
-constexpr size_t class_fixed_lds(int kbm, int ksm, int waves) {
-  return sizeof(float) * (32 * waves) * (16 * ksm + 4) + sizeof(uint32_t) * (32 * waves) +
-         sizeof(uint32_t) * (32 * kbm) + sizeof(float) * (32 * kbm) +
-         3 * sizeof(uint16_t) * (32 * kbm) * (16 * ksm + 8);
-}
-// Two four-wave workgroups per CU when both fit its LDS (the sums of one
-// overlap the MFMA phases of the other); otherwise one eight-wave workgroup
-// (four waves for the widest rows, whose stage and tables take most of it).
-constexpr size_t kTwoGroupLds = 78 * 1024;
-constexpr int class_waves(int kbm, int ksm) {
-  return ksm == 8 ? 4 : (class_fixed_lds(kbm, ksm, 4) <= kTwoGroupLds ? 4 : 8);
-}
-
-// Waves per workgroup a class may run with (the tuner of the batched form
-// tries both): four, or eight where the stage and tables fit.
-constexpr bool class_allows(int kbm, int ksm, int waves) {
-  return (waves == 4 || (waves == 8 && ksm != 8)) &&
-         class_fixed_lds(kbm, ksm, waves) <= kLdsBudget;
-}
-
-template <int KBM, int KSM, int W = class_waves(KBM, KSM)>
+// Shape class of the label kernel: k <= 32 KBM, padded dimension <= 16 KSM;
+// W waves per workgroup, 32 rows each.
+template <int KBM, int KSM, int W = 4>
 struct Cfg {
   static constexpr int LD = 16 * KSM;                    // largest padded dimension
-  static constexpr int LDP = LD + 4;                     // stage row stride (floats)
   static constexpr int KP = 32 * KBM;                    // largest padded k
   static constexpr int WAVES = W;                        // waves per workgroup
-  static constexpr bool TWO_GROUPS =                     // two workgroups per CU
-      KSM != 8 && W == 4 && class_fixed_lds(KBM, KSM, 4) <= kTwoGroupLds;
-  static constexpr int MIN_WAVES_PER_EU = TWO_GROUPS ? 2 : WAVES / 4;
   static constexpr int NTH = 64 * WAVES;                 // threads per workgroup
   static constexpr int CR = 32 * WAVES;                  // rows per chunk
-  static constexpr int SLOTS = (KP + WAVES - 1) / WAVES; // clusters one wave owns
   static constexpr int DPL = (LD + 63) / 64;             // dimensions per lane
-  static constexpr int PFV = CR * LD / 4 / NTH;          // f32x4 loads per thread per chunk
-  // LDS: stage | lab | cnt | c2 | A tables (hi | mid | lo) | [f64 centroids]
-  static constexpr size_t STAGE = sizeof(float) * CR * LDP;
   static constexpr int LDA = LD + 8;                     // A-table row stride: 16 B of padding
   static constexpr size_t ATAB = sizeof(uint16_t) * KP * LDA;
-  static constexpr size_t FIXED = STAGE + sizeof(uint32_t) * CR + sizeof(uint32_t) * KP +
-                                  sizeof(float) * KP + 3 * ATAB;
-  static_assert(FIXED == class_fixed_lds(KBM, KSM, WAVES), "LDS layout");
+  // LDS: c2 | A tables (hi | mid | lo)
+  static constexpr size_t LABEL_LDS = sizeof(float) * KP + 3 * ATAB;
+  // accumulator and row-value registers per lane
+  static constexpr int LABEL_REGS = 16 * KBM + 8 * KSM;
+  // waves per SIMD the label kernel is compiled for (EXACT: fixed loop
+  // bounds, fewer live registers)
+  static constexpr int label_waves(bool exact) {
+    return LABEL_REGS < 32 ? 4 : (LABEL_REGS <= (exact ? 64 : 48) ? 3 : 2);
+  }
   static_assert(kBlockRows % CR == 0, "a block is a whole number of chunks");
-  static_assert(CR % 64 == 0, "chunks are whole ballots");
 };
 
 __device__ __forceinline__ uint32_t bf16_bits(float f) {
@@ -147,8 +124,8 @@ __device__ __forceinline__ void lexmin_d(double& v, uint32_t& i, double v2, uint
 
 // s_c of KMeans::predict for one row (f32, stride 1) and one centroid (f64):
 // the reference's sequential fold.
-template <typename CP>
-__device__ __forceinline__ double exact_s(const float* x, CP c, int ld) {
+template <typename XP, typename CP>
+__device__ __forceinline__ double exact_s(XP x, CP c, int ld) {
   double s = 0.0;
   for (int j = 0; j < ld; ++j) s = s + Term<double, kEuclid>::f(static_cast<double>(x[j]), c[j], 2.0);
   return s;
@@ -159,85 +136,51 @@ __device__ __forceinline__ uint32_t slot_centroid(int sl, int half) {
   return static_cast<uint32_t>(32 * (sl >> 4) + 8 * ((sl & 15) >> 2) + 4 * half + (sl & 3));
 }
 
-// One block b of problem p: labels of its rows and its partial sums.
+// Rows of one label work item: one chunk of a four-wave workgroup.
+constexpr int kSegRows = 128;
+
+// The eight values of the lane's row that it feeds the MFMA in each K step.
+template <int KSM, bool EXACT>
+__device__ __forceinline__ void load_row_parts(const float* xr, int ks_n, int half,
+                                               f32x4 (&xv)[KSM][2]) {
+#pragma unroll
+  for (int ks = 0; ks < KSM; ++ks)
+    if (EXACT || ks < ks_n) {
+      const auto s4 = to_global(reinterpret_cast<const f32x4*>(xr + 16 * ks + 8 * half));
+      xv[ks][0] = s4[0];
+      xv[ks][1] = s4[1];
+    }
+}
+
+// Labels of rows [rb, re) of problem p (one chunk, 32 rows per wave), with
+// the centroid tables of p in LDS and the lane's row values already in xv.
+// next_xr (or null): the lane's row of the workgroup's next chunk, whose
+// values are loaded into xv once this chunk's MFMAs have consumed them.
 // EXACT: the problem has exactly KBM centroid blocks and KSM K steps.
 template <int KBM, int KSM, bool EXACT, int W>
-__device__ __forceinline__ void km_block(const KmProb& p, int64_t b, unsigned char* smem,
-                                         bool ctab_lds, unsigned long long& fallback) {
+__device__ __forceinline__ void km_label_chunk(const KmProb& p, int64_t c0, int64_t re,
+                                               f32x4 (&xv)[KSM][2], const float* next_xr,
+                                               const unsigned char* smem,
+                                               unsigned long long& fb) {
   using C = Cfg<KBM, KSM, W>;
-  constexpr int NTH = C::NTH;
-  constexpr int CR = C::CR;
-  const int tid = threadIdx.x;
-  const int lane = tid & 63;
-  const int wave = tid >> 6;
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
   const int col = lane & 31;
   const int half = lane >> 5;
   const int k = p.k;
   const int ld = p.ld;
-  const int d = p.d;
   const int kb = EXACT ? KBM : (k + 31) / 32;
   const int ks_n = EXACT ? KSM : ld / 16;
-  const int ldq = ld / 4;
-
-  float* stage = reinterpret_cast<float*>(smem);
-  uint32_t* lab = reinterpret_cast<uint32_t*>(smem + C::STAGE);
-  uint32_t* cnt = lab + CR;
-  float* c2s = reinterpret_cast<float*>(cnt + C::KP);
-  uint16_t* ahi = reinterpret_cast<uint16_t*>(c2s + C::KP);  // KP x LDA
-  uint16_t* amd = ahi + C::KP * C::LDA;
-  uint16_t* alo = amd + C::KP * C::LDA;
-  // f64 centroids for the exact paths: the LDS copy when it fits, else the
-  // global array.  Each use branches on the (uniform) choice so that every
-  // load is a typed LDS or global load: a generic load would make the
-  // fallback wait for the chunk prefetch in flight.
-  double* ctab_s = reinterpret_cast<double*>(smem + C::FIXED);
-  auto exact_c = [&](const float* x, uint32_t c) -> double {
-    return ctab_lds ? exact_s(x, ctab_s + static_cast<int64_t>(c) * ld, ld)
-                    : exact_s(x, to_global(p.cent) + static_cast<int64_t>(c) * ld, ld);
-  };
-
-  // the first chunk's rows are requested before the tables are loaded, so
-  // their latency overlaps the table loads
+  const float* c2s = reinterpret_cast<const float*>(smem);
+  const uint16_t* ahi = reinterpret_cast<const uint16_t*>(c2s + C::KP);
+  const uint16_t* amd = ahi + C::KP * C::LDA;
+  const uint16_t* alo = amd + C::KP * C::LDA;
   const float* rows = static_cast<const float*>(p.rows);
-  const int64_t r0 = b * kBlockRows;
-  const int rows_here = static_cast<int>(min<int64_t>(kBlockRows, p.n - r0));
-  f32x4 pf[C::PFV];
-  auto load_chunk = [&](int c0) {
-    const int nr = min(CR, rows_here - c0);
-    const auto src = to_global(reinterpret_cast<const f32x4*>(rows + (r0 + c0) * ld));
-    const int nvec = nr * ldq;
-#pragma unroll
-    for (int i = 0; i < C::PFV; ++i) {
-      const int e = tid + NTH * i;
-      pf[i] = e < nvec ? src[e] : f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-    }
-  };
-  load_chunk(0);
-
-  // ---- per-block tables ----
-  for (int e = tid; e < C::KP; e += NTH) {
-    cnt[e] = 0u;
-    c2s[e] = e < p.kpad ? to_global(p.c2f)[e] : INFINITY;
-  }
-  for (int e = tid; e < C::KP * C::LD / 8; e += NTH) {
-    const int r = e / (C::LD / 8), j8 = e % (C::LD / 8);
-    const bool in = r < p.kpad && 8 * j8 < ld;
-    const int64_t src = static_cast<int64_t>(r) * ld + 8 * j8;
-    const u32x4 z = {0u, 0u, 0u, 0u};
-    const int dst = r * C::LDA + 8 * j8;
-    *reinterpret_cast<u32x4*>(ahi + dst) =
-        in ? *to_global(reinterpret_cast<const u32x4*>(p.bhi + src)) : z;
-    *reinterpret_cast<u32x4*>(amd + dst) =
-        in ? *to_global(reinterpret_cast<const u32x4*>(p.bmd + src)) : z;
-    *reinterpret_cast<u32x4*>(alo + dst) =
-        in ? *to_global(reinterpret_cast<const u32x4*>(p.blo + src)) : z;
-  }
-  if (ctab_lds)
-    for (int64_t e = tid; e < static_cast<int64_t>(k) * ld; e += NTH) ctab_s[e] = to_global(p.cent)[e];
+  const auto grows = to_global(rows);
+  const auto cent = to_global(static_cast<const double*>(p.cent));
   auto frag = [&](const uint16_t* tab, int cb, int ks) -> bf16x8 {
     return load_frag(tab + (32 * cb + col) * C::LDA + 16 * ks + 8 * half);
   };
-
   const float cmax = to_global(p.cmax)[0];
   const float c2max = to_global(p.cmax)[1];
   // cross-term error constant, relative to sum |x_i| |v_i| <= ||x|| ||v||:
@@ -251,39 +194,19 @@ __device__ __forceinline__ void km_block(const KmProb& p, int64_t b, unsigned ch
   // tree vs sequential f64 fold of the same ld non-negative terms
   const double dr = 2.0004 * (ld + 8.0) * 1.1102230246251565e-16;
 
-  // f64 sums of the wave's clusters (wave w owns c = WAVES s + w, slot s),
-  // slot-major, lane = dimension
-  double sacc[C::SLOTS * C::DPL];
-#pragma unroll
-  for (int e = 0; e < C::SLOTS * C::DPL; ++e) sacc[e] = 0.0;
-
-  unsigned long long fb = 0;
-
-  for (int c0 = 0; c0 < rows_here; c0 += CR) {
-    const int nr = min(CR, rows_here - c0);
-    __syncthreads();  // the previous chunk is consumed; tables are written
-#pragma unroll
-    for (int i = 0; i < C::PFV; ++i) {
-      const int e = tid + NTH * i;
-      if (e < nr * ldq) {
-        const int r = e / ldq, q4 = e - r * ldq;
-        *reinterpret_cast<f32x4*>(stage + r * C::LDP + 4 * q4) = pf[i];
-      }
-    }
-    if (tid < CR && tid >= nr) lab[tid] = kUncert;
-    __syncthreads();
-    if (c0 + CR < rows_here) load_chunk(c0 + CR);
+  {
+    const int64_t row = c0 + 32 * wave + col;
+    const bool pvalid = row < re;
+    // the lane's row (an in-range row for lanes past the end: their values
+    // fed only their own MFMA columns, which are never used)
+    const float* xr = rows + (pvalid ? row : c0) * ld;
 
     // ---- labels of this wave's 32 rows ----
     // Three MFMA phases by term weight, smallest first: (lo(c) hi(x),
     // hi(c) lo(x), mid(c) mid(x)) at 2^-16, (mid(c) hi(x), hi(c) mid(x)) at
-    // 2^-8, then hi(c) hi(x).  Each phase splits the staged row again, so no
-    // split is held across phases.
-    const int pr = 32 * wave + col;
-    const bool pvalid = pr < nr;
-    const float* xr = stage + (pvalid ? pr : 0) * C::LDP;
+    // 2^-8, then hi(c) hi(x).  Each phase splits the row's values again, so
+    // no split is held across phases.
     float x2p = 0.0f;
-    bool fin = true;
     f32x16 acc[KBM];
 #pragma unroll
     for (int cb = 0; cb < KBM; ++cb) acc[cb] = f32x16{};
@@ -291,8 +214,8 @@ __device__ __forceinline__ void km_block(const KmProb& p, int64_t b, unsigned ch
     // values per v_cvt_pk_bf16_f32 (round to nearest even); the differences
     // v - hi and (v - hi) - mid are exact in f32
     auto split_x = [&](int ks, int parts, bf16x8& xh, bf16x8& xm, bf16x8& xl, bool norms) {
-      const f32x4 v0 = *reinterpret_cast<const f32x4*>(xr + 16 * ks + 8 * half);
-      const f32x4 v1 = *reinterpret_cast<const f32x4*>(xr + 16 * ks + 8 * half + 4);
+      const f32x4 v0 = xv[ks][0];
+      const f32x4 v1 = xv[ks][1];
       const float v[8] = {v0[0], v0[1], v0[2], v0[3], v1[0], v1[1], v1[2], v1[3]};
       u32x4 H, M = {0u, 0u, 0u, 0u}, L = {0u, 0u, 0u, 0u};
 #pragma unroll
@@ -313,10 +236,7 @@ __device__ __forceinline__ void km_block(const KmProb& p, int64_t b, unsigned ch
       }
       if (norms) {
 #pragma unroll
-        for (int t = 0; t < 8; ++t) {
-          x2p = fmaf(v[t], v[t], x2p);
-          fin = fin && isfinite(v[t]);
-        }
+        for (int t = 0; t < 8; ++t) x2p = fmaf(v[t], v[t], x2p);
       }
       xh = as_frag(H);
       xm = as_frag(M);
@@ -361,8 +281,8 @@ __device__ __forceinline__ void km_block(const KmProb& p, int64_t b, unsigned ch
           if (EXACT || cb < kb)
             acc[cb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(frag(ahi, cb, ks), xh, acc[cb], 0, 0, 0);
       }
+    if (next_xr) load_row_parts<KSM, EXACT>(next_xr, ks_n, half, xv);
     const float x2 = x2p + __shfl_xor(x2p, 32);
-    fin = fin && (__shfl_xor(fin ? 1 : 0, 32) != 0);
 
     // a_c = ||c||^2 + acc; lane (col, half) holds centroids
     // 32 cb + 8 g + 4 half + r for v = 4 g + r, in increasing index order
@@ -410,9 +330,10 @@ __device__ __forceinline__ void km_block(const KmProb& p, int64_t b, unsigned ch
       }
     const int ncand = __popcll(cm) + __shfl_xor(__popcll(cm), 32);
     // a finite threshold keeps padding centroids (a = +inf) out of the
-    // candidates; anything else takes the all-centroid path below
+    // candidates; anything else takes the all-centroid path below.  A NaN
+    // or infinite value makes ||x||^2 NaN or +inf, which fails x2 < 1e30;
     // rows with 0 < ||x||^2 < 1e-30 could lose split terms to underflow
-    const bool sane = fin && x2 < 1e30f && (x2 == 0.0f || x2 >= 1e-30f) && amin < INFINITY &&
+    const bool sane = x2 < 1e30f && (x2 == 0.0f || x2 >= 1e-30f) && amin < INFINITY &&
                       thr < INFINITY;
     const bool certified = sane && ncand == 1;
     const bool cand = pvalid && sane && !certified;
@@ -437,16 +358,13 @@ __device__ __forceinline__ void km_block(const KmProb& p, int64_t b, unsigned ch
         if (lane == L) work &= work - 1;
         const uint32_t c = slot_centroid(sl, L >> 5);
         if (c >= static_cast<uint32_t>(k)) continue;  // never: padding has a = +inf
-        const float* xs = stage + (32 * wave + (L & 31)) * C::LDP;
+        const auto xs = grows + (c0 + 32 * wave + (L & 31)) * ld;
         const int64_t cofs = static_cast<int64_t>(c) * ld;
         double t = 0.0;
 #pragma unroll
         for (int dd = 0; dd < C::DPL; ++dd) {
           const int j = lane + 64 * dd;
-          if (j < ld) {
-            const double cj = ctab_lds ? ctab_s[cofs + j] : to_global(p.cent)[cofs + j];
-            t = t + Term<double, kEuclid>::f(static_cast<double>(xs[j]), cj, 2.0);
-          }
+          if (j < ld) t = t + Term<double, kEuclid>::f(static_cast<double>(xs[j]), cent[cofs + j], 2.0);
         }
 #pragma unroll
         for (int off = 32; off >= 1; off >>= 1) t = t + __shfl_xor(t, off);
@@ -486,7 +404,7 @@ __device__ __forceinline__ void km_block(const KmProb& p, int64_t b, unsigned ch
         cm3 &= cm3 - 1;
         const uint32_t c = slot_centroid(sl, half);
         if (c < static_cast<uint32_t>(k))
-          lexmin_d(bs, bi, exact_c(xr, c), c);
+          lexmin_d(bs, bi, exact_s(to_global(xr), cent + static_cast<int64_t>(c) * ld, ld), c);
       }
     }
     {
@@ -494,13 +412,10 @@ __device__ __forceinline__ void km_block(const KmProb& p, int64_t b, unsigned ch
       const uint32_t i2 = __shfl_xor(bi, 32);
       lexmin_d(bs, bi, v2, i2);
     }
-    if (half == 0 && pvalid) {
-      uint32_t lb = kUncert;
-      if (certified)
-        lb = aidx;
-      else if (cand)
-        lb = ok2 ? bc : (bs < DBL_MAX ? bi : 0u);
-      lab[pr] = lb;
+    if (half == 0 && pvalid && sane) {
+      uint32_t lb = aidx;
+      if (cand) lb = ok2 ? bc : (bs < DBL_MAX ? bi : 0u);
+      to_global(p.labels)[row] = lb;
     }
     // rows outside the bound's domain: all k centroids, lanes = centroids
     uint64_t um = __ballot(half == 0 && pvalid && !sane);
@@ -508,11 +423,11 @@ __device__ __forceinline__ void km_block(const KmProb& p, int64_t b, unsigned ch
     while (um) {
       const int q = __builtin_ctzll(um);
       um &= um - 1;
-      const float* x = stage + (32 * wave + q) * C::LDP;
+      const auto x = grows + (c0 + 32 * wave + q) * ld;
       double bv = INFINITY;
       uint32_t bj = kUncert;
       for (int c = lane; c < k; c += 64)
-        lexmin_d(bv, bj, exact_c(x, static_cast<uint32_t>(c)), static_cast<uint32_t>(c));
+        lexmin_d(bv, bj, exact_s(x, cent + static_cast<int64_t>(c) * ld, ld), static_cast<uint32_t>(c));
 #pragma unroll
       for (int off = 32; off >= 1; off >>= 1) {
         const double v2 = __shfl_xor(bv, off);
@@ -520,106 +435,262 @@ __device__ __forceinline__ void km_block(const KmProb& p, int64_t b, unsigned ch
         lexmin_d(bv, bj, v2, i2);
       }
       // kmeans.rs:334-347: min starts at f64::MAX; nothing below it keeps 0
-      if (lane == 0) lab[32 * wave + q] = bv < DBL_MAX ? bj : 0u;
-    }
-    __syncthreads();
-    if (tid < nr) to_global(p.labels)[r0 + c0 + tid] = lab[tid];
-
-    // ---- sums: wave owns clusters WAVES s + wave, rows in order ----
-    {
-      // one ballot per (slot, group); static slot indices
-      uint32_t lrow[CR / 64];
-#pragma unroll
-      for (int g = 0; g < CR / 64; ++g) lrow[g] = lab[64 * g + lane];
-#pragma unroll
-      for (int s = 0; s < C::SLOTS; ++s) {
-        const uint32_t c = static_cast<uint32_t>(C::WAVES * s + wave);
-        if (c >= static_cast<uint32_t>(k)) break;
-#pragma unroll
-        for (int g = 0; g < CR / 64; ++g) {
-          uint64_t m = __ballot(lrow[g] == c);
-          if (lane == 0 && m) atomicAdd(&cnt[c], static_cast<uint32_t>(__popcll(m)));
-          while (m) {
-            const int i = 64 * g + __builtin_ctzll(m);
-            m &= m - 1;
-#pragma unroll
-            for (int dd = 0; dd < C::DPL; ++dd)
-              if (lane + 64 * dd < ld)
-                sacc[s * C::DPL + dd] =
-                    sacc[s * C::DPL + dd] + static_cast<double>(stage[i * C::LDP + lane + 64 * dd]);
-          }
-        }
-      }
+      if (lane == 0) to_global(p.labels)[c0 + 32 * wave + q] = bv < DBL_MAX ? bj : 0u;
     }
   }
-  __syncthreads();
-  // ---- block partials ----
-  const int64_t kd = static_cast<int64_t>(k) * d;
-#pragma unroll
-  for (int s = 0; s < C::SLOTS; ++s) {
-    const int c = C::WAVES * s + wave;
-    if (c >= k) break;
-#pragma unroll
-    for (int dd = 0; dd < C::DPL; ++dd) {
-      const int dim = lane + 64 * dd;
-      if (dim < d) to_global(p.psum)[b * kd + static_cast<int64_t>(c) * d + dim] = sacc[s * C::DPL + dd];
-    }
-  }
-  for (int c = tid; c < k; c += NTH) to_global(p.pcount)[b * k + c] = cnt[c];
-  fallback += fb;
-  __syncthreads();  // LDS is rewritten by the next block
 }
 
-// Persistent step kernel: items are (problem, block) pairs of one shape
-// class; with items == nullptr the launch covers every block of problem 0.
+// Persistent label kernel over (problem, chunk) items of one shape class;
+// with items == nullptr the launch covers every chunk of problem 0.  The
+// centroid tables are loaded into LDS when the problem changes; the rows of
+// the workgroup's next item are requested while the current one is
+// labelled when both belong to the same problem.
 template <int KBM, int KSM, bool EXACT, int W>
-__global__ __launch_bounds__((Cfg<KBM, KSM, W>::NTH), (Cfg<KBM, KSM, W>::MIN_WAVES_PER_EU)) void km_step_kernel(
-    const KmProb* __restrict__ P, const uint2* __restrict__ items, int64_t nitems, int ctab_lds) {
+__global__ __launch_bounds__((Cfg<KBM, KSM, W>::NTH), (Cfg<KBM, KSM, W>::label_waves(EXACT))) void km_label_kernel(
+    const KmProb* __restrict__ P, const uint2* __restrict__ items, int64_t nitems) {
+  using C = Cfg<KBM, KSM, W>;
+  static_assert(C::CR == kSegRows, "one chunk per item");
   extern __shared__ __align__(16) unsigned char smem[];
+  float* c2s = reinterpret_cast<float*>(smem);
+  uint16_t* ahi = reinterpret_cast<uint16_t*>(c2s + C::KP);
+  uint16_t* amd = ahi + C::KP * C::LDA;
+  uint16_t* alo = amd + C::KP * C::LDA;
+  const int tid = threadIdx.x;
+  const int col = tid & 31;
+  const int half = (tid & 63) >> 5;
+  const int wave = tid >> 6;
   unsigned long long fb = 0;
   int cur = -1;
+  bool skip = false;
+  bool have = false;  // xv holds the rows of item `it`
   KmState* st = nullptr;
+  f32x4 xv[KSM][2];
+  auto item = [&](int64_t it, int& i, int64_t& c0) {
+    i = items ? static_cast<int>(items[it].x) : 0;
+    c0 = (items ? static_cast<int64_t>(items[it].y) : it) * kSegRows;
+  };
+  auto lane_row = [&](const KmProb& q, int64_t c0) -> const float* {
+    const int64_t row = min(c0 + 32 * wave + col, q.n - 1);
+    return static_cast<const float*>(q.rows) + row * q.ld;
+  };
+  for (int64_t it = blockIdx.x; it < nitems; it += gridDim.x) {
+    int i;
+    int64_t c0;
+    item(it, i, c0);
+    const KmProb p = P[i];
+    if (i != cur) {
+      if (st && fb && (tid & 63) == 0) atomicAdd(&st->fallback, fb);
+      fb = 0;
+      // every workgroup of this launch sees the same value: the flag only
+      // changes in the final reduce kernel
+      skip = *reinterpret_cast<volatile uint32_t*>(&p.st->done) != 0;
+      if (!skip) {
+        __syncthreads();  // nobody reads the previous problem's tables
+        for (int e = tid; e < C::KP; e += C::NTH) c2s[e] = e < p.kpad ? to_global(p.c2f)[e] : INFINITY;
+        for (int e = tid; e < C::KP * C::LD / 8; e += C::NTH) {
+          const int r = e / (C::LD / 8), j8 = e % (C::LD / 8);
+          const bool in = r < p.kpad && 8 * j8 < p.ld;
+          const int64_t src = static_cast<int64_t>(r) * p.ld + 8 * j8;
+          const u32x4 z = {0u, 0u, 0u, 0u};
+          const int dst = r * C::LDA + 8 * j8;
+          *reinterpret_cast<u32x4*>(ahi + dst) =
+              in ? *to_global(reinterpret_cast<const u32x4*>(p.bhi + src)) : z;
+          *reinterpret_cast<u32x4*>(amd + dst) =
+              in ? *to_global(reinterpret_cast<const u32x4*>(p.bmd + src)) : z;
+          *reinterpret_cast<u32x4*>(alo + dst) =
+              in ? *to_global(reinterpret_cast<const u32x4*>(p.blo + src)) : z;
+        }
+        __syncthreads();
+      }
+      cur = i;
+      st = p.st;
+    }
+    if (skip) continue;
+    const int ks_n = EXACT ? KSM : p.ld / 16;
+    if (!have) load_row_parts<KSM, EXACT>(lane_row(p, c0), ks_n, half, xv);
+    // the next item's rows, when it is of the same problem
+    const float* next_xr = nullptr;
+    const int64_t nit = it + gridDim.x;
+    if (nit < nitems) {
+      int ni;
+      int64_t nc0;
+      item(nit, ni, nc0);
+      if (ni == i) next_xr = lane_row(p, nc0);
+    }
+    have = next_xr != nullptr;
+    km_label_chunk<KBM, KSM, EXACT, W>(p, c0, min<int64_t>(c0 + kSegRows, p.n), xv, next_xr, smem,
+                                       fb);
+  }
+  // each wave adds its own count
+  if (st && fb && (tid & 63) == 0) atomicAdd(&st->fallback, fb);
+}
+
+// ---- block partial sums from the labels ----
+// One workgroup per (problem, block) item.  The block's rows are sorted by
+// label (stable: row order inside each cluster) in LDS; then wave w adds the
+// runs of clusters w, w + kSumWaves, ... one at a time into one f64
+// accumulator per dimension, loading kSumBatch rows of a run at once.  The
+// adds of each cluster are its rows in row order from +0.0 -- the order the
+// FP64 form's partial kernel uses -- so both forms write the same block
+// partials bit for bit.
+constexpr int kSumWaves = 8;
+constexpr int kSumThreads = 64 * kSumWaves;
+constexpr int kSumBatch = 32;
+constexpr int kSumMaxK = 128;
+constexpr int kSumGroups = kBlockRows / 64 / kSumWaves;  // 64-row groups per wave
+
+// DPL: dimensions per lane (2 when some row is wider than 64 values).
+template <int DPL>
+__global__ __launch_bounds__(kSumThreads, DPL == 1 ? 4 : 2) void km_sums_kernel(
+    const KmProb* __restrict__ P, const uint2* __restrict__ items, int64_t nitems) {
+  __shared__ uint32_t lab[kBlockRows];
+  __shared__ uint16_t perm[kBlockRows];
+  __shared__ uint32_t pos[kSumWaves][kSumMaxK];
+  __shared__ uint32_t start[kSumMaxK + 1];
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
   for (int64_t it = blockIdx.x; it < nitems; it += gridDim.x) {
     const int i = items ? static_cast<int>(items[it].x) : 0;
     const int64_t b = items ? static_cast<int64_t>(items[it].y) : it;
     const KmProb p = P[i];
-    // every workgroup of this launch sees the same value: the flag only
-    // changes in the final reduce kernel
     if (*reinterpret_cast<volatile uint32_t*>(&p.st->done)) continue;
-    if (i != cur) {
-      if (fb && (threadIdx.x & 63) == 0) atomicAdd(&st->fallback, fb);
-      fb = 0;
-      cur = i;
-      st = p.st;
+    const int k = p.k, d = p.d, ld = p.ld;
+    const int64_t r0 = b * kBlockRows;
+    const int nr = static_cast<int>(min<int64_t>(kBlockRows, p.n - r0));
+    const auto labels = to_global(static_cast<const uint32_t*>(p.labels)) + r0;
+    for (int e = tid; e < kBlockRows; e += kSumThreads) lab[e] = e < nr ? labels[e] : kUncert;
+    for (int e = tid; e < kSumWaves * kSumMaxK; e += kSumThreads) (&pos[0][0])[e] = 0u;
+    __syncthreads();
+    // per 64-row group: the lanes holding the same label (eight bits: labels
+    // are below 128, rows past the end get 255), the lane's rank among them
+    // and their number; wave w's groups are rows 64 (4w) .. 64 (4w + 4)
+    uint32_t key[kSumGroups], rk[kSumGroups], ng[kSumGroups];
+#pragma unroll
+    for (int gi = 0; gi < kSumGroups; ++gi) {
+      const int g = kSumGroups * wave + gi;
+      const uint32_t l = lab[64 * g + lane];
+      const uint32_t kk = l < static_cast<uint32_t>(k) ? l : 0xffu;
+      uint64_t eq = ~0ull;
+#pragma unroll
+      for (int bit = 0; bit < 8; ++bit) {
+        const bool on = (kk >> bit) & 1u;
+        const uint64_t B = __ballot(on);
+        eq &= on ? B : ~B;
+      }
+      key[gi] = kk;
+      rk[gi] = __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(eq >> 32),
+                                         __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(eq), 0u));
+      ng[gi] = static_cast<uint32_t>(__popcll(eq));
+      if (kk < static_cast<uint32_t>(k) && rk[gi] == 0) pos[wave][kk] += ng[gi];
     }
-    km_block<KBM, KSM, EXACT, W>(p, b, smem, ctab_lds != 0, fb);
+    __syncthreads();
+    // cluster starts, and the first position of each wave's rows of each
+    // cluster (wave 0; lane = clusters 2 lane, 2 lane + 1)
+    if (wave == 0) {
+      const int ca = 2 * lane, cb = 2 * lane + 1;
+      uint32_t ha[kSumWaves], hb[kSumWaves], ta = 0, tb = 0;
+#pragma unroll
+      for (int w = 0; w < kSumWaves; ++w) {
+        ha[w] = ca < k ? pos[w][ca] : 0u;
+        hb[w] = cb < k ? pos[w][cb] : 0u;
+        ta += ha[w];
+        tb += hb[w];
+      }
+      uint32_t incl = ta + tb;
+#pragma unroll
+      for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t v = __shfl_up(incl, off);
+        if (lane >= off) incl += v;
+      }
+      uint32_t sa = incl - ta - tb, sb = sa + ta;
+      if (ca < k) start[ca] = sa;
+      if (cb < k) start[cb] = sb;
+      if (lane == 63) start[k] = incl;
+#pragma unroll
+      for (int w = 0; w < kSumWaves; ++w) {
+        if (ca < k) pos[w][ca] = sa;
+        if (cb < k) pos[w][cb] = sb;
+        sa += ha[w];
+        sb += hb[w];
+      }
+    }
+    __syncthreads();
+    // stable scatter, the wave's groups in row order
+#pragma unroll
+    for (int gi = 0; gi < kSumGroups; ++gi) {
+      const int g = kSumGroups * wave + gi;
+      if (key[gi] < static_cast<uint32_t>(k)) {
+        perm[pos[wave][key[gi]] + rk[gi]] = static_cast<uint16_t>(64 * g + lane);
+        if (rk[gi] == 0) pos[wave][key[gi]] += ng[gi];
+      }
+    }
+    __syncthreads();
+    // the runs
+    const int64_t kd = static_cast<int64_t>(k) * d;
+    const auto psum = to_global(p.psum) + b * kd;
+    const auto rows = to_global(static_cast<const float*>(p.rows)) + r0 * ld;
+    // two runs at a time: clusters c and c + kSumWaves, kSumBatch rows of
+    // each requested before either is added
+    const int j0 = min(lane, ld - 1), j1 = min(lane + 64, ld - 1);
+    for (int c = wave; c < k; c += 2 * kSumWaves) {
+      const int cc[2] = {c, c + kSumWaves};
+      int e[2], e1[2];
+      double acc[2][2] = {{0.0, 0.0}, {0.0, 0.0}};
+#pragma unroll
+      for (int u = 0; u < 2; ++u) {
+        e[u] = cc[u] < k ? static_cast<int>(start[cc[u]]) : 0;
+        e1[u] = cc[u] < k ? static_cast<int>(start[cc[u] + 1]) : 0;
+      }
+      while (e[0] < e1[0] || e[1] < e1[1]) {
+        int nv[2];
+        float x[2][kSumBatch][DPL];
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+          nv[u] = max(0, min(kSumBatch, e1[u] - e[u]));
+          const uint32_t mine = lane < nv[u] ? perm[e[u] + lane] : 0u;
+          // lanes at or past ld read an in-row value that is never stored
+#pragma unroll
+          for (int t = 0; t < kSumBatch; ++t) {
+            const uint32_t r = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(mine), t));
+            x[u][t][0] = rows[static_cast<int64_t>(r) * ld + j0];
+            if (DPL > 1) x[u][t][1] = rows[static_cast<int64_t>(r) * ld + j1];
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+#pragma unroll
+          for (int t = 0; t < kSumBatch; ++t)
+            if (t < nv[u]) {
+              acc[u][0] = acc[u][0] + static_cast<double>(x[u][t][0]);
+              if (DPL > 1) acc[u][1] = acc[u][1] + static_cast<double>(x[u][t][1]);
+            }
+          e[u] += nv[u];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < 2; ++u)
+        if (cc[u] < k) {
+          if (lane < d) psum[static_cast<int64_t>(cc[u]) * d + lane] = acc[u][0];
+          if (DPL > 1 && lane + 64 < d) psum[static_cast<int64_t>(cc[u]) * d + lane + 64] = acc[u][1];
+          if (lane == 0)
+            to_global(p.pcount)[b * k + cc[u]] = start[cc[u] + 1] - start[cc[u]];
+        }
+    }
+    __syncthreads();  // LDS is rewritten by the next item
   }
-  // each wave adds its own count
-  if (st && fb && (threadIdx.x & 63) == 0) atomicAdd(&st->fallback, fb);
-}
-
-template <int KBM, int KSM, int W>
-size_t lds_bytes(int k, int ld, bool* ctab_lds) {
-  const size_t fixed = Cfg<KBM, KSM, W>::FIXED;
-  const size_t ctab = sizeof(double) * static_cast<size_t>(k) * ld;
-  // the f64 centroid copy goes to LDS only when the workgroups per CU still
-  // fit
-  *ctab_lds = fixed + ctab <= (Cfg<KBM, KSM, W>::TWO_GROUPS ? kTwoGroupLds : kLdsBudget);
-  return fixed + (*ctab_lds ? ctab : 0);
 }
 
 // Shape classes: KBM in {1, 2, 4}, KSM in {1, 2, 4, 8}.
 int kbm_of(int k) { return k <= 32 ? 1 : (k <= 64 ? 2 : 4); }
 int ksm_of(int ld) { return ld <= 16 ? 1 : (ld <= 32 ? 2 : (ld <= 64 ? 4 : 8)); }
 
-// Whether a shape class's fixed LDS fits one workgroup's budget.  The widest
-// rows with k above 64 do not (three centroid tables of 128 x 136 BF16 and a
-// 128-row stage): those problems take the FP64 form.
+// Whether a shape class's centroid tables fit one workgroup's LDS.
 bool class_fits(int k, int ld) {
-  if (k < 1 || k > 128 || ld > 128) return false;
+  if (k < 1 || k > kSumMaxK || ld > 128) return false;
   const int kbm = kbm_of(k), ksm = ksm_of(ld);
 #define SC_KM_FIT(A, B) \
-  if (kbm == A && ksm == B) return Cfg<A, B>::FIXED <= kLdsBudget;
+  if (kbm == A && ksm == B) return Cfg<A, B, 4>::LABEL_LDS <= kLdsBudget;
   SC_KM_FIT(1, 1) SC_KM_FIT(1, 2) SC_KM_FIT(1, 4) SC_KM_FIT(1, 8)
   SC_KM_FIT(2, 1) SC_KM_FIT(2, 2) SC_KM_FIT(2, 4) SC_KM_FIT(2, 8)
   SC_KM_FIT(4, 1) SC_KM_FIT(4, 2) SC_KM_FIT(4, 4) SC_KM_FIT(4, 8)
@@ -628,47 +699,83 @@ bool class_fits(int k, int ld) {
 }
 
 template <int KBM, int KSM, int W>
-int launch_class(const KmProb* dP, const uint2* items, int64_t nitems, unsigned grid, bool exact,
-                 int kmax, int ldmax, hipStream_t s) {
-  bool ctl = false;
-  const size_t lds = lds_bytes<KBM, KSM, W>(kmax, ldmax, &ctl);
-  const dim3 block(Cfg<KBM, KSM, W>::NTH);
+int launch_label_class(const KmProb* dP, const uint2* items, int64_t nitems, unsigned grid,
+                       bool exact, hipStream_t s) {
+  using C = Cfg<KBM, KSM, W>;
+  const size_t lds = C::LABEL_LDS;
+  const dim3 block(C::NTH);
   // allow the largest dynamic LDS this class can ask for (once per kernel)
   static const bool attr = [] {
     const int mx = static_cast<int>(kLdsBudget);
-    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&km_step_kernel<KBM, KSM, true, W>),
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&km_label_kernel<KBM, KSM, true, W>),
                               hipFuncAttributeMaxDynamicSharedMemorySize, mx);
-    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&km_step_kernel<KBM, KSM, false, W>),
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&km_label_kernel<KBM, KSM, false, W>),
                               hipFuncAttributeMaxDynamicSharedMemorySize, mx);
     return true;
   }();
   (void)attr;
   if (exact)
-    hipLaunchKernelGGL((km_step_kernel<KBM, KSM, true, W>), dim3(grid), block, lds, s, dP, items,
-                       nitems, ctl ? 1 : 0);
+    hipLaunchKernelGGL((km_label_kernel<KBM, KSM, true, W>), dim3(grid), block, lds, s, dP, items,
+                       nitems);
   else
-    hipLaunchKernelGGL((km_step_kernel<KBM, KSM, false, W>), dim3(grid), block, lds, s, dP, items,
-                       nitems, ctl ? 1 : 0);
+    hipLaunchKernelGGL((km_label_kernel<KBM, KSM, false, W>), dim3(grid), block, lds, s, dP, items,
+                       nitems);
   SC_HIP(hipGetLastError());
   return SC_OK;
 }
 
-// waves: the preferred waves per workgroup (0: the class's default); a
-// class that does not allow it runs its default.
-int launch_step(int kbm, int ksm, int waves, const KmProb* dP, const uint2* items, int64_t nitems,
-                unsigned grid, bool exact, int kmax, int ldmax, hipStream_t s) {
-  const int w = waves != 0 && class_allows(kbm, ksm, waves) ? waves : class_waves(kbm, ksm);
-#define SC_KM_CASE(A, B)                                                                     \
-  if (kbm == A && ksm == B) {                                                                \
-    if constexpr (class_allows(A, B, 8))                                                     \
-      if (w == 8) return launch_class<A, B, 8>(dP, items, nitems, grid, exact, kmax, ldmax, s); \
-    return launch_class<A, B, 4>(dP, items, nitems, grid, exact, kmax, ldmax, s);           \
-  }
+// Workgroups of a class's label kernel one CU holds.
+template <int KBM, int KSM, int W>
+int label_groups_per_cu(bool exact) {
+  using C = Cfg<KBM, KSM, W>;
+  const int by_waves = C::label_waves(exact) * 4 / W;
+  const int by_lds = static_cast<int>(160 * 1024 / C::LABEL_LDS);
+  return std::max(1, std::min(by_waves, by_lds));
+}
+
+// mult: workgroups per CU in the grid (0: as many as one CU holds).  The
+// grid is trimmed so that every workgroup gets the same number of items.
+int launch_label(int kbm, int ksm, int mult, const KmProb* dP, const uint2* items,
+                 int64_t nitems, bool exact, int cus, hipStream_t s) {
+  auto grid_of = [&](int per) {
+    const int64_t g = std::max<int64_t>(1, std::min<int64_t>(nitems, static_cast<int64_t>(mult ? mult : per) * cus));
+    const int64_t each = (nitems + g - 1) / g;
+    return static_cast<unsigned>(std::max<int64_t>(1, (nitems + each - 1) / each));
+  };
+#define SC_KM_CASE(A, B)                                                                  \
+  if (kbm == A && ksm == B)                                                               \
+    return launch_label_class<A, B, 4>(dP, items, nitems,                                \
+                                       grid_of(label_groups_per_cu<A, B, 4>(exact)), exact, s);
   SC_KM_CASE(1, 1) SC_KM_CASE(1, 2) SC_KM_CASE(1, 4) SC_KM_CASE(1, 8)
   SC_KM_CASE(2, 1) SC_KM_CASE(2, 2) SC_KM_CASE(2, 4) SC_KM_CASE(2, 8)
   SC_KM_CASE(4, 1) SC_KM_CASE(4, 2) SC_KM_CASE(4, 4) SC_KM_CASE(4, 8)
 #undef SC_KM_CASE
   return fail(SC_ERR_UNSUPPORTED, "k-means MFMA form: unsupported shape");
+}
+
+// ldmax: the widest padded row among the items' problems.
+int launch_sums(const KmProb* dP, const uint2* items, int64_t nitems, int ldmax, int cus,
+                hipStream_t s) {
+  const unsigned grid = static_cast<unsigned>(std::max<int64_t>(1, std::min<int64_t>(nitems, 4 * cus)));
+  if (ldmax > 64)
+    hipLaunchKernelGGL(km_sums_kernel<2>, dim3(grid), dim3(kSumThreads), 0, s, dP, items, nitems);
+  else
+    hipLaunchKernelGGL(km_sums_kernel<1>, dim3(grid), dim3(kSumThreads), 0, s, dP, items, nitems);
+  SC_HIP(hipGetLastError());
+  return SC_OK;
+}
+
+int device_cus() {
+  static int cached[64] = {};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+  if (cached[dev] == 0) {
+    hipDeviceProp_t prop;
+    cached[dev] = hipGetDeviceProperties(&prop, dev) == hipSuccess && prop.multiProcessorCount > 0
+                      ? prop.multiProcessorCount
+                      : 256;
+  }
+  return cached[dev];
 }
 
 }  // namespace
@@ -682,9 +789,11 @@ int kmeans_mixed_step_launch(const sc_index* ix, KmWork& w, hipStream_t s) {
   const int k = w.p.k, ld = w.p.ld;
   const int kbm = kbm_of(k), ksm = ksm_of(ld);
   const bool exact = (k + 31) / 32 == kbm && ld / 16 == ksm;
+  const int cus = device_cus();
+  const int64_t nseg = (w.p.n + kSegRows - 1) / kSegRows;
   ProfScope ps("kmeans_mixed_assign", s);
-  return launch_step(kbm, ksm, 0, w.dprob, nullptr, w.p.nb, static_cast<unsigned>(w.p.nb), exact,
-                     k, ld, s);
+  if (int rc = launch_label(kbm, ksm, 0, w.dprob, nullptr, nseg, exact, cus, s)) return rc;
+  return launch_sums(w.dprob, nullptr, w.p.nb, ld, cus, s);
 }
 
 // ------------------------------------------------------------------ batch
@@ -765,8 +874,8 @@ struct Klass {
   DevBuf buf;
 };
 
-// The batched form's launch shape for a set of problems: waves per
-// workgroup (and so rows per chunk, 32 per wave) and workgroups per CU.
+// The batched form's launch shape for a set of problems: label workgroups
+// per CU (four waves each), and the waves per workgroup it was timed with.
 struct TuneEntry {
   std::vector<int64_t> key;
   int waves, mult;
@@ -926,7 +1035,11 @@ extern "C" int sc_kmeans_batch_run(const sc_kmeans_problem* problems, size_t cou
   std::vector<uint2> all;
   std::map<int, Klass> klass;
   int64_t maxnb = 0;
-  for (auto& p : P) maxnb = std::max(maxnb, p.nb);
+  int ldmax_all = 0;
+  for (auto& p : P) {
+    maxnb = std::max(maxnb, p.nb);
+    ldmax_all = std::max(ldmax_all, p.ld);
+  }
   // block-major across problems so concurrent workgroups spread over problems
   for (int64_t b = 0; b < maxnb; ++b)
     for (size_t i = 0; i < count; ++i) {
@@ -942,7 +1055,11 @@ extern "C" int sc_kmeans_batch_run(const sc_kmeans_problem* problems, size_t cou
       K.exact = exact;
       K.kmax = std::max(K.kmax, p.k);
       K.ldmax = std::max(K.ldmax, p.ld);
-      K.items.push_back(it);
+      // the label segments of this block
+      const int64_t seg_end = std::min<int64_t>((b + 1) * (kBlockRows / kSegRows),
+                                                (p.n + kSegRows - 1) / kSegRows);
+      for (int64_t sg = b * (kBlockRows / kSegRows); sg < seg_end; ++sg)
+        K.items.push_back(make_uint2(static_cast<unsigned>(i), static_cast<unsigned>(sg)));
     }
   DevBuf bAll;
   SC_HIP(bAll.alloc(sizeof(uint2) * all.size(), s));
@@ -999,23 +1116,23 @@ extern "C" int sc_kmeans_batch_run(const sc_kmeans_problem* problems, size_t cou
     for (auto& kv : klass) {
       Klass& K = kv.second;
       const int64_t ni = static_cast<int64_t>(K.items.size());
-      const unsigned grid = static_cast<unsigned>(std::min<int64_t>(ni, static_cast<int64_t>(mult) * cus));
       SC_HIP(hipStreamWaitEvent(sides.st[i], sides.fork, 0));
-      if (int rc = launch_step(K.kbm, K.ksm, waves, dP, K.buf.as<uint2>(), ni, grid, K.exact,
-                               K.kmax, K.ldmax, sides.st[i]))
+      if (int rc = launch_label(K.kbm, K.ksm, mult, dP, K.buf.as<uint2>(), ni, K.exact, cus,
+                                sides.st[i]))
         return rc;
       SC_HIP(hipEventRecord(sides.done[i], sides.st[i]));
       ++i;
     }
     for (hipEvent_t e : sides.done) SC_HIP(hipStreamWaitEvent(s, e, 0));
+    if (int rc = launch_sums(dP, bAll.as<uint2>(), static_cast<int64_t>(all.size()), ldmax_all, cus, s))
+      return rc;
     return kmeans_reduce_launch(dP, red, true, true, s);
   };
 
   // ---- launch shape: cached, or timed on the first iterations ----
-  // Candidates: waves per workgroup (4 or 8, where a class allows it; rows
-  // per chunk follow) x workgroups per CU (1, 2, 4).  Every shape gives the
-  // same labels and sums (the sums' order is fixed per block), so the
-  // timed iterations are real ones.
+  // Candidates: label workgroups per CU (1, 2, 3, 4, 6, 8; four waves
+  // each).  Every shape gives the same labels and sums (the sums' order is
+  // fixed per block), so the timed iterations are real ones.
   std::vector<int64_t> key;
   for (auto& kv : klass) {
     key.push_back(kv.first);
@@ -1032,7 +1149,7 @@ extern "C" int sc_kmeans_batch_run(const sc_kmeans_problem* problems, size_t cou
   }
   size_t it_done = 0;
   if (mult == 0) {
-    const int cand[6][2] = {{4, 1}, {4, 2}, {4, 4}, {8, 1}, {8, 2}, {8, 4}};
+    const int cand[6][2] = {{4, 1}, {4, 2}, {4, 3}, {4, 4}, {4, 6}, {4, 8}};
     float best = 1e30f;
     struct TimerPair {
       hipEvent_t a = nullptr, b = nullptr;
