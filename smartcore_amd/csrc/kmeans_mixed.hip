@@ -285,9 +285,10 @@ __device__ __forceinline__ void km_label_chunk(const KmProb& p, int64_t c0, int6
     const float x2 = x2p + __shfl_xor(x2p, 32);
 
     // a_c = ||c||^2 + acc; lane (col, half) holds centroids
-    // 32 cb + 8 g + 4 half + r for v = 4 g + r, in increasing index order
+    // 32 cb + 8 g + 4 half + r for v = 4 g + r, in increasing index order.
+    // The row's minimum over both halves sets the threshold (NaN values are
+    // passed over by the minimum and fail every comparison below)
     float amin = INFINITY;
-    int aslot = 0;  // 16 cb + v of the lane's minimum
 #pragma unroll
     for (int cb = 0; cb < KBM; ++cb) {
       if (EXACT || cb < kb) {
@@ -295,49 +296,60 @@ __device__ __forceinline__ void km_label_chunk(const KmProb& p, int64_t c0, int6
         for (int g = 0; g < 4; ++g) {
           const f32x4 c2q = *reinterpret_cast<const f32x4*>(c2s + 32 * cb + 8 * g + 4 * half);
 #pragma unroll
-          for (int r = 0; r < 4; ++r) {
-            const float a = acc[cb][4 * g + r] + c2q[r];
-            acc[cb][4 * g + r] = a;
-            if (a < amin) {
-              amin = a;
-              aslot = 16 * cb + 4 * g + r;
-            }
-          }
+          for (int r = 0; r < 4; ++r) acc[cb][4 * g + r] += c2q[r];
+          amin = fminf(amin, fminf(fminf(acc[cb][4 * g], acc[cb][4 * g + 1]),
+                                   fminf(acc[cb][4 * g + 2], acc[cb][4 * g + 3])));
         }
       }
     }
-    uint32_t aidx = amin < INFINITY ? slot_centroid(aslot, half) : kUncert;
-    {
-      const float v2 = __shfl_xor(amin, 32);
-      const uint32_t i2 = __shfl_xor(aidx, 32);
-      if (v2 < amin || (v2 == amin && i2 < aidx)) {
-        amin = v2;
-        aidx = i2;
-      }
-    }
+    amin = fminf(amin, __shfl_xor(amin, 32));
     const double x2u = static_cast<double>(x2) * (1.0 + 6.103515625e-05) + 1e-300;
     const double E = 2.0 * c1 * sqrt(x2u) * static_cast<double>(cmax) * (1.0 + 9.5367431640625e-07) +
                      kKappa * (x2u + static_cast<double>(c2max));
     const float thr = __double2float_ru(static_cast<double>(amin) + 2.0 * E * (1.0 + 9.765625e-04));
-    uint64_t cm = 0;  // this lane's candidates: bit 16 cb + v
+    // The candidates a_c <= thr, counted per lane; the row is certified when
+    // its two lanes hold exactly one between them.  That candidate is the
+    // minimum, and its slot is the one the lane recorded.
+    uint32_t ncl = 0;
+    int aslot = -1;  // 16 cb + v of the lane's last candidate
 #pragma unroll
     for (int cb = 0; cb < KBM; ++cb)
       if (EXACT || cb < kb) {
-        uint32_t m16 = 0;
 #pragma unroll
-        for (int v = 0; v < 16; ++v) m16 |= static_cast<uint32_t>(acc[cb][v] <= thr) << v;
-        cm |= static_cast<uint64_t>(m16) << (16 * cb);
+        for (int v = 0; v < 16; ++v) {
+          // le = a <= thr (false for NaN); aslot = le ? slot : aslot;
+          // ncl += le -- one compare result, used while it is in VCC
+          asm("v_cmp_le_f32 vcc, %2, %3\n\t"
+              "v_cndmask_b32_e64 %1, %1, %4, vcc\n\t"
+              "v_addc_co_u32 %0, vcc, 0, %0, vcc"
+              : "+v"(ncl), "+v"(aslot)
+              : "v"(acc[cb][v]), "v"(thr), "i"(16 * cb + v)
+              : "vcc");
+        }
       }
-    const int ncand = __popcll(cm) + __shfl_xor(__popcll(cm), 32);
+    const uint32_t ncand = ncl + static_cast<uint32_t>(__shfl_xor(static_cast<int>(ncl), 32));
+    uint32_t aidx = aslot >= 0 ? slot_centroid(aslot, half) : kUncert;
+    aidx = min(aidx, static_cast<uint32_t>(__shfl_xor(static_cast<int>(aidx), 32)));
+    const bool single = ncand == 1;
     // a finite threshold keeps padding centroids (a = +inf) out of the
     // candidates; anything else takes the all-centroid path below.  A NaN
     // or infinite value makes ||x||^2 NaN or +inf, which fails x2 < 1e30;
     // rows with 0 < ||x||^2 < 1e-30 could lose split terms to underflow
     const bool sane = x2 < 1e30f && (x2 == 0.0f || x2 >= 1e-30f) && amin < INFINITY &&
                       thr < INFINITY;
-    const bool certified = sane && ncand == 1;
+    const bool certified = sane && single;
     const bool cand = pvalid && sane && !certified;
-    if (!cand) cm = 0;
+    uint64_t cm = 0;  // an uncertified row's candidates in this lane: bit 16 cb + v
+    if (cand) {
+#pragma unroll
+      for (int cb = 0; cb < KBM; ++cb)
+        if (EXACT || cb < kb) {
+          uint32_t m16 = 0;
+#pragma unroll
+          for (int v = 0; v < 16; ++v) m16 |= static_cast<uint32_t>(acc[cb][v] <= thr) << v;
+          cm |= static_cast<uint64_t>(m16) << (16 * cb);
+        }
+    }
 
     // Level 2: s of each candidate with lanes = dimensions and a fixed tree
     // over the lanes; the lane owning the candidate keeps its best and
