@@ -545,8 +545,8 @@ def bench_batch(args):
     st = S.last_stats()
     shape = {"waves_per_workgroup": st["launch_waves"], "rows_per_chunk": 32 * st["launch_waves"],
              "workgroups_per_cu": st["launch_groups_per_cu"],
-             "how": "timed on the warmup call's first six iterations (4 or 8 waves x 1, 2, 4 "
-                    "workgroups per CU), reused by the timed calls"}
+             "how": "label grid timed on the warmup call's first six iterations (1, 2, 3, 4, "
+                    "6 or 8 four-wave workgroups per CU), reused by the timed calls"}
     gms, launches = profile_read(lib, "kmeans_batch")
     graph_s = gms / 1e3 / max(launches, 1)
     inst_iters = sum(g[4] for g in got)

@@ -401,7 +401,8 @@ class sc_kmeans_problem(C.Structure):
 
 def kmeans_batch_run(problems, max_iter: int, mode: int = SC_MIXED):
     """sc_kmeans_batch_run over [(rows, init_centroids), ...] (all rows of one
-    dtype).  Returns a list of (centroids, labels, size, distortion, iters)."""
+    dtype).  Returns a list of (centroids, labels, size, distortion, iters);
+    the arrays of all problems are views of three result buffers."""
     keep = []
     arr = (sc_kmeans_problem * max(len(problems), 1))()
     dtype = None
@@ -434,8 +435,8 @@ def kmeans_batch_run(problems, max_iter: int, mode: int = SC_MIXED):
     for i, (x, init) in enumerate(problems):
         k, d = init.shape
         n = x.shape[0]
-        out.append((cent[co:co + k * d].reshape(k, d).copy(), labels[lo:lo + n].copy(),
-                    sizes[so:so + k].copy(), float(dist[i]), int(iters[i])))
+        out.append((cent[co:co + k * d].reshape(k, d), labels[lo:lo + n], sizes[so:so + k],
+                    float(dist[i]), int(iters[i])))
         co += k * d
         lo += n
         so += k

@@ -4,12 +4,14 @@
 
 #include <algorithm>
 #include <atomic>
+#include <condition_variable>
 #include <cstring>
+#include <functional>
 #include <map>
 #include <mutex>
-#include <unordered_map>
 #include <string>
 #include <thread>
+#include <unordered_map>
 #include <vector>
 
 #include "common.hpp"
@@ -139,25 +141,70 @@ Stager& thread_stager() {
   return st;
 }
 
-// memcpy split over a few threads for large spans.
+// Host copy workers shared by every host thread, started once: run(fn)
+// calls fn(t, kCopyThreads) for every t, t = 0 on the caller, and returns
+// when all have returned; one job at a time.  The pool is never destroyed
+// (its workers wait for work until the process ends).
+class CopyPool {
+ public:
+  static CopyPool& get() {
+    static CopyPool* pool = new CopyPool();
+    return *pool;
+  }
+  void run(const std::function<void(int, int)>& fn) {
+    std::lock_guard<std::mutex> one(use_mu_);
+    {
+      std::lock_guard<std::mutex> lk(mu_);
+      job_ = &fn;
+      pending_ = kCopyThreads - 1;
+      ++gen_;
+    }
+    cv_.notify_all();
+    fn(0, kCopyThreads);
+    std::unique_lock<std::mutex> lk(mu_);
+    done_.wait(lk, [&] { return pending_ == 0; });
+    job_ = nullptr;
+  }
+
+ private:
+  CopyPool() {
+    for (int t = 1; t < kCopyThreads; ++t) std::thread([this, t] { loop(t); }).detach();
+  }
+  void loop(int t) {
+    uint64_t seen = 0;
+    for (;;) {
+      const std::function<void(int, int)>* job;
+      {
+        std::unique_lock<std::mutex> lk(mu_);
+        cv_.wait(lk, [&] { return gen_ != seen; });
+        seen = gen_;
+        job = job_;
+      }
+      (*job)(t, kCopyThreads);
+      std::lock_guard<std::mutex> lk(mu_);
+      if (--pending_ == 0) done_.notify_one();
+    }
+  }
+  std::mutex use_mu_, mu_;
+  std::condition_variable cv_, done_;
+  const std::function<void(int, int)>* job_ = nullptr;
+  uint64_t gen_ = 0;
+  int pending_ = 0;
+};
+
+// memcpy split over the copy workers for large spans.
 void parallel_copy(void* dst, const void* src, size_t bytes) {
   if (bytes < (size_t{4} << 20)) {
     std::memcpy(dst, src, bytes);
     return;
   }
   const size_t part = (bytes / kCopyThreads + 4095) / 4096 * 4096;
-  std::thread th[kCopyThreads - 1];
-  for (int t = 1; t < kCopyThreads; ++t) {
-    const size_t off = part * t;
-    if (off >= bytes) break;
-    const size_t len = std::min(part, bytes - off);
-    th[t - 1] = std::thread([=] {
-      std::memcpy(static_cast<char*>(dst) + off, static_cast<const char*>(src) + off, len);
-    });
-  }
-  std::memcpy(dst, src, std::min(part, bytes));
-  for (auto& t : th)
-    if (t.joinable()) t.join();
+  CopyPool::get().run([&](int t, int) {
+    const size_t off = part * static_cast<size_t>(t);
+    if (off >= bytes) return;
+    std::memcpy(static_cast<char*>(dst) + off, static_cast<const char*>(src) + off,
+                std::min(part, bytes - off));
+  });
 }
 
 }  // namespace
@@ -275,14 +322,10 @@ int upload_staged_segments(void* dev, const StageSeg* segs, size_t count, hipStr
         if (e > a) copy_rows(dst + pc.at + a * pc.lb, pc.src + a * pc.rb, e - a, pc.rb, pc.lb);
       }
     };
-    if (fill < (size_t{4} << 20)) {
+    if (fill < (size_t{4} << 20))
       work(0, 1);
-    } else {
-      std::thread th[kCopyThreads - 1];
-      for (int t = 1; t < kCopyThreads; ++t) th[t - 1] = std::thread(work, t, kCopyThreads);
-      work(0, kCopyThreads);
-      for (auto& t : th) t.join();
-    }
+    else
+      CopyPool::get().run([&](int t, int nt) { work(static_cast<size_t>(t), static_cast<size_t>(nt)); });
     SC_HIP(hipMemcpyAsync(static_cast<char*>(dev) + dev_off, st.buf[b], fill, hipMemcpyHostToDevice,
                           stream));
     SC_HIP(hipEventRecord(st.ev[b], stream));
@@ -314,13 +357,10 @@ int upload_staged_rows(void* dev, const void* host, size_t n, size_t row_bytes, 
     if (nr * ld_bytes < (size_t{4} << 20)) {
       pad_rows(0, nr);
     } else {
-      std::thread th[kCopyThreads - 1];
       const size_t part = (nr + kCopyThreads - 1) / kCopyThreads;
-      for (int t = 1; t < kCopyThreads; ++t)
-        th[t - 1] = std::thread(pad_rows, std::min(nr, part * t), std::min(nr, part * (t + 1)));
-      pad_rows(0, std::min(nr, part));
-      for (auto& t : th)
-        if (t.joinable()) t.join();
+      CopyPool::get().run([&](int t, int) {
+        pad_rows(std::min(nr, part * t), std::min(nr, part * (t + 1)));
+      });
     }
     SC_HIP(hipMemcpyAsync(static_cast<char*>(dev) + r0 * ld_bytes, st.buf[b], nr * ld_bytes,
                           hipMemcpyHostToDevice, stream));
