@@ -593,7 +593,7 @@ def bench_batch(args):
                                f"max_iter={iters}, one persistent launch per iteration in a hipGraph"},
         "roofline": {"bound": "hbm", "achieved": byts / graph_s / 1e9, "peak": PEAK_HBM / 1e9,
                      "unit": "GB/s", "frac": byts / graph_s / PEAK_HBM, "traffic": None,
-                     "kernel": "kmb_step_kernel (graph of all iterations)"},
+                     "kernel": "label + sums + reduce kernels (graph of all iterations)"},
         "vs_fp64_form": {"fp64_form_ms_per_call": fp64_s * 1e3,
                          "speedup_vs_fp64_form": fp64_s / wall,
                          "problems_with_label_mismatch": len(bad),
