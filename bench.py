@@ -553,13 +553,18 @@ def bench_batch(args):
     # algorithmic HBM bytes: every active problem's rows (f32, unpadded) and
     # labels once per iteration it runs
     byts = sum(g[4] * (x.shape[0] * x.shape[1] * 4 + 4 * x.shape[0]) for (x, _), g in zip(probs, got))
-    t1 = time.perf_counter()
-    ref = S.kmeans_batch_run(probs, iters, S.SC_PLAIN_FP64)
-    fp64_s = time.perf_counter() - t1
-    bad = [i for i, (a, b) in enumerate(zip(got, ref)) if not np.array_equal(a[1], b[1])]
-    it_bad = sum(1 for a, b in zip(got, ref) if a[4] != b[4])
-    log(f"[bench] batch graph {graph_s * 1e3:.1f} ms, call {wall * 1e3:.1f} ms; "
-        f"FP64 form {fp64_s * 1e3:.0f} ms; label-mismatched problems {len(bad)}")
+    vs_fp64 = None
+    if not args.no_compare:
+        t1 = time.perf_counter()
+        ref = S.kmeans_batch_run(probs, iters, S.SC_PLAIN_FP64)
+        fp64_s = time.perf_counter() - t1
+        bad = [i for i, (a, b) in enumerate(zip(got, ref)) if not np.array_equal(a[1], b[1])]
+        it_bad = sum(1 for a, b in zip(got, ref) if a[4] != b[4])
+        vs_fp64 = {"fp64_form_ms_per_call": fp64_s * 1e3, "speedup_vs_fp64_form": fp64_s / wall,
+                   "problems_with_label_mismatch": len(bad),
+                   "problems_with_iteration_mismatch": it_bad}
+        log(f"[bench] FP64 form {fp64_s * 1e3:.0f} ms; label-mismatched problems {len(bad)}")
+    log(f"[bench] batch graph {graph_s * 1e3:.1f} ms, call {wall * 1e3:.1f} ms")
     cpu = None
     if not args.no_cpu:
         # the oracle's KMeans fit loop (BBD tree) over the first problems one
@@ -594,10 +599,7 @@ def bench_batch(args):
         "roofline": {"bound": "hbm", "achieved": byts / graph_s / 1e9, "peak": PEAK_HBM / 1e9,
                      "unit": "GB/s", "frac": byts / graph_s / PEAK_HBM, "traffic": None,
                      "kernel": "label + sums + reduce kernels (graph of all iterations)"},
-        "vs_fp64_form": {"fp64_form_ms_per_call": fp64_s * 1e3,
-                         "speedup_vs_fp64_form": fp64_s / wall,
-                         "problems_with_label_mismatch": len(bad),
-                         "problems_with_iteration_mismatch": it_bad},
+        "vs_fp64_form": vs_fp64,
         "cpu_baseline": cpu,
     }
 
@@ -616,7 +618,8 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=5.0)
     ap.add_argument("--mode", choices=["plain", "mixed"], default="mixed")
     ap.add_argument("--no-compare", action="store_true",
-                    help="skip the FP64-form comparison step of the mixed kNN bench")
+                    help="skip the FP64-form comparison steps of the mixed kNN and batched "
+                         "k-means benches")
     ap.add_argument("--n", type=int, default=4_000_000)
     ap.add_argument("--q", type=int, default=65_536)
     ap.add_argument("--km-n", type=int, default=1_000_000)

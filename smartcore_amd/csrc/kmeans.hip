@@ -360,9 +360,13 @@ __device__ __forceinline__ void set_cmax(const KmProb& p) {
 // in group order, the count, the distortion term of c with this step's
 // centroid (dot and ||c||^2 added in dimension order), then (update)
 // c = sums_c / n_c when n_c > 0, and the MFMA terms of the resulting
-// centroid when the problem has them.  The last workgroup of a problem adds
-// the terms in cluster order, applies the stop rule (control) and sets the
-// centroid-norm maxima.
+// centroid when the problem has them.  LAST: the last workgroup of a
+// problem (an arrival counter) adds the terms in cluster order, applies the
+// stop rule (control) and sets the centroid-norm maxima; otherwise the close
+// kernel does that after this launch.  Each arrival takes a device-scope
+// release fence, which writes back the L2: cheap for one problem's k
+// workgroups, not for the tens of thousands of a batched call.
+template <bool LAST>
 __global__ __launch_bounds__(128) void kmeans_final_kernel(const KmProb* __restrict__ P,
                                                            const uint2* __restrict__ items,
                                                            int update, int control) {
@@ -447,6 +451,9 @@ __global__ __launch_bounds__(128) void kmeans_final_kernel(const KmProb* __restr
       to_global(p.c2f)[c] = static_cast<float>(n2);
       to_global(p.c2d)[c] = n2;
     }
+  }
+  if constexpr (!LAST) return;
+  if (tid == 0) {
     __threadfence();
     const uint32_t old = atomicAdd(p.arrive, 1u);
     s_last = old == static_cast<uint32_t>(p.k - 1) ? 1 : 0;
@@ -487,6 +494,47 @@ __global__ __launch_bounds__(128) void kmeans_final_kernel(const KmProb* __restr
   }
   if (tables) set_cmax_of(p, mx);
   *p.arrive = 0u;
+}
+
+// One workgroup per problem, after kmeans_final_kernel<false>: the
+// distortion (the terms of all clusters, loaded by every thread 128 at a
+// time and added in cluster order by one, as the last workgroup of the
+// arrival form does), the stop rule (control) and the centroid-norm maxima.
+__global__ __launch_bounds__(128) void kmeans_close_kernel(const KmProb* __restrict__ P, int control) {
+  const KmProb p = P[blockIdx.x];
+  if (p.st->done) return;
+  const int tid = threadIdx.x;
+  const bool tables = p.bhi != nullptr;
+  __shared__ double s_term[128], s_c2[128];
+  double s = 0.0, mx = 0.0;
+  if (tid == 0) s = p.st->x2total;
+  for (int c0 = 0; c0 < p.k; c0 += 128) {
+    const int cc = c0 + tid;
+    if (cc < p.k) {
+      s_term[tid] = to_global(p.term)[cc];
+      if (tables) s_c2[tid] = to_global(p.c2d)[cc];
+    }
+    __syncthreads();
+    if (tid == 0) {
+      const int m = min(128, p.k - c0);
+      for (int j = 0; j < m; ++j) s = s + s_term[j];
+      if (tables)
+        for (int j = 0; j < m; ++j)
+          if (s_c2[j] < 1e300) mx = fmax(mx, s_c2[j]);  // NaN and inf centroids never count
+    }
+    __syncthreads();
+  }
+  if (tid != 0) return;
+  p.st->dist = s;
+  if (control) {
+    p.st->iters += 1;
+    if (p.st->distortion <= s) {
+      p.st->done = 1;
+    } else {
+      p.st->distortion = s;
+    }
+  }
+  if (tables) set_cmax_of(p, mx);
 }
 
 // MFMA terms of the current centroids: one workgroup per problem.  Padding
@@ -648,8 +696,16 @@ int kmeans_reduce_launch(const KmProb* d_probs, const KmItems& items, bool updat
     SC_HIP(hipGetLastError());
   }
   if (items.nfinal > 0) {
-    hipLaunchKernelGGL(kmeans_final_kernel, dim3(static_cast<unsigned>(items.nfinal)), dim3(128), 0,
-                       s, d_probs, items.fin, update ? 1 : 0, control ? 1 : 0);
+    if (items.nprob <= 1) {
+      hipLaunchKernelGGL(kmeans_final_kernel<true>, dim3(static_cast<unsigned>(items.nfinal)),
+                         dim3(128), 0, s, d_probs, items.fin, update ? 1 : 0, control ? 1 : 0);
+    } else {
+      hipLaunchKernelGGL(kmeans_final_kernel<false>, dim3(static_cast<unsigned>(items.nfinal)),
+                         dim3(128), 0, s, d_probs, items.fin, update ? 1 : 0, control ? 1 : 0);
+      SC_HIP(hipGetLastError());
+      hipLaunchKernelGGL(kmeans_close_kernel, dim3(static_cast<unsigned>(items.nprob)), dim3(128),
+                         0, s, d_probs, control ? 1 : 0);
+    }
     SC_HIP(hipGetLastError());
   }
   return SC_OK;
@@ -675,6 +731,7 @@ int KmItems::build(const KmProb* probs, size_t count, hipStream_t s) {
   }
   ngroup = static_cast<int64_t>(g.size());
   nfinal = static_cast<int64_t>(f.size());
+  nprob = static_cast<int64_t>(count);
   if (ngroup) {
     SC_HIP(b_group.alloc(sizeof(uint2) * ngroup, s));
     SC_HIP(hipMemcpyAsync(b_group.p, g.data(), sizeof(uint2) * ngroup, hipMemcpyHostToDevice, s));

@@ -73,6 +73,9 @@ struct KmItems {
   uint2* group = nullptr;
   uint2* fin = nullptr;
   int64_t ngroup = 0, nfinal = 0;
+  // problems (the first nprob of the array); with more than one the final
+  // kernel leaves the distortion and stop rule to a per-problem close kernel
+  int64_t nprob = 0;
   int build(const KmProb* probs, size_t count, hipStream_t s);
 };
 

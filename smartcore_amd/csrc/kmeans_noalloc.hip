@@ -159,6 +159,7 @@ extern "C" int sc_kmeans_step_device_noalloc(const sc_index* index, const double
   w.items.fin = reinterpret_cast<uint2*>(base + o_fin);
   w.items.ngroup = ngroup;
   w.items.nfinal = kk;
+  w.items.nprob = 1;
 
   const unsigned setup_grid =
       static_cast<unsigned>(std::min<int64_t>((std::max(ngroup, kk) + 255) / 256, 1024));
