@@ -398,16 +398,16 @@ __global__ __launch_bounds__(128) void kmeans_final_kernel(const KmProb* __restr
     const int j = j0 + tid;
     if (j < d) {
       const auto gs = to_global(p.gsum) + static_cast<int64_t>(c) * d + j;
+      // group sums in group order, 32 requested before they are added
       double S = 0.0;
-      int64_t g = 0;
-      for (; g + 8 <= p.ng; g += 8) {
-        double v[8];
+      for (int64_t g0 = 0; g0 < p.ng; g0 += 32) {
+        double v[32];
 #pragma unroll
-        for (int i = 0; i < 8; ++i) v[i] = gs[(g + i) * kd];
+        for (int i = 0; i < 32; ++i) v[i] = g0 + i < p.ng ? gs[(g0 + i) * kd] : 0.0;
 #pragma unroll
-        for (int i = 0; i < 8; ++i) S = S + v[i];
+        for (int i = 0; i < 32; ++i)
+          if (g0 + i < p.ng) S = S + v[i];
       }
-      for (; g < p.ng; ++g) S = S + gs[g * kd];
       to_global(p.sums)[static_cast<int64_t>(c) * d + j] = S;
       const double cj = to_global(p.cent)[static_cast<int64_t>(c) * p.ld + j];
       const double nc = upd ? S / static_cast<double>(n) : cj;
