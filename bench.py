@@ -517,11 +517,13 @@ def bench_kmeans(args):
 def bench_batch(args):
     """512 independent k-means problems of uneven size (n ~ log-uniform
     [1,000, 200,000], d in {8, 16, 32, 64}, k ~ U{4..128}, f32 blobs), 20
-    Lloyd iterations each, in one sc_kmeans_batch_run call: one persistent
-    kernel per iteration over every problem's row blocks, the iterations
-    replayed as one hipGraph.  The warmup call picks the launch shape (waves
-    per workgroup, workgroups per CU) by timing; the timed calls reuse it, so
-    their graph covers all iterations."""
+    Lloyd iterations each, in one sc_kmeans_batch_run call.  The problems go
+    in two groups of about equal row bytes: one group's rows are
+    copied while the previous group iterates (persistent label launches per
+    shape class, one sums launch over the group's row blocks, the reduces;
+    the iterations replayed as one hipGraph per group).  The warmup call
+    picks each group's launch shape by timing; the timed calls reuse it, so
+    their graphs cover all iterations."""
     from smartcore_amd import _capi as S
     from smartcore_amd import synth
 
@@ -545,10 +547,14 @@ def bench_batch(args):
     st = S.last_stats()
     shape = {"waves_per_workgroup": st["launch_waves"], "rows_per_chunk": 32 * st["launch_waves"],
              "workgroups_per_cu": st["launch_groups_per_cu"],
-             "how": "label grid timed on the warmup call's first six iterations (1, 2, 3, 4, "
-                    "6 or 8 four-wave workgroups per CU), reused by the timed calls"}
+             "upload_groups": st["upload_groups"],
+             "how": "label grid timed on the warmup call's first six iterations per group of "
+                    "problems (1, 2, 3, 4, 6 or 8 four-wave workgroups per CU), reused by the "
+                    "timed calls; the shape shown is the largest group's"}
     gms, launches = profile_read(lib, "kmeans_batch")
-    graph_s = gms / 1e3 / max(launches, 1)
+    # one graph per group of problems and call: a call's device time is
+    # the sum of its groups' graphs
+    graph_s = gms / 1e3 / max(args.steps, 1)
     inst_iters = sum(g[4] for g in got)
     # algorithmic HBM bytes: every active problem's rows (f32, unpadded) and
     # labels once per iteration it runs
@@ -595,7 +601,9 @@ def bench_batch(args):
         "launch_shape": shape,
         "config": {"workload": f"{len(probs)} k-means problems, n log-uniform[1e3, 2e5] "
                                f"({rows:,} rows), d in {{8,16,32,64}}, k in 4..128, "
-                               f"max_iter={iters}, one persistent launch per iteration in a hipGraph"},
+                               f"max_iter={iters}, problems in groups of about equal row bytes (a group's rows "
+                               f"go up while the previous group iterates), per group one hipGraph of "
+                               f"its iterations"},
         "roofline": {"bound": "hbm", "achieved": byts / graph_s / 1e9, "peak": PEAK_HBM / 1e9,
                      "unit": "GB/s", "frac": byts / graph_s / PEAK_HBM, "traffic": None,
                      "kernel": "label + sums + reduce kernels (graph of all iterations)"},

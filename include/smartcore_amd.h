@@ -79,6 +79,8 @@ typedef struct {
   uint64_t launch_waves;       /* batched k-means: waves per workgroup the launch-shape
                                   tuner chose (0 for other calls) */
   uint64_t launch_groups_per_cu; /* batched k-means: workgroups per CU it chose */
+  uint64_t upload_groups;      /* batched k-means: groups of problems whose rows were
+                                  copied while the previous group iterated */
 } sc_stats;
 
 /* Opaque device-resident reference set (packed row-major copy in HBM).

@@ -74,6 +74,7 @@ class sc_stats(C.Structure):
         ("uncertified", C.c_uint64),
         ("launch_waves", C.c_uint64),
         ("launch_groups_per_cu", C.c_uint64),
+        ("upload_groups", C.c_uint64),
     ]
 
 

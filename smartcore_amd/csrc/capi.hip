@@ -518,6 +518,29 @@ void prof_clear_locked() {
 hipStream_t stream_for(const sc_index* index) { return thread_stream(index->device); }
 hipStream_t stream_for_device(int device) { return thread_stream(device); }
 
+// Created with a CU mask that enables every CU: such a stream gets a hardware
+// queue of its own.  A stream that shares its queue with kernel streams sees
+// every event it records (the copy done, a staging buffer free) complete
+// only after the kernels queued ahead of it there, which halved the rate of
+// staged uploads next to running kernels.
+hipStream_t copy_stream_for_device(int device) {
+  static constexpr int kMaxDev = 64;
+  thread_local hipStream_t streams[kMaxDev] = {};
+  if (device < 0 || device >= kMaxDev) return nullptr;
+  if (!streams[device]) {
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess ||
+        cus <= 0)
+      return nullptr;
+    std::vector<uint32_t> mask((static_cast<size_t>(cus) + 31) / 32, 0u);
+    for (int c = 0; c < cus; ++c) mask[c / 32] |= 1u << (c % 32);
+    if (hipExtStreamCreateWithCUMask(&streams[device], static_cast<uint32_t>(mask.size()),
+                                     mask.data()) != hipSuccess)
+      streams[device] = nullptr;
+  }
+  return streams[device];
+}
+
 }  // namespace sc
 
 using namespace sc;

@@ -196,6 +196,9 @@ bool profiling_enabled();
 // The calling host thread's stream for the index's device.
 hipStream_t stream_for(const sc_index* index);
 hipStream_t stream_for_device(int device);
+// A second stream of the calling host thread for the device, for host-to-
+// device copies that overlap the kernels of the first (no scratch cache).
+hipStream_t copy_stream_for_device(int device);
 
 // ---- kernels entry points shared between translation units ----
 

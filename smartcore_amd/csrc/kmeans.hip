@@ -164,14 +164,7 @@ __global__ __launch_bounds__(NT) void x2_block_kernel(const T* __restrict__ rows
   double s = 0.0;
   for (int j = 0; j < kBlockRows / NT; ++j) {
     const int64_t r = r0 + tid * (kBlockRows / NT) + j;
-    if (r < n) {
-      double t = 0.0;
-      for (int c = 0; c < ld; ++c) {
-        const double v = static_cast<double>(rows[r * ld + c]) - mu[c];
-        t = t + v * v;
-      }
-      s = s + t;
-    }
+    if (r < n) s = s + row_shift_x2(rows + r * ld, mu, ld);
   }
   red[tid] = s;
   __syncthreads();
@@ -732,18 +725,18 @@ int KmItems::build(const KmProb* probs, size_t count, hipStream_t s) {
   ngroup = static_cast<int64_t>(g.size());
   nfinal = static_cast<int64_t>(f.size());
   nprob = static_cast<int64_t>(count);
+  // staged copies: the host vectors may die when they return, and the host
+  // does not wait for work already queued on s
   if (ngroup) {
     SC_HIP(b_group.alloc(sizeof(uint2) * ngroup, s));
-    SC_HIP(hipMemcpyAsync(b_group.p, g.data(), sizeof(uint2) * ngroup, hipMemcpyHostToDevice, s));
+    if (int rc = upload_staged(b_group.p, g.data(), sizeof(uint2) * ngroup, s)) return rc;
     group = b_group.as<uint2>();
   }
   if (nfinal) {
     SC_HIP(b_final.alloc(sizeof(uint2) * nfinal, s));
-    SC_HIP(hipMemcpyAsync(b_final.p, f.data(), sizeof(uint2) * nfinal, hipMemcpyHostToDevice, s));
+    if (int rc = upload_staged(b_final.p, f.data(), sizeof(uint2) * nfinal, s)) return rc;
     fin = b_final.as<uint2>();
   }
-  // the host vectors die here: make the copies complete first
-  SC_HIP(hipStreamSynchronize(s));
   return SC_OK;
 }
 

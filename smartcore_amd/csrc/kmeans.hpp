@@ -24,6 +24,45 @@ constexpr int kMaxLdsAcc = 8192;
 // Elements of one cluster's sum vector handled by one reduce workgroup.
 constexpr int kReduceChunk = 256;
 
+// ||x - mu||^2 of one row over its ld columns in f64, the columns folded in
+// order.  f32 rows with ld a multiple of 16 (every padded f32 row: its start
+// is 64-byte aligned) are read 16 values per step; the fold is the same.
+__device__ inline double row_shift_x2(const double* __restrict__ row,
+                                      const double* __restrict__ mu, int ld) {
+  double t = 0.0;
+  for (int c = 0; c < ld; ++c) {
+    const double v = row[c] - mu[c];
+    t = t + v * v;
+  }
+  return t;
+}
+__device__ inline double row_shift_x2(const float* __restrict__ row,
+                                      const double* __restrict__ mu, int ld) {
+  double t = 0.0;
+  if (ld % 16 != 0) {
+    for (int c = 0; c < ld; ++c) {
+      const double v = static_cast<double>(row[c]) - mu[c];
+      t = t + v * v;
+    }
+    return t;
+  }
+  for (int c0 = 0; c0 < ld; c0 += 16) {
+    float4 q[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) q[u] = *reinterpret_cast<const float4*>(row + c0 + 4 * u);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const float e[4] = {q[u].x, q[u].y, q[u].z, q[u].w};
+#pragma unroll
+      for (int w = 0; w < 4; ++w) {
+        const double v = static_cast<double>(e[w]) - mu[c0 + 4 * u + w];
+        t = t + v * v;
+      }
+    }
+  }
+  return t;
+}
+
 // Device-side Lloyd loop state (kmeans.rs:273, :305-309).
 struct KmState {
   double distortion;  // best distortion so far (f64::MAX at start)

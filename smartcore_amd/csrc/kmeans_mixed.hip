@@ -56,6 +56,7 @@
 #include <cmath>
 #include <cstring>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -849,14 +850,7 @@ __global__ __launch_bounds__(NT) void kmb_x2_kernel(const KmProb* __restrict__ P
     double s = 0.0;
     for (int j = 0; j < kBlockRows / NT; ++j) {
       const int64_t r = r0 + tid * (kBlockRows / NT) + j;
-      if (r < p.n) {
-        double t = 0.0;
-        for (int c = 0; c < p.ld; ++c) {
-          const double v = static_cast<double>(rows[r * p.ld + c]) - p.mu[c];
-          t = t + v * v;
-        }
-        s = s + t;
-      }
+      if (r < p.n) s = s + row_shift_x2(rows + r * p.ld, p.mu, p.ld);
     }
     red[tid] = s;
     __syncthreads();
@@ -883,8 +877,68 @@ struct Klass {
   bool exact;
   int kmax = 0, ldmax = 0;
   std::vector<uint2> items;
-  DevBuf buf;
+  size_t at = 0;  // offset of the items in the call's item buffer
 };
+
+// Consecutive problems of one batched call whose rows go up together; the
+// group's iterations run while the next group's rows are copied.  Its items
+// hold problem indices relative to the group's first problem.
+struct Group {
+  size_t first = 0, count = 0;
+  std::vector<uint2> all;  // (problem, 2048-row block), block-major
+  size_t all_at = 0;       // offset of `all` in the call's item buffer
+  std::map<int, Klass> klass;
+  KmItems red;
+  int ldmax = 0;
+  int64_t rows = 0;
+  int waves = 0, mult = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  Group() = default;
+  Group(const Group&) = delete;
+  Group& operator=(const Group&) = delete;
+  ~Group() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+  }
+};
+
+// Side streams of the calling host thread for a device (default priority),
+// one per shape class, with their join events and the fork event.  Kept for
+// the thread's life like its main stream: creating and destroying a dozen
+// streams in every call costs milliseconds.
+struct SideStreams {
+  std::vector<hipStream_t> st;
+  std::vector<hipEvent_t> done;
+  hipEvent_t fork = nullptr;
+};
+
+int side_streams(int dev, size_t n, SideStreams** out) {
+  static constexpr int kMaxDev = 64;
+  thread_local SideStreams per[kMaxDev];
+  if (dev < 0 || dev >= kMaxDev) return fail(SC_ERR_DEVICE, "device index out of range");
+  SideStreams& ss = per[dev];
+  if (!ss.fork) SC_HIP(hipEventCreateWithFlags(&ss.fork, hipEventDisableTiming));
+  while (ss.st.size() < n) {
+    hipStream_t x = nullptr;
+    SC_HIP(hipStreamCreateWithFlags(&x, hipStreamNonBlocking));
+    ss.st.push_back(x);
+  }
+  while (ss.done.size() < n) {
+    hipEvent_t e = nullptr;
+    SC_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    ss.done.push_back(e);
+  }
+  *out = &ss;
+  return SC_OK;
+}
+
+// Problem groups per batched call, and the fewest row bytes a group holds.
+// Each group adds about 0.35 ms of fixed work per iteration (its own label,
+// sums and reduce launches): on 512 problems (2.7 GB of rows, 20 iterations)
+// calls took 111 / 95 / 98 / 100 ms with 1 / 2 / 3 / 4 groups.
+constexpr int64_t kUploadGroups = 2;
+constexpr int64_t kUploadGroupBytes = int64_t{64} << 20;
 
 // The batched form's launch shape for a set of problems: label workgroups
 // per CU (four waves each), and the waves per workgroup it was timed with.
@@ -1022,7 +1076,45 @@ extern "C" int sc_kmeans_batch_run(const sc_kmeans_problem* problems, size_t cou
     p.c2f = bC2f.as<float>() + o.c2f;
     p.cmax = p.c2f + p.kpad;
   }
-  if (int rc = upload_staged(bP.p, P.data(), sizeof(KmProb) * count, s)) return rc;
+  // Two streams from here on: cs carries every host-to-device copy, s every
+  // kernel.  The problems go in groups of consecutive problems with about
+  // equal row bytes; a group's iterations run on s while the next group's
+  // rows go up on cs (the copy engines need none of the CUs that the
+  // persistent label launches hold), and s waits for a group's rows
+  // through an event.
+  hipStream_t cs = copy_stream_for_device(dev);
+  if (!cs) return fail(SC_ERR_DEVICE, "no copy stream for the device");
+  std::vector<std::unique_ptr<Group>> groups;
+  DevBuf bItems;
+  struct Events {
+    std::vector<hipEvent_t> e;
+    ~Events() {
+      for (hipEvent_t x : e) (void)hipEventDestroy(x);
+    }
+    hipError_t add(hipEvent_t* out) {
+      hipEvent_t x = nullptr;
+      const hipError_t r = hipEventCreateWithFlags(&x, hipEventDisableTiming);
+      if (r == hipSuccess) {
+        e.push_back(x);
+        *out = x;
+      }
+      return r;
+    }
+  } evs;
+  // Declared after every buffer that cs writes: whatever the return, both
+  // streams are drained before those buffers are released.
+  struct Drain {
+    hipStream_t a, b;
+    ~Drain() {
+      (void)hipStreamSynchronize(a);
+      (void)hipStreamSynchronize(b);
+    }
+  } drain{s, cs};
+  hipEvent_t allocated = nullptr;
+  SC_HIP(evs.add(&allocated));
+  SC_HIP(hipEventRecord(allocated, s));
+  SC_HIP(hipStreamWaitEvent(cs, allocated, 0));
+  if (int rc = upload_staged(bP.p, P.data(), sizeof(KmProb) * count, cs)) return rc;
   // the initial centroids of every problem, padded on the host, in one copy
   std::vector<double> hcent0(static_cast<size_t>(tot.cent), 0.0);
   for (size_t i = 0; i < count; ++i) {
@@ -1031,193 +1123,206 @@ extern "C" int sc_kmeans_batch_run(const sc_kmeans_problem* problems, size_t cou
       std::copy(pr.init_centroids + c * pr.d, pr.init_centroids + (c + 1) * pr.d,
                 hcent0.data() + off[i].cent + c * P[i].ld);
   }
-  if (int rc = upload_staged(bCent.p, hcent0.data(), sizeof(double) * tot.cent, s)) return rc;
-  {
-    // the problems' rows are consecutive in the row arena: one staged
-    // stream of rows, padded on the host
-    std::vector<StageSeg> segs(count);
-    for (size_t i = 0; i < count; ++i)
-      segs[i] = StageSeg{problems[i].rows, problems[i].n, problems[i].d * sizeof(float),
-                         static_cast<size_t>(P[i].ld) * sizeof(float)};
-    if (int rc = upload_staged_segments(bRows.p, segs.data(), count, s)) return rc;
-  }
+  if (int rc = upload_staged(bCent.p, hcent0.data(), sizeof(double) * tot.cent, cs)) return rc;
   const KmProb* dP = bP.as<KmProb>();
 
-  // ---- items: x2 over every block; step items per shape class ----
-  std::vector<uint2> all;
-  std::map<int, Klass> klass;
-  int64_t maxnb = 0;
-  int ldmax_all = 0;
-  for (auto& p : P) {
-    maxnb = std::max(maxnb, p.nb);
-    ldmax_all = std::max(ldmax_all, p.ld);
-  }
-  // block-major across problems so concurrent workgroups spread over problems
-  for (int64_t b = 0; b < maxnb; ++b)
-    for (size_t i = 0; i < count; ++i) {
-      const KmProb& p = P[i];
-      if (b >= p.nb) continue;
-      const uint2 it = make_uint2(static_cast<unsigned>(i), static_cast<unsigned>(b));
-      all.push_back(it);
-      const int kbm = kbm_of(p.k), ksm = ksm_of(p.ld);
-      const bool exact = (p.k + 31) / 32 == kbm && p.ld / 16 == ksm;
-      Klass& K = klass[(kbm * 16 + ksm) * 2 + (exact ? 1 : 0)];
-      K.kbm = kbm;
-      K.ksm = ksm;
-      K.exact = exact;
-      K.kmax = std::max(K.kmax, p.k);
-      K.ldmax = std::max(K.ldmax, p.ld);
-      // the label segments of this block
-      const int64_t seg_end = std::min<int64_t>((b + 1) * (kBlockRows / kSegRows),
-                                                (p.n + kSegRows - 1) / kSegRows);
-      for (int64_t sg = b * (kBlockRows / kSegRows); sg < seg_end; ++sg)
-        K.items.push_back(make_uint2(static_cast<unsigned>(i), static_cast<unsigned>(sg)));
-    }
-  DevBuf bAll;
-  SC_HIP(bAll.alloc(sizeof(uint2) * all.size(), s));
-  SC_HIP(hipMemcpyAsync(bAll.p, all.data(), sizeof(uint2) * all.size(), hipMemcpyHostToDevice, s));
-  for (auto& kv : klass) {
-    Klass& K = kv.second;
-    SC_HIP(K.buf.alloc(sizeof(uint2) * K.items.size(), s));
-    SC_HIP(hipMemcpyAsync(K.buf.p, K.items.data(), sizeof(uint2) * K.items.size(),
-                          hipMemcpyHostToDevice, s));
-  }
-  KmItems red;
-  if (int rc = red.build(P.data(), count, s)) return rc;  // waits for the copies above
-
-  const unsigned cgrid = static_cast<unsigned>((count + 255) / 256);
-  hipLaunchKernelGGL(kmb_init_kernel, dim3(cgrid), dim3(256), 0, s, dP, static_cast<int>(count));
-  hipLaunchKernelGGL(kmb_mu_kernel, dim3(static_cast<unsigned>(count)), dim3(128), 0, s, dP,
-                     static_cast<int>(count));
-  hipLaunchKernelGGL(kmb_x2_kernel, dim3(1024), dim3(NT), 0, s, dP, bAll.as<uint2>(),
-                     static_cast<int64_t>(all.size()));
-  hipLaunchKernelGGL(kmb_x2_total_kernel, dim3(cgrid), dim3(256), 0, s, dP, static_cast<int>(count));
-  SC_HIP(hipGetLastError());
-  if (int rc = kmeans_tables_launch(dP, count, s)) return rc;
-
-  int cus = 256;
-  hipDeviceProp_t prop;
-  if (hipGetDeviceProperties(&prop, dev) == hipSuccess && prop.multiProcessorCount > 0)
-    cus = prop.multiProcessorCount;
-  // One side stream per shape class (default priority), forked from s and
-  // joined back in every iteration, so the classes' launches run side by
-  // side; the reduce launches follow the join.  Inside the graph capture
-  // the fork and join become parallel branches.
-  struct Sides {
-    std::vector<hipStream_t> st;
-    std::vector<hipEvent_t> done;
-    hipEvent_t fork = nullptr;
-    ~Sides() {
-      for (hipStream_t x : st) (void)hipStreamDestroy(x);
-      for (hipEvent_t e : done) (void)hipEventDestroy(e);
-      if (fork) (void)hipEventDestroy(fork);
-    }
-  } sides;
-  SC_HIP(hipEventCreateWithFlags(&sides.fork, hipEventDisableTiming));
-  for (size_t i = 0; i < klass.size(); ++i) {
-    hipStream_t x = nullptr;
-    hipEvent_t e = nullptr;
-    SC_HIP(hipStreamCreateWithFlags(&x, hipStreamNonBlocking));
-    sides.st.push_back(x);
-    SC_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-    sides.done.push_back(e);
-  }
-  auto iteration = [&](int waves, int mult) -> int {
-    SC_HIP(hipEventRecord(sides.fork, s));
-    size_t i = 0;
-    for (auto& kv : klass) {
-      Klass& K = kv.second;
-      const int64_t ni = static_cast<int64_t>(K.items.size());
-      SC_HIP(hipStreamWaitEvent(sides.st[i], sides.fork, 0));
-      if (int rc = launch_label(K.kbm, K.ksm, mult, dP, K.buf.as<uint2>(), ni, K.exact, cus,
-                                sides.st[i]))
-        return rc;
-      SC_HIP(hipEventRecord(sides.done[i], sides.st[i]));
-      ++i;
-    }
-    for (hipEvent_t e : sides.done) SC_HIP(hipStreamWaitEvent(s, e, 0));
-    if (int rc = launch_sums(dP, bAll.as<uint2>(), static_cast<int64_t>(all.size()), ldmax_all, cus, s))
-      return rc;
-    return kmeans_reduce_launch(dP, red, true, true, s);
-  };
-
-  // ---- launch shape: cached, or timed on the first iterations ----
-  // Candidates: label workgroups per CU (1, 2, 3, 4, 6, 8; four waves
-  // each).  Every shape gives the same labels and sums (the sums' order is
-  // fixed per block), so the timed iterations are real ones.
-  std::vector<int64_t> key;
-  for (auto& kv : klass) {
-    key.push_back(kv.first);
-    key.push_back(static_cast<int64_t>(kv.second.items.size()));
-  }
-  int waves = 0, mult = 0;
+  // ---- groups: cut where the running row bytes pass g / ngroups ----
   {
-    std::lock_guard<std::mutex> lk(g_tune_mu);
-    for (auto& e : g_tune)
-      if (e.key == key) {
-        waves = e.waves;
-        mult = e.mult;
-      }
-  }
-  size_t it_done = 0;
-  if (mult == 0) {
-    const int cand[6][2] = {{4, 1}, {4, 2}, {4, 3}, {4, 4}, {4, 6}, {4, 8}};
-    float best = 1e30f;
-    struct TimerPair {
-      hipEvent_t a = nullptr, b = nullptr;
-      ~TimerPair() {
-        if (a) (void)hipEventDestroy(a);
-        if (b) (void)hipEventDestroy(b);
-      }
-    } tp;
-    SC_HIP(hipEventCreate(&tp.a));
-    SC_HIP(hipEventCreate(&tp.b));
-    hipEvent_t e0 = tp.a, e1 = tp.b;
-    for (int c = 0; c < 6 && it_done < max_iter; ++c) {
-      SC_HIP(hipEventRecord(e0, s));
-      if (int rc = iteration(cand[c][0], cand[c][1])) return rc;
-      SC_HIP(hipEventRecord(e1, s));
-      SC_HIP(hipEventSynchronize(e1));
-      float ms = 0.0f;
-      SC_HIP(hipEventElapsedTime(&ms, e0, e1));
-      ++it_done;
-      if (ms < best) {
-        best = ms;
-        waves = cand[c][0];
-        mult = cand[c][1];
+    const int64_t row_bytes = static_cast<int64_t>(sizeof(float)) * tot.rows;
+    const int64_t ng = std::max<int64_t>(1, std::min<int64_t>(kUploadGroups, row_bytes / kUploadGroupBytes));
+    int64_t acc = 0;
+    size_t first = 0;
+    for (size_t i = 0; i < count; ++i) {
+      acc += static_cast<int64_t>(sizeof(float)) * P[i].n * P[i].ld;
+      if (i + 1 == count || acc * ng >= row_bytes * static_cast<int64_t>(groups.size() + 1)) {
+        auto g = std::make_unique<Group>();
+        g->first = first;
+        g->count = i + 1 - first;
+        groups.push_back(std::move(g));
+        first = i + 1;
       }
     }
-    if (it_done == 6) {  // a full comparison: kept for later calls
+  }
+
+  // ---- items of every group (problem indices relative to the group):
+  // sums items over every block, label items per shape class; one copy ----
+  std::vector<uint2> hitems;
+  for (auto& gp : groups) {
+    Group& G = *gp;
+    const KmProb* HP = P.data() + G.first;
+    int64_t maxnb = 0;
+    for (size_t i = 0; i < G.count; ++i) {
+      maxnb = std::max(maxnb, HP[i].nb);
+      G.ldmax = std::max(G.ldmax, HP[i].ld);
+      G.rows += HP[i].n;
+    }
+    // block-major across problems so concurrent workgroups spread over problems
+    for (int64_t b = 0; b < maxnb; ++b)
+      for (size_t i = 0; i < G.count; ++i) {
+        const KmProb& p = HP[i];
+        if (b >= p.nb) continue;
+        G.all.push_back(make_uint2(static_cast<unsigned>(i), static_cast<unsigned>(b)));
+        const int kbm = kbm_of(p.k), ksm = ksm_of(p.ld);
+        const bool exact = (p.k + 31) / 32 == kbm && p.ld / 16 == ksm;
+        Klass& K = G.klass[(kbm * 16 + ksm) * 2 + (exact ? 1 : 0)];
+        K.kbm = kbm;
+        K.ksm = ksm;
+        K.exact = exact;
+        K.kmax = std::max(K.kmax, p.k);
+        K.ldmax = std::max(K.ldmax, p.ld);
+        // the label segments of this block
+        const int64_t seg_end = std::min<int64_t>((b + 1) * (kBlockRows / kSegRows),
+                                                  (p.n + kSegRows - 1) / kSegRows);
+        for (int64_t sg = b * (kBlockRows / kSegRows); sg < seg_end; ++sg)
+          K.items.push_back(make_uint2(static_cast<unsigned>(i), static_cast<unsigned>(sg)));
+      }
+    G.all_at = hitems.size();
+    hitems.insert(hitems.end(), G.all.begin(), G.all.end());
+    for (auto& kv : G.klass) {
+      kv.second.at = hitems.size();
+      hitems.insert(hitems.end(), kv.second.items.begin(), kv.second.items.end());
+    }
+    if (int rc = G.red.build(HP, G.count, cs)) return rc;
+  }
+  SC_HIP(bItems.alloc(sizeof(uint2) * hitems.size(), cs));
+  if (int rc = upload_staged(bItems.p, hitems.data(), sizeof(uint2) * hitems.size(), cs)) return rc;
+
+  const int cus = device_cus();
+  // One side stream per shape class, forked from s and joined back in every
+  // iteration, so the classes' launches run side by side; the reduce
+  // launches follow the join.  Inside the graph capture the fork and join
+  // become parallel branches.
+  size_t nsides = 0;
+  for (auto& gp : groups) nsides = std::max(nsides, gp->klass.size());
+  SideStreams* sp = nullptr;
+  if (int rc = side_streams(dev, nsides, &sp)) return rc;
+  SideStreams& sides = *sp;
+
+  for (auto& gp : groups) {
+    Group& G = *gp;
+    // the group's rows: one staged stream, padded on the host
+    {
+      std::vector<StageSeg> segs(G.count);
+      for (size_t i = 0; i < G.count; ++i) {
+        const size_t pi = G.first + i;
+        segs[i] = StageSeg{problems[pi].rows, problems[pi].n, problems[pi].d * sizeof(float),
+                           static_cast<size_t>(P[pi].ld) * sizeof(float)};
+      }
+      if (int rc = upload_staged_segments(bRows.as<float>() + off[G.first].rows, segs.data(),
+                                          G.count, cs))
+        return rc;
+    }
+    hipEvent_t up = nullptr;
+    SC_HIP(evs.add(&up));
+    SC_HIP(hipEventRecord(up, cs));
+    SC_HIP(hipStreamWaitEvent(s, up, 0));
+
+    const KmProb* dG = dP + G.first;
+    const uint2* d_all = bItems.as<uint2>() + G.all_at;
+    const int64_t nall = static_cast<int64_t>(G.all.size());
+    const int gc = static_cast<int>(G.count);
+    const unsigned cgrid = static_cast<unsigned>((G.count + 255) / 256);
+    hipLaunchKernelGGL(kmb_init_kernel, dim3(cgrid), dim3(256), 0, s, dG, gc);
+    hipLaunchKernelGGL(kmb_mu_kernel, dim3(static_cast<unsigned>(gc)), dim3(128), 0, s, dG, gc);
+    hipLaunchKernelGGL(kmb_x2_kernel, dim3(1024), dim3(NT), 0, s, dG, d_all, nall);
+    hipLaunchKernelGGL(kmb_x2_total_kernel, dim3(cgrid), dim3(256), 0, s, dG, gc);
+    SC_HIP(hipGetLastError());
+    if (int rc = kmeans_tables_launch(dG, G.count, s)) return rc;
+
+    auto iteration = [&](int mult) -> int {
+      SC_HIP(hipEventRecord(sides.fork, s));
+      size_t i = 0;
+      for (auto& kv : G.klass) {
+        Klass& K = kv.second;
+        const int64_t ni = static_cast<int64_t>(K.items.size());
+        SC_HIP(hipStreamWaitEvent(sides.st[i], sides.fork, 0));
+        if (int rc = launch_label(K.kbm, K.ksm, mult, dG, bItems.as<uint2>() + K.at, ni, K.exact,
+                                  cus, sides.st[i]))
+          return rc;
+        SC_HIP(hipEventRecord(sides.done[i], sides.st[i]));
+        ++i;
+      }
+      for (size_t j = 0; j < i; ++j) SC_HIP(hipStreamWaitEvent(s, sides.done[j], 0));
+      if (int rc = launch_sums(dG, d_all, nall, G.ldmax, cus, s)) return rc;
+      return kmeans_reduce_launch(dG, G.red, true, true, s);
+    };
+
+    // ---- launch shape: cached, or timed on the first iterations ----
+    // Candidates: label workgroups per CU (1, 2, 3, 4, 6, 8; four waves
+    // each).  Every shape gives the same labels and sums (the sums' order is
+    // fixed per block), so the timed iterations are real ones.
+    std::vector<int64_t> key;
+    for (auto& kv : G.klass) {
+      key.push_back(kv.first);
+      key.push_back(static_cast<int64_t>(kv.second.items.size()));
+    }
+    {
       std::lock_guard<std::mutex> lk(g_tune_mu);
-      g_tune.push_back(TuneEntry{key, waves, mult});
+      for (auto& e : g_tune)
+        if (e.key == key) {
+          G.waves = e.waves;
+          G.mult = e.mult;
+        }
+    }
+    size_t it_done = 0;
+    if (G.mult == 0) {
+      const int cand[6][2] = {{4, 1}, {4, 2}, {4, 3}, {4, 4}, {4, 6}, {4, 8}};
+      float best = 1e30f;
+      struct TimerPair {
+        hipEvent_t a = nullptr, b = nullptr;
+        ~TimerPair() {
+          if (a) (void)hipEventDestroy(a);
+          if (b) (void)hipEventDestroy(b);
+        }
+      } tp;
+      SC_HIP(hipEventCreate(&tp.a));
+      SC_HIP(hipEventCreate(&tp.b));
+      for (int c = 0; c < 6 && it_done < max_iter; ++c) {
+        SC_HIP(hipEventRecord(tp.a, s));
+        if (int rc = iteration(cand[c][1])) return rc;
+        SC_HIP(hipEventRecord(tp.b, s));
+        SC_HIP(hipEventSynchronize(tp.b));
+        float ms = 0.0f;
+        SC_HIP(hipEventElapsedTime(&ms, tp.a, tp.b));
+        ++it_done;
+        if (ms < best) {
+          best = ms;
+          G.waves = cand[c][0];
+          G.mult = cand[c][1];
+        }
+      }
+      if (it_done == 6) {  // a full comparison: kept for later calls
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        g_tune.push_back(TuneEntry{key, G.waves, G.mult});
+      }
+    }
+    // ---- remaining iterations as one graph, launched without waiting ----
+    if (it_done < max_iter) {
+      ProfScope ps("kmeans_batch", s);
+      hipGraph_t graph = nullptr;
+      hipGraphExec_t exec = nullptr;
+      SC_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+      int crc = SC_OK;
+      for (size_t it = it_done; it < max_iter && crc == SC_OK; ++it) crc = iteration(G.mult);
+      hipError_t ce = hipStreamEndCapture(s, &graph);
+      if (crc) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return crc;
+      }
+      if (ce != hipSuccess) return hip_fail(ce, "hipStreamEndCapture");
+      ce = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+      if (ce != hipSuccess) {
+        (void)hipGraphDestroy(graph);
+        return hip_fail(ce, "hipGraphInstantiate");
+      }
+      // kept by the group until both streams are drained
+      G.graph = graph;
+      G.exec = exec;
+      ce = hipGraphLaunch(exec, s);
+      if (ce != hipSuccess) return hip_fail(ce, "hipGraphLaunch");
     }
   }
-  // ---- remaining iterations as one graph ----
-  if (it_done < max_iter) {
-    ProfScope ps("kmeans_batch", s);
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
-    SC_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-    int crc = SC_OK;
-    for (size_t it = it_done; it < max_iter && crc == SC_OK; ++it) crc = iteration(waves, mult);
-    hipError_t ce = hipStreamEndCapture(s, &graph);
-    if (crc) {
-      if (graph) (void)hipGraphDestroy(graph);
-      return crc;
-    }
-    if (ce != hipSuccess) return hip_fail(ce, "hipStreamEndCapture");
-    ce = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-    if (ce != hipSuccess) {
-      (void)hipGraphDestroy(graph);
-      return hip_fail(ce, "hipGraphInstantiate");
-    }
-    ce = hipGraphLaunch(exec, s);
-    const hipError_t se = hipStreamSynchronize(s);
-    (void)hipGraphExecDestroy(exec);
-    (void)hipGraphDestroy(graph);
-    if (ce != hipSuccess) return hip_fail(ce, "hipGraphLaunch");
-    if (se != hipSuccess) return hip_fail(se, "hipStreamSynchronize");
-  }
+  SC_HIP(hipStreamSynchronize(s));
   // ---- results ----
   // the states sit st_stride apart in one arena: one copy for all
   std::vector<char> hst(st_stride * count);
@@ -1246,11 +1351,16 @@ extern "C" int sc_kmeans_batch_run(const sc_kmeans_problem* problems, size_t cou
     iters_out[i] = hs[i].iters;
     fb += hs[i].fallback;
   }
+  // the launch shape reported is the one of the group with the most rows
+  const Group* big = groups.front().get();
+  for (auto& gp : groups)
+    if (gp->rows > big->rows) big = gp.get();
   sc_stats& st = last_stats();
   st = sc_stats{};
   st.queries = count;
   st.uncertified = fb;
-  st.launch_waves = static_cast<uint64_t>(waves);
-  st.launch_groups_per_cu = static_cast<uint64_t>(mult);
+  st.launch_waves = static_cast<uint64_t>(big->waves);
+  st.launch_groups_per_cu = static_cast<uint64_t>(big->mult);
+  st.upload_groups = groups.size();
   return SC_OK;
 }

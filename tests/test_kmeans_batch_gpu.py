@@ -78,8 +78,11 @@ def test_batch_launch_shape_tuner_is_invariant_and_reported():
 def test_large_batch_equals_single_runs():
     """Six problems of 200,000 x 64 f32 rows (307 MB, several hundred
     label items per problem, rows staged through the pinned buffers in
-    many pieces): every problem equals its single-problem fit bit for bit
-    and its labels the FP64 form's."""
+    many pieces, in two groups: the second group's rows upload while the
+    first group iterates): every problem equals its single-problem fit bit for bit
+    and its labels the FP64 form's, both on the first call (six tuning
+    iterations per group, then a graph) and on a second call (the kept
+    shapes, every iteration in the graph)."""
     rng = np.random.default_rng(55)
     probs = []
     for j in range(6):
@@ -89,17 +92,23 @@ def test_large_batch_equals_single_runs():
         init = x[np.sort(rng.choice(n, size=k, replace=False))].astype(np.float64)
         probs.append((x, init))
     assert sum(x.nbytes for x, _ in probs) > 256 << 20
-    got = S.kmeans_batch_run(probs, 5, S.SC_MIXED)
-    assert S.last_stats()["queries"] == len(probs)
-    for (x, init), (c, lab, size, dist, it) in zip(probs, got):
-        c1, l1, s1, d1, i1 = single(x, init, 5, S.SC_MIXED)
-        np.testing.assert_array_equal(lab, l1)
-        np.testing.assert_array_equal(size, s1)
-        assert it == i1
-        np.testing.assert_array_equal(c.view(np.uint64), c1.view(np.uint64))
-        assert np.float64(dist).view(np.uint64) == np.float64(d1).view(np.uint64)
-        c2, l2, s2, d2, i2 = single(x, init, 5, S.SC_PLAIN_FP64)
-        np.testing.assert_array_equal(lab, l2)
+    iters = 9
+    first = S.kmeans_batch_run(probs, iters, S.SC_MIXED)
+    st = S.last_stats()
+    assert st["queries"] == len(probs)
+    assert st["upload_groups"] == 2
+    second = S.kmeans_batch_run(probs, iters, S.SC_MIXED)
+    assert S.last_stats()["upload_groups"] == 2
+    for (x, init), ra, rb in zip(probs, first, second):
+        c1, l1, s1, d1, i1 = single(x, init, iters, S.SC_MIXED)
+        for c, lab, size, dist, it in (ra, rb):
+            np.testing.assert_array_equal(lab, l1)
+            np.testing.assert_array_equal(size, s1)
+            assert it == i1
+            np.testing.assert_array_equal(c.view(np.uint64), c1.view(np.uint64))
+            assert np.float64(dist).view(np.uint64) == np.float64(d1).view(np.uint64)
+        c2, l2, s2, d2, i2 = single(x, init, iters, S.SC_PLAIN_FP64)
+        np.testing.assert_array_equal(ra[1], l2)
 
 
 def test_batch_fp64_mode_runs_problems_one_by_one():
