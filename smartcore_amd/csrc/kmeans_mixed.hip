@@ -903,36 +903,6 @@ struct Group {
   }
 };
 
-// Side streams of the calling host thread for a device (default priority),
-// one per shape class, with their join events and the fork event.  Kept for
-// the thread's life like its main stream: creating and destroying a dozen
-// streams in every call costs milliseconds.
-struct SideStreams {
-  std::vector<hipStream_t> st;
-  std::vector<hipEvent_t> done;
-  hipEvent_t fork = nullptr;
-};
-
-int side_streams(int dev, size_t n, SideStreams** out) {
-  static constexpr int kMaxDev = 64;
-  thread_local SideStreams per[kMaxDev];
-  if (dev < 0 || dev >= kMaxDev) return fail(SC_ERR_DEVICE, "device index out of range");
-  SideStreams& ss = per[dev];
-  if (!ss.fork) SC_HIP(hipEventCreateWithFlags(&ss.fork, hipEventDisableTiming));
-  while (ss.st.size() < n) {
-    hipStream_t x = nullptr;
-    SC_HIP(hipStreamCreateWithFlags(&x, hipStreamNonBlocking));
-    ss.st.push_back(x);
-  }
-  while (ss.done.size() < n) {
-    hipEvent_t e = nullptr;
-    SC_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-    ss.done.push_back(e);
-  }
-  *out = &ss;
-  return SC_OK;
-}
-
 // Problem groups per batched call, and the fewest row bytes a group holds.
 // Each group adds about 0.35 ms of fixed work per iteration (its own label,
 // sums and reduce launches): on 512 problems (2.7 GB of rows, 20 iterations)
@@ -1188,15 +1158,6 @@ extern "C" int sc_kmeans_batch_run(const sc_kmeans_problem* problems, size_t cou
   if (int rc = upload_staged(bItems.p, hitems.data(), sizeof(uint2) * hitems.size(), cs)) return rc;
 
   const int cus = device_cus();
-  // One side stream per shape class, forked from s and joined back in every
-  // iteration, so the classes' launches run side by side; the reduce
-  // launches follow the join.  Inside the graph capture the fork and join
-  // become parallel branches.
-  size_t nsides = 0;
-  for (auto& gp : groups) nsides = std::max(nsides, gp->klass.size());
-  SideStreams* sp = nullptr;
-  if (int rc = side_streams(dev, nsides, &sp)) return rc;
-  SideStreams& sides = *sp;
 
   for (auto& gp : groups) {
     Group& G = *gp;
@@ -1229,20 +1190,24 @@ extern "C" int sc_kmeans_batch_run(const sc_kmeans_problem* problems, size_t cou
     SC_HIP(hipGetLastError());
     if (int rc = kmeans_tables_launch(dG, G.count, s)) return rc;
 
+    // The shape classes' label launches go one after another on s, the
+    // most work first (items x k blocks x row blocks): each launch is
+    // persistent over the whole chip, and the small classes at the end
+    // leave short tails.  (One side stream per class, forked and joined in
+    // every iteration, ran slower: the branches share the process's four
+    // hardware queues, so they queued behind one another anyway.)
+    std::vector<const Klass*> order;
+    for (auto& kv : G.klass) order.push_back(&kv.second);
+    auto work = [](const Klass* K) {
+      return static_cast<double>(K->items.size()) * K->kbm * K->ksm;
+    };
+    std::stable_sort(order.begin(), order.end(),
+                     [&](const Klass* a, const Klass* b) { return work(a) > work(b); });
     auto iteration = [&](int mult) -> int {
-      SC_HIP(hipEventRecord(sides.fork, s));
-      size_t i = 0;
-      for (auto& kv : G.klass) {
-        Klass& K = kv.second;
-        const int64_t ni = static_cast<int64_t>(K.items.size());
-        SC_HIP(hipStreamWaitEvent(sides.st[i], sides.fork, 0));
-        if (int rc = launch_label(K.kbm, K.ksm, mult, dG, bItems.as<uint2>() + K.at, ni, K.exact,
-                                  cus, sides.st[i]))
+      for (const Klass* K : order)
+        if (int rc = launch_label(K->kbm, K->ksm, mult, dG, bItems.as<uint2>() + K->at,
+                                  static_cast<int64_t>(K->items.size()), K->exact, cus, s))
           return rc;
-        SC_HIP(hipEventRecord(sides.done[i], sides.st[i]));
-        ++i;
-      }
-      for (size_t j = 0; j < i; ++j) SC_HIP(hipStreamWaitEvent(s, sides.done[j], 0));
       if (int rc = launch_sums(dG, d_all, nall, G.ldmax, cus, s)) return rc;
       return kmeans_reduce_launch(dG, G.red, true, true, s);
     };
