@@ -349,20 +349,27 @@ __device__ __forceinline__ void set_cmax(const KmProb& p) {
   set_cmax_of(p, mx);
 }
 
-// Final sums.  Item (problem, cluster c), 128 threads: sums_c = group sums
-// in group order, the count, the distortion term of c with this step's
-// centroid (dot and ||c||^2 added in dimension order), then (update)
-// c = sums_c / n_c when n_c > 0, and the MFMA terms of the resulting
-// centroid when the problem has them.  LAST: the last workgroup of a
-// problem (an arrival counter) adds the terms in cluster order, applies the
-// stop rule (control) and sets the centroid-norm maxima; otherwise the close
-// kernel does that after this launch.  Each arrival takes a device-scope
-// release fence, which writes back the L2: cheap for one problem's k
-// workgroups, not for the tens of thousands of a batched call.
-template <bool LAST>
-__global__ __launch_bounds__(128) void kmeans_final_kernel(const KmProb* __restrict__ P,
-                                                           const uint2* __restrict__ items,
-                                                           int update, int control) {
+// Final sums.  Item (problem, cluster c): sums_c = group sums in group
+// order, the count, the distortion term of c with this step's centroid (dot
+// and ||c||^2 added in dimension order), then (update) c = sums_c / n_c when
+// n_c > 0, and the MFMA terms of the resulting centroid when the problem has
+// them.  LAST: the last workgroup of a problem (an arrival counter) adds the
+// terms in cluster order, applies the stop rule (control) and sets the
+// centroid-norm maxima; otherwise the close kernel does that after this
+// launch.  Each arrival takes a device-scope release fence, which writes
+// back the L2: cheap for one problem's k workgroups, not for the tens of
+// thousands of a batched call.
+// FUSED (one problem, ng * d <= kFusedGroupSums): the workgroup first forms
+// the group sums of its cluster itself, from the block partials (each a sum
+// of kGroupBlocks blocks in block order, into LDS), which is the arithmetic
+// of kmeans_group_kernel, so the group launch is left out.
+constexpr int kFusedThreads = 512;
+constexpr int kFusedGroupSums = 8192;  // doubles of LDS for the group sums
+
+template <bool LAST, bool FUSED>
+__global__ __launch_bounds__(FUSED ? kFusedThreads : 128) void kmeans_final_kernel(
+    const KmProb* __restrict__ P, const uint2* __restrict__ items, int update, int control) {
+  constexpr int NTH = FUSED ? kFusedThreads : 128;
   const uint2 it = items[blockIdx.x];
   const KmProb p = P[it.x];
   if (p.st->done) return;
@@ -372,6 +379,7 @@ __global__ __launch_bounds__(128) void kmeans_final_kernel(const KmProb* __restr
   const int64_t kd = static_cast<int64_t>(p.k) * d;
   const bool tables = p.bhi != nullptr;
   __shared__ double s_dot[128], s_c2[128], s_n2[128];
+  __shared__ double s_gs[FUSED ? kFusedGroupSums : 1];  // [group][dimension]
   __shared__ uint64_t s_n;
   __shared__ int s_last;
   // the count: integer adds, so any order gives the same value
@@ -379,8 +387,42 @@ __global__ __launch_bounds__(128) void kmeans_final_kernel(const KmProb* __restr
   __syncthreads();
   {
     uint64_t n = 0;
-    for (int64_t g = tid; g < p.ng; g += 128) n += to_global(p.gcount)[g * p.k + c];
+    if (FUSED) {
+      for (int64_t b = tid; b < p.nb; b += NTH) n += to_global(p.pcount)[b * p.k + c];
+    } else {
+      for (int64_t g = tid; g < p.ng; g += NTH) n += to_global(p.gcount)[g * p.k + c];
+    }
     if (n) atomicAdd(reinterpret_cast<unsigned long long*>(&s_n), static_cast<unsigned long long>(n));
+  }
+  if (FUSED) {
+    // group sums of this cluster: (group, dimension) pairs, dimensions
+    // across consecutive threads; each adds its group's blocks in order.
+    // A thread requests the partials of four pairs before adding any.
+    constexpr int U = 4;
+    const auto ps = to_global(p.psum) + static_cast<int64_t>(c) * d;
+    const int64_t np = p.ng * d;
+    for (int64_t e0 = tid; e0 < np; e0 += static_cast<int64_t>(NTH) * U) {
+      double v[U][kGroupBlocks];
+      int nbk[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int64_t e = e0 + static_cast<int64_t>(u) * NTH;
+        const int64_t g = e / d;
+        const int j = static_cast<int>(e - g * d);
+        const int64_t b0 = g * kGroupBlocks;
+        nbk[u] = e < np ? static_cast<int>(min<int64_t>(kGroupBlocks, p.nb - b0)) : 0;
+#pragma unroll
+        for (int i = 0; i < kGroupBlocks; ++i) v[u][i] = i < nbk[u] ? ps[(b0 + i) * kd + j] : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        double sg = 0.0;
+#pragma unroll
+        for (int i = 0; i < kGroupBlocks; ++i)
+          if (i < nbk[u]) sg = sg + v[u][i];
+        if (nbk[u] > 0) s_gs[e0 + static_cast<int64_t>(u) * NTH] = sg;
+      }
+    }
   }
   __syncthreads();
   const uint64_t n = s_n;
@@ -389,17 +431,22 @@ __global__ __launch_bounds__(128) void kmeans_final_kernel(const KmProb* __restr
   double dot = 0.0, c2 = 0.0, n2 = 0.0;
   for (int j0 = 0; j0 < d; j0 += 128) {
     const int j = j0 + tid;
-    if (j < d) {
-      const auto gs = to_global(p.gsum) + static_cast<int64_t>(c) * d + j;
-      // group sums in group order, 32 requested before they are added
+    if (tid < 128 && j < d) {
       double S = 0.0;
-      for (int64_t g0 = 0; g0 < p.ng; g0 += 32) {
-        double v[32];
+      if (FUSED) {
+        // group sums in group order
+        for (int64_t g = 0; g < p.ng; ++g) S = S + s_gs[g * d + j];
+      } else {
+        const auto gs = to_global(p.gsum) + static_cast<int64_t>(c) * d + j;
+        // group sums in group order, 32 requested before they are added
+        for (int64_t g0 = 0; g0 < p.ng; g0 += 32) {
+          double v[32];
 #pragma unroll
-        for (int i = 0; i < 32; ++i) v[i] = g0 + i < p.ng ? gs[(g0 + i) * kd] : 0.0;
+          for (int i = 0; i < 32; ++i) v[i] = g0 + i < p.ng ? gs[(g0 + i) * kd] : 0.0;
 #pragma unroll
-        for (int i = 0; i < 32; ++i)
-          if (g0 + i < p.ng) S = S + v[i];
+          for (int i = 0; i < 32; ++i)
+            if (g0 + i < p.ng) S = S + v[i];
+        }
       }
       to_global(p.sums)[static_cast<int64_t>(c) * d + j] = S;
       const double cj = to_global(p.cent)[static_cast<int64_t>(c) * p.ld + j];
@@ -455,12 +502,12 @@ __global__ __launch_bounds__(128) void kmeans_final_kernel(const KmProb* __restr
   if (!s_last) return;
   __threadfence();
   // The last workgroup: the terms (and centroid norms) of all clusters,
-  // loaded by every thread 128 at a time and added in cluster order by one.
+  // loaded by 128 threads at a time and added in cluster order by one.
   double s = 0.0, mx = 0.0;
   if (tid == 0) s = p.st->x2total;
   for (int c0 = 0; c0 < p.k; c0 += 128) {
     const int cc = c0 + tid;
-    if (cc < p.k) {
+    if (tid < 128 && cc < p.k) {
       s_dot[tid] = __hip_atomic_load(p.term + cc, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       if (tables)
         s_c2[tid] = __hip_atomic_load(p.c2d + cc, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -683,6 +730,16 @@ int kmeans_partial_launch(const sc_index* ix, KmWork& w, int k, const KmState* s
 int kmeans_reduce_launch(const KmProb* d_probs, const KmItems& items, bool update, bool control,
                          hipStream_t s) {
   ProfScope ps("kmeans_reduce", s);
+  if (items.fused && items.nprob == 1) {
+    // one problem whose group sums fit LDS: the final kernel forms them
+    if (items.nfinal > 0) {
+      hipLaunchKernelGGL((kmeans_final_kernel<true, true>), dim3(static_cast<unsigned>(items.nfinal)),
+                         dim3(kFusedThreads), 0, s, d_probs, items.fin, update ? 1 : 0,
+                         control ? 1 : 0);
+      SC_HIP(hipGetLastError());
+    }
+    return SC_OK;
+  }
   if (items.ngroup > 0) {
     hipLaunchKernelGGL(kmeans_group_kernel, dim3(static_cast<unsigned>(items.ngroup)),
                        dim3(kReduceChunk), 0, s, d_probs, items.group);
@@ -690,10 +747,10 @@ int kmeans_reduce_launch(const KmProb* d_probs, const KmItems& items, bool updat
   }
   if (items.nfinal > 0) {
     if (items.nprob <= 1) {
-      hipLaunchKernelGGL(kmeans_final_kernel<true>, dim3(static_cast<unsigned>(items.nfinal)),
+      hipLaunchKernelGGL((kmeans_final_kernel<true, false>), dim3(static_cast<unsigned>(items.nfinal)),
                          dim3(128), 0, s, d_probs, items.fin, update ? 1 : 0, control ? 1 : 0);
     } else {
-      hipLaunchKernelGGL(kmeans_final_kernel<false>, dim3(static_cast<unsigned>(items.nfinal)),
+      hipLaunchKernelGGL((kmeans_final_kernel<false, false>), dim3(static_cast<unsigned>(items.nfinal)),
                          dim3(128), 0, s, d_probs, items.fin, update ? 1 : 0, control ? 1 : 0);
       SC_HIP(hipGetLastError());
       hipLaunchKernelGGL(kmeans_close_kernel, dim3(static_cast<unsigned>(items.nprob)), dim3(128),
@@ -725,6 +782,7 @@ int KmItems::build(const KmProb* probs, size_t count, hipStream_t s) {
   ngroup = static_cast<int64_t>(g.size());
   nfinal = static_cast<int64_t>(f.size());
   nprob = static_cast<int64_t>(count);
+  fused = count == 1 && probs[0].ng * std::max(probs[0].d, 1) <= kFusedGroupSums;
   // staged copies: the host vectors may die when they return, and the host
   // does not wait for work already queued on s
   if (ngroup) {

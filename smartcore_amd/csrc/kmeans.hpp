@@ -115,6 +115,9 @@ struct KmItems {
   // problems (the first nprob of the array); with more than one the final
   // kernel leaves the distortion and stop rule to a per-problem close kernel
   int64_t nprob = 0;
+  // one problem whose group sums fit the final kernel's LDS: that kernel
+  // forms them, and the group launch is left out
+  bool fused = false;
   int build(const KmProb* probs, size_t count, hipStream_t s);
 };
 
