@@ -45,8 +45,11 @@
 //
 // Batched form.  Many independent problems (uneven n, d, k) run per Lloyd
 // iteration as one persistent label launch per shape class over (problem,
-// chunk) items, one sums launch over every (problem, block), then the shared
-// reduce kernels over every problem; the iterations are captured in one
+// chunk) items (the classes one after another, most work first), one sums
+// launch over every (problem, block), then the shared reduce kernels over
+// every problem.  The problems go in two groups of about equal row bytes:
+// the second group's rows are copied on a stream of their own while the
+// first group iterates, and each group's iterations are captured in one
 // hipGraph.  The label grid size is picked by timing the first iterations.
 // Every problem gets bit-identical results to sc_kmeans_run on it alone.
 #include <hip/hip_runtime.h>
