@@ -291,11 +291,13 @@ __global__ __launch_bounds__(kReduceChunk) void kmeans_group_kernel(
   const int64_t b0 = g * kGroupBlocks;
   const int nbk = static_cast<int>(min<int64_t>(kGroupBlocks, p.nb - b0));
   // the partials are loaded before the stop flag is read, so both requests
-  // are in flight together
+  // are in flight together; every load is unconditional (a block index
+  // clamped into range; values past the group's end are not added), so
+  // that all of them are issued before the first add
   double v[kGroupBlocks];
 #pragma unroll
   for (int i = 0; i < kGroupBlocks; ++i)
-    v[i] = i < nbk ? to_global(p.psum)[(b0 + i) * kd + e] : 0.0;
+    v[i] = to_global(p.psum)[min<int64_t>(b0 + i, p.nb - 1) * kd + e];
   if (*reinterpret_cast<volatile const uint32_t*>(&p.st->done)) return;
   double s = 0.0;
 #pragma unroll
@@ -308,10 +310,11 @@ __global__ __launch_bounds__(kReduceChunk) void kmeans_group_kernel(
     uint32_t w[kGroupBlocks];
 #pragma unroll
     for (int i = 0; i < kGroupBlocks; ++i)
-      w[i] = i < nbk ? to_global(p.pcount)[(b0 + i) * p.k + c] : 0u;
+      w[i] = to_global(p.pcount)[min<int64_t>(b0 + i, p.nb - 1) * p.k + c];
     uint64_t cnt = 0;
 #pragma unroll
-    for (int i = 0; i < kGroupBlocks; ++i) cnt += w[i];
+    for (int i = 0; i < kGroupBlocks; ++i)
+      if (i < nbk) cnt += w[i];
     to_global(p.gcount)[g * p.k + c] = cnt;
   }
 }
@@ -411,8 +414,10 @@ __global__ __launch_bounds__(FUSED ? kFusedThreads : 128) void kmeans_final_kern
         const int j = static_cast<int>(e - g * d);
         const int64_t b0 = g * kGroupBlocks;
         nbk[u] = e < np ? static_cast<int>(min<int64_t>(kGroupBlocks, p.nb - b0)) : 0;
+        // unconditional loads (block index clamped into range; values past
+        // the group's end are not added), all issued before the first add
 #pragma unroll
-        for (int i = 0; i < kGroupBlocks; ++i) v[u][i] = i < nbk[u] ? ps[(b0 + i) * kd + j] : 0.0;
+        for (int i = 0; i < kGroupBlocks; ++i) v[u][i] = ps[min<int64_t>(b0 + i, p.nb - 1) * kd + j];
       }
 #pragma unroll
       for (int u = 0; u < U; ++u) {
@@ -439,10 +444,11 @@ __global__ __launch_bounds__(FUSED ? kFusedThreads : 128) void kmeans_final_kern
       } else {
         const auto gs = to_global(p.gsum) + static_cast<int64_t>(c) * d + j;
         // group sums in group order, 32 requested before they are added
+        // (unconditional loads, the group index clamped into range)
         for (int64_t g0 = 0; g0 < p.ng; g0 += 32) {
           double v[32];
 #pragma unroll
-          for (int i = 0; i < 32; ++i) v[i] = g0 + i < p.ng ? gs[(g0 + i) * kd] : 0.0;
+          for (int i = 0; i < 32; ++i) v[i] = gs[min<int64_t>(g0 + i, p.ng - 1) * kd];
 #pragma unroll
           for (int i = 0; i < 32; ++i)
             if (g0 + i < p.ng) S = S + v[i];
