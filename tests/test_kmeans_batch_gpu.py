@@ -111,6 +111,32 @@ def test_large_batch_equals_single_runs():
         np.testing.assert_array_equal(ra[1], l2)
 
 
+def test_batch_group_of_one_problem():
+    """One problem of 500,000 x 64 rows (128 MB, nine tenths of the bytes)
+    and three small ones: the first group holds the large problem alone, so
+    its reduce forms the group sums inside the final kernel, as a single
+    fit does; the second group's three problems take the per-problem close
+    kernel.  Every problem equals its single-problem fit bit for bit."""
+    rng = np.random.default_rng(57)
+    probs = []
+    for n, k in [(500_000, 48), (20_000, 16), (15_000, 100), (25_000, 7)]:
+        d = 64
+        centres = rng.normal(0.0, 4.0, size=(k, d))
+        x = (centres[rng.integers(0, k, size=n)] + rng.normal(size=(n, d))).astype(np.float32)
+        init = x[np.sort(rng.choice(n, size=k, replace=False))].astype(np.float64)
+        probs.append((x, init))
+    iters = 8
+    got = S.kmeans_batch_run(probs, iters, S.SC_MIXED)
+    assert S.last_stats()["upload_groups"] == 2
+    for (x, init), (c, lab, size, dist, it) in zip(probs, got):
+        c1, l1, s1, d1, i1 = single(x, init, iters, S.SC_MIXED)
+        np.testing.assert_array_equal(lab, l1)
+        np.testing.assert_array_equal(size, s1)
+        assert it == i1
+        np.testing.assert_array_equal(c.view(np.uint64), c1.view(np.uint64))
+        assert np.float64(dist).view(np.uint64) == np.float64(d1).view(np.uint64)
+
+
 def test_batch_fp64_mode_runs_problems_one_by_one():
     probs = synth.kmeans_batch_problems(seed=52, count=4, n_lo=300, n_hi=3_000)
     got = S.kmeans_batch_run(probs, 5, S.SC_PLAIN_FP64)
