@@ -8,8 +8,9 @@ already resident in HBM.  `--workload kmeans` runs the k-means Lloyd loop on
 1,000,000 x 64 f32 Gaussian blobs, k = 64, up to 100 iterations (one step =
 one fit; the reported rate is Lloyd iterations per second).  `--workload
 batch` runs 512 uneven k-means problems in one batched call, `--workload
-serve` the continuous-batching server at fixed Poisson arrival rates, and the
-default `all` reports both beside the kNN and k-means figures.
+serve` the continuous-batching server at fixed Poisson arrival rates,
+`--workload dbscan` a DBSCAN fit over the device radius search, and the
+default `all` reports them all beside the kNN line.
 
 Prints one JSON line (rank 0) with the throughput, a roofline entry for the
 dominant kernel (its duration measured with HIP events on the stream it
@@ -260,6 +261,52 @@ def bench_classifier(args):
                                "sample": f"all {xq.shape[0]:,} queries, fit + predict "
                                          f"({cpu_s:.2f} s); host {cpu_model()}, nproc {os.cpu_count()}"}
         out["label_mismatches_vs_oracle"] = int(np.sum(np.asarray(got) != want))
+    return out
+
+
+def bench_dbscan(args):
+    """DBSCAN::fit (dbscan.rs:274-354), eps = 1.5, min_samples = 10, Euclidian,
+    on 200,000 x 8 f64 rows of a 10-class Gaussian mixture (unit variance,
+    centres ~ N(0, 16 I); about 55 neighbours per row).  One step is one
+    sc_dbscan_fit call on the resident index: the batched device radius
+    search over every row, the neighbour lists copied to the host and the
+    reference's expansion loop there.  The rate counts n^2 distance
+    evaluations per fit, a unit that compares fits of different sizes.  The
+    CPU baseline is the oracle's restatement of the reference loop on the
+    first 40,000 rows, on one core; the device fit of the same rows must give
+    its labels and cluster count."""
+    from smartcore_amd import _capi as S
+    from smartcore_amd import synth
+
+    n, d, eps, min_samples, ns = 200_000, 8, 1.5, 10, 40_000
+    x, _, _, _ = synth.knn_mixture(seed=7, n=n, q=16, d=d)
+    ix = S.Index(x)
+    for _ in range(max(args.warmup, 1)):
+        ix.dbscan_fit(eps, min_samples)
+    reps = max(args.steps, 1)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        lab, nc = ix.dbscan_fit(eps, min_samples)
+    fit_s = (time.perf_counter() - t0) / reps
+    noise = int(np.sum(lab < 0))
+    out = {"workload": f"DBSCAN eps={eps}, min_samples={min_samples}, Euclidian, {n:,}x{d} f64 "
+                       "rows (10-class Gaussian mixture)",
+           "value": float(n) * n / fit_s, "unit": "distance evaluations/s",
+           "ms_per_fit": fit_s * 1e3, "clusters": int(nc), "noise_rows": noise, "dtype": "f64"}
+    if not args.no_cpu:
+        import oracle as O
+
+        xs = np.ascontiguousarray(x[:ns])
+        glab, gnc = S.Index(xs).dbscan_fit(eps, min_samples)
+        t0 = time.perf_counter()
+        olab, onc = O.dbscan_fit(xs, eps, min_samples)
+        cpu_s = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": float(ns) * ns / cpu_s, "unit": "distance evaluations/s",
+                               "cores": 1, "kind": "port",
+                               "sample": f"one fit of the first {ns:,} rows ({cpu_s:.1f} s); host "
+                                         f"{cpu_model()}, nproc {os.cpu_count()}"}
+        out["vs_oracle"] = {"rows": ns, "label_mismatches": int(np.sum(glab != olab)),
+                            "clusters_equal": int(gnc) == int(onc)}
     return out
 
 
@@ -618,7 +665,7 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", choices=["all", "knn", "classifier", "kmeans", "batch", "serve",
-                                           "copy"],
+                                           "dbscan", "copy"],
                     default="all",
                     help="all = the kNN line plus the k-means and serving figures")
     ap.add_argument("--serve-n", type=int, default=2_000_000)
@@ -651,6 +698,8 @@ def main():
         out = bench_serve(args)
     elif args.workload == "classifier":
         out = bench_classifier(args)
+    elif args.workload == "dbscan":
+        out = bench_dbscan(args)
     elif args.workload == "batch":
         out = bench_batch(args)
         out.update({"metric": METRIC, "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -670,6 +719,9 @@ def main():
             b_args = argparse.Namespace(**vars(args))
             b_args.steps, b_args.warmup = 1, 1
             out["kmeans_batch"] = bench_batch(b_args)
+            d_args = argparse.Namespace(**vars(args))
+            d_args.steps, d_args.warmup = 3, 1
+            out["dbscan"] = bench_dbscan(d_args)
             out["serve"] = bench_serve(args)
     print(json.dumps(out), flush=True)
 
