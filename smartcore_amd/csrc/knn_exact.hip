@@ -71,7 +71,7 @@ __device__ inline void heap_push_root(double* hd, int64_t* hi, int k, double d, 
 
 template <typename T, int M>
 __global__ __launch_bounds__(NT) void knn_scan_kernel(
-    const T* __restrict__ refs, int64_t n, int ld, const T* __restrict__ queries, int64_t q, int k,
+    const T* __restrict__ refs, int64_t n, int ld, int d, const T* __restrict__ queries, int64_t q, int k,
     double p, int64_t seg_len, int nseg, uint32_t* __restrict__ cand_idx,
     double* __restrict__ cand_s, uint32_t* __restrict__ cand_count, int cap) {
   __shared__ TileSmem<T> sm;
@@ -100,7 +100,7 @@ __global__ __launch_bounds__(NT) void knn_scan_kernel(
 
   for (int64_t t0 = r_begin; t0 < r_end; t0 += RT) {
     double acc[4][4];
-    tile_sums<T, M>(sm, acc, refs, t0, r_end, queries, q0, q, ld, p);
+    tile_sums<T, M>(sm, acc, refs, t0, r_end, queries, q0, q, ld, d, p);
 
     // Epilogue: mark rows whose sum beats the query's running k-th sum.
 #pragma unroll
@@ -829,7 +829,8 @@ int run_exact(const sc_index* index, const T* d_queries, int64_t q, int k, doubl
       {
         ProfScope ps("knn_scan", stream);
         hipLaunchKernelGGL((knn_scan_kernel<T, M>), grid, dim3(NT), 0, stream, refs, n, ld,
-                           d_queries + b0 * ld, qn, k, p, plan.seg_len, plan.nseg,
+                           static_cast<int>(index->d), d_queries + b0 * ld, qn, k, p,
+                           plan.seg_len, plan.nseg,
                            cidx.as<uint32_t>(), cs.as<double>(), ccnt.as<uint32_t>(), plan.cap);
         SC_HIP(hipGetLastError());
       }

@@ -26,7 +26,7 @@ constexpr int RT = kTileR;
 constexpr int NT = kTileThreads;
 
 template <typename T, int M, bool FILL>
-__global__ __launch_bounds__(NT) void radius_kernel(const T* __restrict__ refs, int64_t n, int ld,
+__global__ __launch_bounds__(NT) void radius_kernel(const T* __restrict__ refs, int64_t n, int ld, int d,
                                                     const T* __restrict__ queries, int64_t q,
                                                     double p, double radius, double sum_bound,
                                                     int64_t seg_len, int nseg,
@@ -50,7 +50,7 @@ __global__ __launch_bounds__(NT) void radius_kernel(const T* __restrict__ refs, 
 
   for (int64_t t0 = r_begin; t0 < r_end; t0 += RT) {
     double acc[4][4];
-    tile_sums<T, M>(sm, acc, refs, t0, r_end, queries, q0, q, ld, p);
+    tile_sums<T, M>(sm, acc, refs, t0, r_end, queries, q0, q, ld, d, p);
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       const int qi = tq * 4 + i;
@@ -115,7 +115,7 @@ int run_radius(const sc_index* ix, const T* dq, int64_t q, double p, double radi
   SC_HIP(cnt.alloc(sizeof(uint64_t) * q * nseg, s));
   dim3 grid(static_cast<unsigned>(nqt), static_cast<unsigned>(nseg));
   hipLaunchKernelGGL((radius_kernel<T, M, false>), grid, dim3(NT), 0, s, static_cast<const T*>(ix->rows),
-                     n, ld, dq, q, p, radius, bound, seg_len, static_cast<int>(nseg),
+                     n, ld, static_cast<int>(ix->d), dq, q, p, radius, bound, seg_len, static_cast<int>(nseg),
                      cnt.as<uint64_t>(), nullptr, nullptr);
   SC_HIP(hipGetLastError());
   std::vector<uint64_t> hc(static_cast<size_t>(q * nseg));
@@ -142,7 +142,7 @@ int run_radius(const sc_index* ix, const T* dq, int64_t q, double p, double radi
   SC_HIP(od.alloc(sizeof(double) * total, s));
   SC_HIP(hipMemcpyAsync(cnt.p, hc.data(), sizeof(uint64_t) * hc.size(), hipMemcpyHostToDevice, s));
   hipLaunchKernelGGL((radius_kernel<T, M, true>), grid, dim3(NT), 0, s, static_cast<const T*>(ix->rows),
-                     n, ld, dq, q, p, radius, bound, seg_len, static_cast<int>(nseg),
+                     n, ld, static_cast<int>(ix->d), dq, q, p, radius, bound, seg_len, static_cast<int>(nseg),
                      cnt.as<uint64_t>(), oi.as<int64_t>(), od.as<double>());
   SC_HIP(hipGetLastError());
   SC_HIP(hipMemcpyAsync(res->idx.data(), oi.p, sizeof(int64_t) * total, hipMemcpyDeviceToHost, s));

@@ -32,7 +32,7 @@ template <typename T, int M>
 __device__ __forceinline__ void tile_sums(TileSmem<T>& sm, double (&acc)[4][4],
                                           const T* __restrict__ refs, int64_t t0, int64_t r_end,
                                           const T* __restrict__ queries, int64_t q0, int64_t q,
-                                          int ld, double p) {
+                                          int ld, int d, double p) {
   const int tid = threadIdx.x;
   const int tq = tid >> 5;
   const int tr = tid & 31;
@@ -40,7 +40,10 @@ __device__ __forceinline__ void tile_sums(TileSmem<T>& sm, double (&acc)[4][4],
   for (int i = 0; i < 4; ++i)
 #pragma unroll
     for (int j = 0; j < 4; ++j) acc[i][j] = 0.0;
-  for (int c0 = 0; c0 < ld; c0 += kTileD) {
+  // columns d .. ld are the row padding (zeros): their terms are exact
+  // zeros added to non-negative sums, so stopping at d changes no bit
+  for (int c0 = 0; c0 < d; c0 += kTileD) {
+    const int kn = min(kTileD, d - c0);
     __syncthreads();
     for (int e = tid; e < kTileR * kTileD; e += kTileThreads) {
       const int r = e / kTileD, c = e % kTileD;
@@ -54,7 +57,7 @@ __device__ __forceinline__ void tile_sums(TileSmem<T>& sm, double (&acc)[4][4],
     }
     __syncthreads();
 #pragma unroll 4
-    for (int kk = 0; kk < kTileD; ++kk) {
+    for (int kk = 0; kk < kn; ++kk) {
       T a[4], b[4];
 #pragma unroll
       for (int i = 0; i < 4; ++i) a[i] = sm.Qs[kk][tq * 4 + i];
