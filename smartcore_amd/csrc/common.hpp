@@ -6,18 +6,47 @@
 
 #include <cstdint>
 #include <cstdio>
+#include <memory>
+#include <new>
 #include <string>
+#include <type_traits>
+#include <utility>
 #include <vector>
 
 #include "../../include/smartcore_amd.h"
+
+namespace sc {
+// A vector allocator whose value-less construct leaves the element
+// uninitialised: buffers that a device copy fills at once are not zeroed
+// first (tens of milliseconds for the neighbour lists of a large radius
+// search).
+template <typename T>
+struct DefaultInitAllocator : std::allocator<T> {
+  template <typename U>
+  struct rebind {
+    using other = DefaultInitAllocator<U>;
+  };
+  DefaultInitAllocator() = default;
+  template <typename U>
+  DefaultInitAllocator(const DefaultInitAllocator<U>&) noexcept {}
+  template <typename U>
+  void construct(U* p) noexcept(std::is_nothrow_default_constructible<U>::value) {
+    ::new (static_cast<void*>(p)) U;
+  }
+  template <typename U, typename... A>
+  void construct(U* p, A&&... a) {
+    ::new (static_cast<void*>(p)) U(std::forward<A>(a)...);
+  }
+};
+}  // namespace sc
 
 // Result of a radius search: per query, the matching rows in index order
 // (CSR offsets over the queries).
 struct sc_radius_result {
   size_t q = 0;
   std::vector<uint64_t> offsets;
-  std::vector<int64_t> idx;
-  std::vector<double> dist;
+  std::vector<int64_t, sc::DefaultInitAllocator<int64_t>> idx;
+  std::vector<double, sc::DefaultInitAllocator<double>> dist;  // empty when not asked for
 };
 
 namespace sc {
@@ -220,8 +249,9 @@ int knn_mixed_device(const sc_index* index, const void* d_queries, size_t q, siz
 
 // find_radius (linear_search.rs:89-108) for q padded device queries (stride
 // index->ld), all at once: res gets each query's rows in index order.
+// want_dist false: res->dist stays empty (callers that need the rows only).
 int radius_search_device(const sc_index* index, const void* d_queries, int64_t q,
                          sc_metric metric, int p, double radius, sc_radius_result* res,
-                         hipStream_t stream);
+                         hipStream_t stream, bool want_dist = true);
 
 }  // namespace sc

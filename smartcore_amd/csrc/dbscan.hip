@@ -46,8 +46,9 @@ int sc_dbscan_fit(const sc_index* index, double eps, size_t min_samples, sc_metr
   SC_HIP(hipSetDevice(index->device));
   hipStream_t s = stream_for(index);
   sc_radius_result nb;
+  // the expansion reads the neighbour rows only, not their distances
   if (int rc = radius_search_device(index, index->rows, static_cast<int64_t>(n), metric, p, eps,
-                                    &nb, s))
+                                    &nb, s, false))
     return rc;
   auto count = [&](size_t i) { return static_cast<size_t>(nb.offsets[i + 1] - nb.offsets[i]); };
   auto at = [&](size_t i, size_t j) { return nb.idx[nb.offsets[i] + j]; };
@@ -112,7 +113,8 @@ int sc_dbscan_predict(const sc_index* index, const int32_t* labels, size_t num_c
     DevBuf dq;
     SC_HIP(dq.alloc(q * index->ld * elem_size(index->dtype), s));
     if (int rc = upload_padded(queries, index->dtype, q, index->d, index->ld, dq.p, s)) return rc;
-    if (int rc = radius_search_device(index, dq.p, static_cast<int64_t>(q), metric, p, eps, &nb, s))
+    if (int rc = radius_search_device(index, dq.p, static_cast<int64_t>(q), metric, p, eps, &nb, s,
+                                      false))
       return rc;
   }
   // dbscan.rs:368-393

@@ -80,7 +80,7 @@ __global__ __launch_bounds__(NT) void radius_kernel(const T* __restrict__ refs, 
           bits &= bits - 1;
           const int rl = w * 32 + b;
           out_idx[run] = t0 + rl;
-          out_dist[run] = dbuf[tid][rl];
+          if (out_dist) out_dist[run] = dbuf[tid][rl];
           ++run;
         }
       }
@@ -91,7 +91,7 @@ __global__ __launch_bounds__(NT) void radius_kernel(const T* __restrict__ refs, 
 
 template <typename T, int M>
 int run_radius(const sc_index* ix, const T* dq, int64_t q, double p, double radius,
-               sc_radius_result* res, hipStream_t s) {
+               sc_radius_result* res, bool want_dist, hipStream_t s) {
   const int64_t n = static_cast<int64_t>(ix->n);
   const int ld = static_cast<int>(ix->ld);
   const int64_t nqt = (q + QT - 1) / QT;
@@ -135,33 +135,35 @@ int run_radius(const sc_index* ix, const T* dq, int64_t q, double p, double radi
   }
   res->offsets[q] = total;
   res->idx.resize(total);
-  res->dist.resize(total);
+  if (want_dist) res->dist.resize(total);
   if (total == 0) return SC_OK;
   DevBuf oi, od;
   SC_HIP(oi.alloc(sizeof(int64_t) * total, s));
-  SC_HIP(od.alloc(sizeof(double) * total, s));
+  if (want_dist) SC_HIP(od.alloc(sizeof(double) * total, s));
   SC_HIP(hipMemcpyAsync(cnt.p, hc.data(), sizeof(uint64_t) * hc.size(), hipMemcpyHostToDevice, s));
   hipLaunchKernelGGL((radius_kernel<T, M, true>), grid, dim3(NT), 0, s, static_cast<const T*>(ix->rows),
                      n, ld, static_cast<int>(ix->d), dq, q, p, radius, bound, seg_len, static_cast<int>(nseg),
-                     cnt.as<uint64_t>(), oi.as<int64_t>(), od.as<double>());
+                     cnt.as<uint64_t>(), oi.as<int64_t>(), want_dist ? od.as<double>() : nullptr);
   SC_HIP(hipGetLastError());
-  SC_HIP(hipMemcpyAsync(res->idx.data(), oi.p, sizeof(int64_t) * total, hipMemcpyDeviceToHost, s));
-  SC_HIP(hipMemcpyAsync(res->dist.data(), od.p, sizeof(double) * total, hipMemcpyDeviceToHost, s));
+  // staged through the pinned buffers (parallel host copies)
+  if (int rc = download_staged(res->idx.data(), oi.p, sizeof(int64_t) * total, s)) return rc;
+  if (want_dist)
+    if (int rc = download_staged(res->dist.data(), od.p, sizeof(double) * total, s)) return rc;
   SC_HIP(hipStreamSynchronize(s));
   return SC_OK;
 }
 
 template <typename T>
 int radius_dispatch(const sc_index* ix, const void* dq, int64_t q, sc_metric metric, int p,
-                    double radius, sc_radius_result* res, hipStream_t s) {
+                    double radius, sc_radius_result* res, bool want_dist, hipStream_t s) {
   const T* qs = static_cast<const T*>(dq);
   switch (metric) {
     case SC_EUCLIDIAN:
-      return run_radius<T, kEuclid>(ix, qs, q, 2.0, radius, res, s);
+      return run_radius<T, kEuclid>(ix, qs, q, 2.0, radius, res, want_dist, s);
     case SC_MANHATTAN:
-      return run_radius<T, kManhattan>(ix, qs, q, 1.0, radius, res, s);
+      return run_radius<T, kManhattan>(ix, qs, q, 1.0, radius, res, want_dist, s);
     case SC_MINKOWSKI:
-      return run_radius<T, kMinkowski>(ix, qs, q, static_cast<double>(p), radius, res, s);
+      return run_radius<T, kMinkowski>(ix, qs, q, static_cast<double>(p), radius, res, want_dist, s);
   }
   return fail(SC_ERR_UNSUPPORTED, "unknown metric");
 }
@@ -170,15 +172,15 @@ int radius_dispatch(const sc_index* ix, const void* dq, int64_t q, sc_metric met
 
 int radius_search_device(const sc_index* index, const void* d_queries, int64_t q,
                          sc_metric metric, int p, double radius, sc_radius_result* res,
-                         hipStream_t s) {
+                         hipStream_t s, bool want_dist) {
   res->q = static_cast<size_t>(q);
   res->offsets.assign(static_cast<size_t>(q) + 1, 0);
   res->idx.clear();
   res->dist.clear();
   if (q == 0 || index->n == 0) return SC_OK;
   return index->dtype == SC_F64
-             ? radius_dispatch<double>(index, d_queries, q, metric, p, radius, res, s)
-             : radius_dispatch<float>(index, d_queries, q, metric, p, radius, res, s);
+             ? radius_dispatch<double>(index, d_queries, q, metric, p, radius, res, want_dist, s)
+             : radius_dispatch<float>(index, d_queries, q, metric, p, radius, res, want_dist, s);
 }
 
 }  // namespace sc
