@@ -38,6 +38,35 @@ def test_radius_matches_oracle(metric, om, dtype):
         np.testing.assert_array_equal(got[r][1].view(np.uint64), od.view(np.uint64))
 
 
+def test_radius_one_segment_capped_pass_and_overflow():
+    """140,000 queries over 140,000 rows: every query block scans all rows
+    (one segment), so the search takes the capped single pass, where each
+    query's matches go to a slot of 256 entries and the queries with more are
+    searched again as a subset.  A dense blob gives its rows several hundred
+    matches.  Every list (indexes and distance bits) must equal the same
+    queries searched as a set of 2,000 (count and fill passes), and three
+    the oracle."""
+    rng = np.random.default_rng(61)
+    n, d = 140_000, 8
+    x = rng.normal(0.0, 4.0, size=(n, d))
+    x[:3000] = rng.normal(0.0, 0.3, size=(3000, d))
+    ix = S.Index(x)
+    radius = 0.9
+    got = ix.radius(x, radius)
+    counts = np.array([len(g[0]) for g in got])
+    over = np.flatnonzero(counts > 256)
+    assert 0 < len(over) < n
+    pick = np.concatenate([over[:1000], rng.choice(n, 1000, replace=False)])
+    ref = ix.radius(x[pick], radius)
+    for j, i in enumerate(pick):
+        np.testing.assert_array_equal(got[i][0], ref[j][0])
+        np.testing.assert_array_equal(got[i][1].view(np.uint64), ref[j][1].view(np.uint64))
+    for i in (int(over[0]), int(pick[-1]), 5_000):
+        oi, od = O.linear_find_radius(x, x[i], radius, O.EUCLIDIAN)
+        np.testing.assert_array_equal(got[i][0], oi)
+        np.testing.assert_array_equal(got[i][1].view(np.uint64), od.view(np.uint64))
+
+
 def test_radius_kat_and_errors():
     ref = np.arange(1, 11, dtype=np.float64)[:, None]
     ix = S.Index(ref)
