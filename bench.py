@@ -652,8 +652,11 @@ def bench_batch(args):
                                f"go up while the previous group iterates), per group one hipGraph of "
                                f"its iterations"},
         "roofline": {"bound": "hbm", "achieved": byts / graph_s / 1e9, "peak": PEAK_HBM / 1e9,
-                     "unit": "GB/s", "frac": byts / graph_s / PEAK_HBM, "traffic": None,
-                     "kernel": "label + sums + reduce kernels (graph of all iterations)"},
+                     "unit": "GB/s", "frac": byts / graph_s / PEAK_HBM,
+                     "traffic": measured_traffic("kmeans_batch")[0],
+                     "traffic_source": measured_traffic("kmeans_batch")[1],
+                     "traffic_per": "call (the iteration kernels of both groups, 20 iterations)",
+                     "kernel": "label + sums + reduce kernels (graphs of all iterations)"},
         "vs_fp64_form": vs_fp64,
         "cpu_baseline": cpu,
     }
