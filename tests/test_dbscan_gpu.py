@@ -46,6 +46,25 @@ def test_digits_equals_oracle(eps, ms):
     np.testing.assert_array_equal(ix.dbscan_predict(lab, nc, eps, q), O.dbscan_predict(fit, olab, onc, eps, q))
 
 
+def test_many_clusters_and_borders_equal_oracle():
+    """20,000 rows: four touching Gaussian blobs and uniform noise, eps 0.45,
+    min_samples 8 -- 37 clusters, most of them small groups on the blobs'
+    fringes, and many non-core rows next to more than one cluster.  The
+    device union-find over the core rows' edges, the numbering by smallest
+    core row and the lowest-cluster rule for the other rows must give the
+    reference loop's labels exactly."""
+    rng = np.random.default_rng(63)
+    parts = [rng.normal(c, 1.0, size=(4000, 4))
+             for c in ([0, 0, 0, 0], [3.2, 0, 0, 0], [0, 6, 0, 0], [9, 9, 0, 0])]
+    parts.append(rng.uniform(-6, 14, size=(4000, 4)))
+    x = np.concatenate(parts)
+    x = x[rng.permutation(len(x))]
+    lab, nc = S.Index(x).dbscan_fit(0.45, 8)
+    olab, onc = O.dbscan_fit(x, 0.45, 8)
+    assert nc == onc and nc > 20
+    np.testing.assert_array_equal(lab, olab)
+
+
 @pytest.mark.parametrize("metric,p,eps", [(S.SC_EUCLIDIAN, 2, 2.2), (S.SC_MANHATTAN, 1, 9.0),
                                           (S.SC_MINKOWSKI, 3, 1.7)])
 def test_blobs_equal_oracle(metric, p, eps):
