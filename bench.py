@@ -280,6 +280,9 @@ def bench_dbscan(args):
 
     n, d, eps, min_samples, ns = 200_000, 8, 1.5, 10, 40_000
     x, _, _, _ = synth.knn_mixture(seed=7, n=n, q=16, d=d)
+    lib = S.lib()
+    lib.sc_profile_enable.argtypes = [C.c_int]
+    lib.sc_profile_read.argtypes = [C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_uint64)]
     ix = S.Index(x)
     for _ in range(max(args.warmup, 1)):
         ix.dbscan_fit(eps, min_samples)
@@ -288,11 +291,24 @@ def bench_dbscan(args):
     for _ in range(reps):
         lab, nc = ix.dbscan_fit(eps, min_samples)
     fit_s = (time.perf_counter() - t0) / reps
+    # the dominant kernel (the capped radius pass) timed with HIP events on
+    # its stream in one more fit
+    lib.sc_profile_enable(1)
+    ix.dbscan_fit(eps, min_samples)
+    lib.sc_profile_enable(0)
+    kms, launches = profile_read(lib, "radius_capped")
+    k_s = kms / 1e3 / max(launches, 1)
+    flops = 3.0 * d * float(n) * n  # per pair: d subtractions and d fused multiply-adds
     noise = int(np.sum(lab < 0))
     out = {"workload": f"DBSCAN eps={eps}, min_samples={min_samples}, Euclidian, {n:,}x{d} f64 "
                        "rows (10-class Gaussian mixture)",
            "value": float(n) * n / fit_s, "unit": "distance evaluations/s",
-           "ms_per_fit": fit_s * 1e3, "clusters": int(nc), "noise_rows": noise, "dtype": "f64"}
+           "ms_per_fit": fit_s * 1e3, "clusters": int(nc), "noise_rows": noise, "dtype": "f64",
+           "roofline": {"bound": "fp64-valu", "achieved": flops / k_s / 1e12 if k_s > 0 else None,
+                        "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s",
+                        "frac": flops / k_s / PEAK_FP64 if k_s > 0 else None, "traffic": None,
+                        "kernel": "radius_kernel (capped pass over all rows)",
+                        "avg_ms": k_s * 1e3}}
     if not args.no_cpu:
         import oracle as O
 

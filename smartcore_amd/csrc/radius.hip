@@ -160,11 +160,14 @@ int run_radius_capped(const sc_index* ix, const T* dq, int64_t q, double p, doub
   SC_HIP(cnt.alloc(sizeof(uint64_t) * q, s));
   SC_HIP(ci.alloc(sizeof(int64_t) * q * kCap, s));
   if (want_dist) SC_HIP(cd.alloc(sizeof(double) * q * kCap, s));
-  hipLaunchKernelGGL((radius_kernel<T, M, kCapped>), dim3(static_cast<unsigned>(nqt), 1), dim3(NT), 0,
-                     s, static_cast<const T*>(ix->rows), n, ld, static_cast<int>(ix->d), dq, q, p,
-                     radius, bound, tiles * RT, 1, cnt.as<uint64_t>(), ci.as<int64_t>(),
-                     want_dist ? cd.as<double>() : nullptr, kCap);
-  SC_HIP(hipGetLastError());
+  {
+    ProfScope ps("radius_capped", s);
+    hipLaunchKernelGGL((radius_kernel<T, M, kCapped>), dim3(static_cast<unsigned>(nqt), 1), dim3(NT),
+                       0, s, static_cast<const T*>(ix->rows), n, ld, static_cast<int>(ix->d), dq, q,
+                       p, radius, bound, tiles * RT, 1, cnt.as<uint64_t>(), ci.as<int64_t>(),
+                       want_dist ? cd.as<double>() : nullptr, kCap);
+    SC_HIP(hipGetLastError());
+  }
   std::vector<uint64_t> hc(static_cast<size_t>(q));
   SC_HIP(hipMemcpyAsync(hc.data(), cnt.p, sizeof(uint64_t) * hc.size(), hipMemcpyDeviceToHost, s));
   SC_HIP(hipStreamSynchronize(s));
