@@ -39,7 +39,9 @@ constexpr int64_t kCap = 256;
 // Largest capped buffer (index and distance slots) one search may take.
 constexpr double kCapBytes = 1024.0 * 1024.0 * 1024.0;
 
-template <typename T, int M, int MODE>
+// DIST: the matches' distances are written too (their LDS buffer, 32 KB,
+// would otherwise cost occupancy).
+template <typename T, int M, int MODE, bool DIST>
 __global__ __launch_bounds__(NT) void radius_kernel(const T* __restrict__ refs, int64_t n, int ld, int d,
                                                     const T* __restrict__ queries, int64_t q,
                                                     double p, double radius, double sum_bound,
@@ -48,7 +50,7 @@ __global__ __launch_bounds__(NT) void radius_kernel(const T* __restrict__ refs, 
                                                     int64_t* __restrict__ out_idx,
                                                     double* __restrict__ out_dist, int64_t cap) {
   __shared__ TileSmem<T> sm;
-  __shared__ double dbuf[QT][RT];
+  __shared__ double dbuf[DIST ? QT : 1][DIST ? RT : 1];
   __shared__ uint32_t mask[QT][RT / 32];
   const int tid = threadIdx.x;
   const int tq = tid >> 5;
@@ -74,7 +76,7 @@ __global__ __launch_bounds__(NT) void radius_kernel(const T* __restrict__ refs, 
         if (q0 + qi < q && t0 + rl < r_end && acc[i][j] <= sum_bound) {
           const double dv = finalize<M>(acc[i][j], p);
           if (dv <= radius) {
-            dbuf[qi][rl] = dv;
+            if constexpr (DIST) dbuf[qi][rl] = dv;
             atomicOr(&mask[qi][rl >> 5], 1u << (rl & 31));
           }
         }
@@ -95,11 +97,11 @@ __global__ __launch_bounds__(NT) void radius_kernel(const T* __restrict__ refs, 
           const int rl = w * 32 + b;
           if (MODE == kFill) {
             out_idx[run] = t0 + rl;
-            if (out_dist) out_dist[run] = dbuf[tid][rl];
+            if constexpr (DIST) out_dist[run] = dbuf[tid][rl];
           } else if (run < static_cast<uint64_t>(cap)) {
             const size_t o = static_cast<size_t>(my_q) * static_cast<size_t>(cap) + run;
             out_idx[o] = t0 + rl;
-            if (out_dist) out_dist[o] = dbuf[tid][rl];
+            if constexpr (DIST) out_dist[o] = dbuf[tid][rl];
           }
           ++run;
         }
@@ -162,10 +164,16 @@ int run_radius_capped(const sc_index* ix, const T* dq, int64_t q, double p, doub
   if (want_dist) SC_HIP(cd.alloc(sizeof(double) * q * kCap, s));
   {
     ProfScope ps("radius_capped", s);
-    hipLaunchKernelGGL((radius_kernel<T, M, kCapped>), dim3(static_cast<unsigned>(nqt), 1), dim3(NT),
-                       0, s, static_cast<const T*>(ix->rows), n, ld, static_cast<int>(ix->d), dq, q,
-                       p, radius, bound, tiles * RT, 1, cnt.as<uint64_t>(), ci.as<int64_t>(),
-                       want_dist ? cd.as<double>() : nullptr, kCap);
+    if (want_dist)
+      hipLaunchKernelGGL((radius_kernel<T, M, kCapped, true>), dim3(static_cast<unsigned>(nqt), 1),
+                         dim3(NT), 0, s, static_cast<const T*>(ix->rows), n, ld,
+                         static_cast<int>(ix->d), dq, q, p, radius, bound, tiles * RT, 1,
+                         cnt.as<uint64_t>(), ci.as<int64_t>(), cd.as<double>(), kCap);
+    else
+      hipLaunchKernelGGL((radius_kernel<T, M, kCapped, false>), dim3(static_cast<unsigned>(nqt), 1),
+                         dim3(NT), 0, s, static_cast<const T*>(ix->rows), n, ld,
+                         static_cast<int>(ix->d), dq, q, p, radius, bound, tiles * RT, 1,
+                         cnt.as<uint64_t>(), ci.as<int64_t>(), nullptr, kCap);
     SC_HIP(hipGetLastError());
   }
   std::vector<uint64_t> hc(static_cast<size_t>(q));
@@ -255,7 +263,7 @@ int run_radius(const sc_index* ix, const T* dq, int64_t q, double p, double radi
   DevBuf cnt;
   SC_HIP(cnt.alloc(sizeof(uint64_t) * q * nseg, s));
   dim3 grid(static_cast<unsigned>(nqt), static_cast<unsigned>(nseg));
-  hipLaunchKernelGGL((radius_kernel<T, M, kCount>), grid, dim3(NT), 0, s, static_cast<const T*>(ix->rows),
+  hipLaunchKernelGGL((radius_kernel<T, M, kCount, false>), grid, dim3(NT), 0, s, static_cast<const T*>(ix->rows),
                      n, ld, static_cast<int>(ix->d), dq, q, p, radius, bound, seg_len, static_cast<int>(nseg),
                      cnt.as<uint64_t>(), nullptr, nullptr, int64_t{0});
   SC_HIP(hipGetLastError());
@@ -282,10 +290,16 @@ int run_radius(const sc_index* ix, const T* dq, int64_t q, double p, double radi
   SC_HIP(oi.alloc(sizeof(int64_t) * total, s));
   if (want_dist) SC_HIP(od.alloc(sizeof(double) * total, s));
   SC_HIP(hipMemcpyAsync(cnt.p, hc.data(), sizeof(uint64_t) * hc.size(), hipMemcpyHostToDevice, s));
-  hipLaunchKernelGGL((radius_kernel<T, M, kFill>), grid, dim3(NT), 0, s, static_cast<const T*>(ix->rows),
-                     n, ld, static_cast<int>(ix->d), dq, q, p, radius, bound, seg_len, static_cast<int>(nseg),
-                     cnt.as<uint64_t>(), oi.as<int64_t>(), want_dist ? od.as<double>() : nullptr,
-                     int64_t{0});
+  if (want_dist)
+    hipLaunchKernelGGL((radius_kernel<T, M, kFill, true>), grid, dim3(NT), 0, s,
+                       static_cast<const T*>(ix->rows), n, ld, static_cast<int>(ix->d), dq, q, p,
+                       radius, bound, seg_len, static_cast<int>(nseg), cnt.as<uint64_t>(),
+                       oi.as<int64_t>(), od.as<double>(), int64_t{0});
+  else
+    hipLaunchKernelGGL((radius_kernel<T, M, kFill, false>), grid, dim3(NT), 0, s,
+                       static_cast<const T*>(ix->rows), n, ld, static_cast<int>(ix->d), dq, q, p,
+                       radius, bound, seg_len, static_cast<int>(nseg), cnt.as<uint64_t>(),
+                       oi.as<int64_t>(), nullptr, int64_t{0});
   SC_HIP(hipGetLastError());
   // staged through the pinned buffers (parallel host copies)
   if (int rc = download_staged(res->idx.data(), oi.p, sizeof(int64_t) * total, s)) return rc;
